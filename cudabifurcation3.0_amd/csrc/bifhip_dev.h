@@ -1,0 +1,220 @@
+// bifhip_dev.h -- device-side building blocks shared by the gfx950 kernels.
+//
+// Everything here is compiled with -ffp-contract=off: an expression written
+// as a*b + c is a multiply and an add, each rounded, exactly as in the
+// reference's host-compiled semantics.  Fast paths ask for fused
+// multiply-adds explicitly with fma()/fmaf().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bifhip {
+
+constexpr int kMaxVal = 16;   // parameter vector capacity
+constexpr int kWave = 64;     // CDNA wavefront width
+
+// Model ids (the public numbering, see include/bifhip.h).
+enum ModelId : int { kRossler = 0, kLinear = 1 };
+
+// Grid description handed to every map kernel by value.
+//   value of swept parameter k for flattened cell idx (getValueByIdx,
+//   Library/cudaLibrary.cu:210-216):
+//     start[k] + (double)(((int)((float)idx / pk[k])) % nPts) * step[k]
+//   pk[k] = powf(nPts, k) and step[k] = (end-start)/(nPts-1) are evaluated
+//   on the host with the same IEEE operations the reference uses.
+struct Grid {
+    int nPts;
+    int dimension;
+    int mut[2];
+    float pk[2];
+    double start[2];
+    double step[2];
+    int nIC;
+    int nVal;
+    double ic[8];
+    double values[kMaxVal];
+};
+
+// Arguments of the bifurcation kernels (k_bif.hip).
+struct BifArgs {
+    Grid g;
+    double h;
+    double maxValue;
+    double hpf;          // peakFinder's h (Library/cudaLibrary.cuh:204)
+    int cellBase;        // flattened index of launch cell 0 (amountOfCalculatedPoints)
+    const int* cellList; // optional launch-cell -> flattened index map
+    int nCells;
+    int skip;            // transient steps
+    int N;               // recorded samples (amountOfPointsInBlock)
+    int pre;             // preScaller
+    int cap;             // peak slots per cell row
+    long long rowStride; // doubles between cell rows
+};
+
+// Arguments of the Lyapunov kernels (k_lyap.hip).
+struct LyapArgs {
+    Grid g;
+    double h;
+    double eps;
+    double tMax;
+    double maxValue;
+    int cellBase;
+    const int* cellList;
+    int nCells;
+    int skip;            // transient steps
+    int nNT;             // steps between renormalisations, (size_t)(NT/h)
+    int nBlocks;         // renormalisations, (int)(tMax/NT)
+    const double* dirs;  // per launch cell: 3 (LLE) or 9 (LS) doubles; null = axis
+};
+
+// Flattened cell index -> parameter value (float division is IEEE-correct
+// under -fhip-fp32-correctly-rounded-divide-sqrt).
+__device__ __forceinline__ double grid_value(const Grid& g, int idx, int k)
+{
+    float q = (float)idx / g.pk[k];
+    int t = (int)q;
+    return g.start[k] + (double)(t % g.nPts) * g.step[k];
+}
+
+// ---------------------------------------------------------------- models
+//
+// Rossler, Library/cudaLibrary.cu:15-17: sequential explicit Euler.  The
+// exact form keeps the reference's operation order (12 roundings/step).
+struct Rossler {
+    static constexpr int D = 3;
+    struct P64 { double a, b, c; };
+    struct P32 { float a, b, c, h; float ha1p, hb; };
+
+    __device__ __forceinline__ static P64 params(const double* v) { return {v[0], v[1], v[2]}; }
+
+    __device__ __forceinline__ static void step(double& x0, double& x1, double& x2,
+                                                const P64& p, double h)
+    {
+        x0 = x0 + h * (-x1 - x2);
+        x1 = x1 + h * (x0 + p.a * x1);
+        x2 = x2 + h * (p.b + x2 * (x0 - p.c));
+    }
+
+    // FP32 fast form of the same map with fused multiply-adds (7 ops):
+    //   x0 <- x0 - h(x1+x2)
+    //   x1 <- x1(1+ha) + h x0
+    //   x2 <- x2(1 + h(x0-c)) + hb
+    __device__ __forceinline__ static P32 params32(const double* v, double h)
+    {
+        P32 q;
+        q.a = (float)v[0]; q.b = (float)v[1]; q.c = (float)v[2]; q.h = (float)h;
+        q.ha1p = (float)(1.0 + h * v[0]);
+        q.hb = (float)(h * v[1]);
+        return q;
+    }
+    __device__ __forceinline__ static void step32(float& x0, float& x1, float& x2, const P32& p)
+    {
+        x0 = fmaf(-p.h, x1 + x2, x0);
+        x1 = fmaf(x1, p.ha1p, p.h * x0);
+        x2 = fmaf(x2, fmaf(p.h, x0 - p.c, 1.0f), p.hb);
+    }
+    // Deviation u (true deviation = s*u) of the FP32 map along (x -> x'),
+    // exact difference of two trajectories x and x + s*u, given the
+    // pre-step x2 (x2o) and post-step x0 (x0n):
+    //   u0' = u0 - h(u1+u2)
+    //   u1' = u1(1+ha) + h u0'
+    //   u2' = u2 + h(u2 (x0n + s u0' - c) + x2o u0')
+    __device__ __forceinline__ static void dev32(float& u0, float& u1, float& u2, float x0n,
+                                                 float x2o, float s, const P32& p)
+    {
+        u0 = fmaf(-p.h, u1 + u2, u0);
+        u1 = fmaf(u1, p.ha1p, p.h * u0);
+        float g = fmaf(s, u0, x0n - p.c);
+        u2 = fmaf(p.h, fmaf(u2, g, x2o * u0), u2);
+    }
+    // Tangent map (variational equation of the discrete step), FP32.
+    //   v0' = v0 - h(v1+v2); v1' = v1(1+ha) + h v0'; v2' = v2(1+h(x0n-c)) + h x2o v0'
+    __device__ __forceinline__ static void tan32(float& v0, float& v1, float& v2, float x0n,
+                                                 float x2o, const P32& p)
+    {
+        v0 = fmaf(-p.h, v1 + v2, v0);
+        v1 = fmaf(v1, p.ha1p, p.h * v0);
+        v2 = fmaf(v2, fmaf(p.h, x0n - p.c, 1.0f), (p.h * x2o) * v0);
+    }
+};
+
+// Linear diagonal test model x_i <- x_i + h*(a_i*x_i); its Lyapunov
+// spectrum is log|1+h a_i|/h.  Used only by the closed-form tests.
+struct Linear {
+    static constexpr int D = 3;
+    struct P64 { double a, b, c; };
+    struct P32 { float a, b, c, h; float ha1p, hb; };
+    __device__ __forceinline__ static P64 params(const double* v) { return {v[0], v[1], v[2]}; }
+    __device__ __forceinline__ static void step(double& x0, double& x1, double& x2,
+                                                const P64& p, double h)
+    {
+        x0 = x0 + h * (p.a * x0);
+        x1 = x1 + h * (p.b * x1);
+        x2 = x2 + h * (p.c * x2);
+    }
+    __device__ __forceinline__ static P32 params32(const double* v, double h)
+    {
+        P32 q;
+        q.a = (float)(1.0 + h * v[0]); q.b = (float)(1.0 + h * v[1]);
+        q.c = (float)(1.0 + h * v[2]); q.h = (float)h; q.ha1p = 0; q.hb = 0;
+        return q;
+    }
+    __device__ __forceinline__ static void step32(float& x0, float& x1, float& x2, const P32& p)
+    {
+        x0 *= p.a; x1 *= p.b; x2 *= p.c;
+    }
+    __device__ __forceinline__ static void dev32(float& u0, float& u1, float& u2, float, float,
+                                                 float, const P32& p)
+    {
+        u0 *= p.a; u1 *= p.b; u2 *= p.c;
+    }
+    __device__ __forceinline__ static void tan32(float& v0, float& v1, float& v2, float, float,
+                                                 const P32& p)
+    {
+        v0 *= p.a; v1 *= p.b; v2 *= p.c;
+    }
+};
+
+// Load the cell's parameter vector with the swept entries substituted
+// (calculateDiscreteModelCUDA, Library/cudaLibrary.cu:96-105).
+__device__ __forceinline__ void cell_values(const Grid& g, int idx, double* v)
+{
+#pragma unroll
+    for (int i = 0; i < kMaxVal; ++i) v[i] = g.values[i];
+    for (int k = 0; k < g.dimension; ++k) {
+        const double val = grid_value(g, idx, k);
+        // dynamic index into a small register array: keep it a select chain
+#pragma unroll
+        for (int i = 0; i < kMaxVal; ++i)
+            if (i == g.mut[k]) v[i] = val;
+    }
+}
+
+// Divergence test of loopCalculateDiscreteModel (Library/cudaLibrary.cu:69-71):
+// |(float)x| > maxValue in double, only when maxValue != 0.
+__device__ __forceinline__ bool diverged(double xw, double maxValue)
+{
+    return (double)fabsf((float)xw) > maxValue;
+}
+
+// ------------------------------------------------------- wave utilities
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffull), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+}  // namespace bifhip

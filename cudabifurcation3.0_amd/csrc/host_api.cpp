@@ -1,0 +1,1246 @@
+// host_api.cpp -- C ABI of libbifhip.so: context, drivers, CSV writers.
+//
+// The drivers keep the reference's host-function surface
+// (Library/hostLibrary.cu) -- argument meaning, derived step counts and
+// output files -- and drive the gfx950 kernels in k_bif.hip / k_lyap.hip.
+// Unlike the reference they never stage a full trajectory in HBM: the
+// integrator streams samples straight into the peak finder, so memory per
+// cell is a small peak row and a whole map usually fits in one launch.
+#include "bifhip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "bifhip_dev.h"
+
+// kernel launchers (k_bif.hip, k_lyap.hip) -------------------------------
+namespace bifhip {
+hipError_t launch_bif(int model, const BifArgs& A, int wv, bool check, int* counts, double* amps,
+                      double* ivls, hipStream_t st);
+hipError_t launch_series(int model, const BifArgs& A, int wv, double* data, int* flags,
+                         hipStream_t st);
+hipError_t launch_peak_finder(const double* data, int sizeOfBlock, int nBlocks, int* counts,
+                              double* outPeaks, double* outIntervals, double hpf, hipStream_t st);
+hipError_t launch_dbscan_wave(int nCells, const int* counts, const double* amps,
+                              const double* ivls, long long rowStride, int cap, int maxPeaks,
+                              double T, int* out, int* overflow, int* nOverflow, int listBase,
+                              hipStream_t st);
+hipError_t launch_dbscan_lane(int nList, const int* list, const int* counts, const double* amps,
+                              const double* ivls, long long rowStride, double T, int* scratch,
+                              long long scratchStride, int* out, const int* outIndex,
+                              hipStream_t st);
+hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, long long rowStride,
+                           int cap, int maxPeaks, int kdeSampling, double I1, double I2, double H,
+                           int* out, const int* outIndex, int* overflow, int* nOverflow,
+                           int listBase, hipStream_t st);
+hipError_t launch_clamp(int n, int* v, int clamp, hipStream_t st);
+hipError_t launch_fill(long long n, int* v, int value, hipStream_t st);
+hipError_t launch_lle(int model, int precision, const LyapArgs& A, bool check, double* out,
+                      hipStream_t st);
+hipError_t launch_ls(int model, const LyapArgs& A, bool check, double* out, hipStream_t st);
+hipError_t launch_gs(const double* a, double* b, double* den, int n, int count, hipStream_t st);
+}  // namespace bifhip
+
+using namespace bifhip;
+
+// ------------------------------------------------------------- context
+
+namespace {
+
+struct Ctx {
+    std::string outPath = "mat.csv";
+    int model = BIFHIP_MODEL_ROSSLER;
+    int precision = BIFHIP_PREC_FP64;
+    int pert = BIFHIP_PERT_REFERENCE;
+    long long dirChunk = -1;
+    long long maxCellsPerLaunch = 0;
+    int peakCap = 256;
+    int clusterClamp = 0;
+    int verbose = 0;
+    int timing = 0;
+    std::string lastError;
+    hipStream_t stream = nullptr;
+};
+Ctx g;
+
+// per-kernel-family timing: pending event pairs resolved after each call
+struct TimerSlot {
+    double ms = 0;
+    long long launches = 0;
+};
+std::map<std::string, TimerSlot> g_times;
+struct PendingEv {
+    std::string name;
+    hipEvent_t a, b;
+};
+std::vector<PendingEv> g_pending;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g.lastError = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((int)e_, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                        \
+    } while (0)
+
+#define RC_TRY(expr)                  \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != BIFHIP_OK) return rc_; \
+    } while (0)
+
+// scoped timing of one launch family
+struct Timed {
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit Timed(const char* n) : name(n)
+    {
+        if (!g.timing) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            a = b = nullptr;
+            return;
+        }
+        hipEventRecord(a, g.stream);
+    }
+    ~Timed()
+    {
+        if (!a) return;
+        hipEventRecord(b, g.stream);
+        g_pending.push_back({name, a, b});
+    }
+};
+
+void resolve_timers()
+{
+    for (auto& p : g_pending) {
+        float ms = 0;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto& s = g_times[p.name];
+            s.ms += ms;
+            s.launches += 1;
+        }
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    g_pending.clear();
+}
+
+// device buffer that frees itself
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n)
+    {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        bytes = n;
+        if (n == 0) return hipSuccess;
+        return hipMalloc(&p, n);
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// cached workspaces reused across calls (released by bifhip_release)
+struct Pool {
+    std::map<std::string, void*> ptr;
+    std::map<std::string, size_t> size;
+    hipError_t get(const char* key, size_t bytes, void** out)
+    {
+        auto it = ptr.find(key);
+        if (it != ptr.end() && size[key] >= bytes) {
+            *out = it->second;
+            return hipSuccess;
+        }
+        if (it != ptr.end()) {
+            (void)hipFree(it->second);
+            ptr.erase(it);
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+        if (e != hipSuccess) return e;
+        ptr[key] = p;
+        size[key] = bytes ? bytes : 16;
+        *out = p;
+        return hipSuccess;
+    }
+    void release()
+    {
+        for (auto& kv : ptr) (void)hipFree(kv.second);
+        ptr.clear();
+        size.clear();
+    }
+};
+Pool g_pool;
+
+template <class T>
+int pool_get(const char* key, size_t count, T** out)
+{
+    void* p = nullptr;
+    HIP_TRY(g_pool.get(key, count * sizeof(T), &p));
+    *out = static_cast<T*>(p);
+    return BIFHIP_OK;
+}
+
+int ensure_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(BIFHIP_E_NODEV, "no HIP device visible");
+    return BIFHIP_OK;
+}
+
+// getValueByIdx on the host (Library/cudaLibrary.cu:210-216)
+double value_by_idx(int idx, int nPts, double s, double f, int k)
+{
+    float p = powf((float)nPts, (float)k);
+    float q = (float)idx / p;
+    int t = (int)q;
+    return s + ((double)(t % nPts) * ((double)(f - s) / (double)(nPts - 1)));
+}
+
+int model_dim(int model) { return model == BIFHIP_MODEL_LINEAR ? 3 : 3; }
+
+int make_grid(Grid& G, int nPts, int dimension, const double* ranges, const int* mut,
+              const double* ic, int nIC, const double* values, int nVal)
+{
+    std::memset(&G, 0, sizeof(G));
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1 (got %d)", nPts);
+    if (dimension < 1 || dimension > 2) return fail(BIFHIP_E_ARG, "dimension must be 1 or 2");
+    if (!ranges || !mut || !ic || !values) return fail(BIFHIP_E_ARG, "null input array");
+    if (nIC != model_dim(g.model))
+        return fail(BIFHIP_E_ARG, "model %d has %d state variables, got %d initial conditions",
+                    g.model, model_dim(g.model), nIC);
+    if (nVal < 3 || nVal > kMaxVal)
+        return fail(BIFHIP_E_ARG, "amountOfValues must be in [3, %d] (got %d)", kMaxVal, nVal);
+    const long long cells = dimension == 1 ? (long long)nPts : (long long)nPts * nPts;
+    if (cells > (1LL << 30)) return fail(BIFHIP_E_LIMIT, "grid of %lld cells is too large", cells);
+    G.nPts = nPts;
+    G.dimension = dimension;
+    G.nIC = nIC;
+    G.nVal = nVal;
+    for (int i = 0; i < nIC; ++i) G.ic[i] = ic[i];
+    for (int i = 0; i < nVal; ++i) G.values[i] = values[i];
+    for (int k = 0; k < dimension; ++k) {
+        if (mut[k] < 0 || mut[k] >= nVal)
+            return fail(BIFHIP_E_ARG, "indicesOfMutVars[%d] = %d out of range", k, mut[k]);
+        G.mut[k] = mut[k];
+        G.pk[k] = powf((float)nPts, (float)k);
+        G.start[k] = ranges[2 * k];
+        G.step[k] = (double)(ranges[2 * k + 1] - ranges[2 * k]) / (double)(nPts - 1);
+    }
+    return BIFHIP_OK;
+}
+
+long long cells_of(int nPts, int dimension)
+{
+    return dimension == 1 ? (long long)nPts : (long long)nPts * nPts;
+}
+
+// Least double T with (float)sqrt(T) >= eps: the exact DBSCAN neighbour
+// threshold (hypotf(a,b) < eps  <=>  (double)a*a + (double)b*b < T).
+double dbscan_threshold(double eps)
+{
+    if (!(eps > 0)) return 0.0;  // never a neighbour (also NaN eps)
+    if (std::isinf(eps)) return INFINITY;
+    auto ok = [&](double s) { return (double)(float)std::sqrt(s) >= eps; };
+    // search over the ordered bit patterns of non-negative doubles
+    uint64_t lo = 0, hi;
+    double inf = INFINITY;
+    std::memcpy(&hi, &inf, sizeof(hi));
+    // invariant: ok(hi) true (sqrt(inf) = inf >= eps), lo may be false
+    if (ok(0.0)) return 0.0;
+    while (hi - lo > 1) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        double m;
+        std::memcpy(&m, &mid, sizeof(m));
+        if (ok(m)) hi = mid; else lo = mid;
+    }
+    double T;
+    std::memcpy(&T, &hi, sizeof(T));
+    return T;
+}
+
+long long launch_chunk(long long cells)
+{
+    long long c = g.maxCellsPerLaunch > 0 ? g.maxCellsPerLaunch : cells;
+    if (c > (1LL << 26)) c = 1LL << 26;
+    return c < 1 ? 1 : c;
+}
+
+// ------------------------------------------------------- peak-map core
+
+struct PeakMapSpec {
+    Grid G;
+    int dimension;
+    long long cells;
+    double h, maxValue;
+    int skip, N, pre, wv;
+};
+
+int peak_spec(PeakMapSpec& S, int dimension, double tMax, int nPts, double h, int nIC,
+              const double* ic, const double* ranges, const int* mut, int wv, double maxValue,
+              double transientTime, const double* values, int nVal, int pre)
+{
+    RC_TRY(make_grid(S.G, nPts, dimension, ranges, mut, ic, nIC, values, nVal));
+    if (wv < 0 || wv >= nIC) return fail(BIFHIP_E_ARG, "writableVar %d out of range", wv);
+    if (pre == 0) return fail(BIFHIP_E_ARG, "preScaller must not be 0 (tMax/h/preScaller)");
+    // derived counts exactly as Library/hostLibrary.cu:22-23, :417-418, :588-589
+    const int N = (int)(tMax / h / pre);
+    const int skip = (int)(transientTime / h);
+    if (N < 0 || skip < 0) return fail(BIFHIP_E_ARG, "negative step counts");
+    S.dimension = dimension;
+    S.cells = cells_of(nPts, dimension);
+    S.h = h;
+    S.maxValue = maxValue;
+    S.skip = skip;
+    S.N = N;
+    S.pre = pre < 1 ? 1 : pre;
+    S.wv = wv;
+    return BIFHIP_OK;
+}
+
+BifArgs bif_args(const PeakMapSpec& S, int cellBase, const int* list, int n, int cap)
+{
+    BifArgs A;
+    A.g = S.G;
+    A.h = S.h;
+    A.maxValue = S.maxValue;
+    A.hpf = 0.01;  // peakFinder's default h (Library/cudaLibrary.cuh:204)
+    A.cellBase = cellBase;
+    A.cellList = list;
+    A.nCells = n;
+    A.skip = S.skip;
+    A.N = S.N;
+    A.pre = S.pre;
+    A.cap = cap;
+    A.rowStride = cap;
+    return A;
+}
+
+// Integrate cells [base, base+n) (or the listed ones) into peak rows.
+int integrate_peaks(const PeakMapSpec& S, int base, const int* d_list, int n, int cap,
+                    int* d_counts, double* d_amps, double* d_ivls)
+{
+    const long long chunk = launch_chunk(n);
+    for (long long off = 0; off < n; off += chunk) {
+        const int m = (int)std::min<long long>(chunk, n - off);
+        BifArgs A = bif_args(S, base + (int)off, d_list ? d_list + off : nullptr, m, cap);
+        Timed t("bif_peaks");
+        HIP_TRY(launch_bif(g.model, A, S.wv, S.maxValue != 0, d_counts + off,
+                           d_amps ? d_amps + off * cap : nullptr,
+                           d_ivls ? d_ivls + off * cap : nullptr, g.stream));
+    }
+    return BIFHIP_OK;
+}
+
+// Whole 2D/1D cluster map (DBSCAN when kind==0, KDE when kind==1) into d_out.
+struct ClusterParams {
+    int kind;
+    int maxPeaks;
+    double eps;
+    int kdeSampling;
+    double I1, I2, H;
+};
+
+int cluster_map_device(const PeakMapSpec& S, const ClusterParams& C, int* d_out)
+{
+    const long long cells = S.cells;
+    const int capMax = std::max(2, std::min(g.peakCap, S.N / 2 + 1));
+    int cap = capMax;
+    if ((long long)C.maxPeaks + 1 < cap) cap = (int)std::max(2LL, (long long)C.maxPeaks + 1);
+    // cells per pass bounded so the peak rows stay within ~8 GiB
+    const long long rowBytes = (long long)cap * 16 + 8;
+    long long pass = std::max<long long>(1, (8LL << 30) / rowBytes);
+    pass = std::min(pass, cells);
+    if (g.maxCellsPerLaunch > 0) pass = std::min(pass, g.maxCellsPerLaunch);
+
+    int *d_counts, *d_ovf, *d_novf;
+    double *d_amps, *d_ivls;
+    RC_TRY(pool_get("counts", (size_t)pass, &d_counts));
+    RC_TRY(pool_get("amps", (size_t)(pass * cap), &d_amps));
+    RC_TRY(pool_get("ivls", (size_t)(C.kind == 0 ? pass * cap : 1), &d_ivls));
+    RC_TRY(pool_get("ovf", (size_t)cells, &d_ovf));
+    RC_TRY(pool_get("novf", 1, &d_novf));
+    HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
+    const double T = dbscan_threshold(C.eps);
+
+    for (long long off = 0; off < cells; off += pass) {
+        const int n = (int)std::min(pass, cells - off);
+        RC_TRY(integrate_peaks(S, (int)off, nullptr, n, cap, d_counts, d_amps,
+                               C.kind == 0 ? d_ivls : nullptr));
+        if (C.kind == 0) {
+            Timed t("dbscan");
+            HIP_TRY(launch_dbscan_wave(n, d_counts, d_amps, d_ivls, cap, cap, C.maxPeaks, T,
+                                       d_out + off, d_ovf, d_novf, (int)off, g.stream));
+        } else {
+            Timed t("kde");
+            HIP_TRY(launch_kde_wave(n, d_counts, d_amps, cap, cap, C.maxPeaks, C.kdeSampling,
+                                    C.I1, C.I2, C.H, d_out + off, nullptr, d_ovf, d_novf,
+                                    (int)off, g.stream));
+        }
+    }
+    int novf = 0;
+    HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (novf > 0) {
+        // Rare: cells with more peaks than the rows hold (or than the wave
+        // kernel's 256 registers slots).  Recompute them with full rows.
+        std::vector<int> list(novf);
+        HIP_TRY(hipMemcpy(list.data(), d_ovf, sizeof(int) * novf, hipMemcpyDeviceToHost));
+        const int cap2 = (int)std::max(2LL, std::min((long long)S.N / 2 + 1, (long long)C.maxPeaks + 1));
+        DBuf cnt, am, iv, scr;
+        HIP_TRY(cnt.alloc(sizeof(int) * novf));
+        HIP_TRY(am.alloc(sizeof(double) * (size_t)novf * cap2));
+        HIP_TRY(iv.alloc(sizeof(double) * (size_t)novf * cap2));
+        RC_TRY(integrate_peaks(S, 0, d_ovf, novf, cap2, cnt.as<int>(), am.as<double>(),
+                               iv.as<double>()));
+        if (C.kind == 0) {
+            HIP_TRY(scr.alloc(sizeof(int) * (size_t)novf * 2 * cap2));
+            Timed t("dbscan");
+            HIP_TRY(launch_dbscan_lane(novf, nullptr, cnt.as<int>(), am.as<double>(),
+                                       iv.as<double>(), cap2, T, scr.as<int>(), 2LL * cap2, d_out,
+                                       d_ovf, g.stream));
+        } else {
+            // rows now hold every peak (cap2 >= P), so this pass cannot overflow
+            int *d_o2, *d_n2;
+            RC_TRY(pool_get("ovf2", (size_t)novf, &d_o2));
+            RC_TRY(pool_get("novf2", 1, &d_n2));
+            HIP_TRY(launch_fill(1, d_n2, 0, g.stream));
+            Timed t("kde");
+            HIP_TRY(launch_kde_wave(novf, cnt.as<int>(), am.as<double>(), cap2, cap2, C.maxPeaks,
+                                    C.kdeSampling, C.I1, C.I2, C.H, d_out, d_ovf, d_o2, d_n2, 0,
+                                    g.stream));
+        }
+        HIP_TRY(hipStreamSynchronize(g.stream));
+    }
+    if (C.kind == 0 && g.clusterClamp > 0) HIP_TRY(launch_clamp((int)cells, d_out, g.clusterClamp, g.stream));
+    return BIFHIP_OK;
+}
+
+// ------------------------------------------------------- Lyapunov core
+
+struct LyapSpec {
+    Grid G;
+    int dimension;
+    long long cells;
+    double h, eps, tMax, maxValue;
+    int skip, nNT, nBlocks;
+    long long dirChunk;
+};
+
+int lyap_spec(LyapSpec& S, int dimension, double tMax, double NT, int nPts, double h,
+              double eps, const double* ic, int nIC, const double* ranges, const int* mut,
+              double maxValue, double transientTime, const double* values, int nVal)
+{
+    RC_TRY(make_grid(S.G, nPts, dimension, ranges, mut, ic, nIC, values, nVal));
+    // derived counts exactly as Library/hostLibrary.cu:1286-1288 and
+    // LLEKernelCUDA's (size_t)(NT/h) (Library/cudaLibrary.cu:697)
+    const size_t nNT = (size_t)(NT / h);
+    const int nBlocks = (int)(tMax / NT);
+    const int skip = (int)(transientTime / h);
+    if (nNT > (size_t)INT32_MAX || skip < 0)
+        return fail(BIFHIP_E_ARG, "invalid NT/h or transient");
+    S.dimension = dimension;
+    S.cells = cells_of(nPts, dimension);
+    S.h = h;
+    S.eps = eps;
+    S.tMax = tMax;
+    S.maxValue = maxValue;
+    S.skip = skip;
+    S.nNT = (int)nNT;
+    S.nBlocks = nBlocks < 0 ? 0 : nBlocks;
+    // reference chunking of the direction index: 200000 cells in LLE2D
+    // (Library/hostLibrary.cu:1552), one chunk in the other drivers
+    return BIFHIP_OK;
+}
+
+// Host table of reference perturbation directions for cells [off, off+n):
+// LLE: 0.5*sinf((int)(idx*(i*idx+1)+1)) normalised (Library/cudaLibrary.cu:720-732)
+// LS:  0.5*sinf(0.2171828(i+1)(j+1) + idx + (0.2171828 + i*j*idx)) rows
+//      normalised (Library/cudaLibrary.cu:1025-1040)
+void directions(bool ls, long long off, int n, long long chunk, std::vector<double>& out)
+{
+    const int per = ls ? 9 : 3;
+    out.resize((size_t)n * per);
+    for (int t = 0; t < n; ++t) {
+        const long long gi = off + t;
+        const long long local = chunk > 0 ? gi % chunk : gi;
+        double* z = out.data() + (size_t)t * per;
+        if (!ls) {
+            const uint32_t id = (uint32_t)local;
+            double zp = 0;
+            for (int i = 0; i < 3; ++i) {
+                const int32_t arg = (int32_t)(id * ((uint32_t)i * id + 1u) + 1u);
+                z[i] = 0.5 * (double)sinf((float)arg);
+                zp += z[i] * z[i];
+            }
+            zp = std::sqrt(zp);
+            for (int i = 0; i < 3; ++i) z[i] /= zp;
+        } else {
+            const int idx = (int)local;
+            for (int j = 0; j < 3; ++j) {
+                double zp = 0;
+                for (int i = 0; i < 3; ++i) {
+                    const double arg = 0.2171828 * (i + 1) * (j + 1) + idx + (0.2171828 + i * j * idx);
+                    z[j * 3 + i] = sinf((float)arg) * 0.5;
+                    zp += z[j * 3 + i] * z[j * 3 + i];
+                }
+                zp = std::sqrt(zp);
+                for (int i = 0; i < 3; ++i) z[j * 3 + i] /= zp;
+            }
+        }
+    }
+}
+
+int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* d_out)
+{
+    const long long cells = S.cells;
+    const long long chunk = launch_chunk(cells);
+    const long long dirChunk = g.dirChunk >= 0 ? g.dirChunk : defaultChunk;
+    const bool refDirs = g.pert == BIFHIP_PERT_REFERENCE;
+    const int per = ls ? 9 : 3;
+    double* d_dirs = nullptr;
+    std::vector<double> hdirs;
+    if (refDirs) RC_TRY(pool_get("dirs", (size_t)std::min(chunk, cells) * per, &d_dirs));
+    for (long long off = 0; off < cells; off += chunk) {
+        const int n = (int)std::min(chunk, cells - off);
+        LyapArgs A;
+        A.g = S.G;
+        A.h = S.h;
+        A.eps = S.eps;
+        A.tMax = S.tMax;
+        A.maxValue = S.maxValue;
+        A.cellBase = (int)off;
+        A.cellList = nullptr;
+        A.nCells = n;
+        A.skip = S.skip;
+        A.nNT = S.nNT;
+        A.nBlocks = S.nBlocks;
+        A.dirs = nullptr;
+        if (refDirs) {
+            directions(ls, off, n, dirChunk, hdirs);
+            HIP_TRY(hipMemcpyAsync(d_dirs, hdirs.data(), sizeof(double) * hdirs.size(),
+                                   hipMemcpyHostToDevice, g.stream));
+            A.dirs = d_dirs;
+        }
+        if (ls) {
+            Timed t("ls");
+            HIP_TRY(launch_ls(g.model, A, S.maxValue != 0, d_out + off * 3, g.stream));
+        } else {
+            Timed t(g.precision == BIFHIP_PREC_FP32 ? "lle32" : "lle");
+            HIP_TRY(launch_lle(g.model, g.precision, A, S.maxValue != 0, d_out + off, g.stream));
+        }
+        if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
+    }
+    return BIFHIP_OK;
+}
+
+// ------------------------------------------------------------ CSV output
+
+struct Out {
+    FILE* f = nullptr;
+    std::string buf;
+    ~Out()
+    {
+        if (f) fclose(f);
+    }
+    bool open(const std::string& path, const char* mode)
+    {
+        f = fopen(path.c_str(), mode);
+        return f != nullptr;
+    }
+    void num(double v, int prec)
+    {
+        char t[64];
+        int n = snprintf(t, sizeof(t), "%.*g", prec, v);
+        buf.append(t, (size_t)n);
+    }
+    void num(int v)
+    {
+        char t[32];
+        int n = snprintf(t, sizeof(t), "%d", v);
+        buf.append(t, (size_t)n);
+    }
+    void str(const char* s) { buf.append(s); }
+    bool flush()
+    {
+        if (!f) return false;
+        if (!buf.empty() && fwrite(buf.data(), 1, buf.size(), f) != buf.size()) return false;
+        buf.clear();
+        return true;
+    }
+    bool close()
+    {
+        bool ok = flush();
+        if (f && fclose(f) != 0) ok = false;
+        f = nullptr;
+        return ok;
+    }
+};
+
+void header(Out& o, const double* r, int prec)
+{
+    o.num(r[0], prec); o.str(" "); o.num(r[1], prec); o.str("\n");
+    o.num(r[2], prec); o.str(" "); o.num(r[3], prec); o.str("\n");
+}
+
+// the reference's row logic: ", " before every value but the first, a
+// line break once nPts values are on the row (Library/hostLibrary.cu:708-725)
+struct RowWriter {
+    int counter = 0;
+    int nPts;
+    explicit RowWriter(int n) : nPts(n) {}
+    void sep(Out& o)
+    {
+        if (counter != 0) o.str(", ");
+        if (counter == nPts) {
+            o.str("\n");
+            counter = 0;
+        }
+        ++counter;
+    }
+};
+
+int file_error(const std::string& path)
+{
+    return fail(BIFHIP_E_FILE, "Output file open error: %s", path.c_str());
+}
+
+int write_map_int(const std::string& path, int nPts, const double* ranges, int hdr, int prec,
+                  const int* v, long long n)
+{
+    Out o;
+    if (!o.open(path, "w")) return file_error(path);
+    if (hdr) header(o, ranges, prec);
+    RowWriter rw(nPts);
+    for (long long i = 0; i < n; ++i) {
+        rw.sep(o);
+        o.num(v[i]);
+        if (o.buf.size() > (1u << 22) && !o.flush()) return file_error(path);
+    }
+    if (!o.close()) return file_error(path);
+    return BIFHIP_OK;
+}
+
+int write_map_double(const std::string& path, int nPts, const double* ranges, int hdr, int prec,
+                     const double* v, long long n, int stride, int offset)
+{
+    Out o;
+    if (!o.open(path, "w")) return file_error(path);
+    if (hdr) header(o, ranges, prec);
+    RowWriter rw(nPts);
+    for (long long i = 0; i < n; ++i) {
+        rw.sep(o);
+        o.num(v[i * stride + offset], prec);
+        if (o.buf.size() > (1u << 22) && !o.flush()) return file_error(path);
+    }
+    if (!o.close()) return file_error(path);
+    return BIFHIP_OK;
+}
+
+int write_bif1d(const std::string& path, int nPts, const double* ranges, const int* counts,
+                const double* peaks, int stride)
+{
+    Out o;
+    if (!o.open(path, "w")) return file_error(path);
+    for (int k = 0; k < nPts; ++k) {
+        const double pv = value_by_idx(k, nPts, ranges[0], ranges[1], 0);
+        for (int j = 0; j < counts[k]; ++j) {
+            o.num(pv, 12);
+            o.str(", ");
+            o.num(peaks[(long long)k * stride + j], 12);
+            o.str("\n");
+        }
+        if (o.buf.size() > (1u << 22) && !o.flush()) return file_error(path);
+    }
+    if (!o.close()) return file_error(path);
+    return BIFHIP_OK;
+}
+
+int write_1d_values(const std::string& path, int nPts, const double* ranges, const double* v,
+                    int per)
+{
+    Out o;
+    if (!o.open(path, "w")) return file_error(path);
+    for (int k = 0; k < nPts; ++k) {
+        o.num(value_by_idx(k, nPts, ranges[0], ranges[1], 0), 6);
+        for (int j = 0; j < per; ++j) {
+            o.str(", ");
+            o.num(v[(long long)k * per + j], 6);
+        }
+        o.str("\n");
+    }
+    if (!o.close()) return file_error(path);
+    return BIFHIP_OK;
+}
+
+void progress(const char* name, long long cells)
+{
+    if (!g.verbose) return;
+    printf("%s\n", name);
+    printf("nPtsLimiter : %lld\n", cells);
+    printf("Amount of iterations %d: \n", 1);
+}
+
+int finish()
+{
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    resolve_timers();
+    return BIFHIP_OK;
+}
+
+// ----------------------------------------------------------- drivers
+
+int bif1d_impl(double tMax, int nPts, double h, int nIC, const double* ic, const double* ranges,
+               const int* mut, int wv, double maxValue, double transientTime,
+               const double* values, int nVal, int pre, std::vector<int>& counts,
+               std::vector<double>& peaks, int& stride)
+{
+    RC_TRY(ensure_device());
+    PeakMapSpec S;
+    RC_TRY(peak_spec(S, 1, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime,
+                     values, nVal, pre));
+    progress("BIFURCATION1D", S.cells);
+    stride = std::max(1, S.N / 2 + 1);
+    DBuf c, a;
+    HIP_TRY(c.alloc(sizeof(int) * (size_t)S.cells));
+    HIP_TRY(a.alloc(sizeof(double) * (size_t)S.cells * stride));
+    RC_TRY(integrate_peaks(S, 0, nullptr, (int)S.cells, stride, c.as<int>(), a.as<double>(),
+                           nullptr));
+    counts.resize((size_t)S.cells);
+    peaks.resize((size_t)S.cells * stride);
+    HIP_TRY(hipMemcpyAsync(counts.data(), c.p, sizeof(int) * counts.size(), hipMemcpyDeviceToHost,
+                           g.stream));
+    HIP_TRY(hipMemcpyAsync(peaks.data(), a.p, sizeof(double) * peaks.size(),
+                           hipMemcpyDeviceToHost, g.stream));
+    RC_TRY(finish());
+    // peakFinderCUDA turns the divergence flag into 0 peaks
+    for (auto& v : counts)
+        if (v < 0) v = 0;
+    return BIFHIP_OK;
+}
+
+int cluster_impl(int kind, double tMax, int nPts, double h, int nIC, const double* ic,
+                 const double* ranges, const int* mut, int wv, double maxValue, int maxPeaks,
+                 double transientTime, const double* values, int nVal, int pre, double eps,
+                 int kdeSampling, double I1, double I2, double H, int* d_out)
+{
+    RC_TRY(ensure_device());
+    PeakMapSpec S;
+    RC_TRY(peak_spec(S, 2, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime,
+                     values, nVal, pre));
+    progress("BIFURCATION2D", S.cells);
+    ClusterParams C{kind, maxPeaks, eps, kdeSampling, I1, I2, H};
+    RC_TRY(cluster_map_device(S, C, d_out));
+    return finish();
+}
+
+int cluster_to_host(int kind, double tMax, int nPts, double h, int nIC, const double* ic,
+                    const double* ranges, const int* mut, int wv, double maxValue, int maxPeaks,
+                    double transientTime, const double* values, int nVal, int pre, double eps,
+                    int kdeSampling, double I1, double I2, double H, int* out)
+{
+    RC_TRY(ensure_device());
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    const long long cells = (long long)nPts * nPts;
+    DBuf d;
+    HIP_TRY(d.alloc(sizeof(int) * (size_t)cells));
+    RC_TRY(cluster_impl(kind, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                        transientTime, values, nVal, pre, eps, kdeSampling, I1, I2, H,
+                        d.as<int>()));
+    HIP_TRY(hipMemcpy(out, d.p, sizeof(int) * (size_t)cells, hipMemcpyDeviceToHost));
+    return BIFHIP_OK;
+}
+
+int lyap_impl(bool ls, int dimension, double tMax, double NT, int nPts, double h, double eps,
+              const double* ic, int nIC, const double* ranges, const int* mut, double maxValue,
+              double transientTime, const double* values, int nVal, double* d_out)
+{
+    RC_TRY(ensure_device());
+    LyapSpec S;
+    RC_TRY(lyap_spec(S, dimension, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                     transientTime, values, nVal));
+    progress(ls ? (dimension == 1 ? "LS1D" : "LS2D") : (dimension == 1 ? "LLE1D" : "LLE2D"),
+             S.cells);
+    const long long defChunk = (!ls && dimension == 2) ? 200000 : 0;
+    RC_TRY(lyap_map_device(S, ls, defChunk, d_out));
+    return finish();
+}
+
+int lyap_to_host(bool ls, int dimension, double tMax, double NT, int nPts, double h, double eps,
+                 const double* ic, int nIC, const double* ranges, const int* mut,
+                 double maxValue, double transientTime, const double* values, int nVal,
+                 double* out)
+{
+    RC_TRY(ensure_device());
+    if (nPts < 1 || dimension < 1 || dimension > 2) return fail(BIFHIP_E_ARG, "bad grid");
+    const long long cells = cells_of(nPts, dimension);
+    const size_t n = (size_t)cells * (ls ? nIC : 1);
+    if (ls && nIC != 3) return fail(BIFHIP_E_ARG, "LS supports 3 state variables");
+    DBuf d;
+    HIP_TRY(d.alloc(sizeof(double) * n));
+    RC_TRY(lyap_impl(ls, dimension, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                     transientTime, values, nVal, d.as<double>()));
+    HIP_TRY(hipMemcpy(out, d.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return BIFHIP_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+
+extern "C" {
+
+int bifhip_set_output_path(const char* path)
+{
+    if (!path || !*path) return fail(BIFHIP_E_ARG, "empty output path");
+    g.outPath = path;
+    return BIFHIP_OK;
+}
+const char* bifhip_get_output_path(void) { return g.outPath.c_str(); }
+
+int bifhip_set_model(int model)
+{
+    if (model != BIFHIP_MODEL_ROSSLER && model != BIFHIP_MODEL_LINEAR)
+        return fail(BIFHIP_E_ARG, "unknown model %d", model);
+    g.model = model;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_precision(int p)
+{
+    if (p != BIFHIP_PREC_FP64 && p != BIFHIP_PREC_FP32)
+        return fail(BIFHIP_E_ARG, "unknown precision %d", p);
+    g.precision = p;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_perturbation(int m)
+{
+    if (m != BIFHIP_PERT_REFERENCE && m != BIFHIP_PERT_AXIS_X)
+        return fail(BIFHIP_E_ARG, "unknown perturbation mode %d", m);
+    g.pert = m;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_direction_chunk(long long cells)
+{
+    if (cells < -1) return fail(BIFHIP_E_ARG, "direction chunk must be >= -1");
+    g.dirChunk = cells;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_max_cells_per_launch(long long cells)
+{
+    if (cells < 0) return fail(BIFHIP_E_ARG, "max cells per launch must be >= 0");
+    g.maxCellsPerLaunch = cells;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_peak_capacity(int cap)
+{
+    if (cap < 2) return fail(BIFHIP_E_ARG, "peak capacity must be >= 2");
+    g.peakCap = cap;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_cluster_clamp(int clamp)
+{
+    if (clamp < 0) return fail(BIFHIP_E_ARG, "clamp must be >= 0");
+    g.clusterClamp = clamp;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_verbose(int on)
+{
+    g.verbose = on;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_timing(int on)
+{
+    g.timing = on;
+    return BIFHIP_OK;
+}
+
+int bifhip_kernel_time(const char* kernel, double* total_ms, long long* launches)
+{
+    if (!kernel) {
+        g_times.clear();
+        return BIFHIP_OK;
+    }
+    auto it = g_times.find(kernel);
+    if (total_ms) *total_ms = it == g_times.end() ? 0.0 : it->second.ms;
+    if (launches) *launches = it == g_times.end() ? 0 : it->second.launches;
+    return BIFHIP_OK;
+}
+
+int bifhip_release(void)
+{
+    g_pool.release();
+    return BIFHIP_OK;
+}
+
+const char* bifhip_last_error(void) { return g.lastError.c_str(); }
+
+int bifhip_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* bifhip_version(void) { return "bifhip 0.1 gfx950"; }
+
+double bifhip_value_by_idx(int idx, int nPts, double s, double f, int k)
+{
+    return value_by_idx(idx, nPts, s, f, k);
+}
+
+// ----------------------------------------------------------- file drivers
+
+int bifhip_bifurcation1D(double tMax, int nPts, double h, int nIC, const double* ic,
+                         const double* ranges, const int* mut, int wv, double maxValue,
+                         double transientTime, const double* values, int nVal, int pre)
+{
+    std::vector<int> counts;
+    std::vector<double> peaks;
+    int stride = 0;
+    RC_TRY(bif1d_impl(tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime, values,
+                      nVal, pre, counts, peaks, stride));
+    return write_bif1d(g.outPath, nPts, ranges, counts.data(), peaks.data(), stride);
+}
+
+int bifhip_bifurcation2DKDE(double tMax, int nPts, double h, int nIC, const double* ic,
+                            const double* ranges, const int* mut, int wv, double maxValue,
+                            int maxPeaks, double transientTime, const double* values, int nVal,
+                            int pre, int kdeSampling, double I1, double I2, double H)
+{
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<int> out((size_t)nPts * nPts);
+    RC_TRY(cluster_to_host(1, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                           transientTime, values, nVal, pre, 0.0, kdeSampling, I1, I2, H,
+                           out.data()));
+    // Library/hostLibrary.cu:474-480: setprecision(12) before the header
+    return write_map_int(g.outPath, nPts, ranges, 1, 12, out.data(), (long long)out.size());
+}
+
+int bifhip_bifurcation2DDBSCAN(double tMax, int nPts, double h, int nIC, const double* ic,
+                               const double* ranges, const int* mut, int wv, double maxValue,
+                               int maxPeaks, double transientTime, const double* values,
+                               int nVal, int pre, double eps)
+{
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<int> out((size_t)nPts * nPts);
+    RC_TRY(cluster_to_host(0, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                           transientTime, values, nVal, pre, eps, 0, 0, 0, 0, out.data()));
+    return write_map_int(g.outPath, nPts, ranges, 1, 6, out.data(), (long long)out.size());
+}
+
+int bifhip_LLE1D(double tMax, double NT, int nPts, double h, double eps, const double* ic,
+                 int nIC, const double* ranges, const int* mut, int wv, double maxValue,
+                 double transientTime, const double* values, int nVal)
+{
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<double> out((size_t)nPts);
+    RC_TRY(lyap_to_host(false, 1, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                        transientTime, values, nVal, out.data()));
+    (void)wv;
+    return write_1d_values(g.outPath, nPts, ranges, out.data(), 1);
+}
+
+int bifhip_LLE2D(double tMax, double NT, int nPts, double h, double eps, const double* ic,
+                 int nIC, const double* ranges, const int* mut, int wv, double maxValue,
+                 double transientTime, const double* values, int nVal)
+{
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<double> out((size_t)nPts * nPts);
+    RC_TRY(lyap_to_host(false, 2, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                        transientTime, values, nVal, out.data()));
+    (void)wv;
+    return write_map_double(g.outPath, nPts, ranges, 1, 6, out.data(), (long long)out.size(), 1,
+                            0);
+}
+
+int bifhip_LS1D(double tMax, double NT, int nPts, double h, double eps, const double* ic,
+                int nIC, const double* ranges, const int* mut, int wv, double maxValue,
+                double transientTime, const double* values, int nVal)
+{
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<double> out((size_t)nPts * nIC);
+    RC_TRY(lyap_to_host(true, 1, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                        transientTime, values, nVal, out.data()));
+    (void)wv;
+    return write_1d_values(g.outPath, nPts, ranges, out.data(), nIC);
+}
+
+int bifhip_LS2D(double tMax, double NT, int nPts, double h, double eps, const double* ic,
+                int nIC, const double* ranges, const int* mut, int wv, double maxValue,
+                double transientTime, const double* values, int nVal)
+{
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    const long long cells = (long long)nPts * nPts;
+    std::vector<double> out((size_t)cells * nIC);
+    RC_TRY(lyap_to_host(true, 2, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                        transientTime, values, nVal, out.data()));
+    (void)wv;
+    // Library/hostLibrary.cu:2003-2066: one file per exponent, path + "k.csv"
+    for (int k = 0; k < nIC; ++k) {
+        const std::string path = g.outPath + std::to_string(k + 1) + ".csv";
+        RC_TRY(write_map_double(path, nPts, ranges, 1, 6, out.data(), cells, nIC, k));
+    }
+    return BIFHIP_OK;
+}
+
+// ----------------------------------------------------- host-buffer twins
+
+int bifhip_bifurcation1D_to_buffer(double tMax, int nPts, double h, int nIC, const double* ic,
+                                   const double* ranges, const int* mut, int wv, double maxValue,
+                                   double transientTime, const double* values, int nVal, int pre,
+                                   int* counts, double* peaks, int peakStride)
+{
+    if (!counts) return fail(BIFHIP_E_ARG, "null counts");
+    std::vector<int> c;
+    std::vector<double> p;
+    int stride = 0;
+    RC_TRY(bif1d_impl(tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime, values,
+                      nVal, pre, c, p, stride));
+    for (int k = 0; k < nPts; ++k) {
+        counts[k] = c[k];
+        if (peaks) {
+            if (c[k] > peakStride)
+                return fail(BIFHIP_E_LIMIT, "cell %d has %d peaks > peakStride %d", k, c[k],
+                            peakStride);
+            for (int j = 0; j < c[k]; ++j)
+                peaks[(long long)k * peakStride + j] = p[(long long)k * stride + j];
+        }
+    }
+    return BIFHIP_OK;
+}
+
+int bifhip_bifurcation2DKDE_to_buffer(double tMax, int nPts, double h, int nIC, const double* ic,
+                                      const double* ranges, const int* mut, int wv,
+                                      double maxValue, int maxPeaks, double transientTime,
+                                      const double* values, int nVal, int pre, int kdeSampling,
+                                      double I1, double I2, double H, int* out)
+{
+    if (!out) return fail(BIFHIP_E_ARG, "null out");
+    return cluster_to_host(1, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                           transientTime, values, nVal, pre, 0.0, kdeSampling, I1, I2, H, out);
+}
+
+int bifhip_bifurcation2DDBSCAN_to_buffer(double tMax, int nPts, double h, int nIC,
+                                         const double* ic, const double* ranges, const int* mut,
+                                         int wv, double maxValue, int maxPeaks,
+                                         double transientTime, const double* values, int nVal,
+                                         int pre, double eps, int* out)
+{
+    if (!out) return fail(BIFHIP_E_ARG, "null out");
+    return cluster_to_host(0, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                           transientTime, values, nVal, pre, eps, 0, 0, 0, 0, out);
+}
+
+int bifhip_LLE_to_buffer(int dimension, double tMax, double NT, int nPts, double h, double eps,
+                         const double* ic, int nIC, const double* ranges, const int* mut, int wv,
+                         double maxValue, double transientTime, const double* values, int nVal,
+                         double* out)
+{
+    (void)wv;
+    if (!out) return fail(BIFHIP_E_ARG, "null out");
+    return lyap_to_host(false, dimension, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut,
+                        maxValue, transientTime, values, nVal, out);
+}
+
+int bifhip_LS_to_buffer(int dimension, double tMax, double NT, int nPts, double h, double eps,
+                        const double* ic, int nIC, const double* ranges, const int* mut, int wv,
+                        double maxValue, double transientTime, const double* values, int nVal,
+                        double* out)
+{
+    (void)wv;
+    if (!out) return fail(BIFHIP_E_ARG, "null out");
+    return lyap_to_host(true, dimension, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                        transientTime, values, nVal, out);
+}
+
+// --------------------------------------------------- device-buffer twins
+
+int bifhip_bifurcation2DDBSCAN_device(double tMax, int nPts, double h, int nIC, const double* ic,
+                                      const double* ranges, const int* mut, int wv,
+                                      double maxValue, int maxPeaks, double transientTime,
+                                      const double* values, int nVal, int pre, double eps,
+                                      int* d_out)
+{
+    if (!d_out) return fail(BIFHIP_E_ARG, "null d_out");
+    return cluster_impl(0, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                        transientTime, values, nVal, pre, eps, 0, 0, 0, 0, d_out);
+}
+
+int bifhip_LLE_device(int dimension, double tMax, double NT, int nPts, double h, double eps,
+                      const double* ic, int nIC, const double* ranges, const int* mut, int wv,
+                      double maxValue, double transientTime, const double* values, int nVal,
+                      double* d_out)
+{
+    (void)wv;
+    if (!d_out) return fail(BIFHIP_E_ARG, "null d_out");
+    return lyap_impl(false, dimension, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                     transientTime, values, nVal, d_out);
+}
+
+// ------------------------------------------------ kernel-level entry points
+
+int bifhip_peak_finder_device(const double* d_data, int sizeOfBlock, int nBlocks, int* d_counts,
+                              double* d_outPeaks, double* d_timeOfPeaks, double h)
+{
+    RC_TRY(ensure_device());
+    if (!d_data || !d_counts || sizeOfBlock < 0 || nBlocks < 0)
+        return fail(BIFHIP_E_ARG, "bad peak finder arguments");
+    {
+        Timed t("peak_finder");
+        HIP_TRY(launch_peak_finder(d_data, sizeOfBlock, nBlocks, d_counts, d_outPeaks,
+                                   d_timeOfPeaks, h, g.stream));
+    }
+    return finish();
+}
+
+int bifhip_dbscan_device(const double* d_peaks, const double* d_intervals, int sizeOfBlock,
+                         int nBlocks, const int* d_counts, int maxPeaks, double eps, int* d_out)
+{
+    RC_TRY(ensure_device());
+    if (!d_peaks || !d_intervals || !d_counts || !d_out || sizeOfBlock < 1 || nBlocks < 0)
+        return fail(BIFHIP_E_ARG, "bad dbscan arguments");
+    const double T = dbscan_threshold(eps);
+    int *d_ovf, *d_novf;
+    RC_TRY(pool_get("ovf", (size_t)std::max(1, nBlocks), &d_ovf));
+    RC_TRY(pool_get("novf", 1, &d_novf));
+    HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
+    {
+        Timed t("dbscan");
+        HIP_TRY(launch_dbscan_wave(nBlocks, d_counts, d_peaks, d_intervals, sizeOfBlock,
+                                   sizeOfBlock, maxPeaks, T, d_out, d_ovf, d_novf, 0, g.stream));
+    }
+    int novf = 0;
+    HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (novf > 0) {
+        DBuf scr;
+        HIP_TRY(scr.alloc(sizeof(int) * (size_t)novf * 2 * sizeOfBlock));
+        Timed t("dbscan");
+        HIP_TRY(launch_dbscan_lane(novf, d_ovf, d_counts, d_peaks, d_intervals, sizeOfBlock, T,
+                                   scr.as<int>(), 2LL * sizeOfBlock, d_out, nullptr, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+    }
+    return finish();
+}
+
+int bifhip_kde_device(const double* d_peaks, int sizeOfBlock, int nBlocks, const int* d_counts,
+                      int maxPeaks, int kdeSampling, double I1, double I2, double H, int* d_out)
+{
+    RC_TRY(ensure_device());
+    if (!d_peaks || !d_counts || !d_out || sizeOfBlock < 1 || nBlocks < 0)
+        return fail(BIFHIP_E_ARG, "bad kde arguments");
+    int *d_ovf, *d_novf;
+    RC_TRY(pool_get("ovf", (size_t)std::max(1, nBlocks), &d_ovf));
+    RC_TRY(pool_get("novf", 1, &d_novf));
+    HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
+    {
+        Timed t("kde");
+        // rows hold every peak (cap = sizeOfBlock), so nothing overflows
+        HIP_TRY(launch_kde_wave(nBlocks, d_counts, d_peaks, sizeOfBlock, sizeOfBlock, maxPeaks,
+                                kdeSampling, I1, I2, H, d_out, nullptr, d_ovf, d_novf, 0,
+                                g.stream));
+    }
+    return finish();
+}
+
+int bifhip_gram_schmidt_device(const double* d_a, double* d_b, double* d_den, int n, int count)
+{
+    RC_TRY(ensure_device());
+    if (!d_a || !d_b || n < 1 || n > 8 || count < 0)
+        return fail(BIFHIP_E_ARG, "bad Gram-Schmidt arguments (n must be 1..8)");
+    {
+        Timed t("gram_schmidt");
+        HIP_TRY(launch_gs(d_a, d_b, d_den, n, count, g.stream));
+    }
+    return finish();
+}
+
+int bifhip_series_device(int dimension, double tMax, int nPts, double h, int nIC,
+                         const double* ic, const double* ranges, const int* mut, int wv,
+                         double maxValue, double transientTime, const double* values, int nVal,
+                         int pre, double* d_data, int* d_flags)
+{
+    RC_TRY(ensure_device());
+    if (!d_data || !d_flags) return fail(BIFHIP_E_ARG, "null device buffer");
+    PeakMapSpec S;
+    RC_TRY(peak_spec(S, dimension, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue,
+                     transientTime, values, nVal, pre));
+    BifArgs A = bif_args(S, 0, nullptr, (int)S.cells, S.N);
+    A.rowStride = S.N;
+    {
+        Timed t("series");
+        HIP_TRY(launch_series(g.model, A, wv, d_data, d_flags, g.stream));
+    }
+    return finish();
+}
+
+// ------------------------------------------------------------ writers
+
+int bifhip_write_map_int(const char* path, int nPts, const double* ranges, int hdr, int prec,
+                         const int* values, long long count)
+{
+    if (!path || !values || (hdr && !ranges) || nPts < 1) return fail(BIFHIP_E_ARG, "bad writer args");
+    return write_map_int(path, nPts, ranges, hdr, prec, values, count);
+}
+
+int bifhip_write_map_double(const char* path, int nPts, const double* ranges, int hdr, int prec,
+                            const double* values, long long count, int stride, int offset)
+{
+    if (!path || !values || (hdr && !ranges) || nPts < 1 || stride < 1)
+        return fail(BIFHIP_E_ARG, "bad writer args");
+    return write_map_double(path, nPts, ranges, hdr, prec, values, count, stride, offset);
+}
+
+int bifhip_write_bif1d(const char* path, int nPts, const double* ranges, const int* counts,
+                       const double* peaks, int stride)
+{
+    if (!path || !ranges || !counts || !peaks || nPts < 1) return fail(BIFHIP_E_ARG, "bad writer args");
+    return write_bif1d(path, nPts, ranges, counts, peaks, stride);
+}
+
+int bifhip_write_1d_values(const char* path, int nPts, const double* ranges,
+                           const double* values, int perCell)
+{
+    if (!path || !ranges || !values || nPts < 1 || perCell < 1) return fail(BIFHIP_E_ARG, "bad writer args");
+    return write_1d_values(path, nPts, ranges, values, perCell);
+}
+
+}  // extern "C"

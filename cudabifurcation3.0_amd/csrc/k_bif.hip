@@ -1,0 +1,591 @@
+// k_bif.hip -- bifurcation kernels for gfx950.
+//
+//  k_bif_peaks      one trajectory per lane, state in VGPRs: transient, then
+//                   the recorded window fed sample by sample into a streaming
+//                   restatement of peakFinder (Library/cudaLibrary.cu:220-266)
+//                   so the 50 000-sample series never touches HBM.  Peaks
+//                   (amplitude, interval) land in a per-cell row.
+//  k_series         the reference's record pass (calculateDiscreteModelCUDA,
+//                   Library/cudaLibrary.cu:78-117) writing the raw series;
+//                   used by the peak-finder parity tests.
+//  k_peak_finder    peakFinderCUDA (Library/cudaLibrary.cu:270-284) over
+//                   series already in HBM, same streaming state machine.
+//  k_dbscan_wave    dbscan (Library/cudaLibrary.cu:457-529), one wavefront
+//                   per cell: neighbour rows built with __ballot, the
+//                   reference's sequential label/stack machine replayed on
+//                   64-bit masks.  Exact for P <= 256 peaks.
+//  k_dbscan_lane    the same algorithm one lane per cell on HBM scratch, for
+//                   the rare cells with more than 256 peaks.
+//  k_kde_wave       kde (Library/cudaLibrary.cu:356-422), one wavefront per
+//                   cell: lanes evaluate 64 sample points at a time (each sum
+//                   in the reference's peak order), the local-maximum state
+//                   machine runs wave-uniformly over them.
+#include "bifhip_dev.h"
+
+namespace bifhip {
+
+// ------------------------------------------------------------ peak stream
+//
+// Streaming form of peakFinder.  The batch scan evaluates candidate i when
+// d[i+1] is known; a plateau scan compares d[j] with d[j+1]; after a peak
+// or a rising plateau exit at j the next candidate is j+2.  Modes:
+//   kSeek  evaluate candidate t-1 on sample t
+//   kPlat  inside a plateau: compare d[t-1] with d[t]
+//   kSkip  candidate t-1 is skipped (the loop's ++i after i = j+1)
+// The first peak found only provides the time origin of the first interval
+// (peakFinder drops it); intervals are (j_k - j_{k-1}) * hpf.
+enum : int { kSeek = 0, kPlat = 1, kSkip = 2 };
+
+struct PeakSink {
+    double* amps;     // row base (nullptr: count only)
+    double* ivls;     // row base (nullptr: no intervals)
+    int cap;          // slots in the row
+    double hpf;       // peakFinder's h (0.01 by default)
+};
+
+struct PeakStream {
+    double p1, p2;
+    int mode, found, lastj;
+
+    __device__ __forceinline__ void init() { p1 = p2 = 0; mode = kSeek; found = 0; lastj = 0; }
+
+    __device__ __forceinline__ void emit(double amp, int j, const PeakSink& s)
+    {
+        if (found >= 1) {
+            const int slot = found - 1;
+            if (slot < s.cap) {
+                if (s.amps) s.amps[slot] = amp;
+                if (s.ivls) s.ivls[slot] = (double)(j - lastj) * s.hpf;
+            }
+        }
+        lastj = j;
+        ++found;
+    }
+
+    // sample v has index t (t >= 2 is checked by the caller's loop shape)
+    __device__ __forceinline__ void push(double v, int t, const PeakSink& s)
+    {
+        if (mode == kSeek) {
+            if (p1 > p2 && p1 >= v) {
+                if (p1 > v) { emit(p1, t - 1, s); mode = kSkip; }
+                else mode = kPlat;
+            }
+        } else if (mode == kPlat) {
+            if (p1 < v) mode = kSkip;
+            else if (p1 > v) { emit(p1, t - 1, s); mode = kSkip; }
+        } else {
+            mode = kSeek;
+        }
+        p2 = p1;
+        p1 = v;
+    }
+
+    __device__ __forceinline__ int count() const { return found > 1 ? found - 1 : 0; }
+};
+
+
+template <class M, int WV>
+__device__ __forceinline__ double pick(double x0, double x1, double x2)
+{
+    return WV == 0 ? x0 : (WV == 1 ? x1 : x2);
+}
+
+// One trajectory per lane.  counts[t] = P (after dropping the first peak)
+// or -1 when the divergence check fired (the reference's flag value).
+template <class M, int WV, bool CHECK>
+__global__ __launch_bounds__(256) void k_bif_peaks(const BifArgs A, int* __restrict__ counts,
+                                                   double* __restrict__ amps,
+                                                   double* __restrict__ ivls)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nCells) return;
+    const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
+
+    double v[kMaxVal];
+    cell_values(A.g, idx, v);
+    const typename M::P64 p = M::params(v);
+    const double h = A.h;
+    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
+
+    // transient, no divergence check (Library/cudaLibrary.cu:107-108)
+    int s = 0;
+    for (; s + 4 <= A.skip; s += 4) {
+        M::step(x0, x1, x2, p, h);
+        M::step(x0, x1, x2, p, h);
+        M::step(x0, x1, x2, p, h);
+        M::step(x0, x1, x2, p, h);
+    }
+    for (; s < A.skip; ++s) M::step(x0, x1, x2, p, h);
+
+    PeakSink sink;
+    sink.amps = amps ? amps + (long long)t * A.rowStride : nullptr;
+    sink.ivls = ivls ? ivls + (long long)t * A.rowStride : nullptr;
+    sink.cap = A.cap;
+    sink.hpf = A.hpf;
+
+    PeakStream ps;
+    ps.init();
+    bool bad = false;
+    const int N = A.N;
+    const int pre = A.pre;
+    for (int i = 0; i < N; ++i) {
+        const double d = pick<M, WV>(x0, x1, x2);
+        if (i >= 2) ps.push(d, i, sink);
+        else { ps.p2 = ps.p1; ps.p1 = d; }
+        for (int j = 0; j < pre - 1; ++j) M::step(x0, x1, x2, p, h);
+        M::step(x0, x1, x2, p, h);
+        if (CHECK && diverged(pick<M, WV>(x0, x1, x2), A.maxValue)) { bad = true; break; }
+    }
+    counts[t] = bad ? -1 : ps.count();
+}
+
+// Raw recorded series (the reference's data[] block) for parity tests.
+template <class M>
+__global__ __launch_bounds__(256) void k_series(const BifArgs A, int wv, double* __restrict__ data,
+                                                int* __restrict__ flags)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nCells) return;
+    const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
+    double v[kMaxVal];
+    cell_values(A.g, idx, v);
+    const typename M::P64 p = M::params(v);
+    const double h = A.h;
+    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
+    for (int s = 0; s < A.skip; ++s) M::step(x0, x1, x2, p, h);
+    double* row = data + (long long)t * A.rowStride;
+    int flag = 0;
+    for (int i = 0; i < A.N; ++i) {
+        row[i] = wv == 0 ? x0 : (wv == 1 ? x1 : x2);
+        for (int j = 0; j < A.pre - 1; ++j) M::step(x0, x1, x2, p, h);
+        M::step(x0, x1, x2, p, h);
+        if (A.maxValue != 0 && diverged(wv == 0 ? x0 : (wv == 1 ? x1 : x2), A.maxValue)) {
+            flag = -1;
+            break;
+        }
+    }
+    flags[t] = flag;
+}
+
+// peakFinderCUDA over series in HBM: counts[c] == -1 -> 0, else the peak
+// count; peaks/intervals written from the start of the cell's block.
+// outPeaks may alias data (the 2D drivers find peaks in place): the write
+// slot never passes the read index, so no __restrict__ here.
+__global__ __launch_bounds__(256) void k_peak_finder(const double* data, int sizeOfBlock,
+                                                     int nBlocks, int* __restrict__ counts,
+                                                     double* outPeaks, double* outIntervals,
+                                                     double hpf)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nBlocks) return;
+    if (counts[c] == -1) { counts[c] = 0; return; }
+    const double* d = data + (long long)c * sizeOfBlock;
+    PeakSink sink;
+    sink.amps = outPeaks ? outPeaks + (long long)c * sizeOfBlock : nullptr;
+    sink.ivls = outIntervals ? outIntervals + (long long)c * sizeOfBlock : nullptr;
+    sink.cap = sizeOfBlock;
+    sink.hpf = hpf;
+    PeakStream ps;
+    ps.init();
+    if (sizeOfBlock >= 1) ps.p1 = d[0];
+    if (sizeOfBlock >= 2) { ps.p2 = ps.p1; ps.p1 = d[1]; }
+    for (int i = 2; i < sizeOfBlock; ++i) ps.push(d[i], i, sink);
+    counts[c] = ps.count();
+}
+
+// ------------------------------------------------------------- DBSCAN
+//
+// Neighbour test of distance() (Library/cudaLibrary.cu:445-453):
+//   hypotf((float)dx, (float)dy) < eps
+// glibc's hypotf is (float)sqrt((double)fx*fx + (double)fy*fy) and the
+// squares are exact in double, so the test is s < T with
+//   s = (double)fx*fx + (double)fy*fy   (one rounding)
+//   T = least double with (float)sqrt(T) >= eps   (computed on the host)
+// Identical points give s = 0 < T exactly when eps > 0, as in the reference.
+__device__ __forceinline__ bool near(double xa, double ya, double xb, double yb, double T)
+{
+    const float fx = (float)(xb - xa);
+    const float fy = (float)(yb - ya);
+    const double sx = (double)fx * (double)fx;
+    const double sy = (double)fy * (double)fy;
+    return sx + sy < T;
+}
+
+constexpr int kDbR = 4;                  // register sets: P <= 64*kDbR
+constexpr int kDbMaxP = kWave * kDbR;
+
+// One wavefront per cell.  Peak k lives in lane k%64 of register set k/64.
+// Neighbour row i (bits over k) lives in lane i%64 of set i/64, one
+// 64-bit word per k-set.  The label/stack machine of the reference:
+//   U = unlabelled k (label == 0), S = stack slots holding their own index
+//   (slot k is written with value k; popping slot NumNeibor-1 yields that
+//   slot's index if set, else 0; the popped value becomes the loop
+//   variable, then ++).  Result cluster-1.
+// Cells whose peaks do not fit (P > min(cap, 256), P <= maxPeaks) are
+// appended to overflow[] as listBase + cell and left to k_dbscan_lane.
+__global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __restrict__ counts,
+                                                     const double* __restrict__ amps,
+                                                     const double* __restrict__ ivls,
+                                                     long long rowStride, int cap, int maxPeaks,
+                                                     double T, int* __restrict__ out,
+                                                     int* __restrict__ overflow,
+                                                     int* __restrict__ nOverflow, int listBase)
+{
+    const int cell = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (cell >= nCells) return;          // wave-uniform
+    const int lane = lane_id();
+    const int P = __builtin_amdgcn_readfirstlane(counts[cell]);
+    int res;
+    if (P <= 0) res = 0;                 // -1 (diverged) and 0
+    else if (P == 1) res = 1;
+    else if (P > maxPeaks) res = 0;
+    else if (P > cap || P > kDbMaxP) {
+        // handled by k_dbscan_lane
+        if (lane == 0) {
+            const int k = atomicAdd(nOverflow, 1);
+            overflow[k] = listBase + cell;
+        }
+        return;
+    } else {
+        const double* ra = amps + (long long)cell * rowStride;
+        const double* rb = ivls + (long long)cell * rowStride;
+        double a[kDbR], b[kDbR];
+#pragma unroll
+        for (int r = 0; r < kDbR; ++r) {
+            const int k = r * kWave + lane;
+            a[r] = k < P ? ra[k] : 0.0;
+            b[r] = k < P ? rb[k] : 0.0;
+        }
+        const int nsets = (P + kWave - 1) / kWave;
+        uint64_t nb[kDbR][kDbR];
+#pragma unroll
+        for (int r = 0; r < kDbR; ++r)
+#pragma unroll
+            for (int w = 0; w < kDbR; ++w) nb[r][w] = 0;
+
+        // neighbour rows
+#pragma unroll
+        for (int ri = 0; ri < kDbR; ++ri) {
+            if (ri >= nsets) break;
+            const int iend = min(kWave, P - ri * kWave);
+            for (int li = 0; li < iend; ++li) {
+                const int i = ri * kWave + li;
+                const double ai = readlane_d(a[ri], li);
+                const double bi = readlane_d(b[ri], li);
+#pragma unroll
+                for (int w = 0; w < kDbR; ++w) {
+                    if (w >= nsets) break;
+                    const int k = w * kWave + lane;
+                    const bool c = (k < P - 1) && (k != i) && near(ai, bi, a[w], b[w], T);
+                    const uint64_t m = __ballot(c);
+                    if (lane == li) nb[ri][w] = m;
+                }
+            }
+        }
+
+        // label / stack machine, wave-uniform
+        uint64_t U[kDbR], S[kDbR];
+#pragma unroll
+        for (int w = 0; w < kDbR; ++w) {
+            const int lo = w * kWave;
+            const int n = P - lo;
+            U[w] = n >= kWave ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
+            S[w] = 0;
+        }
+        int nn = 0, cluster = 0;
+        for (int i = 0; i < P; ++i) {
+            bool scan = false;
+            if (nn >= 1) {
+                const int slot = nn - 1;
+                const int ws = slot >> 6;
+                const uint64_t bit = 1ull << (slot & 63);
+                bool set = false;
+#pragma unroll
+                for (int w = 0; w < kDbR; ++w)
+                    if (w == ws) { set = (S[w] & bit) != 0; S[w] &= ~bit; }
+                i = set ? slot : 0;
+                nn = nn - 1;
+                scan = true;
+            } else {
+                const int wi = i >> 6;
+                const uint64_t bit = 1ull << (i & 63);
+                bool unl = false;
+#pragma unroll
+                for (int w = 0; w < kDbR; ++w)
+                    if (w == wi) unl = (U[w] & bit) != 0;
+                if (unl) {
+                    nn = 0;
+                    ++cluster;
+#pragma unroll
+                    for (int w = 0; w < kDbR; ++w)
+                        if (w == wi) U[w] &= ~bit;
+                    scan = true;
+                }
+            }
+            if (scan) {
+                const int ri = i >> 6;
+                const int li = i & 63;
+#pragma unroll
+                for (int w = 0; w < kDbR; ++w) {
+                    uint64_t row = 0;
+#pragma unroll
+                    for (int r = 0; r < kDbR; ++r)
+                        if (r == ri) row = readlane_u64(nb[r][w], li);
+                    const uint64_t m = row & U[w];
+                    U[w] &= ~m;
+                    S[w] |= m;
+                    nn += __popcll(m);
+                }
+            }
+        }
+        res = cluster - 1;
+    }
+    if (lane == 0) out[cell] = res;
+}
+
+// Lane-per-cell fallback for large P, a direct restatement of the reference
+// loop on an int scratch row (labels [0,P), slots [P,2P)).  Row r = list[t]
+// (or t); result to out[outIndex[t]] (or out[r]).  Rows must hold all P
+// peaks and P must not exceed maxPeaks (the caller filters).
+__global__ __launch_bounds__(64) void k_dbscan_lane(int nList, const int* __restrict__ list,
+                                                    const int* __restrict__ counts,
+                                                    const double* __restrict__ amps,
+                                                    const double* __restrict__ ivls,
+                                                    long long rowStride, double T,
+                                                    int* __restrict__ scratch,
+                                                    long long scratchStride,
+                                                    int* __restrict__ out,
+                                                    const int* __restrict__ outIndex)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nList) return;
+    const int cell = list ? list[t] : t;
+    const int o = outIndex ? outIndex[t] : cell;
+    const int P = counts[cell];
+    if (P > rowStride || 2LL * P > scratchStride) {  // counts larger than the rows: invalid input
+        out[o] = -1;
+        return;
+    }
+    const double* A = amps + (long long)cell * rowStride;
+    const double* B = ivls + (long long)cell * rowStride;
+    int* H = scratch + (long long)t * scratchStride;
+    for (int i = 0; i < 2 * P; ++i) H[i] = 0;
+    int nn = 0, cluster = 0;
+    for (int i = 0; i < P; i++) {
+        bool scan = false;
+        if (nn >= 1) {
+            i = H[P + nn - 1];
+            H[P + nn - 1] = 0;
+            nn = nn - 1;
+            scan = true;
+        } else if (H[i] == 0) {
+            nn = 0;
+            ++cluster;
+            H[i] = cluster;
+            scan = true;
+        }
+        if (scan) {
+            const double ai = A[i], bi = B[i];
+            for (int k = 0; k < P - 1; k++) {
+                if (i != k && H[k] == 0 && near(ai, bi, A[k], B[k], T)) {
+                    H[k] = cluster;
+                    H[P + k] = k;
+                    ++nn;
+                }
+            }
+        }
+    }
+    out[o] = cluster - 1;
+}
+
+// ---------------------------------------------------------------- KDE
+//
+// expf of the reference's KDE kernel evaluated as exp in double rounded to
+// float (correctly rounded except within ~1 double ulp of a float midpoint;
+// glibc's expf differs from correct rounding on ~1e-4 of its inputs, so
+// KDE counts are reported as agreement, not claimed bit-exact).
+__device__ __forceinline__ double kde_term(double pm, double delt, double H)
+{
+    const double t = (pm - delt) / H;
+    const float arg = (float)(-((t * t) / 2));
+    return (double)(float)exp((double)arg);
+}
+
+// Rows hold up to cap peaks; cells with cap < P <= maxPeaks are appended to
+// overflow[] (listBase + cell) for recomputation.  Output to
+// out[outIndex[cell]] when outIndex is given.
+__global__ __launch_bounds__(256) void k_kde_wave(int nCells, const int* __restrict__ counts,
+                                                  const double* __restrict__ amps,
+                                                  long long rowStride, int cap, int maxPeaks,
+                                                  int kdeSampling, double I1, double I2,
+                                                  double H, int* __restrict__ out,
+                                                  const int* __restrict__ outIndex,
+                                                  int* __restrict__ overflow,
+                                                  int* __restrict__ nOverflow, int listBase)
+{
+    const int cell = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (cell >= nCells) return;
+    const int lane = lane_id();
+    const int P = __builtin_amdgcn_readfirstlane(counts[cell]);
+    int res;
+    if (P <= 0) res = 0;                         // -1 (diverged) and 0
+    else if (P == 1 || P == 2) res = 1;
+    else if (P > maxPeaks) res = maxPeaks;
+    else if (P > cap) {
+        if (lane == 0) {
+            const int k = atomicAdd(nOverflow, 1);
+            overflow[k] = listBase + cell;
+        }
+        return;
+    } else {
+        const double* pk = amps + (long long)cell * rowStride;
+        const double k1 = (double)(kdeSampling * P);
+        const double k2 = (I2 - I1) / (k1 - 1);
+        double pp = 0, p = 0, cur = 0;
+        bool strange = false;
+        int result = 0;
+        for (int base = 0; (double)base < k1 - 1; base += kWave) {
+            const int w = base + lane;
+            double val = 0;
+            if ((double)w < k1 - 1) {
+                const double delt = w * k2 + I1;
+                for (int m = 0; m < P; ++m) val += kde_term(pk[m], delt, H);
+            }
+            const int wend = min(kWave, (int)ceil(k1 - 1) - base);
+            for (int l = 0; l < wend; ++l) {
+                const int ww = base + l;
+                if (!((double)ww < k1 - 1)) break;
+                pp = p;
+                p = cur;
+                cur = readlane_d(val, l);
+                if (ww < 2) continue;
+                if (strange) {
+                    if (p == cur) continue;
+                    else if (p < cur) { strange = false; continue; }
+                    else if (p > cur) { strange = false; ++result; continue; }
+                } else if (p > pp && p > cur) { ++result; continue; }
+                else if (p > pp && p == cur) { strange = true; continue; }
+            }
+        }
+        if (p < cur) ++result;
+        res = result;
+    }
+    if (lane == 0) out[outIndex ? outIndex[cell] : cell] = res;
+}
+
+// --------------------------------------------------- integer post-ops
+
+// Cluster-count clamp min(v, clamp) applied in place (clamp > 0).
+__global__ void k_clamp_i32(int n, int* __restrict__ v, int clamp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && v[i] > clamp) v[i] = clamp;
+}
+
+// Fill kernel (used instead of hipMemset so graph-captured work stays
+// kernel-only).
+__global__ void k_fill_i32(long long n, int* __restrict__ v, int value)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = value;
+}
+
+// --------------------------------------------------------- launchers
+
+template <class M>
+static hipError_t launch_bif_m(const BifArgs& A, int wv, bool check, int* counts, double* amps,
+                               double* ivls, hipStream_t st)
+{
+    const dim3 blk(256), grd((A.nCells + 255) / 256);
+    if (A.nCells <= 0) return hipSuccess;
+#define BIF_CASE(W, C) hipLaunchKernelGGL((k_bif_peaks<M, W, C>), grd, blk, 0, st, A, counts, amps, ivls)
+    if (check) {
+        if (wv == 0) BIF_CASE(0, true); else if (wv == 1) BIF_CASE(1, true); else BIF_CASE(2, true);
+    } else {
+        if (wv == 0) BIF_CASE(0, false); else if (wv == 1) BIF_CASE(1, false); else BIF_CASE(2, false);
+    }
+#undef BIF_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_bif(int model, const BifArgs& A, int wv, bool check, int* counts, double* amps,
+                      double* ivls, hipStream_t st)
+{
+    if (model == kLinear) return launch_bif_m<Linear>(A, wv, check, counts, amps, ivls, st);
+    return launch_bif_m<Rossler>(A, wv, check, counts, amps, ivls, st);
+}
+
+hipError_t launch_series(int model, const BifArgs& A, int wv, double* data, int* flags,
+                         hipStream_t st)
+{
+    if (A.nCells <= 0) return hipSuccess;
+    const dim3 blk(256), grd((A.nCells + 255) / 256);
+    if (model == kLinear)
+        hipLaunchKernelGGL(k_series<Linear>, grd, blk, 0, st, A, wv, data, flags);
+    else
+        hipLaunchKernelGGL(k_series<Rossler>, grd, blk, 0, st, A, wv, data, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_peak_finder(const double* data, int sizeOfBlock, int nBlocks, int* counts,
+                              double* outPeaks, double* outIntervals, double hpf, hipStream_t st)
+{
+    if (nBlocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_peak_finder, dim3((nBlocks + 255) / 256), dim3(256), 0, st, data,
+                       sizeOfBlock, nBlocks, counts, outPeaks, outIntervals, hpf);
+    return hipGetLastError();
+}
+
+hipError_t launch_dbscan_wave(int nCells, const int* counts, const double* amps,
+                              const double* ivls, long long rowStride, int cap, int maxPeaks,
+                              double T, int* out, int* overflow, int* nOverflow, int listBase,
+                              hipStream_t st)
+{
+    if (nCells <= 0) return hipSuccess;
+    const long long threads = (long long)nCells * kWave;
+    hipLaunchKernelGGL(k_dbscan_wave, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
+                       nCells, counts, amps, ivls, rowStride, cap, maxPeaks, T, out, overflow,
+                       nOverflow, listBase);
+    return hipGetLastError();
+}
+
+hipError_t launch_dbscan_lane(int nList, const int* list, const int* counts, const double* amps,
+                              const double* ivls, long long rowStride, double T, int* scratch,
+                              long long scratchStride, int* out, const int* outIndex,
+                              hipStream_t st)
+{
+    if (nList <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dbscan_lane, dim3((nList + 63) / 64), dim3(64), 0, st, nList, list,
+                       counts, amps, ivls, rowStride, T, scratch, scratchStride, out, outIndex);
+    return hipGetLastError();
+}
+
+hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, long long rowStride,
+                           int cap, int maxPeaks, int kdeSampling, double I1, double I2, double H,
+                           int* out, const int* outIndex, int* overflow, int* nOverflow,
+                           int listBase, hipStream_t st)
+{
+    if (nCells <= 0) return hipSuccess;
+    const long long threads = (long long)nCells * kWave;
+    hipLaunchKernelGGL(k_kde_wave, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
+                       nCells, counts, amps, rowStride, cap, maxPeaks, kdeSampling, I1, I2, H,
+                       out, outIndex, overflow, nOverflow, listBase);
+    return hipGetLastError();
+}
+
+hipError_t launch_clamp(int n, int* v, int clamp, hipStream_t st)
+{
+    if (n <= 0 || clamp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clamp_i32, dim3((n + 255) / 256), dim3(256), 0, st, n, v, clamp);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(long long n, int* v, int value, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, v,
+                       value);
+    return hipGetLastError();
+}
+
+}  // namespace bifhip

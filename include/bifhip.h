@@ -1,0 +1,266 @@
+/*
+ * bifhip.h -- C ABI of libbifhip.so, the MI355X (gfx950) bifurcation-analysis
+ * library.  Plain C types only; no HIP or torch types cross this boundary.
+ *
+ * Every driver keeps the parameter order, types and meaning of the
+ * reference's host function it replaces (KiShiVi/CUDABifurcation3.0,
+ * Library/hostLibrary.cuh), returns a status instead of calling exit(),
+ * and writes the same CSV files to the configured output path.
+ *
+ * Status: 0 success; > 0 a hipError_t value; < 0 one of BIFHIP_E_*.
+ * bifhip_last_error() describes the last failure.
+ *
+ * Threading: one context per process, calls are synchronous on one HIP
+ * stream and not re-entrant (as in the reference, which also shares one
+ * output file).
+ */
+#ifndef BIFHIP_H
+#define BIFHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BIFHIP_OK 0
+#define BIFHIP_E_ARG (-1)      /* invalid argument                          */
+#define BIFHIP_E_FILE (-2)     /* output file could not be opened/written   */
+#define BIFHIP_E_NODEV (-3)    /* no usable gfx950 device                   */
+#define BIFHIP_E_LIMIT (-4)    /* size beyond what the library supports     */
+
+#define BIFHIP_MODEL_ROSSLER 0 /* Library/cudaLibrary.cu:15-17 (values {a,b,c}) */
+#define BIFHIP_MODEL_LINEAR 1  /* x_i += h*a_i*x_i: closed-form test model      */
+
+#define BIFHIP_PREC_FP64 0     /* reference operation order, no contraction     */
+#define BIFHIP_PREC_FP32 1     /* FP32 bulk integration, FP64 renorm/log        */
+
+#define BIFHIP_PERT_REFERENCE 0 /* 0.5*sinf(...) of the chunk-local index      */
+#define BIFHIP_PERT_AXIS_X 1    /* unit perturbation along x[0]                */
+
+/* ------------------------------------------------------------ context */
+
+/* Output file path.  Replaces the compile-time OUT_FILE_PATH macro
+ * (Library/hostLibrary.cu:3).  Default "mat.csv". */
+int bifhip_set_output_path(const char* path);
+const char* bifhip_get_output_path(void);
+/* Model plug-in.  Replaces editing calculateDiscreteModel
+ * (Library/cudaLibrary.cu:2-44).  Default BIFHIP_MODEL_ROSSLER. */
+int bifhip_set_model(int model);
+/* Integration precision of the LLE drivers.  Default BIFHIP_PREC_FP64. */
+int bifhip_set_precision(int precision);
+/* LLE/LS initial perturbation directions.  Default BIFHIP_PERT_REFERENCE
+ * (Library/cudaLibrary.cu:720-732, :1025-1040). */
+int bifhip_set_perturbation(int mode);
+/* Chunk whose local index seeds the reference perturbation directions
+ * (the reference's nPtsLimiter): -1 = the driver's own (200000 in LLE2D,
+ * Library/hostLibrary.cu:1552; one chunk elsewhere), 0 = none. */
+int bifhip_set_direction_chunk(long long cells);
+/* Upper bound on cells per kernel launch (0 = automatic).  Results do not
+ * depend on it; tests use it to force multi-launch execution. */
+int bifhip_set_max_cells_per_launch(long long cells);
+/* Peak slots per cell kept in HBM by the 2D peak-count drivers (>= 2;
+ * default 256).  Cells with more peaks are recomputed exactly. */
+int bifhip_set_peak_capacity(int cap);
+/* Clamp applied to 2D DBSCAN cluster counts (0 = none, the reference). */
+int bifhip_set_cluster_clamp(int clamp);
+/* Progress messages on stdout like the reference's DEBUG build. */
+int bifhip_set_verbose(int on);
+/* HIP events around every kernel launch for bifhip_kernel_time(). */
+int bifhip_set_timing(int on);
+/* Accumulated event time (ms) and launch count of a kernel family
+ * ("bif_peaks", "dbscan", "kde", "lle", "ls", ...) since the last reset;
+ * kernel == NULL resets all counters. */
+int bifhip_kernel_time(const char* kernel, double* total_ms, long long* launches);
+/* Release cached device workspaces. */
+int bifhip_release(void);
+const char* bifhip_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int bifhip_device_count(void);
+/* Library version string. */
+const char* bifhip_version(void);
+
+/* ------------------------------------------------------ drivers (files) */
+
+/* Library/hostLibrary.cuh:29-42 (bifurcation1D).  Writes "param, peak\n"
+ * lines, 12 significant digits. */
+int bifhip_bifurcation1D(double tMax, int nPts, double h, int amountOfInitialConditions,
+                         const double* initialConditions, const double* ranges,
+                         const int* indicesOfMutVars, int writableVar, double maxValue,
+                         double transientTime, const double* values, int amountOfValues,
+                         int preScaller);
+
+/* Library/hostLibrary.cuh:129-147 (bifurcation2DKDE). */
+int bifhip_bifurcation2DKDE(double tMax, int nPts, double h, int amountOfInitialConditions,
+                            const double* initialConditions, const double* ranges,
+                            const int* indicesOfMutVars, int writableVar, double maxValue,
+                            int maxAmountOfPeaks, double transientTime, const double* values,
+                            int amountOfValues, int preScaller, int kdeSampling,
+                            double kdeSamplesInterval1, double kdeSamplesInterval2,
+                            double kdeSamplesSmooth);
+
+/* Library/hostLibrary.cuh:171-186 (bifurcation2DDBSCAN). */
+int bifhip_bifurcation2DDBSCAN(double tMax, int nPts, double h, int amountOfInitialConditions,
+                               const double* initialConditions, const double* ranges,
+                               const int* indicesOfMutVars, int writableVar, double maxValue,
+                               int maxAmountOfPeaks, double transientTime, const double* values,
+                               int amountOfValues, int preScaller, double eps);
+
+/* Library/hostLibrary.cuh:334-348 (LLE1D). */
+int bifhip_LLE1D(double tMax, double NT, int nPts, double h, double eps,
+                 const double* initialConditions, int amountOfInitialConditions,
+                 const double* ranges, const int* indicesOfMutVars, int writableVar,
+                 double maxValue, double transientTime, const double* values,
+                 int amountOfValues);
+
+/* Library/hostLibrary.cuh:408-422 (LLE2D). */
+int bifhip_LLE2D(double tMax, double NT, int nPts, double h, double eps,
+                 const double* initialConditions, int amountOfInitialConditions,
+                 const double* ranges, const int* indicesOfMutVars, int writableVar,
+                 double maxValue, double transientTime, const double* values,
+                 int amountOfValues);
+
+/* Library/hostLibrary.cuh:482-496 (LS1D). */
+int bifhip_LS1D(double tMax, double NT, int nPts, double h, double eps,
+                const double* initialConditions, int amountOfInitialConditions,
+                const double* ranges, const int* indicesOfMutVars, int writableVar,
+                double maxValue, double transientTime, const double* values,
+                int amountOfValues);
+
+/* Library/hostLibrary.cuh:520-534 (LS2D).  Writes <path>1.csv .. <path>n.csv. */
+int bifhip_LS2D(double tMax, double NT, int nPts, double h, double eps,
+                const double* initialConditions, int amountOfInitialConditions,
+                const double* ranges, const int* indicesOfMutVars, int writableVar,
+                double maxValue, double transientTime, const double* values,
+                int amountOfValues);
+
+/* ------------------------------------------- drivers (host buffers) */
+
+/* Same computations, results in caller-owned host memory instead of a file.
+ * counts[nPts]; peaks[nPts * peakStride] (row k holds counts[k] peaks;
+ * peakStride >= (int)(tMax/h/preScaller)/2 + 1 always suffices). */
+int bifhip_bifurcation1D_to_buffer(double tMax, int nPts, double h,
+                                   int amountOfInitialConditions,
+                                   const double* initialConditions, const double* ranges,
+                                   const int* indicesOfMutVars, int writableVar,
+                                   double maxValue, double transientTime, const double* values,
+                                   int amountOfValues, int preScaller, int* counts,
+                                   double* peaks, int peakStride);
+
+/* out[nPts*nPts], row-major with the first swept parameter fastest. */
+int bifhip_bifurcation2DKDE_to_buffer(double tMax, int nPts, double h,
+                                      int amountOfInitialConditions,
+                                      const double* initialConditions, const double* ranges,
+                                      const int* indicesOfMutVars, int writableVar,
+                                      double maxValue, int maxAmountOfPeaks,
+                                      double transientTime, const double* values,
+                                      int amountOfValues, int preScaller, int kdeSampling,
+                                      double kdeSamplesInterval1, double kdeSamplesInterval2,
+                                      double kdeSamplesSmooth, int* out);
+
+int bifhip_bifurcation2DDBSCAN_to_buffer(double tMax, int nPts, double h,
+                                         int amountOfInitialConditions,
+                                         const double* initialConditions, const double* ranges,
+                                         const int* indicesOfMutVars, int writableVar,
+                                         double maxValue, int maxAmountOfPeaks,
+                                         double transientTime, const double* values,
+                                         int amountOfValues, int preScaller, double eps,
+                                         int* out);
+
+/* dimension 1 (LLE1D) or 2 (LLE2D); out[nPts^dimension]. */
+int bifhip_LLE_to_buffer(int dimension, double tMax, double NT, int nPts, double h, double eps,
+                         const double* initialConditions, int amountOfInitialConditions,
+                         const double* ranges, const int* indicesOfMutVars, int writableVar,
+                         double maxValue, double transientTime, const double* values,
+                         int amountOfValues, double* out);
+
+/* dimension 1 (LS1D) or 2 (LS2D); out[nPts^dimension * nIC], cell-major. */
+int bifhip_LS_to_buffer(int dimension, double tMax, double NT, int nPts, double h, double eps,
+                        const double* initialConditions, int amountOfInitialConditions,
+                        const double* ranges, const int* indicesOfMutVars, int writableVar,
+                        double maxValue, double transientTime, const double* values,
+                        int amountOfValues, double* out);
+
+/* ----------------------------------------- drivers (device buffers) */
+
+/* Device-resident outputs for composition and benchmarking: d_out is a
+ * device pointer the caller owns; nothing is copied back. */
+int bifhip_bifurcation2DDBSCAN_device(double tMax, int nPts, double h,
+                                      int amountOfInitialConditions,
+                                      const double* initialConditions, const double* ranges,
+                                      const int* indicesOfMutVars, int writableVar,
+                                      double maxValue, int maxAmountOfPeaks,
+                                      double transientTime, const double* values,
+                                      int amountOfValues, int preScaller, double eps,
+                                      int* d_out);
+
+int bifhip_LLE_device(int dimension, double tMax, double NT, int nPts, double h, double eps,
+                      const double* initialConditions, int amountOfInitialConditions,
+                      const double* ranges, const int* indicesOfMutVars, int writableVar,
+                      double maxValue, double transientTime, const double* values,
+                      int amountOfValues, double* d_out);
+
+/* ------------------------------------ reference kernels on device data */
+
+/* peakFinderCUDA (Library/cudaLibrary.cu:270-284) with peakFinder's h
+ * (Library/cudaLibrary.cuh:204, default 0.01): series d_data[nBlocks *
+ * sizeOfBlock]; d_counts holds -1 for flagged blocks on entry and the
+ * peak count on exit; peaks and intervals are written from each block's
+ * start (d_outPeaks may equal d_data). */
+int bifhip_peak_finder_device(const double* d_data, int sizeOfBlock, int amountOfBlocks,
+                              int* d_counts, double* d_outPeaks, double* d_timeOfPeaks,
+                              double h);
+
+/* dbscanCUDA (Library/cudaLibrary.cu:609-624) over per-block peak rows of
+ * stride sizeOfBlock: d_out[b] = dbscan(...) (-1 counts give 0). */
+int bifhip_dbscan_device(const double* d_peaks, const double* d_intervals, int sizeOfBlock,
+                         int amountOfBlocks, const int* d_counts, int maxAmountOfPeaks,
+                         double eps, int* d_out);
+
+/* kdeCUDA (Library/cudaLibrary.cu:426-441). */
+int bifhip_kde_device(const double* d_peaks, int sizeOfBlock, int amountOfBlocks,
+                      const int* d_counts, int maxAmountOfPeaks, int kdeSampling,
+                      double kdeSamplesInterval1, double kdeSamplesInterval2,
+                      double kdeSamplesSmooth, int* d_out);
+
+/* gramSchmidtProcess (Library/cudaLibrary.cu:931-957), batched: count
+ * row-major n x n matrices, n <= 8; d_denominators may be NULL. */
+int bifhip_gram_schmidt_device(const double* d_a, double* d_b, double* d_denominators, int n,
+                               int count);
+
+/* calculateDiscreteModelCUDA record pass (Library/cudaLibrary.cu:78-117):
+ * the recorded series of every cell of a 1D/2D grid, d_data[cells * N]
+ * with N = (int)(tMax/h/preScaller); d_flags[cell] = -1 on divergence. */
+int bifhip_series_device(int dimension, double tMax, int nPts, double h,
+                         int amountOfInitialConditions, const double* initialConditions,
+                         const double* ranges, const int* indicesOfMutVars, int writableVar,
+                         double maxValue, double transientTime, const double* values,
+                         int amountOfValues, int preScaller, double* d_data, int* d_flags);
+
+/* ------------------------------------------------ CSV writers (host) */
+
+/* The reference's file formats, usable without a GPU.
+ * 2D maps: header "r0 r1\nr2 r3\n" (when header != 0), values joined by
+ * ", " with a line break after every nPts values, the ", " written before
+ * the break (Library/hostLibrary.cu:651-655, :708-725); precision = the
+ * stream precision (6 default, 12 for bifurcation2DKDE). */
+int bifhip_write_map_int(const char* path, int nPts, const double* ranges, int header,
+                         int precision, const int* values, long long count);
+int bifhip_write_map_double(const char* path, int nPts, const double* ranges, int header,
+                            int precision, const double* values, long long count,
+                            int stride, int offset);
+/* bifurcation1D lines "param, peak\n" (Library/hostLibrary.cu:125-133). */
+int bifhip_write_bif1d(const char* path, int nPts, const double* ranges, const int* counts,
+                       const double* peaks, int peakStride);
+/* LLE1D / LS1D lines "param, v1[, v2 ...]\n" (Library/hostLibrary.cu:1368-1373,
+ * :1900-1908). */
+int bifhip_write_1d_values(const char* path, int nPts, const double* ranges,
+                           const double* values, int perCell);
+/* getValueByIdx (Library/cudaLibrary.cu:210-216) on the host. */
+double bifhip_value_by_idx(int idx, int nPts, double startRange, double finishRange,
+                           int valueNumber);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

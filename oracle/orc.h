@@ -1,0 +1,147 @@
+/*
+ * orc.h -- CPU oracle for the bifurcation-analysis hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's per-trajectory algorithms (KiShiVi/CUDABifurcation3.0,
+ * Library/cudaLibrary.cu and the driver arithmetic of Library/hostLibrary.cu).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it, and only as the checker or as the timed CPU baseline.  The product
+ * (libbifhip.so) never links or calls it.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp, no -march),
+ * so every double operation is one IEEE-754 binary64 rounding in the
+ * reference's source order, with no FMA contraction.
+ *
+ * Pinning (what checks each function; see tests/test_oracle_*.py):
+ *   orc_step                 closed form of the linear model; brute-force
+ *                            hex fixtures of Rossler samples (tests/golden)
+ *   orc_loop                 record-before-step / divergence KATs
+ *   orc_value_by_idx         closed form a_i = s + i*(f-s)/(n-1)
+ *   orc_peak_finder          survey KAT {0,1,3,3,2,0,1,4,1,...}; brute-force
+ *                            textbook comparison on plateau-free series
+ *   orc_distance             glibc hypotf == (float)sqrt(double) check
+ *   orc_dbscan               survey KATs ({1,2}->1, {1..8}->7, ...); Rossler
+ *                            cluster counts at c=2.5,3.5,4.0,5.7,9.0
+ *   orc_kde                  single-cluster / separated-cluster closed cases
+ *   orc_gram_schmidt         orthonormality, |det| = prod(norms), numpy QR
+ *   orc_lle_cell             linear model: LLE = log|1+h*lambda|/h
+ *   orc_ls_cell              linear model spectrum; volume-contraction sum
+ */
+#ifndef BIFHIP_ORACLE_H
+#define BIFHIP_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Model ids shared with the product (same numbering). */
+#define ORC_MODEL_ROSSLER 0   /* sequential explicit Euler, cudaLibrary.cu:15-17 */
+#define ORC_MODEL_LINEAR  1   /* x_i += h*(a_i*x_i): closed-form test model     */
+
+int  orc_model_dim(int model);
+void orc_step(int model, double *x, const double *a, double h);
+
+int  orc_loop(int model, double *x, const double *values, double h,
+              int nIter, int preScaller, int writableVar, double maxValue,
+              double *data, long start, int writeStep);
+
+double orc_value_by_idx(int idx, int nPts, double startRange,
+                        double finishRange, int valueNumber);
+
+int  orc_peak_finder(double *data, long start, int amountOfPoints,
+                     double *outPeaks, double *timeOfPeaks, double h);
+
+int  orc_kde(const double *data, long start, int amountOfPoints,
+             int maxAmountOfPeaks, int kdeSampling, double I1, double I2,
+             double smoothH);
+
+double orc_distance(double x1, double y1, double x2, double y2);
+
+int  orc_dbscan(const double *data, const double *intervals, double *helpful,
+                long start, int amountOfPeaks, int sizeOfHelpful,
+                int maxAmountOfPeaks, double eps);
+
+void orc_gram_schmidt(const double *a, double *b, int n, double *denoms);
+
+/* One cell of calculateDiscreteModelCUDA (cudaLibrary.cu:78-117).
+ * Writes N recorded samples to series; returns 0 on divergence, 1 otherwise.
+ * cellIdx is the global flattened grid index. */
+int  orc_bif_cell(int model, int nPts, long cellIdx, int dimension,
+                  const double *ranges, const int *mutIdx, const double *ic,
+                  int nIC, const double *values, int nVal, double h,
+                  int skip, int N, int preScaller, int writableVar,
+                  double maxValue, double *series);
+
+/* Perturbation-direction modes. */
+#define ORC_PERT_REFERENCE 0  /* 0.5*sinf(...) of the chunk-local index */
+#define ORC_PERT_AXIS_X    1  /* unit vector along x[0] */
+
+double orc_lle_cell(int model, int nPts, long cellIdx, long localIdx,
+                    int dimension, const double *ranges, const int *mutIdx,
+                    const double *ic, int nIC, const double *values, int nVal,
+                    double NT, double tMax, double h, double eps, int skip,
+                    double maxValue, int pertMode);
+
+void orc_ls_cell(int model, int nPts, long cellIdx, long localIdx,
+                 int dimension, const double *ranges, const int *mutIdx,
+                 const double *ic, int nIC, const double *values, int nVal,
+                 double NT, double tMax, double h, double eps, int skip,
+                 double maxValue, int pertMode, double *out);
+
+/* ---- map-level drivers (OpenMP over cells; in-memory results) ---- */
+
+/* bifurcation1D (hostLibrary.cu:7-158): counts[nPts], peaks[nPts*cap]. */
+void orc_bifurcation1d(int model, double tMax, int nPts, double h, int nIC,
+                       const double *ic, const double *ranges,
+                       const int *mutIdx, int writableVar, double maxValue,
+                       double transientTime, const double *values, int nVal,
+                       int preScaller, int *counts, double *peaks, int cap);
+
+/* Per-cell peak counts and peaks for a 2D (or 1D) grid slice [c0, c0+nc). */
+void orc_peaks_cells(int model, int dimension, double tMax, int nPts, double h,
+                     int nIC, const double *ic, const double *ranges,
+                     const int *mutIdx, int writableVar, double maxValue,
+                     double transientTime, const double *values, int nVal,
+                     int preScaller, long c0, long nc, int *counts,
+                     double *peaks, double *intervals, int cap);
+
+/* bifurcation2DDBSCAN (hostLibrary.cu:571-744) over cells [c0, c0+nc). */
+void orc_dbscan_cells(int model, double tMax, int nPts, double h, int nIC,
+                      const double *ic, const double *ranges,
+                      const int *mutIdx, int writableVar, double maxValue,
+                      int maxAmountOfPeaks, double transientTime,
+                      const double *values, int nVal, int preScaller,
+                      double eps, long c0, long nc, int *out);
+
+/* bifurcation2DKDE (hostLibrary.cu:397-567) over cells [c0, c0+nc). */
+void orc_kde_cells(int model, double tMax, int nPts, double h, int nIC,
+                   const double *ic, const double *ranges, const int *mutIdx,
+                   int writableVar, double maxValue, int maxAmountOfPeaks,
+                   double transientTime, const double *values, int nVal,
+                   int preScaller, int kdeSampling, double I1, double I2,
+                   double smoothH, long c0, long nc, int *out);
+
+/* LLE1D / LLE2D (hostLibrary.cu:1270-1393, 1524-1663) over [c0, c0+nc).
+ * dirChunk: chunk size whose local index seeds the reference direction. */
+void orc_lle_cells(int model, int dimension, double tMax, double NT, int nPts,
+                   double h, double eps, const double *ic, int nIC,
+                   const double *ranges, const int *mutIdx, double maxValue,
+                   double transientTime, const double *values, int nVal,
+                   int pertMode, long dirChunk, long c0, long nc, double *out);
+
+/* LS1D / LS2D (hostLibrary.cu:1802-2082) over [c0, c0+nc); out[nc*nIC]. */
+void orc_ls_cells(int model, int dimension, double tMax, double NT, int nPts,
+                  double h, double eps, const double *ic, int nIC,
+                  const double *ranges, const int *mutIdx, double maxValue,
+                  double transientTime, const double *values, int nVal,
+                  int pertMode, long dirChunk, long c0, long nc, double *out);
+
+/* Number of OpenMP threads the map drivers use. */
+int  orc_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

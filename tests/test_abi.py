@@ -1,0 +1,129 @@
+"""C-ABI and host-side behaviour of libbifhip.so (CPU only, no GPU calls).
+
+The library must load and export every function include/bifhip.h declares;
+the host-side pieces (CSV writers, getValueByIdx, argument validation,
+no-device errors) are exercised without touching a GPU.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import bifhip as B
+
+
+def test_library_exports_every_declared_symbol():
+    lib = B.load()
+    names = B.header_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", B.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert set(names) <= exported
+
+
+def test_value_by_idx_matches_oracle(orc):
+    for n, s, f in [(512, 0.05, 0.45), (1000, 2.0, 7.0), (4096, 2.0, 10.0), (7, -1.0, 1.0)]:
+        for idx in range(0, n * n, max(1, n * n // 997)):
+            for k in (0, 1):
+                assert B.value_by_idx(idx, n, s, f, k) == orc.value_by_idx(idx, n, s, f, k)
+
+
+def _ref_map_text(nPts, ranges, vals, prec, header=True, fmt=None):
+    """The reference's 2D writer (Library/hostLibrary.cu:651-655, :708-725)."""
+    fmt = fmt or (lambda v: str(int(v)))
+    out = []
+    if header:
+        out.append(f"{ranges[0]:.{prec}g} {ranges[1]:.{prec}g}\n")
+        out.append(f"{ranges[2]:.{prec}g} {ranges[3]:.{prec}g}\n")
+    counter = 0
+    for v in vals:
+        if counter != 0:
+            out.append(", ")
+        if counter == nPts:
+            out.append("\n")
+            counter = 0
+        out.append(fmt(v))
+        counter += 1
+    return "".join(out)
+
+
+def test_map_int_writer_format(tmp_path):
+    ranges = [0.05, 0.45, 2, 10]
+    vals = np.arange(12, dtype=np.int32) - 3
+    p = tmp_path / "m.csv"
+    B.write_map_int(p, 4, ranges, vals, header=True, precision=6)
+    assert p.read_text() == _ref_map_text(4, ranges, vals, 6)
+    assert p.read_text().startswith("0.05 0.45\n2 10\n-3, -2, -1, 0, \n1, 2")
+    B.write_map_int(p, 4, [0.123456789012345, 1, 2, 3], vals, header=True, precision=12)
+    assert p.read_text().startswith("0.123456789012 1\n")
+
+
+def test_map_double_writer_format_and_stride(tmp_path):
+    ranges = [0.05, 0.45, 2, 10]
+    rng = np.random.default_rng(3)
+    vals = rng.normal(size=(9, 3)) * 10.0 ** rng.integers(-8, 8, size=(9, 3))
+    for k in range(3):
+        p = tmp_path / f"ls{k}.csv"
+        B.write_map_double(p, 3, ranges, vals.ravel(), precision=6, stride=3, offset=k)
+        assert p.read_text() == _ref_map_text(3, ranges, vals[:, k], 6, fmt=lambda v: f"{v:.6g}")
+
+
+def test_bif1d_writer_format(tmp_path, orc):
+    ranges = [2.0, 7.0]
+    counts = np.array([2, 0, 3], dtype=np.int32)
+    peaks = np.array([[1.5, 2.25, 0], [0, 0, 0], [1 / 3, 2 / 3, 12345.678901234567]])
+    p = tmp_path / "b.csv"
+    B.write_bif1d(p, 3, ranges, counts, peaks)
+    lines = []
+    for k in range(3):
+        pv = orc.value_by_idx(k, 3, 2.0, 7.0, 0)
+        for j in range(counts[k]):
+            lines.append(f"{pv:.12g}, {peaks[k, j]:.12g}\n")
+    assert p.read_text() == "".join(lines)
+
+
+def test_1d_values_writer_format(tmp_path, orc):
+    ranges = [0.05, 0.35]
+    v = np.array([[0.1, -0.02, -5.0], [1e-9, 0, 3.14159265]])
+    p = tmp_path / "ls1d.csv"
+    B.write_1d_values(p, 2, ranges, v.ravel(), per_cell=3)
+    want = ""
+    for k in range(2):
+        want += f"{orc.value_by_idx(k, 2, 0.05, 0.35, 0):g}" + "".join(f", {x:g}" for x in v[k]) + "\n"
+    assert p.read_text() == want
+
+
+def test_writer_reports_unwritable_path(tmp_path):
+    with pytest.raises(B.BifhipError) as e:
+        B.write_map_int(tmp_path / "no" / "such" / "dir.csv", 2, [0, 1, 0, 1], [1, 2, 3, 4])
+    assert e.value.code == B.E_FILE
+
+
+def test_context_knob_validation():
+    with pytest.raises(B.BifhipError):
+        B.set_model(7)
+    with pytest.raises(B.BifhipError):
+        B.set_precision(5)
+    with pytest.raises(B.BifhipError):
+        B.set_peak_capacity(1)
+    B.reset_context()
+    B.set_output_path("/tmp/x.csv")
+    assert B.load().bifhip_get_output_path() == b"/tmp/x.csv"
+    B.set_output_path("mat.csv")
+
+
+def test_drivers_fail_cleanly_without_device():
+    """On a host with no GPU every driver returns BIFHIP_E_NODEV (no crash,
+    no CPU fallback)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(B.BifhipError) as e:
+        B.dbscan_map(500, 4, 0.01, [0.1] * 3, [0.05, 0.45, 2, 10], [0, 2], 0, 0.0, 32, 1000,
+                     [0.2, 0.2, 5.7], 1, 0.01)
+    assert e.value.code == B.E_NODEV

@@ -1,0 +1,407 @@
+"""GPU parity: the HIP kernels through the C ABI against the CPU oracle.
+
+Standards:
+  * integer / index results and the polynomial FP64 path (samples, peaks,
+    intervals, DBSCAN counts, Gram-Schmidt): bit-exact;
+  * LLE / LS (FP64 path): every operation is IEEE-exact except `log`, whose
+    device and glibc versions may differ in the last ulp, so |gpu - oracle|
+    <= 1e-12 absolute, and the bit-identical fraction is reported;
+  * KDE: glibc's expf is not correctly rounded, so count agreement >= 99.5%;
+  * FP32 bulk LLE: judged against the FP64 path (error and sign agreement).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+IC = [0.1, 0.1, 0.1]
+VALS = [0.2, 0.2, 5.7]
+R2 = [0.05, 0.45, 2.0, 10.0]
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _t(dev, a, dtype):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+
+# ------------------------------------------------------------ record pass
+
+@pytest.mark.parametrize("wv,pre", [(0, 1), (1, 1), (2, 3)])
+def test_series_bit_exact(orc, bh, dev, wv, pre):
+    import torch
+    n, tmax, h, trans = 16, 20.0, 0.01, 10.0
+    N = int(tmax / h / pre)
+    say("series", n, wv, pre)
+    data = torch.zeros(n * n * N, dtype=torch.float64, device=dev)
+    flags = torch.zeros(n * n, dtype=torch.int32, device=dev)
+    bh.series_device(data.data_ptr(), flags.data_ptr(), 2, tmax, n, h, IC, R2, [0, 2], wv, 0.0,
+                     trans, VALS, pre)
+    torch.cuda.synchronize()
+    got = data.cpu().numpy().reshape(n * n, N)
+    for cell in range(0, n * n, 5):
+        ok, s = orc.bif_cell(cell, n, R2, [0, 2], IC, VALS, h, int(trans / h), N, pre, wv)
+        assert ok == 1
+        np.testing.assert_array_equal(got[cell], s, err_msg=f"cell {cell}")
+    assert not flags.any()
+
+
+# ------------------------------------------------------------ peak finder
+
+def _plateau_walks(rng, nb, n):
+    d = np.round(np.cumsum(rng.normal(size=(nb, n)), axis=1) * 1.5) / 1.5
+    d[::7, 100:140] = d[::7, 100:101]          # long plateaus
+    return d
+
+
+@pytest.mark.parametrize("inplace", [False, True])
+def test_peak_finder_device_fuzz(orc, bh, dev, inplace):
+    import torch
+    rng = np.random.default_rng(5)
+    nb, n = 384, 3000
+    d = _plateau_walks(rng, nb, n)
+    cnt0 = np.zeros(nb, dtype=np.int32)
+    cnt0[::11] = -1
+    say("peak_finder", nb, n, inplace)
+    D = _t(dev, d.ravel(), torch.float64)
+    C = _t(dev, cnt0, torch.int32)
+    P = D if inplace else torch.zeros_like(D)
+    T = torch.zeros_like(D)
+    bh.peak_finder_device(D.data_ptr(), n, nb, C.data_ptr(), P.data_ptr(), T.data_ptr(), 0.01)
+    torch.cuda.synchronize()
+    cnt, pk, tm = C.cpu().numpy(), P.cpu().numpy().reshape(nb, n), T.cpu().numpy().reshape(nb, n)
+    for b in range(nb):
+        if cnt0[b] == -1:
+            assert cnt[b] == 0
+            continue
+        Pw, pkw, tw = orc.peak_finder(d[b], 0.01)
+        assert cnt[b] == Pw, b
+        np.testing.assert_array_equal(pk[b, :Pw], pkw)
+        np.testing.assert_array_equal(tm[b, :Pw], tw)
+
+
+def test_peak_finder_reference_kat_gpu(bh, dev):
+    import json, os, torch
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+    case = gold["peak_finder"][0]
+    s = np.array(case["series"], dtype=np.float64)
+    D = _t(dev, s, torch.float64)
+    C = torch.zeros(1, dtype=torch.int32, device=dev)
+    P = torch.zeros_like(D)
+    T = torch.zeros_like(D)
+    bh.peak_finder_device(D.data_ptr(), len(s), 1, C.data_ptr(), P.data_ptr(), T.data_ptr(), 0.01)
+    torch.cuda.synchronize()
+    assert int(C[0]) == case["count"]
+    assert P.cpu().numpy()[:2].tolist() == case["peaks"]
+    assert T.cpu().numpy()[:2].tolist() == case["intervals"]
+
+
+# ------------------------------------------------------- bifurcation 1D
+
+def test_bifurcation1d_config1_bit_exact(orc, bh):
+    """1D diagram, Rossler c in [2,7] x 1000, transient 1000, record 500."""
+    say("bifurcation1D config-1")
+    cnt, pk = bh.bifurcation1D_to_buffer(500.0, 1000, 0.01, IC, [2.0, 7.0], [2], 0, 0.0, 1000.0,
+                                         [0.2, 0.2, 0.0], 1)
+    cap = pk.shape[1]
+    wc, wp, _ = orc.peaks_cells(1, 500.0, 1000, 0.01, IC, [2.0, 7.0], [2], 0, 0.0, 1000.0,
+                                [0.2, 0.2, 0.0], 1, 0, 1000, cap)
+    np.testing.assert_array_equal(cnt, np.maximum(wc, 0))
+    for k in range(1000):
+        np.testing.assert_array_equal(pk[k, :cnt[k]], wp[k, :cnt[k]], err_msg=str(k))
+    assert 80 <= cnt.min() and cnt.max() <= 90
+
+
+def test_bifurcation1d_file_matches_reference_format(orc, bh, tmp_path):
+    say("bifurcation1D file")
+    out = tmp_path / "mat.csv"
+    bh.set_output_path(out)
+    bh.bifurcation1D(100.0, 40, 0.01, 3, IC, [2.0, 7.0], [2], 0, 0.0, 200.0, [0.2, 0.2, 0.0], 3,
+                     1)
+    N = int(100.0 / 0.01)
+    wc, wp, _ = orc.peaks_cells(1, 100.0, 40, 0.01, IC, [2.0, 7.0], [2], 0, 0.0, 200.0,
+                                [0.2, 0.2, 0.0], 1, 0, 40, N)
+    lines = []
+    for k in range(40):
+        pv = orc.value_by_idx(k, 40, 2.0, 7.0, 0)
+        for j in range(max(wc[k], 0)):
+            lines.append(f"{pv:.12g}, {wp[k, j]:.12g}\n")
+    assert out.read_text() == "".join(lines)
+
+
+# ---------------------------------------------------------------- DBSCAN
+
+def _fuzz_peak_sets(rng, nb, stride):
+    amps = np.zeros((nb, stride))
+    ivls = np.zeros((nb, stride))
+    cnt = np.zeros(nb, dtype=np.int32)
+    for b in range(nb):
+        P = int(rng.choice([0, 1, 2, 3, 5, 17, 63, 64, 65, 130, 200, 256, 257, 300]))
+        centers = rng.normal(size=rng.integers(1, 9)) * 0.05
+        a = rng.choice(centers, P) + rng.normal(size=P) * rng.choice([0, 0.002, 0.006])
+        iv = rng.integers(1, 4, size=P) * 0.01
+        if b % 5 == 0:      # pairs placed on the eps boundary
+            a[1::2] = a[::2][: len(a[1::2])] + 0.006
+            iv[1::2] = iv[::2][: len(iv[1::2])] + 0.008
+        amps[b, :P], ivls[b, :P], cnt[b] = a, iv, P
+    cnt[::13] = -1
+    return amps, ivls, cnt
+
+
+@pytest.mark.parametrize("max_peaks", [100000, 120])
+def test_dbscan_device_fuzz_bit_exact(orc, bh, dev, max_peaks):
+    import torch
+    rng = np.random.default_rng(7)
+    nb, stride = 600, 320
+    amps, ivls, cnt = _fuzz_peak_sets(rng, nb, stride)
+    say("dbscan fuzz", nb, max_peaks)
+    out = torch.full((nb,), -99, dtype=torch.int32, device=dev)
+    A, I, Cn = _t(dev, amps.ravel(), torch.float64), _t(dev, ivls.ravel(), torch.float64), _t(
+        dev, cnt, torch.int32)
+    bh.dbscan_device(A.data_ptr(), I.data_ptr(), stride, nb, Cn.data_ptr(), max_peaks, 0.01,
+                     out.data_ptr())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    for b in range(nb):
+        want = 0 if cnt[b] == -1 else orc.dbscan(amps[b, :cnt[b]], ivls[b, :cnt[b]], 0.01,
+                                                 max_peaks)
+        assert got[b] == want, (b, cnt[b])
+
+
+def test_dbscan_reference_kats_gpu(bh, dev):
+    import json, os, torch
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+    cases = gold["dbscan_eps001_intervals0"]
+    stride = 16
+    amps = np.zeros((len(cases), stride))
+    cnt = np.zeros(len(cases), dtype=np.int32)
+    for i, c in enumerate(cases):
+        amps[i, :len(c["amps"])] = c["amps"]
+        cnt[i] = len(c["amps"])
+    A = _t(dev, amps.ravel(), torch.float64)
+    I = torch.zeros_like(A)
+    Cn = _t(dev, cnt, torch.int32)
+    out = torch.zeros(len(cases), dtype=torch.int32, device=dev)
+    bh.dbscan_device(A.data_ptr(), I.data_ptr(), stride, len(cases), Cn.data_ptr(), 1 << 30,
+                     0.01, out.data_ptr())
+    torch.cuda.synchronize()
+    assert out.cpu().numpy().tolist() == [c["result"] for c in cases]
+
+
+@pytest.mark.parametrize("max_peaks", [50000, 32])
+def test_dbscan_map_config2_rows_bit_exact(orc, bh, max_peaks):
+    """Full 512x512 DBSCAN map on the GPU; 6 whole rows (3072 cells spread
+    over c) against the oracle."""
+    say("dbscan map 512^2", max_peaks)
+    m = bh.dbscan_map(500.0, 512, 0.01, IC, R2, [0, 2], 0, 0.0, max_peaks, 1000.0, VALS, 1, 0.01)
+    m = m.reshape(512, 512)
+    for row in (0, 101, 230, 311, 420, 511):
+        want = orc.dbscan_cells(500.0, 512, 0.01, IC, R2, [0, 2], 0, 0.0, max_peaks, 1000.0,
+                                VALS, 1, 0.01, row * 512, 512)
+        np.testing.assert_array_equal(m[row], want, err_msg=f"row {row}")
+    if max_peaks == 32:
+        assert (m == 0).mean() > 0.99
+    else:
+        assert len(np.unique(m)) > 10
+
+
+def test_dbscan_overflow_recompute(orc, bh):
+    """Rows of 8 slots force every cell through the exact recompute path."""
+    say("dbscan overflow recompute")
+    bh.set_peak_capacity(8)
+    m = bh.dbscan_map(100.0, 12, 0.01, IC, R2, [0, 2], 0, 0.0, 100000, 100.0, VALS, 1, 0.01)
+    want = orc.dbscan_cells(100.0, 12, 0.01, IC, R2, [0, 2], 0, 0.0, 100000, 100.0, VALS, 1,
+                            0.01, 0, 144)
+    np.testing.assert_array_equal(m, want)
+
+
+def test_dbscan_divergence_and_chunking(orc, bh):
+    say("dbscan divergence + chunking")
+    args = (100.0, 20, 0.01, IC, [0.05, 0.45, 2, 10], [0, 2], 0, 5.0, 100000, 50.0, VALS, 1, 0.01)
+    full = bh.dbscan_map(*args)
+    bh.set_max_cells_per_launch(37)
+    chunked = bh.dbscan_map(*args)
+    want = orc.dbscan_cells(*args, 0, 400)
+    np.testing.assert_array_equal(full, want)
+    np.testing.assert_array_equal(chunked, want)
+    assert (want == 0).any()
+
+
+# ------------------------------------------------------------------- KDE
+
+def test_kde_device_agreement(orc, bh, dev):
+    import torch
+    rng = np.random.default_rng(9)
+    nb, stride = 400, 128
+    amps = np.zeros((nb, stride))
+    cnt = rng.integers(0, 120, size=nb).astype(np.int32)
+    for b in range(nb):
+        centers = rng.uniform(0, 10, size=rng.integers(1, 6))
+        amps[b, :cnt[b]] = rng.choice(centers, cnt[b]) + rng.normal(size=cnt[b]) * 0.05
+    cnt[::17] = -1
+    say("kde fuzz", nb)
+    out = torch.zeros(nb, dtype=torch.int32, device=dev)
+    A, Cn = _t(dev, amps.ravel(), torch.float64), _t(dev, cnt, torch.int32)
+    bh.kde_device(A.data_ptr(), stride, nb, Cn.data_ptr(), 100, 10, 0.0, 10.0, 0.05,
+                  out.data_ptr())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = np.array([0 if cnt[b] == -1 else orc.kde(amps[b, :cnt[b]], 100, 10, 0.0, 10.0, 0.05)
+                     for b in range(nb)])
+    agree = (got == want).mean()
+    say("kde agreement", agree)
+    assert agree >= 0.995
+
+
+def test_kde_map_agreement(orc, bh):
+    say("kde map 24^2")
+    m = bh.kde_map(200.0, 24, 0.01, IC, R2, [0, 2], 0, 0.0, 1000, 200.0, VALS, 1, 10, -15.0,
+                   15.0, 0.05)
+    want = orc.kde_cells(200.0, 24, 0.01, IC, R2, [0, 2], 0, 0.0, 1000, 200.0, VALS, 1, 10,
+                         -15.0, 15.0, 0.05, 0, 576)
+    assert (m == want).mean() >= 0.995
+
+
+# --------------------------------------------------------- Gram-Schmidt
+
+def test_gram_schmidt_device_bit_exact(orc, bh, dev):
+    import torch
+    rng = np.random.default_rng(11)
+    for n in (2, 3, 5, 8):
+        cnt = 200
+        a = rng.normal(size=(cnt, n, n))
+        a[::9, 1] = 0.0
+        say("gram_schmidt", n)
+        A = _t(dev, a.ravel(), torch.float64)
+        Bm = torch.zeros_like(A)
+        Dn = torch.zeros(cnt * n, dtype=torch.float64, device=dev)
+        bh.gram_schmidt_device(A.data_ptr(), Bm.data_ptr(), Dn.data_ptr(), n, cnt)
+        torch.cuda.synchronize()
+        b, d = Bm.cpu().numpy().reshape(cnt, n, n), Dn.cpu().numpy().reshape(cnt, n)
+        for i in range(cnt):
+            wb, wd = orc.gram_schmidt(a[i])
+            np.testing.assert_array_equal(b[i], wb)
+            np.testing.assert_array_equal(d[i], wd)
+
+
+# ------------------------------------------------------------- Lyapunov
+
+@pytest.mark.parametrize("dir_chunk", [-1, 50])
+def test_lle2d_fp64_matches_oracle(orc, bh, dir_chunk):
+    say("LLE2D fp64", dir_chunk)
+    n = 16
+    bh.set_direction_chunk(dir_chunk)
+    got = bh.lle_map(2, 100.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 50.0, VALS)
+    want = orc.lle_cells(2, 100.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 50.0, VALS,
+                         orc.PERT_REFERENCE, 200000 if dir_chunk < 0 else dir_chunk, 0, n * n)
+    err = np.abs(got - want)
+    say("LLE max abs err", err.max(), "bit-identical", (got == want).mean())
+    assert err.max() <= 1e-12
+
+
+def test_lle_fp64_divergence_check_and_axis(orc, bh):
+    say("LLE1D fp64 maxValue + axis")
+    bh.set_perturbation(bh.PERT_AXIS_X)
+    got = bh.lle_map(1, 60.0, 0.5, 24, 0.01, 1e-8, IC, [2.0, 12.0], [2], 8.0, 20.0, VALS)
+    want = orc.lle_cells(1, 60.0, 0.5, 24, 0.01, 1e-8, IC, [2.0, 12.0], [2], 8.0, 20.0, VALS,
+                         orc.PERT_AXIS_X, 0, 0, 24)
+    assert np.abs(got - want).max() <= 1e-12
+
+
+def test_ls2d_fp64_matches_oracle(orc, bh):
+    say("LS2D fp64")
+    n = 8
+    got = bh.ls_map(2, 50.0, 0.5, n, 0.01, 1e-6, IC, R2, [0, 2], 0.0, 20.0, VALS)
+    want = orc.ls_cells(2, 50.0, 0.5, n, 0.01, 1e-6, IC, R2, [0, 2], 0.0, 20.0, VALS,
+                        orc.PERT_REFERENCE, 0, 0, n * n)
+    err = np.abs(got - want)
+    say("LS max abs err", err.max(), "bit-identical", (got == want).mean())
+    assert err.max() <= 1e-12
+
+
+def test_linear_model_closed_forms_gpu(bh):
+    import math
+    say("linear model closed forms")
+    bh.set_model(bh.MODEL_LINEAR)
+    bh.set_perturbation(bh.PERT_AXIS_X)
+    lam = [0.3, -0.5, -1.0]
+    h = 0.01
+    lle = bh.lle_map(1, 200.0, 0.5, 4, h, 1e-8, [0, 0, 0], [0.3, 0.3], [0], 0.0, 20.0, lam)
+    np.testing.assert_allclose(lle, math.log(1 + h * 0.3) / h, rtol=1e-9)
+    ls = bh.ls_map(1, 200.0, 0.5, 3, h, 1e-6, [0, 0, 0], [0.3, 0.3], [0], 0.0, 20.0, lam)
+    for row in ls:
+        np.testing.assert_allclose(row, [math.log(1 + h * v) / h for v in lam], rtol=1e-9)
+    bh.set_precision(bh.PREC_FP32)
+    lle32 = bh.lle_map(1, 200.0, 0.5, 4, h, 1e-8, [0, 0, 0], [0.3, 0.3], [0], 0.0, 20.0, lam)
+    np.testing.assert_allclose(lle32, math.log(1 + h * 0.3) / h, rtol=1e-5)
+
+
+def test_lle_fp32_bulk_vs_fp64(bh):
+    """FP32 bulk integration with FP64 renormalisation against the FP64
+    path on a 48x48 map: regime (chaotic LLE > 0.01) agreement and error."""
+    say("LLE fp32 vs fp64 48^2")
+    n = 48
+    bh.set_perturbation(bh.PERT_AXIS_X)
+    f64 = bh.lle_map(2, 300.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 100.0, VALS)
+    bh.set_precision(bh.PREC_FP32)
+    f32 = bh.lle_map(2, 300.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 100.0, VALS)
+    ok = np.isfinite(f64)
+    agree = ((f64[ok] > 0.01) == (f32[ok] > 0.01)).mean()
+    err = np.abs(f64[ok] - f32[ok])
+    say("fp32 regime agreement", agree, "median err", np.median(err), "max", err.max())
+    assert agree >= 0.95
+    assert np.median(err) < 5e-3
+
+
+# ------------------------------------------------------------- drivers
+
+def test_driver_files_match_reference_format(orc, bh, tmp_path):
+    """Every 2D/1D file driver writes the reference's CSV layout with the
+    oracle's numbers."""
+    from test_abi import _ref_map_text
+    say("driver files")
+    out = tmp_path / "mat.csv"
+    bh.set_output_path(out)
+    n = 10
+    bh.bifurcation2DDBSCAN(80.0, n, 0.01, 3, IC, R2, [0, 2], 0, 0.0, 100000, 50.0, VALS, 3, 1,
+                           0.01)
+    want = orc.dbscan_cells(80.0, n, 0.01, IC, R2, [0, 2], 0, 0.0, 100000, 50.0, VALS, 1, 0.01,
+                            0, n * n)
+    assert out.read_text() == _ref_map_text(n, R2, want, 6)
+
+    bh.LLE2D(40.0, 0.5, n, 0.01, 1e-8, IC, 3, R2, [0, 2], 0, 0.0, 20.0, VALS, 3)
+    lle = orc.lle_cells(2, 40.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 20.0, VALS,
+                        orc.PERT_REFERENCE, 200000, 0, n * n)
+    lines = out.read_text().split("\n")
+    assert lines[:2] == ["0.05 0.45", "2 10"]
+    body = "".join(lines[2:]).split(", ")
+    assert body == [f"{v:g}" for v in lle]
+
+    bh.LS2D(30.0, 0.5, n, 0.01, 1e-6, IC, 3, R2, [0, 2], 0, 0.0, 10.0, VALS, 3)
+    ls = orc.ls_cells(2, 30.0, 0.5, n, 0.01, 1e-6, IC, R2, [0, 2], 0.0, 10.0, VALS,
+                      orc.PERT_REFERENCE, 0, 0, n * n)
+    for k in range(3):
+        p = tmp_path / f"mat.csv{k + 1}.csv"
+        lines = p.read_text().split("\n")
+        assert lines[:2] == ["0.05 0.45", "2 10"]
+        body = [float(v) for v in "".join(lines[2:]).split(", ")]
+        np.testing.assert_allclose(body, [float(f"{v:g}") for v in ls[:, k]], rtol=1e-5,
+                                   atol=1e-12)
+
+    bh.LLE1D(40.0, 0.5, n, 0.01, 1e-8, IC, 3, [2.0, 7.0], [2], 0, 0.0, 20.0, VALS, 3)
+    lines = out.read_text().splitlines()
+    assert len(lines) == n and lines[0].startswith("2, ")
+
+    bh.LS1D(30.0, 0.5, n, 0.01, 1e-6, IC, 3, [2.0, 7.0], [2], 0, 0.0, 10.0, VALS, 3)
+    lines = out.read_text().splitlines()
+    assert len(lines) == n and len(lines[0].split(", ")) == 4
+
+    bh.bifurcation2DKDE(80.0, n, 0.01, 3, IC, R2, [0, 2], 0, 0.0, 1000, 50.0, VALS, 3, 1, 10,
+                        -15.0, 15.0, 0.05)
+    assert out.read_text().split("\n")[:2] == ["0.05 0.45", "2 10"]

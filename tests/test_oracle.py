@@ -1,0 +1,272 @@
+"""Pins the CPU oracle (CPU only, no GPU).
+
+Each oracle function is checked against something other than a retyping
+of its own formula: the reference's known-answer values (golden file),
+closed forms of the linear test model, invariants of Gram-Schmidt and of
+the Lyapunov spectrum, numpy's correctly rounded arithmetic, and brute
+force on small inputs.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+IC = [0.1, 0.1, 0.1]
+
+
+# ------------------------------------------------------------ golden KATs
+
+def test_peak_finder_reference_kat(orc):
+    for case in GOLD["peak_finder"]:
+        P, pk, iv = orc.peak_finder(case["series"], case["h"])
+        assert P == case["count"]
+        assert list(pk) == case["peaks"]
+        assert list(iv) == case["intervals"]
+
+
+def test_dbscan_reference_kats(orc):
+    for case in GOLD["dbscan_eps001_intervals0"]:
+        a = case["amps"]
+        assert orc.dbscan(a, [0.0] * len(a), 0.01) == case["result"], a
+
+
+def test_hypotf_reference_kat(orc):
+    (c,) = GOLD["hypotf"]
+    d = orc.distance(0.0, 0.0, c["x"], c["y"])
+    assert float(np.float32(d)).hex() == float.fromhex(c["float_result_hex"]).hex()
+    assert (d < 0.01) == c["neighbour_at_eps_0.01"]
+
+
+@pytest.mark.parametrize("case", GOLD["rossler_dbscan_cells"]["maxAmountOfPeaks_1e5"],
+                         ids=lambda c: f"c={c['c']}")
+def test_rossler_dbscan_reference_counts(orc, case):
+    c = case["c"]
+    # 2D grid whose cell 0 is (a, c) = (0.2, c)
+    rng2 = [0.2, 0.2, c, c]
+    r = orc.dbscan_cells(500, 2, 0.01, IC, rng2, [0, 2], 0, 0.0, 100000, 1000, [0.2, 0.2, c], 1,
+                         0.01, 0, 1)
+    assert r[0] == case["result"]
+    r32 = orc.dbscan_cells(500, 2, 0.01, IC, rng2, [0, 2], 0, 0.0, 32, 1000, [0.2, 0.2, c], 1,
+                           0.01, 0, 1)
+    assert r32[0] == 0
+    cnt, _, _ = orc.peaks_cells(1, 500, 2, 0.01, IC, [c, c], [2], 0, 0.0, 1000, [0.2, 0.2, c], 1,
+                                0, 1, 256)
+    lo, hi = GOLD["rossler_peak_counts"]["approximate_range"]
+    assert lo - 2 <= cnt[0] <= hi + 2
+
+
+# ------------------------------------------------------ float semantics
+
+def test_distance_is_double_evaluated_hypotf(orc):
+    """glibc hypotf == (float)sqrt((double)x*x + (double)y*y) -- the identity
+    the GPU neighbour test relies on.  numpy's sqrt is correctly rounded."""
+    rng = np.random.default_rng(0)
+    x1 = rng.normal(size=20000) * rng.choice([1e-3, 1e-2, 1, 1e3], 20000)
+    y1 = rng.normal(size=20000) * 1e-2
+    x2 = x1 + rng.normal(size=20000) * 1e-2
+    y2 = y1 + rng.choice([0.0, 0.01, 0.02], 20000)
+    fx = (x2 - x1).astype(np.float32).astype(np.float64)
+    fy = (y2 - y1).astype(np.float32).astype(np.float64)
+    want = np.sqrt(fx * fx + fy * fy).astype(np.float32).astype(np.float64)
+    got = np.array([orc.distance(*v) for v in zip(x1, y1, x2, y2)])
+    np.testing.assert_array_equal(got, want)
+
+
+def test_value_by_idx_closed_form(orc):
+    """getValueByIdx == start + i*((end-start)/(n-1)) with i = idx/n^k mod n
+    for grids up to 4096^2 (float division exact there)."""
+    for n, s, f in [(512, 0.05, 0.45), (512, 2.0, 10.0), (1000, 2.0, 7.0), (4096, 2.0, 10.0)]:
+        step = (f - s) / (n - 1)
+        for idx in list(range(0, n * n, max(1, n * n // 4099))) + [n * n - 1]:
+            for k in (0, 1):
+                i = (idx // n ** k) % n
+                want = s + float(i) * step
+                assert orc.value_by_idx(idx, n, s, f, k) == want, (n, idx, k)
+
+
+# ------------------------------------------------------------ loop / step
+
+def test_loop_records_before_step_and_stops_on_divergence(orc):
+    """Sample 0 is the state after the transient; the divergence check stops
+    the record loop and reports 0 (Library/cudaLibrary.cu:55-74)."""
+    ok, s = orc.bif_cell(0, 2, [5.7, 5.7], [2], IC, [0.2, 0.2, 5.7], 0.01, 0, 10)
+    assert ok == 1 and s[0] == 0.1
+    x = orc.step(IC, [0.2, 0.2, 5.7], 0.01)
+    assert s[1] == x[0]
+    ok, s = orc.bif_cell(0, 2, [5.7, 5.7], [2], IC, [0.2, 0.2, 5.7], 0.01, 100, 50,
+                         max_value=1e-3)
+    assert ok == 0
+    # NaN never trips the check (comparison is false)
+    ok, _ = orc.bif_cell(0, 2, [5.7, 5.7], [2], [float("nan")] * 3, [0.2, 0.2, 5.7], 0.01, 0,
+                         10, max_value=1.0)
+    assert ok == 1
+
+
+def test_rossler_step_is_sequential_euler(orc):
+    """One step against an exact rational evaluation of the sequential
+    scheme (x1 and x2 use the new x0), rounded once at the end: the oracle's
+    per-operation rounding keeps it within a few ulps, while the
+    simultaneous-Euler variant misses by far more."""
+    from fractions import Fraction as F
+    x = [0.7, -1.3, 0.4]
+    a, b, c, h = 0.2, 0.2, 5.7, 0.01
+    X = [F(v) for v in x]
+    A, B, Cc, H = F(a), F(b), F(c), F(h)
+    n0 = X[0] + H * (-X[1] - X[2])
+    n1 = X[1] + H * (n0 + A * X[1])
+    n2 = X[2] + H * (B + X[2] * (n0 - Cc))
+    got = orc.step(x, [a, b, c], h)
+    for g, w in zip(got, (n0, n1, n2)):
+        assert abs(F(g) - w) <= F(4) * F(math.ulp(float(w)))
+    s1 = X[1] + H * (X[0] + A * X[1])  # simultaneous variant
+    assert abs(F(got[1]) - s1) > F(1000) * F(math.ulp(float(s1)))
+
+
+# ------------------------------------------------------------ peak finder
+
+def test_peak_finder_plateau_free_is_textbook(orc):
+    """Without equal neighbours the state machine reduces to strict local
+    maxima with the first one dropped; intervals are index gaps * h."""
+    rng = np.random.default_rng(1)
+    for n in (3, 5, 17, 200, 2001):
+        d = rng.normal(size=n).cumsum()
+        i = np.nonzero((d[1:-1] > d[:-2]) & (d[1:-1] > d[2:]))[0] + 1
+        P, pk, iv = orc.peak_finder(d, 0.01)
+        if len(i) <= 1:
+            assert P == 0
+            continue
+        assert P == len(i) - 1
+        np.testing.assert_array_equal(pk, d[i[1:]])
+        np.testing.assert_array_equal(iv, np.diff(i).astype(np.float64) * 0.01)
+
+
+def test_peak_finder_edge_cases(orc):
+    assert orc.peak_finder([])[0] == 0
+    assert orc.peak_finder([1.0])[0] == 0
+    assert orc.peak_finder([0, 1, 0, 1, 0])[0] == 1
+    # a rising plateau exit skips the next candidate (the reference quirk)
+    P, pk, _ = orc.peak_finder([0, 9, 0, 5, 5, 6, 1, 0])
+    assert P == 0
+
+
+# ------------------------------------------------------------------- KDE
+
+def test_kde_small_cases(orc):
+    assert orc.kde([], 10, 10, 0, 1, 0.1) == 0
+    assert orc.kde([0.5], 10, 10, 0, 1, 0.1) == 1
+    assert orc.kde([0.5, 0.6], 10, 10, 0, 1, 0.1) == 1
+    assert orc.kde([0.1] * 20, 5, 10, 0, 1, 0.1) == 5          # P > max returns max
+    assert orc.kde([0.5, 0.5, 0.5], 10, 10, 0, 1, 0.05) == 1    # one Gaussian bump
+    assert orc.kde([0.2, 0.2, 0.8, 0.8], 10, 10, 0, 1, 0.05) == 2
+    assert orc.kde([0.1, 0.1, 0.5, 0.5, 0.9, 0.9], 10, 10, 0, 1, 0.03) == 3
+
+
+# ---------------------------------------------------------- Gram-Schmidt
+
+def test_gram_schmidt_invariants(orc):
+    rng = np.random.default_rng(2)
+    for n in (2, 3, 5, 8):
+        for _ in range(20):
+            a = rng.normal(size=(n, n))
+            b, den = orc.gram_schmidt(a)
+            np.testing.assert_allclose(b @ b.T, np.eye(n), atol=1e-10)
+            assert abs(abs(np.linalg.det(a)) - np.prod(den)) <= 1e-9 * np.prod(den)
+            q, r = np.linalg.qr(a.T)
+            np.testing.assert_allclose(np.abs(b), np.abs(q.T), atol=1e-9)
+            np.testing.assert_allclose(den, np.abs(np.diag(r)), rtol=1e-10)
+    # zero vector -> zero row and zero norm
+    b, den = orc.gram_schmidt(np.array([[1.0, 0, 0], [0, 0, 0], [0, 1.0, 0]]))
+    assert den[1] == 0 and not b[1].any()
+
+
+# ---------------------------------------------------------- Lyapunov
+
+LAM = [0.3, -0.5, -1.0]
+
+
+def _ref_lle_dir(idx):
+    """0.5*sinf of the wrapped int idx*(i*idx+1)+1, normalised (float32 sin;
+    a last-ulp difference from glibc's sinf moves the LLE by < 1e-9)."""
+    z = []
+    for i in range(3):
+        arg = (idx * (i * idx + 1) + 1) & 0xFFFFFFFF
+        arg = arg - (1 << 32) if arg >= (1 << 31) else arg
+        z.append(0.5 * float(np.sin(np.float32(arg))))
+    z = np.array(z)
+    return z / np.linalg.norm(z)
+
+
+def test_lle_linear_model_closed_form(orc):
+    """x_i += h*a_i*x_i: with the perturbation along x the LLE is exactly
+    log(1+h*a_0)/h; with the reference directions z it is the finite-time
+    value log|M^N z|/T, M = diag(1+h*a).  The zero state keeps x exact so the
+    perturbation evolves linearly."""
+    h, T, NT = 0.01, 200, 0.5
+    out = orc.lle_cells(1, T, NT, 4, h, 1e-8, [0, 0, 0], [0.3, 0.3], [0], 0.0, 20, LAM,
+                        orc.PERT_AXIS_X, 0, 0, 4, model=orc.MODEL_LINEAR)
+    np.testing.assert_allclose(out, math.log(1 + h * 0.3) / h, rtol=1e-9)
+    out = orc.lle_cells(1, T, NT, 4, h, 1e-8, [0, 0, 0], [0.3, 0.3], [0], 0.0, 20, LAM,
+                        orc.PERT_REFERENCE, 0, 0, 4, model=orc.MODEL_LINEAR)
+    N = int(T / NT) * int(NT / h)
+    lg = np.log1p(h * np.array(LAM)) * N
+    for idx in range(4):
+        z = _ref_lle_dir(idx)
+        m = lg.max()
+        want = (m + 0.5 * np.log(np.sum((z * np.exp(lg - m)) ** 2))) / T
+        assert abs(out[idx] - want) < 1e-8, (idx, out[idx], want)
+
+
+def test_ls_linear_model_closed_form(orc):
+    """Axis directions stay aligned under a diagonal map: the spectrum is
+    exactly log(1+h*a_i)/h; reference directions converge to the same set."""
+    h = 0.01
+    want = [math.log(1 + h * v) / h for v in LAM]
+    out = orc.ls_cells(1, 200, 0.5, 3, h, 1e-6, [0, 0, 0], [0.3, 0.3], [0], 0.0, 20, LAM,
+                       orc.PERT_AXIS_X, 0, 0, 3, model=orc.MODEL_LINEAR)
+    for row in out:
+        np.testing.assert_allclose(row, want, rtol=1e-9)
+    out = orc.ls_cells(1, 200, 0.5, 3, h, 1e-6, [0, 0, 0], [0.3, 0.3], [0], 0.0, 20, LAM,
+                       orc.PERT_REFERENCE, 0, 0, 3, model=orc.MODEL_LINEAR)
+    for row in out:
+        np.testing.assert_allclose(row, want, atol=2e-2)
+
+
+def test_ls_sum_equals_volume_contraction(orc):
+    """Sum of the spectrum == time average of log|det J|/h along the orbit.
+    For the sequential Euler Rossler step det J = (1+h a)(1+h(x0'-c))."""
+    a, b, c, h = 0.2, 0.2, 5.7, 0.01
+    tmax, trans = 400, 100
+    ls = orc.ls_cells(1, tmax, 0.5, 2, h, 1e-8, IC, [c, c], [2], 0.0, trans, [a, b, c],
+                      orc.PERT_REFERENCE, 0, 0, 1)[0]
+    ok, s = orc.bif_cell(0, 2, [c, c], [2], IC, [a, b, c], h, int(trans / h), int(tmax / h) + 1)
+    x0_new = s[1:]  # x0 after each step of the measured window
+    vol = np.mean(np.log(np.abs((1 + h * a) * (1 + h * (x0_new - c))))) / h
+    assert abs(ls.sum() - vol) < 2e-3 * abs(vol)
+    assert ls[0] > 0.03 and abs(ls[1]) < 0.02 and ls[2] < -1.0
+
+
+def test_lle_rossler_sanity(orc):
+    """Chaotic c=5.7: LLE ~ 0.07; period-1 c=2.5: LLE ~ 0 (neutral flow
+    direction), within finite-time noise."""
+    vals = [0.2, 0.2, 5.7]
+    lle = orc.lle_cells(1, 500, 0.5, 2, 0.01, 1e-8, IC, [5.7, 5.7], [2], 0.0, 200, vals,
+                        orc.PERT_REFERENCE, 0, 0, 1)[0]
+    assert 0.04 < lle < 0.11
+    vals = [0.2, 0.2, 2.5]
+    lle = orc.lle_cells(1, 500, 0.5, 2, 0.01, 1e-8, IC, [2.5, 2.5], [2], 0.0, 200, vals,
+                        orc.PERT_REFERENCE, 0, 0, 1)[0]
+    assert abs(lle) < 0.01
+
+
+def test_lle_direction_chunk_changes_direction_only(orc):
+    """The reference seeds the direction from the chunk-local index; a
+    different chunk must give a slightly different (finite-time) value."""
+    args = (2, 50, 0.5, 8, 0.01, 1e-8, IC, [0.05, 0.45, 2, 10], [0, 2], 0.0, 10,
+            [0.2, 0.2, 5.7], orc.PERT_REFERENCE)
+    full = orc.lle_cells(*args, 0, 0, 64)
+    chunked = orc.lle_cells(*args, 10, 0, 64)
+    np.testing.assert_array_equal(full[:10], chunked[:10])
+    assert (full[10:] != chunked[10:]).any()

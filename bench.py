@@ -46,9 +46,9 @@ def peak_flop_per_cell():
 
 def run_plain(bh, n, d_peak, d_lle):
     N = int(PEAK["tMax"] / H)
-    bh.dbscan_map_device(d_peak.data_ptr(), PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, N,
+    bh.dbscan_map_device(d_peak.ptr, PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, N,
                          PEAK["transient"], VALS, 1, PEAK["eps"])
-    bh.lle_map_device(d_lle.data_ptr(), 2, LLE["tMax"], LLE["NT"], n, H, LLE["eps"], IC, RANGES,
+    bh.lle_map_device(d_lle.ptr, 2, LLE["tMax"], LLE["NT"], n, H, LLE["eps"], IC, RANGES,
                       [0, 2], 0.0, LLE["transient"], VALS)
 
 
@@ -88,27 +88,29 @@ def main():
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    import numpy as np
     import bifhip as bh
 
     bh.load()
-    dev = torch.device("cuda:0")
+    if bh.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
     bh.reset_context()
     bh.set_perturbation(bh.PERT_AXIS_X)
     n = args.n
-    d_peak = torch.zeros(n * n, dtype=torch.int32, device=dev)
-    d_lle = torch.zeros(n * n, dtype=torch.float64, device=dev)
+    d_peak = bh.DeviceArray((n * n,), np.int32)
+    d_lle = bh.DeviceArray((n * n,), np.float64)
 
     for _ in range(args.warmup):
         run_plain(bh, n, d_peak, d_lle)
-    torch.cuda.synchronize()
+    bh.synchronize()
     bh.set_timing(True)
     bh.reset_kernel_times()
-    torch.cuda.synchronize()
+    bh.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run_plain(bh, n, d_peak, d_lle)
-    torch.cuda.synchronize()
+    bh.synchronize()
     dt = time.perf_counter() - t0
     bh.set_timing(False)
 

@@ -3,8 +3,9 @@
 Mirrors the reference's host API (Library/hostLibrary.cuh) one to one:
 the reference-named functions below take the same arguments in the same
 order and raise BifhipError where the reference would print and exit().
-Device buffers are passed as integer addresses (e.g. torch tensor
-data_ptr()).  There is no CPU fallback: a missing library raises.
+Device buffers are passed as integer addresses (DeviceArray.ptr, memory
+from the library's own HIP runtime).  There is no CPU fallback: a missing
+library raises.
 """
 import ctypes as C
 import os
@@ -86,6 +87,24 @@ def _declare(L):
         "bifhip_write_map_double": [C.c_char_p, _I, _PD, _I, _I, _PD, _LL, _I, _I],
         "bifhip_write_bif1d": [C.c_char_p, _I, _PD, _PI, _PD, _I],
         "bifhip_write_1d_values": [C.c_char_p, _I, _PD, _PD, _I],
+        "bifhip_device_malloc": [C.POINTER(_VP), C.c_size_t],
+        "bifhip_device_free": [_VP],
+        "bifhip_memcpy_h2d": [_VP, _VP, C.c_size_t],
+        "bifhip_memcpy_d2h": [_VP, _VP, C.c_size_t],
+        "bifhip_memcpy_d2d": [_VP, _VP, C.c_size_t],
+        "bifhip_device_memset": [_VP, _I, C.c_size_t],
+        "bifhip_synchronize": [],
+        "bifhip_tiles_generate": [_I, _I, _I, _I, C.c_ulonglong, C.c_ulonglong, _D, _D, _D,
+                                  _PI, _PI, _PI],
+        "bifhip_ls_tiles_device": [_I, _I, _I, _PI, _PI, _PI, _I, _PD, _PI, _PD, _I, _PD, _I,
+                                   _D, _D, _D, _VP],
+        "bifhip_ls_tiles_autotune": [_I, _I, _I, _PI, _PI, _PI, _I, _PD, _PI, _PD, _I, _PD,
+                                     _I, _D, _D, _D, _VP],
+        "bifhip_set_tile_config": [_I, _I, _I],
+        "bifhip_get_tile_config": [_PI, _PI, _PI],
+        "bifhip_set_graph": [_I],
+        "bifhip_lle_peak_map_device": [_I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I, _I, _D,
+                                       _D, _D, _D, _D, _VP, _VP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -117,6 +136,56 @@ def _pd(a):
 
 def _pi(a):
     return a.ctypes.data_as(_PI)
+
+
+# ----------------------------------------------------------- device memory
+
+class DeviceArray:
+    """A device buffer on the library's own HIP runtime (numpy dtype/shape)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(np.atleast_1d(np.asarray(shape, dtype=np.int64)).tolist()) \
+            if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(load().bifhip_device_malloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, a.dtype)
+        _check(load().bifhip_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    @classmethod
+    def full(cls, shape, value, dtype):
+        return cls.from_host(np.full(shape, value, dtype=dtype))
+
+    def to_host(self):
+        a = np.empty(self.shape, dtype=self.dtype)
+        _check(load().bifhip_memcpy_d2h(a.ctypes.data, self.ptr, self.nbytes))
+        return a
+
+    def zero_(self):
+        _check(load().bifhip_device_memset(self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            load().bifhip_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(load().bifhip_synchronize())
 
 
 # ------------------------------------------------------------ context
@@ -355,6 +424,52 @@ def series_device(d_data, d_flags, dimension, tMax, nPts, h, initialConditions, 
     _check(load().bifhip_series_device(dimension, tMax, nPts, h, len(ic), _pd(ic), _pd(r), _pi(m),
                                        writableVar, maxValue, transientTime, _pd(v), len(v),
                                        preScaller, d_data, d_flags))
+
+
+def lle_peak_map_device(d_peak, d_lle, nPts, h, initialConditions, ranges, indicesOfMutVars,
+                        values, peak_tmax, peak_transient, writableVar, maxAmountOfPeaks,
+                        dbscan_eps, lle_tmax, NT, lle_transient, lle_eps):
+    ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(load().bifhip_lle_peak_map_device(nPts, h, len(ic), _pd(ic), _pd(r), _pi(m), _pd(v),
+                                             len(v), peak_tmax, peak_transient, writableVar,
+                                             maxAmountOfPeaks, dbscan_eps, lle_tmax, NT,
+                                             lle_transient, lle_eps, d_peak, d_lle))
+
+
+# ------------------------------------------------------- uneven LS tiles
+
+def tiles_generate(n_tiles, tile_w, tile_h, n_pts, seed_origins, seed_lengths, t_min, t_max,
+                   NT):
+    oi = np.zeros(n_tiles, dtype=np.int32)
+    oj = np.zeros(n_tiles, dtype=np.int32)
+    nb = np.zeros(n_tiles, dtype=np.int32)
+    _check(load().bifhip_tiles_generate(n_tiles, tile_w, tile_h, n_pts, seed_origins,
+                                        seed_lengths, t_min, t_max, NT, _pi(oi), _pi(oj),
+                                        _pi(nb)))
+    return oi, oj, nb
+
+
+def ls_tiles_device(d_out, tiles, tile_w, tile_h, n_pts, ranges, indicesOfMutVars,
+                    initialConditions, values, h, NT, transient, autotune=False):
+    oi, oj, nb = (_i32(t) for t in tiles)
+    r, m, ic, v = _f64(ranges), _i32(indicesOfMutVars), _f64(initialConditions), _f64(values)
+    fn = load().bifhip_ls_tiles_autotune if autotune else load().bifhip_ls_tiles_device
+    _check(fn(len(oi), tile_w, tile_h, _pi(oi), _pi(oj), _pi(nb), n_pts, _pd(r), _pi(m), _pd(ic),
+              len(ic), _pd(v), len(v), h, NT, transient, d_out))
+
+
+def set_tile_config(block, cells_per_lane=1, blocks_per_cu=0):
+    _check(load().bifhip_set_tile_config(block, cells_per_lane, blocks_per_cu))
+
+
+def get_tile_config():
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    _check(load().bifhip_get_tile_config(C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
+
+
+def set_graph(on):
+    _check(load().bifhip_set_graph(1 if on else 0))
 
 
 # --------------------------------------------------------------- writers

@@ -52,6 +52,14 @@ struct BifArgs {
     long long rowStride; // doubles between cell rows
 };
 
+// Arguments of the fused LLE + peak-count kernel (k_bif.hip).
+struct FusedArgs {
+    BifArgs pk;           // peak side: grid, h, skip (window start), N, cap
+    double eps, tMax;
+    int skipL, nNT, nBlocks;
+    const double* dirs;   // LLE directions per launch cell (null = along x)
+};
+
 // Arguments of the Lyapunov kernels (k_lyap.hip).
 struct LyapArgs {
     Grid g;
@@ -66,6 +74,24 @@ struct LyapArgs {
     int nNT;             // steps between renormalisations, (size_t)(NT/h)
     int nBlocks;         // renormalisations, (int)(tMax/NT)
     const double* dirs;  // per launch cell: 3 (LLE) or 9 (LS) doubles; null = axis
+};
+
+// Arguments of the persistent tile kernel (k_tiles.hip).
+struct TileArgs {
+    Grid g;                 // lattice the tile cells live on (nPts x nPts)
+    int nTiles;
+    int tileW, tileH;
+    int itemsPerTile;       // work items per tile
+    int itemCells;          // cells per work item = blockDim * cells per lane
+    const int* order;       // queue position -> tile id (longest first)
+    const int* originI;     // tile origin on the lattice (first swept axis)
+    const int* originJ;     // (second swept axis)
+    const int* nBlocks;     // renormalisations per tile
+    double h;
+    int nNT;
+    int skip;
+    double* out;            // [tile][cell][3]
+    int* counter;           // work-queue head, zeroed by the launch
 };
 
 // Flattened cell index -> parameter value (float division is IEEE-correct

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "bifhip_dev.h"
+#include "host_internal.h"
 
 // kernel launchers (k_bif.hip, k_lyap.hip) -------------------------------
 namespace bifhip {
@@ -41,6 +42,8 @@ hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, lo
                            int cap, int maxPeaks, int kdeSampling, double I1, double I2, double H,
                            int* out, const int* outIndex, int* overflow, int* nOverflow,
                            int listBase, hipStream_t st);
+hipError_t launch_lle_peaks(int model, int wv, const FusedArgs& F, double* lle, int* counts,
+                            double* amps, double* ivls, hipStream_t st);
 hipError_t launch_clamp(int n, int* v, int clamp, hipStream_t st);
 hipError_t launch_fill(long long n, int* v, int value, hipStream_t st);
 hipError_t launch_lle(int model, int precision, const LyapArgs& A, bool check, double* out,
@@ -49,26 +52,11 @@ hipError_t launch_ls(int model, const LyapArgs& A, bool check, double* out, hipS
 hipError_t launch_gs(const double* a, double* b, double* den, int n, int count, hipStream_t st);
 }  // namespace bifhip
 
-using namespace bifhip;
-
 // ------------------------------------------------------------- context
 
-namespace {
+namespace bifhip {
+namespace host {
 
-struct Ctx {
-    std::string outPath = "mat.csv";
-    int model = BIFHIP_MODEL_ROSSLER;
-    int precision = BIFHIP_PREC_FP64;
-    int pert = BIFHIP_PERT_REFERENCE;
-    long long dirChunk = -1;
-    long long maxCellsPerLaunch = 0;
-    int peakCap = 256;
-    int clusterClamp = 0;
-    int verbose = 0;
-    int timing = 0;
-    std::string lastError;
-    hipStream_t stream = nullptr;
-};
 Ctx g;
 
 // per-kernel-family timing: pending event pairs resolved after each call
@@ -94,40 +82,10 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail((int)e_, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                        __LINE__);                                                        \
-    } while (0)
-
-#define RC_TRY(expr)                  \
-    do {                              \
-        int rc_ = (expr);             \
-        if (rc_ != BIFHIP_OK) return rc_; \
-    } while (0)
-
-// scoped timing of one launch family
-struct Timed {
-    const char* name;
-    hipEvent_t a = nullptr, b = nullptr;
-    explicit Timed(const char* n) : name(n)
-    {
-        if (!g.timing) return;
-        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
-            a = b = nullptr;
-            return;
-        }
-        hipEventRecord(a, g.stream);
-    }
-    ~Timed()
-    {
-        if (!a) return;
-        hipEventRecord(b, g.stream);
-        g_pending.push_back({name, a, b});
-    }
-};
+void add_pending_timer(const char* name, hipEvent_t a, hipEvent_t b)
+{
+    g_pending.push_back({name, a, b});
+}
 
 void resolve_timers()
 {
@@ -143,31 +101,6 @@ void resolve_timers()
     }
     g_pending.clear();
 }
-
-// device buffer that frees itself
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t n)
-    {
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
-        bytes = n;
-        if (n == 0) return hipSuccess;
-        return hipMalloc(&p, n);
-    }
-    template <class T>
-    T* as() const { return static_cast<T*>(p); }
-};
 
 // cached workspaces reused across calls (released by bifhip_release)
 struct Pool {
@@ -201,13 +134,9 @@ struct Pool {
 };
 Pool g_pool;
 
-template <class T>
-int pool_get(const char* key, size_t count, T** out)
+hipError_t pool_get_raw(const char* key, size_t bytes, void** out)
 {
-    void* p = nullptr;
-    HIP_TRY(g_pool.get(key, count * sizeof(T), &p));
-    *out = static_cast<T*>(p);
-    return BIFHIP_OK;
+    return g_pool.get(key, bytes, out);
 }
 
 int ensure_device()
@@ -564,6 +493,110 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
     return BIFHIP_OK;
 }
 
+// ------------------------------------------- fused LLE + peak-count map
+
+// LLE (LLE2D semantics) and DBSCAN peak count (bifurcation2DDBSCAN
+// semantics) of n cells in one trajectory pass.  Cells are the flattened
+// indices h_list[0..n) (or base..base+n when h_list is null); results go
+// to d_lle[i], d_peak[i] for launch position i.
+int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, double epsDb,
+                       const int* h_list, const int* d_list, int base, int n, double* d_lle,
+                       int* d_peak)
+{
+    if (S.pre != 1) return fail(BIFHIP_E_ARG, "the fused map needs preScaller 1");
+    if (S.maxValue != 0 || L.maxValue != 0)
+        return fail(BIFHIP_E_ARG, "the fused map runs without divergence checks (maxValue 0)");
+    const long long span = (long long)L.skip + (long long)L.nBlocks * L.nNT;
+    if ((long long)S.skip + S.N > span)
+        return fail(BIFHIP_E_ARG, "peak window [%d, %lld) is not inside the LLE trajectory (%lld steps)",
+                    S.skip, (long long)S.skip + S.N, span);
+    const int capMax = std::max(2, std::min(g.peakCap, S.N / 2 + 1));
+    int cap = capMax;
+    if ((long long)maxPeaks + 1 < cap) cap = (int)std::max(2LL, (long long)maxPeaks + 1);
+    const long long rowBytes = (long long)cap * 16 + 8;
+    long long pass = std::max<long long>(1, (8LL << 30) / rowBytes);
+    pass = std::min<long long>(pass, std::max(1, n));
+    if (g.maxCellsPerLaunch > 0) pass = std::min(pass, g.maxCellsPerLaunch);
+
+    int *d_counts, *d_ovf, *d_novf;
+    double *d_amps, *d_ivls, *d_dirs = nullptr;
+    RC_TRY(pool_get("f_counts", (size_t)pass, &d_counts));
+    RC_TRY(pool_get("f_amps", (size_t)(pass * cap), &d_amps));
+    RC_TRY(pool_get("f_ivls", (size_t)(pass * cap), &d_ivls));
+    RC_TRY(pool_get("f_ovf", (size_t)std::max(1, n), &d_ovf));
+    RC_TRY(pool_get("f_novf", 1, &d_novf));
+    HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
+    const double T = dbscan_threshold(epsDb);
+    const bool refDirs = g.pert == BIFHIP_PERT_REFERENCE;
+    const long long dirChunk = g.dirChunk >= 0 ? g.dirChunk : 200000;
+    std::vector<double> hdirs;
+    if (refDirs) RC_TRY(pool_get("f_dirs", (size_t)pass * 3, &d_dirs));
+
+    for (long long off = 0; off < n; off += pass) {
+        const int m = (int)std::min<long long>(pass, n - off);
+        FusedArgs F;
+        std::memset(&F, 0, sizeof(F));
+        F.pk = bif_args(S, base + (int)off, d_list ? d_list + off : nullptr, m, cap);
+        F.eps = L.eps;
+        F.tMax = L.tMax;
+        F.skipL = L.skip;
+        F.nNT = L.nNT;
+        F.nBlocks = L.nBlocks;
+        if (refDirs) {
+            // directions of the listed cells (their flattened indices)
+            hdirs.resize((size_t)m * 3);
+            std::vector<double> one;
+            for (int i = 0; i < m; ++i) {
+                const long long gi = h_list ? h_list[off + i] : base + off + i;
+                directions(false, gi, 1, dirChunk, one);
+                std::memcpy(hdirs.data() + (size_t)i * 3, one.data(), sizeof(double) * 3);
+            }
+            HIP_TRY(hipMemcpyAsync(d_dirs, hdirs.data(), sizeof(double) * hdirs.size(),
+                                   hipMemcpyHostToDevice, g.stream));
+            F.dirs = d_dirs;
+        }
+        {
+            Timed t("lle_peaks");
+            HIP_TRY(launch_lle_peaks(g.model, S.wv, F, d_lle + off, d_counts, d_amps, d_ivls,
+                                     g.stream));
+        }
+        {
+            Timed t("dbscan");
+            HIP_TRY(launch_dbscan_wave(m, d_counts, d_amps, d_ivls, cap, cap, maxPeaks, T,
+                                       d_peak + off, d_ovf, d_novf, (int)off, g.stream));
+        }
+        if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));
+    }
+    int novf = 0;
+    HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (novf > 0) {
+        // rare: more peaks than the rows hold -- recompute those cells' peaks
+        std::vector<int> pos(novf), flat(novf);
+        HIP_TRY(hipMemcpy(pos.data(), d_ovf, sizeof(int) * novf, hipMemcpyDeviceToHost));
+        for (int i = 0; i < novf; ++i) flat[i] = h_list ? h_list[pos[i]] : base + pos[i];
+        const int cap2 = (int)std::max(2LL, std::min((long long)S.N / 2 + 1, (long long)maxPeaks + 1));
+        DBuf dflat, dpos, cnt, am, iv, scr;
+        HIP_TRY(dflat.alloc(sizeof(int) * novf));
+        HIP_TRY(dpos.alloc(sizeof(int) * novf));
+        HIP_TRY(hipMemcpy(dflat.p, flat.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dpos.p, pos.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
+        HIP_TRY(cnt.alloc(sizeof(int) * novf));
+        HIP_TRY(am.alloc(sizeof(double) * (size_t)novf * cap2));
+        HIP_TRY(iv.alloc(sizeof(double) * (size_t)novf * cap2));
+        HIP_TRY(scr.alloc(sizeof(int) * (size_t)novf * 2 * cap2));
+        RC_TRY(integrate_peaks(S, 0, dflat.as<int>(), novf, cap2, cnt.as<int>(), am.as<double>(),
+                               iv.as<double>()));
+        Timed t("dbscan");
+        HIP_TRY(launch_dbscan_lane(novf, nullptr, cnt.as<int>(), am.as<double>(), iv.as<double>(),
+                                   cap2, T, scr.as<int>(), 2LL * cap2, d_peak, dpos.as<int>(),
+                                   g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+    }
+    if (g.clusterClamp > 0) HIP_TRY(launch_clamp(n, d_peak, g.clusterClamp, g.stream));
+    return BIFHIP_OK;
+}
+
 // ------------------------------------------------------------ CSV output
 
 struct Out {
@@ -814,7 +847,11 @@ int lyap_to_host(bool ls, int dimension, double tMax, double NT, int nPts, doubl
     return BIFHIP_OK;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace bifhip
+
+using namespace bifhip;
+using namespace bifhip::host;
 
 // ================================================================ C ABI
 
@@ -920,6 +957,52 @@ int bifhip_device_count(void)
 }
 
 const char* bifhip_version(void) { return "bifhip 0.1 gfx950"; }
+
+int bifhip_device_malloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(BIFHIP_E_ARG, "null pointer");
+    RC_TRY(ensure_device());
+    *ptr = nullptr;
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 16));
+    return BIFHIP_OK;
+}
+
+int bifhip_device_free(void* ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return BIFHIP_OK;
+}
+
+int bifhip_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return BIFHIP_OK;
+}
+
+int bifhip_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return BIFHIP_OK;
+}
+
+int bifhip_memcpy_d2d(void* dst, const void* src, size_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return BIFHIP_OK;
+}
+
+int bifhip_device_memset(void* ptr, int value, size_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemset(ptr, value, bytes));
+    return BIFHIP_OK;
+}
+
+int bifhip_synchronize(void)
+{
+    HIP_TRY(hipDeviceSynchronize());
+    resolve_timers();
+    return BIFHIP_OK;
+}
 
 double bifhip_value_by_idx(int idx, int nPts, double s, double f, int k)
 {
@@ -1113,6 +1196,26 @@ int bifhip_LLE_device(int dimension, double tMax, double NT, int nPts, double h,
     if (!d_out) return fail(BIFHIP_E_ARG, "null d_out");
     return lyap_impl(false, dimension, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
                      transientTime, values, nVal, d_out);
+}
+
+int bifhip_lle_peak_map_device(int nPts, double h, int nIC, const double* ic,
+                               const double* ranges, const int* mut, const double* values,
+                               int nVal, double peakTMax, double peakTransient, int writableVar,
+                               int maxAmountOfPeaks, double dbscanEps, double lleTMax, double NT,
+                               double lleTransient, double lleEps, int* d_peak, double* d_lle)
+{
+    RC_TRY(ensure_device());
+    if (!d_peak || !d_lle) return fail(BIFHIP_E_ARG, "null device output");
+    PeakMapSpec S;
+    RC_TRY(peak_spec(S, 2, peakTMax, nPts, h, nIC, ic, ranges, mut, writableVar, 0.0,
+                     peakTransient, values, nVal, 1));
+    LyapSpec L;
+    RC_TRY(lyap_spec(L, 2, lleTMax, NT, nPts, h, lleEps, ic, nIC, ranges, mut, 0.0, lleTransient,
+                     values, nVal));
+    progress("LLE2D+BIFURCATION2D", S.cells);
+    RC_TRY(fused_cells_device(S, L, maxAmountOfPeaks, dbscanEps, nullptr, nullptr, 0,
+                              (int)S.cells, d_lle, d_peak));
+    return finish();
 }
 
 // ------------------------------------------------ kernel-level entry points

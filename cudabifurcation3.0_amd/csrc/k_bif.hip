@@ -473,6 +473,111 @@ __global__ __launch_bounds__(256) void k_kde_wave(int nCells, const int* __restr
     if (lane == 0) out[outIndex ? outIndex[cell] : cell] = res;
 }
 
+// ------------------------------------------- fused LLE + peak-count map
+//
+// The LLE's reference trajectory x starts from the same state, with the
+// same parameters and step, as the bifurcation trajectory, so it passes
+// through every state the peak window records (e.g. transient 500 + 1000
+// time units of x cover the peak map's samples at 1000..1500).  One lane
+// therefore produces both results with the operation sequence of each
+// reference kernel: LLEKernelCUDA (Library/cudaLibrary.cu:666-785) for the
+// exponent, calculateDiscreteModelCUDA + peakFinder (:78-117, :220-266)
+// for the peaks -- bit for bit, with 250 000 instead of 400 000 steps per
+// cell at the benchmark's settings.  No divergence check (maxValue == 0);
+// preScaller 1; the window must lie inside x's span (checked on the host).
+template <class M, int WV>
+__global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __restrict__ lle,
+                                                   int* __restrict__ counts,
+                                                   double* __restrict__ amps,
+                                                   double* __restrict__ ivls)
+{
+    const BifArgs& A = F.pk;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nCells) return;
+    const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
+    double v[kMaxVal];
+    cell_values(A.g, idx, v);
+    const typename M::P64 p = M::params(v);
+    const double h = A.h, eps = F.eps;
+    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
+
+    PeakSink sink;
+    sink.amps = amps + (long long)t * A.rowStride;
+    sink.ivls = ivls + (long long)t * A.rowStride;
+    sink.cap = A.cap;
+    sink.hpf = A.hpf;
+    PeakStream ps;
+    ps.init();
+    const int w0 = A.skip, w1 = A.skip + A.N;   // window of recorded states [w0, w1)
+
+    // sample the state reached after k steps of x, if k is in the window
+    auto record = [&](int k) {
+        if (k >= w0 && k < w1) {
+            const double d = WV == 0 ? x0 : (WV == 1 ? x1 : x2);
+            const int i = k - w0;
+            if (i >= 2) ps.push(d, i, sink);
+            else { ps.p2 = ps.p1; ps.p1 = d; }
+        }
+    };
+
+    int k = 0;
+    for (; k < F.skipL; ++k) {
+        record(k);
+        M::step(x0, x1, x2, p, h);
+    }
+    double z0 = 1, z1 = 0, z2 = 0;
+    if (F.dirs) {
+        const double* d = F.dirs + (long long)t * 3;
+        z0 = d[0]; z1 = d[1]; z2 = d[2];
+    }
+    double y0 = z0 * eps + x0, y1 = z1 * eps + x1, y2 = z2 * eps + x2;
+    double result = 0;
+    for (int b = 0; b < F.nBlocks; ++b) {
+        if (k + F.nNT <= w0 || k >= w1) {
+            // no recorded state in this block: plain interleaved stepping
+            for (int i = 0; i < F.nNT; ++i) {
+                M::step(x0, x1, x2, p, h);
+                M::step(y0, y1, y2, p, h);
+            }
+            k += F.nNT;
+        } else {
+            for (int i = 0; i < F.nNT; ++i, ++k) {
+                record(k);
+                M::step(x0, x1, x2, p, h);
+                M::step(y0, y1, y2, p, h);
+            }
+        }
+        double s = 0;
+        s += (x0 - y0) * (x0 - y0);
+        s += (x1 - y1) * (x1 - y1);
+        s += (x2 - y2) * (x2 - y2);
+        s = sqrt(s) / eps;
+        result += log(s);
+        if (s != 0) s = (1 / s);
+        y0 = x0 - ((x0 - y0) * s);
+        y1 = x1 - ((x1 - y1) * s);
+        y2 = x2 - ((x2 - y2) * s);
+    }
+    lle[t] = result / F.tMax;
+    counts[t] = ps.count();
+}
+
+hipError_t launch_lle_peaks(int model, int wv, const FusedArgs& F, double* lle, int* counts,
+                            double* amps, double* ivls, hipStream_t st)
+{
+    if (F.pk.nCells <= 0) return hipSuccess;
+    const dim3 blk(256), grd((F.pk.nCells + 255) / 256);
+#define FUSED_CASE(MOD, W) \
+    hipLaunchKernelGGL((k_lle_peaks<MOD, W>), grd, blk, 0, st, F, lle, counts, amps, ivls)
+    if (model == kLinear) {
+        if (wv == 0) FUSED_CASE(Linear, 0); else if (wv == 1) FUSED_CASE(Linear, 1); else FUSED_CASE(Linear, 2);
+    } else {
+        if (wv == 0) FUSED_CASE(Rossler, 0); else if (wv == 1) FUSED_CASE(Rossler, 1); else FUSED_CASE(Rossler, 2);
+    }
+#undef FUSED_CASE
+    return hipGetLastError();
+}
+
 // --------------------------------------------------- integer post-ops
 
 // Cluster-count clamp min(v, clamp) applied in place (clamp > 0).

@@ -80,6 +80,17 @@ int bifhip_device_count(void);
 /* Library version string. */
 const char* bifhip_version(void);
 
+/* Device memory on the library's HIP runtime, for callers that bring no
+ * GPU framework of their own (no counterpart in the reference, whose
+ * drivers only take host arrays). */
+int bifhip_device_malloc(void** ptr, size_t bytes);
+int bifhip_device_free(void* ptr);
+int bifhip_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int bifhip_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int bifhip_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int bifhip_device_memset(void* ptr, int value, size_t bytes);
+int bifhip_synchronize(void);
+
 /* ------------------------------------------------------ drivers (files) */
 
 /* Library/hostLibrary.cuh:29-42 (bifurcation1D).  Writes "param, peak\n"
@@ -200,6 +211,22 @@ int bifhip_LLE_device(int dimension, double tMax, double NT, int nPts, double h,
                       double maxValue, double transientTime, const double* values,
                       int amountOfValues, double* d_out);
 
+/* LLE map and DBSCAN peak-count map of one 2D grid in one pass.  No single
+ * reference function: it produces exactly what LLE2D
+ * (Library/hostLibrary.cuh:408-422) and bifurcation2DDBSCAN
+ * (Library/hostLibrary.cuh:171-186) produce for the same grid, because the
+ * LLE's reference trajectory passes through every state the peak window
+ * records.  Requires preScaller 1, no divergence check, and the peak window
+ * (peakTransient .. peakTransient + peakTMax) inside the LLE trajectory
+ * (lleTransient + lleTMax).  d_peak[nPts^2] int, d_lle[nPts^2] double. */
+int bifhip_lle_peak_map_device(int nPts, double h, int amountOfInitialConditions,
+                               const double* initialConditions, const double* ranges,
+                               const int* indicesOfMutVars, const double* values,
+                               int amountOfValues, double peakTMax, double peakTransient,
+                               int writableVar, int maxAmountOfPeaks, double dbscanEps,
+                               double lleTMax, double NT, double lleTransient, double lleEps,
+                               int* d_peak, double* d_lle);
+
 /* ------------------------------------ reference kernels on device data */
 
 /* peakFinderCUDA (Library/cudaLibrary.cu:270-284) with peakFinder's h
@@ -236,6 +263,48 @@ int bifhip_series_device(int dimension, double tMax, int nPts, double h,
                          const double* ranges, const int* indicesOfMutVars, int writableVar,
                          double maxValue, double transientTime, const double* values,
                          int amountOfValues, int preScaller, double* d_data, int* d_flags);
+
+/* ------------------------------- Lyapunov spectrum on uneven tiles */
+
+/* No counterpart in the reference (whose LS2D, Library/hostLibrary.cu:
+ * 1933-2082, integrates one uniform length with finite-difference copies):
+ * the full 3-exponent spectrum of tiles of tileW x tileH cells placed on an
+ * nPts x nPts lattice over `ranges`, each tile with its own length, in one
+ * persistent launch replayed from a captured HIP graph.  FP32 state and
+ * tangent vectors (variational equation of the discrete step), FP64
+ * Gram-Schmidt (the reference's, Library/cudaLibrary.cu:913-957) and log
+ * accumulation every NT.  Tile tables are host arrays; d_out is device
+ * memory [nTiles][tileW*tileH][3]. */
+
+/* Seeded tile table: origins (half uniform, half clustered), lengths
+ * uniform in [tMinTime, tMaxTime] as whole NT blocks. */
+int bifhip_tiles_generate(int nTiles, int tileW, int tileH, int nPts,
+                          unsigned long long seedOrigins, unsigned long long seedLengths,
+                          double tMinTime, double tMaxTime, double NT, int* originI,
+                          int* originJ, int* nBlocks);
+
+int bifhip_ls_tiles_device(int nTiles, int tileW, int tileH, const int* originI,
+                           const int* originJ, const int* nBlocks, int nPts, const double* ranges,
+                           const int* indicesOfMutVars, const double* initialConditions,
+                           int amountOfInitialConditions, const double* values,
+                           int amountOfValues, double h, double NT, double transientTime,
+                           double* d_out);
+
+/* Same, after timing every (block size, cells per lane) candidate on a
+ * sample of the queue and keeping the fastest as the tile configuration. */
+int bifhip_ls_tiles_autotune(int nTiles, int tileW, int tileH, const int* originI,
+                             const int* originJ, const int* nBlocks, int nPts,
+                             const double* ranges, const int* indicesOfMutVars,
+                             const double* initialConditions, int amountOfInitialConditions,
+                             const double* values, int amountOfValues, double h, double NT,
+                             double transientTime, double* d_out);
+
+/* Tile configuration: block size 0 (autotune on next use), 64, 128, 256;
+ * cells per lane 1 or 2; blocks per CU 0 = occupancy maximum. */
+int bifhip_set_tile_config(int blockSize, int cellsPerLane, int blocksPerCU);
+int bifhip_get_tile_config(int* blockSize, int* cellsPerLane, int* blocksPerCU);
+/* Launch the tile work through a captured HIP graph (default 1). */
+int bifhip_set_graph(int on);
 
 /* ------------------------------------------------ CSV writers (host) */
 

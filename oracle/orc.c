@@ -442,6 +442,59 @@ void orc_ls_cell(int model, int nPts, long cellIdx, long localIdx,
         out[i] = res[i] / tMax;
 }
 
+/* Variational (tangent-map) Lyapunov spectrum of the Rossler step, FP64.
+ * Not in the reference, which uses finite-difference copies
+ * (Library/cudaLibrary.cu:961-1100); this is the oracle of the variational
+ * tile path.  The tangent map of the sequential Euler step
+ *   x0' = x0 - h(x1+x2); x1' = x1 + h(x0' + a x1); x2' = x2 + h(b + x2(x0'-c))
+ * is  v0' = v0 - h(v1+v2); v1' = (1+ha)v1 + h v0';
+ *     v2' = (1 + h(x0'-c)) v2 + h x2 v0'    (x2 the pre-step value).
+ * Every nNT steps the three tangent vectors are orthonormalised with the
+ * reference's classical Gram-Schmidt (orc_gram_schmidt) and log(norm)
+ * accumulates; out = sum / (nBlocks * nNT * h). */
+void orc_ls_var_cell(double a, double b, double c, const double *ic, double h,
+                     int skip, int nNT, int nBlocks, double *out)
+{
+    double x0 = ic[0], x1 = ic[1], x2 = ic[2];
+    double v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, w[9], den[3];
+    double res[3] = {0, 0, 0};
+    for (int s = 0; s < skip; ++s) {
+        x0 = x0 + h * (-x1 - x2);
+        x1 = x1 + h * (x0 + a * x1);
+        x2 = x2 + h * (b + x2 * (x0 - c));
+    }
+    for (int blk = 0; blk < nBlocks; ++blk) {
+        for (int s = 0; s < nNT; ++s) {
+            const double x2o = x2;
+            x0 = x0 + h * (-x1 - x2);
+            x1 = x1 + h * (x0 + a * x1);
+            x2 = x2 + h * (b + x2 * (x0 - c));
+            for (int k = 0; k < 3; ++k) {
+                double *q = v + 3 * k;
+                q[0] = q[0] - h * (q[1] + q[2]);
+                q[1] = (1 + h * a) * q[1] + h * q[0];
+                q[2] = (1 + h * (x0 - c)) * q[2] + h * x2o * q[0];
+            }
+        }
+        orc_gram_schmidt(v, w, 3, den);
+        for (int k = 0; k < 3; ++k)
+            res[k] += log(den[k]);
+        memcpy(v, w, sizeof(v));
+    }
+    const double T = (double)nBlocks * nNT * h;
+    for (int k = 0; k < 3; ++k)
+        out[k] = res[k] / T;
+}
+
+void orc_ls_var_cells(const double *a, const double *b, const double *c,
+                      const double *ic, double h, int skip, int nNT,
+                      const int *nBlocks, long n, double *out)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < n; ++i)
+        orc_ls_var_cell(a[i], b[i], c[i], ic, h, skip, nNT, nBlocks[i], out + 3 * i);
+}
+
 /* ---------------- map-level drivers ---------------- */
 
 void orc_peaks_cells(int model, int dimension, double tMax, int nPts, double h,

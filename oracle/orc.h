@@ -90,6 +90,16 @@ void orc_ls_cell(int model, int nPts, long cellIdx, long localIdx,
                  double NT, double tMax, double h, double eps, int skip,
                  double maxValue, int pertMode, double *out);
 
+/* Variational (tangent-map) spectrum of the Rossler step in FP64 with the
+ * reference's Gram-Schmidt every nNT steps; out[3] = exponents.  Oracle of
+ * the FP32 variational tile kernels.  Pinned by: the finite-difference LS
+ * above (same cells, small eps) and the volume-contraction sum. */
+void orc_ls_var_cell(double a, double b, double c, const double *ic, double h,
+                     int skip, int nNT, int nBlocks, double *out);
+void orc_ls_var_cells(const double *a, const double *b, const double *c,
+                      const double *ic, double h, int skip, int nNT,
+                      const int *nBlocks, long n, double *out);
+
 /* ---- map-level drivers (OpenMP over cells; in-memory results) ---- */
 
 /* bifurcation1D (hostLibrary.cu:7-158): counts[nPts], peaks[nPts*cap]. */

@@ -30,10 +30,12 @@ def bh():
 
 
 @pytest.fixture(scope="session")
-def dev():
-    import torch
-    assert torch.cuda.is_available(), "GPU test collected on a host without a GPU"
-    return torch.device("cuda:0")
+def dev(bh):
+    """Device 0 through the library's own HIP runtime.  GPU tests never
+    import torch: torch bundles a second HIP runtime, and two runtimes in one
+    process do not share the device."""
+    assert bh.device_count() > 0, "GPU test collected on a host without a GPU"
+    return 0
 
 
 @pytest.fixture(autouse=True)

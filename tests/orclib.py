@@ -77,6 +77,8 @@ def _declare(L):
     L.orc_ls_cells.argtypes = [_I, _I, _D, _D, _I, _D, _D, _PD, _I, _PD, _PI,
                                _D, _D, _PD, _I, _I, _L, _L, _L, _PD]
     L.orc_threads.restype = _I
+    L.orc_ls_var_cell.argtypes = [_D, _D, _D, _PD, _D, _I, _I, _I, _PD]
+    L.orc_ls_var_cells.argtypes = [_PD, _PD, _PD, _PD, _D, _I, _I, _PI, _L, _PD]
 
 
 def pd(a):
@@ -212,3 +214,11 @@ def ls_cells(dimension, tMax, NT, nPts, h, eps, ic, ranges, mut, max_value,
                        pd(values), len(values), pert, dir_chunk, c0, nc,
                        pd(out))
     return out.reshape(nc, len(ic))
+
+
+def ls_var_cells(a, b, c, ic, h, skip, nNT, nBlocks):
+    a, b, c, ic = f64(a), f64(b), f64(c), f64(ic)
+    nb = i32(nBlocks)
+    out = np.zeros((len(a), 3))
+    lib().orc_ls_var_cells(pd(a), pd(b), pd(c), pd(ic), h, skip, nNT, pi(nb), len(a), pd(out))
+    return out

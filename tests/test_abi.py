@@ -117,13 +117,62 @@ def test_context_knob_validation():
     B.set_output_path("mat.csv")
 
 
+MAIN_CPP = r"""
+#include "bifhip.hpp"
+int main() {
+    bifhip_set_output_path(OUT);
+    // the reference's own call shape (CudaConsole/CudaConsole/main.cpp templates)
+    bifurcation2DDBSCAN(500, 4, 0.01, 3, new double[3]{0.1, 0.1, 0.1},
+                        new double[4]{0.05, 0.45, 2, 10}, new int[2]{0, 2}, 0, 0, 32, 1000,
+                        new double[3]{0.2, 0.2, 5.7}, 3, 1, 0.01);
+    LLE2D(1000, 0.5, 4, 0.01, 1e-8, new double[3]{0.1, 0.1, 0.1}, 3,
+          new double[4]{0.05, 0.45, 2, 10}, new int[2]{0, 2}, 0, 0, 500,
+          new double[3]{0.2, 0.2, 5.7}, 3);
+    return 0;
+}
+"""
+
+
+def test_reference_style_caller_compiles_and_links(tmp_path):
+    """A main.cpp written against the reference's header compiles against
+    include/bifhip.hpp; without a GPU it exits through the reference's
+    GPUassert path instead of crashing."""
+    src = tmp_path / "main.cpp"
+    src.write_text(MAIN_CPP)
+    exe = tmp_path / "main"
+    inc = os.path.join(os.path.dirname(B.HEADER))
+    subprocess.run(["g++", "-std=c++17", f"-DOUT=\"{tmp_path / 'mat.csv'}\"", "-I", inc,
+                    str(src), "-o", str(exe), f"-L{os.path.dirname(B.LIB_PATH)}", "-lbifhip",
+                    f"-Wl,-rpath,{os.path.dirname(B.LIB_PATH)}"], check=True)
+    if B.device_count() > 0:
+        return
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "GPUassert: no HIP device visible" in r.stderr
+
+
 def test_drivers_fail_cleanly_without_device():
     """On a host with no GPU every driver returns BIFHIP_E_NODEV (no crash,
     no CPU fallback)."""
-    import torch
-    if torch.cuda.is_available():
+    if B.device_count() > 0:
         pytest.skip("a GPU is visible")
     with pytest.raises(B.BifhipError) as e:
         B.dbscan_map(500, 4, 0.01, [0.1] * 3, [0.05, 0.45, 2, 10], [0, 2], 0, 0.0, 32, 1000,
                      [0.2, 0.2, 5.7], 1, 0.01)
     assert e.value.code == B.E_NODEV
+
+
+def test_tile_generator_is_seeded_and_in_bounds():
+    a = B.tiles_generate(4096, 16, 16, 4096, 0, 1, 200.0, 2000.0, 0.5)
+    b = B.tiles_generate(4096, 16, 16, 4096, 0, 1, 200.0, 2000.0, 0.5)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    oi, oj, nb = a
+    assert oi.min() >= 0 and oi.max() <= 4096 - 16 and oj.min() >= 0 and oj.max() <= 4096 - 16
+    assert nb.min() >= 400 and nb.max() <= 4000            # [200, 2000] t.u. in 0.5 blocks
+    assert abs(nb.mean() * 0.5 - 1100) < 40                # uniform mean 1100 t.u.
+    # half of the origins cluster: the origin histogram is far from uniform
+    h, _ = np.histogram(oj, bins=16, range=(0, 4080))
+    assert h.max() > 3 * h.min()
+    c = B.tiles_generate(64, 16, 16, 4096, 5, 1, 200.0, 2000.0, 0.5)
+    assert not np.array_equal(c[0], oi[:64])

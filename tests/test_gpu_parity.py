@@ -26,30 +26,44 @@ def say(*a):
     sys.stdout.flush()
 
 
-def _t(dev, a, dtype):
-    import torch
-    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+def _t(bh, a, dtype):
+    return bh.DeviceArray.from_host(np.asarray(a), dtype)
+
+
+def _close_with_nan_class(got, want, atol, what):
+    """NaN cells (escaping trajectories) must coincide; NaN sign and payload
+    are not compared (x86 and AMD produce different default NaNs).  Finite
+    cells within atol; returns the bit-identical fraction."""
+    got, want = np.asarray(got), np.asarray(want)
+    gn, wn = np.isnan(got), np.isnan(want)
+    np.testing.assert_array_equal(gn, wn, err_msg=f"{what}: NaN cells differ")
+    fin = ~gn
+    err = np.abs(got[fin] - want[fin])
+    ident = float((got[fin] == want[fin]).mean()) if fin.any() else 1.0
+    say(what, "finite cells", int(fin.sum()), "NaN cells", int(gn.sum()), "max abs err",
+        float(err.max()) if err.size else 0.0, "bit-identical", ident)
+    assert err.size == 0 or err.max() <= atol
+    return ident
 
 
 # ------------------------------------------------------------ record pass
 
 @pytest.mark.parametrize("wv,pre", [(0, 1), (1, 1), (2, 3)])
 def test_series_bit_exact(orc, bh, dev, wv, pre):
-    import torch
     n, tmax, h, trans = 16, 20.0, 0.01, 10.0
     N = int(tmax / h / pre)
     say("series", n, wv, pre)
-    data = torch.zeros(n * n * N, dtype=torch.float64, device=dev)
-    flags = torch.zeros(n * n, dtype=torch.int32, device=dev)
-    bh.series_device(data.data_ptr(), flags.data_ptr(), 2, tmax, n, h, IC, R2, [0, 2], wv, 0.0,
+    data = bh.DeviceArray((n * n * N,), np.float64).zero_()
+    flags = bh.DeviceArray((n * n,), np.int32).zero_()
+    bh.series_device(data.ptr, flags.ptr, 2, tmax, n, h, IC, R2, [0, 2], wv, 0.0,
                      trans, VALS, pre)
-    torch.cuda.synchronize()
-    got = data.cpu().numpy().reshape(n * n, N)
+    bh.synchronize()
+    got = data.to_host().reshape(n * n, N)
     for cell in range(0, n * n, 5):
         ok, s = orc.bif_cell(cell, n, R2, [0, 2], IC, VALS, h, int(trans / h), N, pre, wv)
         assert ok == 1
         np.testing.assert_array_equal(got[cell], s, err_msg=f"cell {cell}")
-    assert not flags.any()
+    assert not flags.to_host().any()
 
 
 # ------------------------------------------------------------ peak finder
@@ -62,20 +76,19 @@ def _plateau_walks(rng, nb, n):
 
 @pytest.mark.parametrize("inplace", [False, True])
 def test_peak_finder_device_fuzz(orc, bh, dev, inplace):
-    import torch
     rng = np.random.default_rng(5)
     nb, n = 384, 3000
     d = _plateau_walks(rng, nb, n)
     cnt0 = np.zeros(nb, dtype=np.int32)
     cnt0[::11] = -1
     say("peak_finder", nb, n, inplace)
-    D = _t(dev, d.ravel(), torch.float64)
-    C = _t(dev, cnt0, torch.int32)
-    P = D if inplace else torch.zeros_like(D)
-    T = torch.zeros_like(D)
-    bh.peak_finder_device(D.data_ptr(), n, nb, C.data_ptr(), P.data_ptr(), T.data_ptr(), 0.01)
-    torch.cuda.synchronize()
-    cnt, pk, tm = C.cpu().numpy(), P.cpu().numpy().reshape(nb, n), T.cpu().numpy().reshape(nb, n)
+    D = _t(bh, d.ravel(), np.float64)
+    C = _t(bh, cnt0, np.int32)
+    P = D if inplace else bh.DeviceArray(D.shape, np.float64).zero_()
+    T = bh.DeviceArray(D.shape, np.float64).zero_()
+    bh.peak_finder_device(D.ptr, n, nb, C.ptr, P.ptr, T.ptr, 0.01)
+    bh.synchronize()
+    cnt, pk, tm = C.to_host(), P.to_host().reshape(nb, n), T.to_host().reshape(nb, n)
     for b in range(nb):
         if cnt0[b] == -1:
             assert cnt[b] == 0
@@ -87,19 +100,19 @@ def test_peak_finder_device_fuzz(orc, bh, dev, inplace):
 
 
 def test_peak_finder_reference_kat_gpu(bh, dev):
-    import json, os, torch
+    import json, os
     gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
     case = gold["peak_finder"][0]
     s = np.array(case["series"], dtype=np.float64)
-    D = _t(dev, s, torch.float64)
-    C = torch.zeros(1, dtype=torch.int32, device=dev)
-    P = torch.zeros_like(D)
-    T = torch.zeros_like(D)
-    bh.peak_finder_device(D.data_ptr(), len(s), 1, C.data_ptr(), P.data_ptr(), T.data_ptr(), 0.01)
-    torch.cuda.synchronize()
-    assert int(C[0]) == case["count"]
-    assert P.cpu().numpy()[:2].tolist() == case["peaks"]
-    assert T.cpu().numpy()[:2].tolist() == case["intervals"]
+    D = _t(bh, s, np.float64)
+    C = bh.DeviceArray((1,), np.int32).zero_()
+    P = bh.DeviceArray(D.shape, np.float64).zero_()
+    T = bh.DeviceArray(D.shape, np.float64).zero_()
+    bh.peak_finder_device(D.ptr, len(s), 1, C.ptr, P.ptr, T.ptr, 0.01)
+    bh.synchronize()
+    assert int(C.to_host()[0]) == case["count"]
+    assert P.to_host()[:2].tolist() == case["peaks"]
+    assert T.to_host()[:2].tolist() == case["intervals"]
 
 
 # ------------------------------------------------------- bifurcation 1D
@@ -156,18 +169,17 @@ def _fuzz_peak_sets(rng, nb, stride):
 
 @pytest.mark.parametrize("max_peaks", [100000, 120])
 def test_dbscan_device_fuzz_bit_exact(orc, bh, dev, max_peaks):
-    import torch
     rng = np.random.default_rng(7)
     nb, stride = 600, 320
     amps, ivls, cnt = _fuzz_peak_sets(rng, nb, stride)
     say("dbscan fuzz", nb, max_peaks)
-    out = torch.full((nb,), -99, dtype=torch.int32, device=dev)
-    A, I, Cn = _t(dev, amps.ravel(), torch.float64), _t(dev, ivls.ravel(), torch.float64), _t(
-        dev, cnt, torch.int32)
-    bh.dbscan_device(A.data_ptr(), I.data_ptr(), stride, nb, Cn.data_ptr(), max_peaks, 0.01,
-                     out.data_ptr())
-    torch.cuda.synchronize()
-    got = out.cpu().numpy()
+    out = bh.DeviceArray.full((nb,), -99, np.int32)
+    A, I, Cn = _t(bh, amps.ravel(), np.float64), _t(bh, ivls.ravel(), np.float64), _t(
+        bh, cnt, np.int32)
+    bh.dbscan_device(A.ptr, I.ptr, stride, nb, Cn.ptr, max_peaks, 0.01,
+                     out.ptr)
+    bh.synchronize()
+    got = out.to_host()
     for b in range(nb):
         want = 0 if cnt[b] == -1 else orc.dbscan(amps[b, :cnt[b]], ivls[b, :cnt[b]], 0.01,
                                                  max_peaks)
@@ -175,7 +187,7 @@ def test_dbscan_device_fuzz_bit_exact(orc, bh, dev, max_peaks):
 
 
 def test_dbscan_reference_kats_gpu(bh, dev):
-    import json, os, torch
+    import json, os
     gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
     cases = gold["dbscan_eps001_intervals0"]
     stride = 16
@@ -184,14 +196,14 @@ def test_dbscan_reference_kats_gpu(bh, dev):
     for i, c in enumerate(cases):
         amps[i, :len(c["amps"])] = c["amps"]
         cnt[i] = len(c["amps"])
-    A = _t(dev, amps.ravel(), torch.float64)
-    I = torch.zeros_like(A)
-    Cn = _t(dev, cnt, torch.int32)
-    out = torch.zeros(len(cases), dtype=torch.int32, device=dev)
-    bh.dbscan_device(A.data_ptr(), I.data_ptr(), stride, len(cases), Cn.data_ptr(), 1 << 30,
-                     0.01, out.data_ptr())
-    torch.cuda.synchronize()
-    assert out.cpu().numpy().tolist() == [c["result"] for c in cases]
+    A = _t(bh, amps.ravel(), np.float64)
+    I = bh.DeviceArray(A.shape, np.float64).zero_()
+    Cn = _t(bh, cnt, np.int32)
+    out = bh.DeviceArray((len(cases),), np.int32).zero_()
+    bh.dbscan_device(A.ptr, I.ptr, stride, len(cases), Cn.ptr, 1 << 30,
+                     0.01, out.ptr)
+    bh.synchronize()
+    assert out.to_host().tolist() == [c["result"] for c in cases]
 
 
 @pytest.mark.parametrize("max_peaks", [50000, 32])
@@ -236,7 +248,6 @@ def test_dbscan_divergence_and_chunking(orc, bh):
 # ------------------------------------------------------------------- KDE
 
 def test_kde_device_agreement(orc, bh, dev):
-    import torch
     rng = np.random.default_rng(9)
     nb, stride = 400, 128
     amps = np.zeros((nb, stride))
@@ -246,12 +257,12 @@ def test_kde_device_agreement(orc, bh, dev):
         amps[b, :cnt[b]] = rng.choice(centers, cnt[b]) + rng.normal(size=cnt[b]) * 0.05
     cnt[::17] = -1
     say("kde fuzz", nb)
-    out = torch.zeros(nb, dtype=torch.int32, device=dev)
-    A, Cn = _t(dev, amps.ravel(), torch.float64), _t(dev, cnt, torch.int32)
-    bh.kde_device(A.data_ptr(), stride, nb, Cn.data_ptr(), 100, 10, 0.0, 10.0, 0.05,
-                  out.data_ptr())
-    torch.cuda.synchronize()
-    got = out.cpu().numpy()
+    out = bh.DeviceArray((nb,), np.int32).zero_()
+    A, Cn = _t(bh, amps.ravel(), np.float64), _t(bh, cnt, np.int32)
+    bh.kde_device(A.ptr, stride, nb, Cn.ptr, 100, 10, 0.0, 10.0, 0.05,
+                  out.ptr)
+    bh.synchronize()
+    got = out.to_host()
     want = np.array([0 if cnt[b] == -1 else orc.kde(amps[b, :cnt[b]], 100, 10, 0.0, 10.0, 0.05)
                      for b in range(nb)])
     agree = (got == want).mean()
@@ -271,19 +282,18 @@ def test_kde_map_agreement(orc, bh):
 # --------------------------------------------------------- Gram-Schmidt
 
 def test_gram_schmidt_device_bit_exact(orc, bh, dev):
-    import torch
     rng = np.random.default_rng(11)
     for n in (2, 3, 5, 8):
         cnt = 200
         a = rng.normal(size=(cnt, n, n))
         a[::9, 1] = 0.0
         say("gram_schmidt", n)
-        A = _t(dev, a.ravel(), torch.float64)
-        Bm = torch.zeros_like(A)
-        Dn = torch.zeros(cnt * n, dtype=torch.float64, device=dev)
-        bh.gram_schmidt_device(A.data_ptr(), Bm.data_ptr(), Dn.data_ptr(), n, cnt)
-        torch.cuda.synchronize()
-        b, d = Bm.cpu().numpy().reshape(cnt, n, n), Dn.cpu().numpy().reshape(cnt, n)
+        A = _t(bh, a.ravel(), np.float64)
+        Bm = bh.DeviceArray(A.shape, np.float64).zero_()
+        Dn = bh.DeviceArray((cnt * n,), np.float64).zero_()
+        bh.gram_schmidt_device(A.ptr, Bm.ptr, Dn.ptr, n, cnt)
+        bh.synchronize()
+        b, d = Bm.to_host().reshape(cnt, n, n), Dn.to_host().reshape(cnt, n)
         for i in range(cnt):
             wb, wd = orc.gram_schmidt(a[i])
             np.testing.assert_array_equal(b[i], wb)
@@ -300,9 +310,7 @@ def test_lle2d_fp64_matches_oracle(orc, bh, dir_chunk):
     got = bh.lle_map(2, 100.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 50.0, VALS)
     want = orc.lle_cells(2, 100.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 50.0, VALS,
                          orc.PERT_REFERENCE, 200000 if dir_chunk < 0 else dir_chunk, 0, n * n)
-    err = np.abs(got - want)
-    say("LLE max abs err", err.max(), "bit-identical", (got == want).mean())
-    assert err.max() <= 1e-12
+    _close_with_nan_class(got, want, 1e-12, "LLE2D")
 
 
 def test_lle_fp64_divergence_check_and_axis(orc, bh):
@@ -311,7 +319,7 @@ def test_lle_fp64_divergence_check_and_axis(orc, bh):
     got = bh.lle_map(1, 60.0, 0.5, 24, 0.01, 1e-8, IC, [2.0, 12.0], [2], 8.0, 20.0, VALS)
     want = orc.lle_cells(1, 60.0, 0.5, 24, 0.01, 1e-8, IC, [2.0, 12.0], [2], 8.0, 20.0, VALS,
                          orc.PERT_AXIS_X, 0, 0, 24)
-    assert np.abs(got - want).max() <= 1e-12
+    _close_with_nan_class(got, want, 1e-12, "LLE1D maxValue")
 
 
 def test_ls2d_fp64_matches_oracle(orc, bh):
@@ -320,9 +328,7 @@ def test_ls2d_fp64_matches_oracle(orc, bh):
     got = bh.ls_map(2, 50.0, 0.5, n, 0.01, 1e-6, IC, R2, [0, 2], 0.0, 20.0, VALS)
     want = orc.ls_cells(2, 50.0, 0.5, n, 0.01, 1e-6, IC, R2, [0, 2], 0.0, 20.0, VALS,
                         orc.PERT_REFERENCE, 0, 0, n * n)
-    err = np.abs(got - want)
-    say("LS max abs err", err.max(), "bit-identical", (got == want).mean())
-    assert err.max() <= 1e-12
+    _close_with_nan_class(got, want, 1e-12, "LS2D")
 
 
 def test_linear_model_closed_forms_gpu(bh):
@@ -381,7 +387,13 @@ def test_driver_files_match_reference_format(orc, bh, tmp_path):
     lines = out.read_text().split("\n")
     assert lines[:2] == ["0.05 0.45", "2 10"]
     body = "".join(lines[2:]).split(", ")
-    assert body == [f"{v:g}" for v in lle]
+    want_txt = [f"{v:g}" for v in lle]
+    assert len(body) == len(want_txt)
+    for b_, w_ in zip(body, want_txt):
+        # NaN cells print as "nan" or "-nan" depending on the sign bit, which
+        # differs between x86 and AMD default NaNs; finite cells to 6 digits
+        assert b_ == w_ or (b_ in ("nan", "-nan") and w_ == "nan") or \
+            abs(float(b_) - float(w_)) <= 1e-12, (b_, w_)
 
     bh.LS2D(30.0, 0.5, n, 0.01, 1e-6, IC, 3, R2, [0, 2], 0, 0.0, 10.0, VALS, 3)
     ls = orc.ls_cells(2, 30.0, 0.5, n, 0.01, 1e-6, IC, R2, [0, 2], 0.0, 10.0, VALS,

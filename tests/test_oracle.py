@@ -270,3 +270,14 @@ def test_lle_direction_chunk_changes_direction_only(orc):
     chunked = orc.lle_cells(*args, 10, 0, 64)
     np.testing.assert_array_equal(full[:10], chunked[:10])
     assert (full[10:] != chunked[10:]).any()
+
+
+def test_variational_ls_matches_finite_difference(orc):
+    """The FP64 tangent-map spectrum (oracle of the tile path) agrees with the
+    reference-style finite-difference LS on the same cells."""
+    h, T, trans = 0.01, 400.0, 100.0
+    for c in (2.5, 4.0, 5.7):
+        fd = orc.ls_cells(1, T, 0.5, 2, h, 1e-8, IC, [c, c], [2], 0.0, trans, [0.2, 0.2, c],
+                          orc.PERT_AXIS_X, 0, 0, 1)[0]
+        var = orc.ls_var_cells([0.2], [0.2], [c], IC, h, int(trans / h), 50, [int(T / 0.5)])[0]
+        np.testing.assert_allclose(var, fd, atol=2e-4)

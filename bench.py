@@ -1,16 +1,22 @@
 #!/usr/bin/env python3
-"""Headline benchmark: grid points/s of the 2D LLE + peak-count map.
+"""Headline benchmark: grid points/s of the 2D LLE + peak-count map,
+surrogate-refined versus plain FP64, at >= 99% per-cell regime agreement.
 
-One step = one whole map over the synthetic (a, c) parameter lattice of
-the Rossler system, inputs resident on the GPU, outputs left in HBM:
+Rossler system, b = 0.2, x0 = (0.1, 0.1, 0.1), h = 0.01, (a, c) lattice over
+[0.05, 0.45] x [2, 10] (synthetic parameter lattice, nothing is loaded):
   * peak count: DBSCAN cluster count of (amplitude, interval) peak pairs,
-    transient 1000, record 500 time units, h = 0.01, eps = 0.01;
+    transient 1000, record 500 time units, eps = 0.01, no peak threshold;
   * LLE: Benettin two-trajectory, transient 500, measure 1000 time units,
     renormalisation every 0.5, perturbation 1e-8 along x.
-Workloads:
-  plain    plain FP64 path (reference operation order) on an n x n grid
-Prints one JSON line (rank 0).  --gpus above 1 is refused: this build
-targets one GPU.
+Regime of a cell = (peak-count class clamp(count, 1, 16), LLE > 0.01).
+
+Workloads (one step = one whole map, inputs resident, outputs in HBM):
+  surrogate  (default) 4096^2 fine map: plain FP64 coarse map on 256^2, a
+             20-epoch BF16-MFMA network, inference, FP64 re-solve of the
+             flagged cells.  Agreement and speedup are measured against the
+             plain FP64 map of the same 4096^2 lattice (computed once).
+  plain      the plain FP64 fused LLE + peak-count map on an n x n lattice.
+Prints one JSON line.  --gpus above 1 is refused: this build uses one GPU.
 """
 import argparse
 import json
@@ -28,9 +34,11 @@ RANGES = [0.05, 0.45, 2.0, 10.0]
 H = 0.01
 PEAK = dict(tMax=500.0, transient=1000.0, eps=0.01)
 LLE = dict(tMax=1000.0, NT=0.5, transient=500.0, eps=1e-8)
+CHAOS = 0.01
 FP64_VALU_PEAK = 78.6e12     # MI355X FP64 vector, FMA counted as 2
 STEP_FLOP = 12               # Rossler sequential Euler step, no contraction
 RENORM_FLOP = 20
+METRIC = "grid points/s, 2D LLE+peak map; surrogate-refined vs plain FP64 @≥99% agree"
 
 
 def lle_flop_per_cell():
@@ -40,47 +48,91 @@ def lle_flop_per_cell():
     return STEP_FLOP * (skip + 2 * nb * nNT) + RENORM_FLOP * nb
 
 
-def peak_flop_per_cell():
-    return STEP_FLOP * (int(PEAK["transient"] / H) + int(PEAK["tMax"] / H))
+def n_rec():
+    return int(PEAK["tMax"] / H)
 
 
-def run_plain(bh, n, d_peak, d_lle):
-    N = int(PEAK["tMax"] / H)
-    bh.dbscan_map_device(d_peak.ptr, PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, N,
-                         PEAK["transient"], VALS, 1, PEAK["eps"])
-    bh.lle_map_device(d_lle.ptr, 2, LLE["tMax"], LLE["NT"], n, H, LLE["eps"], IC, RANGES,
-                      [0, 2], 0.0, LLE["transient"], VALS)
+def plain_map(bh, n, d_peak, d_lle):
+    """Plain FP64 map: one fused pass gives both the LLE (LLE2D semantics)
+    and the DBSCAN peak count (bifurcation2DDBSCAN semantics)."""
+    bh.lle_peak_map_device(d_peak.ptr, d_lle.ptr, n, H, IC, RANGES, [0, 2], VALS, PEAK["tMax"],
+                           PEAK["transient"], 0, n_rec(), PEAK["eps"], LLE["tMax"], LLE["NT"],
+                           LLE["transient"], LLE["eps"])
+
+
+def surrogate_map(bh, n, nc, d_peak, d_lle, d_res, seed, conf, validation, batch):
+    return bh.surrogate_map_device(d_peak.ptr, d_lle.ptr, d_res.ptr, n, nc, H, IC, RANGES,
+                                   [0, 2], VALS, PEAK["tMax"], PEAK["transient"], 0, n_rec(),
+                                   PEAK["eps"], LLE["tMax"], LLE["NT"], LLE["transient"],
+                                   LLE["eps"], epochs=20, batch=batch, lr=1e-3, seed=seed,
+                                   conf=conf, chaos=CHAOS, validation=validation)
+
+
+def regimes(np, peak, lle):
+    """(peak-count class, chaotic) per cell; a NaN LLE (escaping orbit)
+    counts as not chaotic."""
+    with np.errstate(invalid="ignore"):
+        return np.clip(peak, 1, 16), lle > CHAOS
+
+
+def boundary_fraction(np, cls, ch, n):
+    """Cells with a 4-neighbour of another regime in the reference map."""
+    c = cls.reshape(n, n)
+    h = ch.reshape(n, n)
+    b = np.zeros((n, n), dtype=bool)
+    for a in (c, h):
+        d0 = a[:, 1:] != a[:, :-1]
+        d1 = a[1:, :] != a[:-1, :]
+        b[:, 1:] |= d0
+        b[:, :-1] |= d0
+        b[1:, :] |= d1
+        b[:-1, :] |= d1
+    return float(b.mean())
 
 
 def cpu_baseline(n, rows):
-    """Oracle (the C restatement of the reference's per-trajectory code,
-    OpenMP over cells) on `rows` whole rows of the same map."""
+    """Oracle (C restatement of the reference's per-trajectory code, OpenMP
+    over cells) on `rows` whole rows of the same map: DBSCAN peak count and
+    LLE as the reference computes them (two separate trajectories)."""
     import orclib as O
     O.lib()
-    N = int(PEAK["tMax"] / H)
     step = max(1, n // rows)
     cells = 0
     t0 = time.perf_counter()
-    for r in range(0, n, step)[:rows]:
-        O.dbscan_cells(PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, N, PEAK["transient"], VALS,
-                       1, PEAK["eps"], r * n, n)
+    for r in list(range(0, n, step))[:rows]:
+        O.dbscan_cells(PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(),
+                       PEAK["transient"], VALS, 1, PEAK["eps"], r * n, n)
         O.lle_cells(2, LLE["tMax"], LLE["NT"], n, H, LLE["eps"], IC, RANGES, [0, 2], 0.0,
                     LLE["transient"], VALS, O.PERT_AXIS_X, 0, r * n, n)
         cells += n
     dt = time.perf_counter() - t0
     return {"value": cells / dt, "unit": "grid points/s", "cores": O.lib().orc_threads(),
             "kind": "port",
-            "sample": f"{cells} cells ({rows} rows of the {n}x{n} map), DBSCAN peak count + LLE, "
-                      f"{dt:.1f} s"}
+            "sample": f"{cells} cells ({rows} evenly spaced rows of the {n}x{n} lattice), "
+                      f"DBSCAN peak count + LLE per cell, {dt:.1f} s"}
+
+
+def roofline(bh, cells_per_launch, kernel="lle_peaks"):
+    ms, n = bh.kernel_time(kernel)
+    avg = ms / max(n, 1)
+    flop = lle_flop_per_cell() * cells_per_launch
+    ach = flop / (avg / 1000) / 1e12 if avg > 0 else 0.0
+    return {"bound": "valu_fp64", "kernel": "k_lle_peaks", "achieved": ach,
+            "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s", "frac": ach / (FP64_VALU_PEAK / 1e12),
+            "traffic": None, "flop_per_launch": flop, "avg_launch_ms": avg, "launches": n}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
-    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--workload", default="surrogate", choices=["surrogate", "plain"])
+    ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
+    ap.add_argument("--coarse", type=int, default=256)
+    ap.add_argument("--conf", type=float, default=0.95)
+    ap.add_argument("--validation", type=float, default=0.002)
+    ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-rows", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -97,56 +149,86 @@ def main():
         sys.exit(1)
     bh.reset_context()
     bh.set_perturbation(bh.PERT_AXIS_X)
-    n = args.n
-    d_peak = bh.DeviceArray((n * n,), np.int32)
-    d_lle = bh.DeviceArray((n * n,), np.float64)
-
-    for _ in range(args.warmup):
-        run_plain(bh, n, d_peak, d_lle)
-    bh.synchronize()
-    bh.set_timing(True)
-    bh.reset_kernel_times()
-    bh.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run_plain(bh, n, d_peak, d_lle)
-    bh.synchronize()
-    dt = time.perf_counter() - t0
-    bh.set_timing(False)
-
-    ms_step = dt * 1000 / args.steps
+    n = args.n or (4096 if args.workload == "surrogate" else 1024)
     cells = n * n
-    lle_ms, lle_n = bh.kernel_time("lle")
-    pk_ms, pk_n = bh.kernel_time("bif_peaks")
-    db_ms, db_n = bh.kernel_time("dbscan")
-    avg_lle = lle_ms / max(lle_n, 1)
-    flop_launch = lle_flop_per_cell() * cells  # one launch covers the whole map
-    achieved = flop_launch / (avg_lle / 1000) / 1e12
-    out = {
-        "metric": "grid points/s, 2D LLE+peak map; surrogate-refined vs plain FP64 @>=99% agree",
-        "value": cells / (ms_step / 1000),
-        "unit": "grid points/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (a,c) parameter lattice of the Rossler system",
-        "config": {"workload": f"plain FP64 2D LLE + DBSCAN peak-count map, Rossler {n}x{n}",
-                   "grid": n, "a_range": RANGES[:2], "c_range": RANGES[2:], "h": H,
-                   "peak": PEAK, "lle": LLE, "parallelism": "single GPU"},
-        "roofline": {"bound": "valu_fp64", "kernel": "k_lle64", "achieved": achieved,
-                     "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / (FP64_VALU_PEAK / 1e12), "traffic": None,
-                     "flop_per_launch": flop_launch, "avg_launch_ms": avg_lle},
-        "kernels_ms_per_step": {"lle": lle_ms / args.steps, "bif_peaks": pk_ms / args.steps,
-                                "dbscan": db_ms / args.steps},
-    }
+    d_peak = bh.DeviceArray((cells,), np.int32)
+    d_lle = bh.DeviceArray((cells,), np.float64)
+    out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (a, c) parameter lattice of the Rossler system"}
+
+    if args.workload == "plain":
+        for _ in range(args.warmup):
+            plain_map(bh, n, d_peak, d_lle)
+        bh.synchronize()
+        bh.set_timing(True)
+        bh.reset_kernel_times()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plain_map(bh, n, d_peak, d_lle)
+        bh.synchronize()
+        dt = time.perf_counter() - t0
+        ms = dt * 1000 / args.steps
+        out.update(value=cells / (ms / 1000), unit="grid points/s", ms_per_step=ms,
+                   config={"workload": f"plain FP64 2D LLE + DBSCAN peak-count map, Rossler {n}x{n}",
+                           "grid": n, "ranges": RANGES, "h": H, "peak": PEAK, "lle": LLE})
+        out["roofline"] = roofline(bh, cells)
+        out["kernels_ms_per_step"] = {k: bh.kernel_time(k)[0] / args.steps
+                                      for k in ("lle_peaks", "dbscan")}
+    else:
+        # reference: the plain FP64 map of the same lattice, timed once
+        bh.set_timing(True)
+        bh.reset_kernel_times()
+        bh.synchronize()
+        t0 = time.perf_counter()
+        plain_map(bh, n, d_peak, d_lle)
+        bh.synchronize()
+        plain_ms = (time.perf_counter() - t0) * 1000
+        rf = roofline(bh, cells)
+        ref_peak, ref_lle = d_peak.to_host(), d_lle.to_host()
+        t_cls, t_ch = regimes(np, ref_peak, ref_lle)
+        bh.set_timing(False)
+        d_res = bh.DeviceArray((cells,), np.uint8)
+        for w in range(args.warmup):
+            surrogate_map(bh, n, args.coarse, d_peak, d_lle, d_res, 100 + w, args.conf,
+                          args.validation, args.batch)
+        bh.synchronize()
+        stats = []
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            stats.append(surrogate_map(bh, n, args.coarse, d_peak, d_lle, d_res, s, args.conf,
+                                       args.validation, args.batch))
+        bh.synchronize()
+        ms = (time.perf_counter() - t0) * 1000 / args.steps
+        s_cls, s_ch = regimes(np, d_peak.to_host(), d_lle.to_host())
+        agree = float(((s_cls == t_cls) & (s_ch == t_ch)).mean())
+        last = stats[-1]
+        out.update(value=cells / (ms / 1000), unit="grid points/s", ms_per_step=ms,
+                   config={"workload": f"surrogate-refined 2D LLE + DBSCAN peak-count map, "
+                                       f"Rossler {n}x{n} (coarse {args.coarse}x{args.coarse}, "
+                                       f"20 epochs, conf < {args.conf} or class/regime "
+                                       f"boundary re-solved in FP64)",
+                           "grid": n, "coarse": args.coarse, "ranges": RANGES, "h": H,
+                           "peak": PEAK, "lle": LLE, "chaos_threshold": CHAOS})
+        out["surrogate"] = {
+            "regime_agreement": agree,
+            "reference_boundary_fraction": boundary_fraction(np, t_cls, t_ch, n),
+            "batch": args.batch,
+            "plain_fp64_ms": plain_ms,
+            "plain_fp64_value": cells / (plain_ms / 1000),
+            "speedup_vs_plain_fp64": plain_ms / ms,
+            "resolved_fraction": last["resolved_frac"],
+            "phase_ms": {k: last[k] for k in ("ms_coarse", "ms_train", "ms_infer", "ms_resolve")},
+            "surrogate_on_flagged": {"class_acc": last["flagged_class_acc"],
+                                     "regime_acc": last["flagged_regime_acc"]},
+            "surrogate_on_validation_sample": {"cells": last["validation"],
+                                               "class_acc": last["valid_class_acc"],
+                                               "regime_acc": last["valid_regime_acc"]},
+            "train_loss": {"ce": last["loss_ce"], "mse": last["loss_mse"]},
+        }
+        out["roofline"] = rf
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_rows)
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_rows if args.workload == "plain" else 1)
     print(json.dumps(out), flush=True)
 
 

@@ -105,6 +105,17 @@ def _declare(L):
         "bifhip_set_graph": [_I],
         "bifhip_lle_peak_map_device": [_I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I, _I, _D,
                                        _D, _D, _D, _D, _VP, _VP],
+        "bifhip_surrogate_map_device": [_I, _I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I,
+                                        _I, _D, _D, _D, _D, _D, _I, _I, _D, C.c_ulonglong,
+                                        _D, _D, _D, _VP, _VP, _VP, _PD],
+        "bifhip_mlp_param_count": [],
+        "bifhip_mlp_train_device": [_I, _VP, _VP, _I, _I, _D, _D, C.c_ulonglong, _D, _D, _I,
+                                    C.POINTER(C.c_float)],
+        "bifhip_mlp_infer_device": [_I, _D, _D, _D, _VP, _VP, _VP, _VP],
+        "bifhip_mlp_get_params": [C.POINTER(C.c_float), _I, C.POINTER(C.c_float), _I],
+        "bifhip_mlp_gradient": [_I, C.POINTER(C.c_float), C.POINTER(C.c_float), _PI, _PI,
+                                C.POINTER(C.c_float), _I, _D, _D, C.POINTER(C.c_float),
+                                C.POINTER(C.c_float), C.POINTER(C.c_float)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -434,6 +445,72 @@ def lle_peak_map_device(d_peak, d_lle, nPts, h, initialConditions, ranges, indic
                                              len(v), peak_tmax, peak_transient, writableVar,
                                              maxAmountOfPeaks, dbscan_eps, lle_tmax, NT,
                                              lle_transient, lle_eps, d_peak, d_lle))
+
+
+# ------------------------------------------------------- surrogate
+
+SSTAT = ["ms_total", "ms_coarse", "ms_train", "ms_infer", "ms_resolve", "resolved",
+         "resolved_frac", "validation", "flagged_class_acc", "flagged_regime_acc",
+         "flagged_lle_mae", "valid_class_acc", "valid_regime_acc", "valid_lle_mae",
+         "loss_ce", "loss_mse", "lle_mean", "lle_sd"]
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def surrogate_map_device(d_peak, d_lle, d_resolved, n_fine, n_coarse, h, initialConditions,
+                         ranges, indicesOfMutVars, values, peak_tmax, peak_transient,
+                         writableVar, maxAmountOfPeaks, dbscan_eps, lle_tmax, NT, lle_transient,
+                         lle_eps, epochs=20, batch=1024, lr=1e-3, seed=0, conf=0.95,
+                         chaos=0.01, validation=0.0):
+    ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    st = np.zeros(len(SSTAT))
+    _check(load().bifhip_surrogate_map_device(n_fine, n_coarse, h, len(ic), _pd(ic), _pd(r),
+                                              _pi(m), _pd(v), len(v), peak_tmax, peak_transient,
+                                              writableVar, maxAmountOfPeaks, dbscan_eps, lle_tmax,
+                                              NT, lle_transient, lle_eps, epochs, batch, lr, seed,
+                                              conf, chaos, validation, d_peak, d_lle, d_resolved,
+                                              _pd(st)))
+    return dict(zip(SSTAT, st.tolist()))
+
+
+def mlp_param_count():
+    return load().bifhip_mlp_param_count()
+
+
+def mlp_train_device(n_lat, d_cls, d_lle, epochs, batch, lr, w_lle, seed, sigma, chaos_std=0.0,
+                     graph=True):
+    loss = np.zeros(3, dtype=np.float32)
+    _check(load().bifhip_mlp_train_device(n_lat, d_cls, d_lle, epochs, batch, lr, w_lle, seed,
+                                          sigma, chaos_std, 1 if graph else 0, _pf(loss)))
+    return loss
+
+
+def mlp_infer_device(n_lat, mean, sd, chaos_thr, d_cls, d_chaos, d_conf, d_lle):
+    _check(load().bifhip_mlp_infer_device(n_lat, mean, sd, chaos_thr, d_cls, d_chaos, d_conf,
+                                          d_lle))
+
+
+def mlp_get_params():
+    p = np.zeros(mlp_param_count(), dtype=np.float32)
+    f = np.zeros(64, dtype=np.float32)
+    _check(load().bifhip_mlp_get_params(_pf(p), len(p), _pf(f), len(f)))
+    return p, f
+
+
+def mlp_gradient(n_lat, params, fourier, cells, cls, lle, w_lle=1.0, chaos_std=0.0):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    fourier = np.ascontiguousarray(fourier, dtype=np.float32)
+    cells, cls = _i32(cells), _i32(cls)
+    lle = np.ascontiguousarray(lle, dtype=np.float32)
+    g = np.zeros_like(params)
+    loss = np.zeros(3, dtype=np.float32)
+    out = np.zeros((len(cells), 32), dtype=np.float32)
+    _check(load().bifhip_mlp_gradient(n_lat, _pf(params), _pf(fourier), _pi(cells), _pi(cls),
+                                      _pf(lle), len(cells), w_lle, chaos_std, _pf(g), _pf(loss),
+                                      _pf(out)))
+    return g, loss, out
 
 
 # ------------------------------------------------------- uneven LS tiles

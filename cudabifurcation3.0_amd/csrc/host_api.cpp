@@ -597,6 +597,24 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
     return BIFHIP_OK;
 }
 
+// Plain FP64 fused LLE + peak count of listed cells (surrogate pipeline):
+// cells = h_list/d_list (flattened indices on an nPts x nPts lattice), or
+// base..base+n when both lists are null.
+int surrogate_resolve(int nPts, double h, int nIC, const double* ic, const double* ranges,
+                      const int* mut, const double* values, int nVal, double peakTMax,
+                      double peakTransient, int wv, int maxPeaks, double dbEps, double lleTMax,
+                      double NT, double lleTransient, double lleEps, const int* h_list,
+                      const int* d_list, int n, double* d_lle, int* d_peak)
+{
+    PeakMapSpec S;
+    RC_TRY(peak_spec(S, 2, peakTMax, nPts, h, nIC, ic, ranges, mut, wv, 0.0, peakTransient,
+                     values, nVal, 1));
+    LyapSpec L;
+    RC_TRY(lyap_spec(L, 2, lleTMax, NT, nPts, h, lleEps, ic, nIC, ranges, mut, 0.0, lleTransient,
+                     values, nVal));
+    return fused_cells_device(S, L, maxPeaks, dbEps, h_list, d_list, 0, n, d_lle, d_peak);
+}
+
 // ------------------------------------------------------------ CSV output
 
 struct Out {

@@ -1,0 +1,643 @@
+// host_mlp.cpp -- surrogate-refined LLE + peak-count map.
+//
+// 1. coarse map: the fused FP64 kernel on an nCoarse x nCoarse lattice;
+// 2. training: 64 Fourier features -> 3 x 128 ReLU -> head (LLE + 16
+//    peak-count classes), BF16 MFMA products with FP32 accumulation
+//    (k_mlp.hip), Adam on FP32 master weights; one epoch is captured as a
+//    HIP graph and replayed;
+// 3. inference of the nFine x nFine lattice in chunks;
+// 4. cells with low confidence or a neighbour of another class / LLE regime
+//    are re-solved with the plain FP64 fused kernel, which also scores the
+//    surrogate on them; a seeded sample of the remaining cells is re-solved
+//    as validation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "bifhip.h"
+#include "host_internal.h"
+
+namespace bifhip {
+typedef __bf16 bf16;
+hipError_t launch_gemm(int epi, int M, int N, int K, const bf16* A, long long sAm, long long sAk,
+                       const bf16* B, long long sBk, long long sBn, const float* bias,
+                       const bf16* mask, long long sMm, void* C, long long sCm, hipStream_t st,
+                       int splits = 1);
+hipError_t launch_sum_splits(int n, int splits, const float* part, float* out, hipStream_t st);
+hipError_t launch_features(int rows, int n, const int* list, long long base, const float* Bff,
+                           bf16* X, hipStream_t st);
+hipError_t launch_loss_grad(int rows, const int* list, const float* out, const int* cls,
+                            const float* lle, float wLle, float chaosStd, bf16* dOut, float* loss,
+                            hipStream_t st);
+hipError_t launch_colsum(int rows, int N, const bf16* g, float* part, int splits, float* out,
+                         hipStream_t st);
+hipError_t launch_adam(int n, float* p, float* m, float* v, const float* g,
+                       const unsigned char* keep, const int* bfIndex, bf16* wbf, float lr,
+                       float b1, float b2, float eps, const int* step, hipStream_t st);
+hipError_t launch_step_inc(int* step, hipStream_t st);
+hipError_t launch_fill_f32(int n, float* p, float v, hipStream_t st);
+hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, float chaosThr,
+                             unsigned char* cls, unsigned char* chaos, float* conf, float* lle,
+                             hipStream_t st);
+hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
+                       const float* conf, float confThr, int* list, int* count,
+                       unsigned char* flag, hipStream_t st);
+hipError_t launch_scatter(int n, const int* list, const int* peak, const double* lle,
+                          int* peakMap, double* lleMap, hipStream_t st);
+hipError_t launch_fill_pred(long long n, const unsigned char* cls, const float* lle,
+                            int* peakMap, double* lleMap, hipStream_t st);
+namespace host {
+struct PeakMapSpec;
+struct LyapSpec;
+int surrogate_resolve(int nPts, double h, int nIC, const double* ic, const double* ranges,
+                      const int* mut, const double* values, int nVal, double peakTMax,
+                      double peakTransient, int wv, int maxPeaks, double dbEps, double lleTMax,
+                      double NT, double lleTransient, double lleEps, const int* h_list,
+                      const int* d_list, int n, double* d_lle, int* d_peak);
+}  // namespace host
+}  // namespace bifhip
+
+using namespace bifhip;
+using namespace bifhip::host;
+
+namespace {
+
+constexpr int kFeat = 64, kHid = 128, kHead = 32, kClasses = 16;
+constexpr int kLiveHead = 2 + kClasses;   // LLE, 16 classes, chaos logit
+constexpr int kMaxSplits = 32;
+
+// K-splits of the weight-gradient products over a batch of `rows`
+int splits_for(int rows) { return std::max(1, std::min(kMaxSplits, rows / 256)); }
+
+// flat FP32 parameter layout (row-major [out][in] matrices)
+constexpr int oW1 = 0, oB1 = oW1 + kHid * kFeat;
+constexpr int oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid;
+constexpr int oW3 = oB2 + kHid, oB3 = oW3 + kHid * kHid;
+constexpr int oWh = oB3 + kHid, oBh = oWh + kHead * kHid;
+constexpr int kParams = oBh + kHead;
+// BF16 weight copies
+constexpr int bW1 = 0, bW2 = bW1 + kHid * kFeat, bW3 = bW2 + kHid * kHid,
+              bWh = bW3 + kHid * kHid, kBf = bWh + kHead * kHid;
+
+struct Net {
+    float *p, *m, *v, *g, *bff, *loss, *part;
+    bf16* wbf;
+    unsigned char* keep;
+    int *bfIndex, *step;
+};
+
+int net_alloc(Net& N)
+{
+    RC_TRY(pool_get("mlp_p", kParams, &N.p));
+    RC_TRY(pool_get("mlp_m", kParams, &N.m));
+    RC_TRY(pool_get("mlp_v", kParams, &N.v));
+    RC_TRY(pool_get("mlp_g", kParams, &N.g));
+    RC_TRY(pool_get("mlp_part", (size_t)kMaxSplits * kHid * kHid, &N.part));
+    RC_TRY(pool_get("mlp_bff", kFeat, &N.bff));
+    RC_TRY(pool_get("mlp_loss", 3, &N.loss));
+    RC_TRY(pool_get("mlp_wbf", kBf, &N.wbf));
+    RC_TRY(pool_get("mlp_keep", kParams, &N.keep));
+    RC_TRY(pool_get("mlp_bfidx", kParams, &N.bfIndex));
+    RC_TRY(pool_get("mlp_step", 1, &N.step));
+    return BIFHIP_OK;
+}
+
+// seeded initialisation: He-uniform weights, zero biases, Fourier matrix
+// N(0, sigma^2); padded head rows (17..31) stay zero
+int net_init(Net& N, unsigned long long seed, float sigma)
+{
+    std::mt19937_64 rng(seed);
+    auto u01 = [&]() { return (double)(rng() >> 11) * (1.0 / 9007199254740992.0); };
+    std::vector<float> p(kParams, 0.0f);
+    std::vector<unsigned char> keep(kParams, 1);
+    std::vector<int> bfi(kParams, -1);
+    auto fill = [&](int off, int rows, int cols, int bfoff, double scale, int liveRows) {
+        const double lim = scale * std::sqrt(6.0 / cols);
+        for (int r = 0; r < rows; ++r)
+            for (int c = 0; c < cols; ++c) {
+                const int i = off + r * cols + c;
+                p[i] = r < liveRows ? (float)((2 * u01() - 1) * lim) : 0.0f;
+                keep[i] = r < liveRows;
+                bfi[i] = bfoff + r * cols + c;
+            }
+    };
+    fill(oW1, kHid, kFeat, bW1, 1.0, kHid);
+    fill(oW2, kHid, kHid, bW2, 1.0, kHid);
+    fill(oW3, kHid, kHid, bW3, 1.0, kHid);
+    fill(oWh, kHead, kHid, bWh, 0.5, kLiveHead);
+    for (int i = kLiveHead; i < kHead; ++i) keep[oBh + i] = 0;
+    std::vector<float> bff(kFeat);
+    for (int i = 0; i < kFeat; i += 2) {
+        const double r = std::sqrt(-2.0 * std::log(1.0 - u01()));
+        const double th = 2.0 * M_PI * u01();
+        bff[i] = (float)(sigma * r * std::cos(th));
+        bff[i + 1] = (float)(sigma * r * std::sin(th));
+    }
+    std::vector<bf16> wbf(kBf);
+    for (int i = 0; i < kParams; ++i)
+        if (bfi[i] >= 0) wbf[bfi[i]] = (bf16)p[i];
+    HIP_TRY(hipMemcpy(N.p, p.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(N.m, 0, sizeof(float) * kParams));
+    HIP_TRY(hipMemset(N.v, 0, sizeof(float) * kParams));
+    HIP_TRY(hipMemcpy(N.bff, bff.data(), sizeof(float) * kFeat, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(N.wbf, wbf.data(), sizeof(bf16) * kBf, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(N.keep, keep.data(), kParams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(N.bfIndex, bfi.data(), sizeof(int) * kParams, hipMemcpyHostToDevice));
+    const int zero = 0;
+    HIP_TRY(hipMemcpy(N.step, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return BIFHIP_OK;
+}
+
+struct Acts {
+    bf16 *x0, *h1, *h2, *h3, *dout, *d3, *d2, *d1;
+    float* out;
+};
+
+int acts_alloc(Acts& A, int rows, const char* tag)
+{
+    std::string t(tag);
+    RC_TRY(pool_get((t + "x0").c_str(), (size_t)rows * kFeat, &A.x0));
+    RC_TRY(pool_get((t + "h1").c_str(), (size_t)rows * kHid, &A.h1));
+    RC_TRY(pool_get((t + "h2").c_str(), (size_t)rows * kHid, &A.h2));
+    RC_TRY(pool_get((t + "h3").c_str(), (size_t)rows * kHid, &A.h3));
+    RC_TRY(pool_get((t + "out").c_str(), (size_t)rows * kHead, &A.out));
+    A.dout = A.d3 = A.d2 = A.d1 = nullptr;
+    return BIFHIP_OK;
+}
+
+int acts_alloc_grad(Acts& A, int rows)
+{
+    RC_TRY(pool_get("tr_dout", (size_t)rows * kHead, &A.dout));
+    RC_TRY(pool_get("tr_d3", (size_t)rows * kHid, &A.d3));
+    RC_TRY(pool_get("tr_d2", (size_t)rows * kHid, &A.d2));
+    RC_TRY(pool_get("tr_d1", (size_t)rows * kHid, &A.d1));
+    return BIFHIP_OK;
+}
+
+// forward of `rows` lattice cells (list or base..base+rows) into A.out
+int forward(const Net& N, Acts& A, int rows, int nLat, const int* list, long long base,
+            hipStream_t s)
+{
+    HIP_TRY(launch_features(rows, nLat, list, base, N.bff, A.x0, s));
+    HIP_TRY(launch_gemm(0, rows, kHid, kFeat, A.x0, kFeat, 1, N.wbf + bW1, 1, kFeat, N.p + oB1,
+                        nullptr, 0, A.h1, kHid, s));
+    HIP_TRY(launch_gemm(0, rows, kHid, kHid, A.h1, kHid, 1, N.wbf + bW2, 1, kHid, N.p + oB2,
+                        nullptr, 0, A.h2, kHid, s));
+    HIP_TRY(launch_gemm(0, rows, kHid, kHid, A.h2, kHid, 1, N.wbf + bW3, 1, kHid, N.p + oB3,
+                        nullptr, 0, A.h3, kHid, s));
+    HIP_TRY(launch_gemm(1, rows, kHead, kHid, A.h3, kHid, 1, N.wbf + bWh, 1, kHid, N.p + oBh,
+                        nullptr, 0, A.out, kHead, s));
+    return BIFHIP_OK;
+}
+
+// backward of the loss gradient A.dout into N.g (weights and biases)
+// weight gradient dW[M x Nn] = dZ^T X over the batch, split over K = rows
+// into partials (fixed-order sum keeps it deterministic)
+int weight_grad(const Net& N, int M, int Nn, int rows, const bf16* dz, int ldz, const bf16* x,
+                int ldx, float* out, hipStream_t s)
+{
+    const int S = splits_for(rows);
+    HIP_TRY(launch_gemm(3, M, Nn, rows, dz, 1, ldz, x, ldx, 1, nullptr, nullptr, 0,
+                        S > 1 ? N.part : out, Nn, s, S));
+    if (S > 1) HIP_TRY(launch_sum_splits(M * Nn, S, N.part, out, s));
+    return BIFHIP_OK;
+}
+
+int bias_grad(const Net& N, int rows, int Nn, const bf16* dz, float* out, hipStream_t s)
+{
+    HIP_TRY(launch_colsum(rows, Nn, dz, N.part, splits_for(rows), out, s));
+    return BIFHIP_OK;
+}
+
+int backward(const Net& N, Acts& A, int rows, hipStream_t s)
+{
+    // head: dWh = dOut^T H3, dbh = colsum(dOut), dH3 = (dOut Wh) * (H3 > 0)
+    RC_TRY(weight_grad(N, kHead, kHid, rows, A.dout, kHead, A.h3, kHid, N.g + oWh, s));
+    RC_TRY(bias_grad(N, rows, kHead, A.dout, N.g + oBh, s));
+    HIP_TRY(launch_gemm(2, rows, kHid, kHead, A.dout, kHead, 1, N.wbf + bWh, kHid, 1, nullptr,
+                        A.h3, kHid, A.d3, kHid, s));
+    // layer 3
+    RC_TRY(weight_grad(N, kHid, kHid, rows, A.d3, kHid, A.h2, kHid, N.g + oW3, s));
+    RC_TRY(bias_grad(N, rows, kHid, A.d3, N.g + oB3, s));
+    HIP_TRY(launch_gemm(2, rows, kHid, kHid, A.d3, kHid, 1, N.wbf + bW3, kHid, 1, nullptr, A.h2,
+                        kHid, A.d2, kHid, s));
+    // layer 2
+    RC_TRY(weight_grad(N, kHid, kHid, rows, A.d2, kHid, A.h1, kHid, N.g + oW2, s));
+    RC_TRY(bias_grad(N, rows, kHid, A.d2, N.g + oB2, s));
+    HIP_TRY(launch_gemm(2, rows, kHid, kHid, A.d2, kHid, 1, N.wbf + bW2, kHid, 1, nullptr, A.h1,
+                        kHid, A.d1, kHid, s));
+    // layer 1 (no input gradient needed)
+    RC_TRY(weight_grad(N, kHid, kFeat, rows, A.d1, kHid, A.x0, kFeat, N.g + oW1, s));
+    RC_TRY(bias_grad(N, rows, kHid, A.d1, N.g + oB1, s));
+    return BIFHIP_OK;
+}
+
+struct TrainCfg {
+    int epochs = 20, batch = 1024;
+    float lr = 1e-3f, wLle = 1.0f, chaosStd = 0.0f;
+    unsigned long long seed = 0;
+    bool graph = true;
+};
+
+// one optimisation step on rows of the permuted sample list
+int train_step(const Net& N, Acts& A, const TrainCfg& T, int nLat, const int* list,
+               const int* cls, const float* lle, hipStream_t s)
+{
+    RC_TRY(forward(N, A, T.batch, nLat, list, 0, s));
+    HIP_TRY(launch_loss_grad(T.batch, list, A.out, cls, lle, T.wLle, T.chaosStd, A.dout, N.loss,
+                             s));
+    RC_TRY(backward(N, A, T.batch, s));
+    HIP_TRY(launch_step_inc(N.step, s));
+    HIP_TRY(launch_adam(kParams, N.p, N.m, N.v, N.g, N.keep, N.bfIndex, N.wbf, T.lr, 0.9f,
+                        0.999f, 1e-8f, N.step, s));
+    return BIFHIP_OK;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+        .count();
+}
+
+}  // namespace
+
+extern "C" {
+
+// Train the surrogate on a labelled lattice already in device memory:
+// d_cls[nLat^2] class 0..15, d_lle[nLat^2] standardised LLE (NaN = no
+// target); chaosStd = the chaos threshold in standardised units.  Mean
+// losses (CE, MSE, chaos BCE) of the last epoch to loss_out[3] when not NULL.
+int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int epochs,
+                            int batch, double lr, double wLle, unsigned long long seed,
+                            double fourierSigma, double chaosStd, int useGraph, float* loss_out)
+{
+    RC_TRY(ensure_device());
+    const int n = nLat * nLat;
+    if (nLat < 2 || batch < 16 || batch > n || epochs < 0 || !d_cls || !d_lle)
+        return fail(BIFHIP_E_ARG, "bad training arguments");
+    Net N;
+    RC_TRY(net_alloc(N));
+    RC_TRY(net_init(N, seed, (float)fourierSigma));
+    Acts A;
+    RC_TRY(acts_alloc(A, batch, "tr_"));
+    RC_TRY(acts_alloc_grad(A, batch));
+    int* d_perm;
+    const int steps = n / batch;
+    RC_TRY(pool_get("tr_perm", (size_t)steps * batch, &d_perm));
+    TrainCfg T;
+    T.epochs = epochs;
+    T.batch = batch;
+    T.lr = (float)lr;
+    T.wLle = (float)wLle;
+    T.chaosStd = (float)chaosStd;
+    T.seed = seed;
+    hipStream_t s = g.stream;
+    std::mt19937_64 rng(seed ^ 0x9e3779b97f4a7c15ull);
+    std::vector<int> perm(n);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cs = nullptr;
+    if (useGraph) HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int rc = BIFHIP_OK;
+    for (int e = 0; e < epochs && rc == BIFHIP_OK; ++e) {
+        std::iota(perm.begin(), perm.end(), 0);
+        for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[rng() % (uint64_t)(i + 1)]);
+        if (useGraph) {
+            rc = hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * steps * batch,
+                                hipMemcpyHostToDevice, cs) == hipSuccess ? BIFHIP_OK
+                                                                         : fail(1, "perm upload");
+            if (rc) break;
+            if (!exec) {
+                if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                    rc = fail(1, "graph capture failed");
+                    break;
+                }
+                int crc = BIFHIP_OK;
+                if (launch_fill_f32(3, N.loss, 0.0f, cs) != hipSuccess) crc = fail(1, "fill");
+                for (int k = 0; k < steps && crc == BIFHIP_OK; ++k)
+                    crc = train_step(N, A, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, cs);
+                hipError_t ce = hipStreamEndCapture(cs, &graph);
+                if (crc != BIFHIP_OK) { rc = crc; break; }
+                if (ce != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
+                                            hipSuccess) {
+                    rc = fail(1, "graph instantiation failed");
+                    break;
+                }
+            }
+            Timed t("mlp_train", cs);
+            if (hipGraphLaunch(exec, cs) != hipSuccess) rc = fail(1, "graph launch failed");
+        } else {
+            HIP_TRY(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * steps * batch,
+                                   hipMemcpyHostToDevice, s));
+            Timed t("mlp_train", s);
+            HIP_TRY(launch_fill_f32(3, N.loss, 0.0f, s));
+            for (int k = 0; k < steps && rc == BIFHIP_OK; ++k)
+                rc = train_step(N, A, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, s);
+        }
+    }
+    if (cs) {
+        (void)hipStreamSynchronize(cs);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipStreamDestroy(cs);
+    }
+    RC_TRY(rc);
+    HIP_TRY(hipStreamSynchronize(s));
+    if (loss_out) {
+        HIP_TRY(hipMemcpy(loss_out, N.loss, sizeof(float) * 3, hipMemcpyDeviceToHost));
+        if (steps > 0) {
+            loss_out[0] /= steps;
+            loss_out[1] /= steps;
+            loss_out[2] /= steps;
+        }
+    }
+    resolve_timers();
+    return BIFHIP_OK;
+}
+
+// Surrogate inference of a whole nLat x nLat lattice: class, chaos regime,
+// confidence and LLE (de-standardised with mean/sd) per cell.
+int bifhip_mlp_infer_device(int nLat, double lleMean, double lleSd, double chaosThr,
+                            unsigned char* d_cls, unsigned char* d_chaos, float* d_conf,
+                            float* d_lle)
+{
+    RC_TRY(ensure_device());
+    Net N;
+    RC_TRY(net_alloc(N));
+    const long long n = (long long)nLat * nLat;
+    const int chunk = (int)std::min<long long>(n, 1 << 21);
+    Acts A;
+    RC_TRY(acts_alloc(A, chunk, "inf_"));
+    for (long long off = 0; off < n; off += chunk) {
+        const int rows = (int)std::min<long long>(chunk, n - off);
+        Timed t("mlp_infer");
+        RC_TRY(forward(N, A, rows, nLat, nullptr, off, g.stream));
+        HIP_TRY(launch_infer_post(rows, A.out, (float)lleMean, (float)lleSd, (float)chaosThr,
+                                  d_cls + off, d_chaos + off, d_conf + off, d_lle + off,
+                                  g.stream));
+    }
+    return finish();
+}
+
+// Parameters of the current network (FP32, layout W1 b1 W2 b2 W3 b3 Wh bh)
+// and the Fourier matrix, for checks against a CPU reference.
+int bifhip_mlp_get_params(float* params, int nParams, float* fourier, int nFourier)
+{
+    if (nParams != kParams || nFourier != kFeat) return fail(BIFHIP_E_ARG, "size mismatch");
+    Net N;
+    RC_TRY(net_alloc(N));
+    HIP_TRY(hipMemcpy(params, N.p, sizeof(float) * kParams, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(fourier, N.bff, sizeof(float) * kFeat, hipMemcpyDeviceToHost));
+    return BIFHIP_OK;
+}
+
+// One forward + backward pass of given parameters on given lattice cells;
+// gradients (same layout as the parameters) and the three losses back to
+// the host.  Used to check the MFMA kernels against a CPU reference.
+int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, const int* cells,
+                        const int* cls, const float* lle, int rows, double wLle, double chaosStd,
+                        float* grads, float* losses, float* outputs)
+{
+    RC_TRY(ensure_device());
+    if (rows < 1 || !params || !fourier || !cells || !cls || !lle || !grads)
+        return fail(BIFHIP_E_ARG, "bad gradient arguments");
+    Net N;
+    RC_TRY(net_alloc(N));
+    std::vector<int> bfi(kParams, -1);
+    std::vector<bf16> wbf(kBf);
+    auto idx = [&](int off, int rows_, int cols, int bfoff) {
+        for (int i = 0; i < rows_ * cols; ++i) bfi[off + i] = bfoff + i;
+    };
+    idx(oW1, kHid, kFeat, bW1);
+    idx(oW2, kHid, kHid, bW2);
+    idx(oW3, kHid, kHid, bW3);
+    idx(oWh, kHead, kHid, bWh);
+    for (int i = 0; i < kParams; ++i)
+        if (bfi[i] >= 0) wbf[bfi[i]] = (bf16)params[i];
+    HIP_TRY(hipMemcpy(N.p, params, sizeof(float) * kParams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(N.bff, fourier, sizeof(float) * kFeat, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(N.wbf, wbf.data(), sizeof(bf16) * kBf, hipMemcpyHostToDevice));
+    Acts A;
+    RC_TRY(acts_alloc(A, rows, "gr_"));
+    RC_TRY(acts_alloc_grad(A, rows));
+    DBuf dc, dcls, dl, seq;
+    HIP_TRY(dc.alloc(sizeof(int) * rows));
+    HIP_TRY(dcls.alloc(sizeof(int) * rows));
+    HIP_TRY(dl.alloc(sizeof(float) * rows));
+    HIP_TRY(seq.alloc(sizeof(int) * rows));
+    std::vector<int> iota(rows);
+    std::iota(iota.begin(), iota.end(), 0);
+    HIP_TRY(hipMemcpy(dc.p, cells, sizeof(int) * rows, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dcls.p, cls, sizeof(int) * rows, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dl.p, lle, sizeof(float) * rows, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(seq.p, iota.data(), sizeof(int) * rows, hipMemcpyHostToDevice));
+    hipStream_t s = g.stream;
+    HIP_TRY(launch_fill_f32(3, N.loss, 0.0f, s));
+    RC_TRY(forward(N, A, rows, nLat, dc.as<int>(), 0, s));
+    HIP_TRY(launch_loss_grad(rows, seq.as<int>(), A.out, dcls.as<int>(), dl.as<float>(),
+                             (float)wLle, (float)chaosStd, A.dout, N.loss, s));
+    RC_TRY(backward(N, A, rows, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(grads, N.g, sizeof(float) * kParams, hipMemcpyDeviceToHost));
+    if (losses) HIP_TRY(hipMemcpy(losses, N.loss, sizeof(float) * 3, hipMemcpyDeviceToHost));
+    if (outputs)
+        HIP_TRY(hipMemcpy(outputs, A.out, sizeof(float) * rows * kHead, hipMemcpyDeviceToHost));
+    return BIFHIP_OK;
+}
+
+int bifhip_mlp_param_count(void) { return kParams; }
+
+int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const double* ic,
+                                const double* ranges, const int* mut, const double* values,
+                                int nVal, double peakTMax, double peakTransient, int wv,
+                                int maxPeaks, double dbEps, double lleTMax, double NT,
+                                double lleTransient, double lleEps, int epochs, int batch,
+                                double lr, unsigned long long seed, double confThreshold,
+                                double chaosThreshold, double validationFraction, int* d_peak,
+                                double* d_lle, unsigned char* d_resolved, double* stats)
+{
+    RC_TRY(ensure_device());
+    if (nFine < 2 || nCoarse < 2 || !d_peak || !d_lle) return fail(BIFHIP_E_ARG, "bad grid");
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = g.stream;
+    double st[BIFHIP_SSTAT_COUNT];
+    std::fill(st, st + BIFHIP_SSTAT_COUNT, 0.0);
+
+    // 1. coarse map with the plain FP64 fused kernel
+    const int nc = nCoarse * nCoarse;
+    int* d_cpeak;
+    double* d_clle;
+    RC_TRY(pool_get("sg_cpeak", nc, &d_cpeak));
+    RC_TRY(pool_get("sg_clle", nc, &d_clle));
+    RC_TRY(surrogate_resolve(nCoarse, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
+                             peakTransient, wv, maxPeaks, dbEps, lleTMax, NT, lleTransient,
+                             lleEps, nullptr, nullptr, nc, d_clle, d_cpeak));
+    std::vector<int> cpeak(nc);
+    std::vector<double> clle(nc);
+    HIP_TRY(hipMemcpy(cpeak.data(), d_cpeak, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(clle.data(), d_clle, sizeof(double) * nc, hipMemcpyDeviceToHost));
+    st[BIFHIP_SSTAT_MS_COARSE] = ms_since(t0);
+
+    // labels: class = clamp(count, 1, 16) - 1; LLE standardised over finite cells
+    double sum = 0, sum2 = 0;
+    long long nfin = 0;
+    for (int i = 0; i < nc; ++i)
+        if (std::isfinite(clle[i])) {
+            sum += clle[i];
+            sum2 += clle[i] * clle[i];
+            ++nfin;
+        }
+    const double mean = nfin ? sum / nfin : 0.0;
+    const double sd = nfin ? std::max(1e-6, std::sqrt(std::max(0.0, sum2 / nfin - mean * mean)))
+                           : 1.0;
+    std::vector<int> cls(nc);
+    std::vector<float> lstd(nc);
+    for (int i = 0; i < nc; ++i) {
+        cls[i] = std::min(kClasses, std::max(1, cpeak[i])) - 1;
+        lstd[i] = std::isfinite(clle[i]) ? (float)((clle[i] - mean) / sd) : NAN;
+    }
+    int* d_cls;
+    float* d_lstd;
+    RC_TRY(pool_get("sg_cls", nc, &d_cls));
+    RC_TRY(pool_get("sg_lstd", nc, &d_lstd));
+    HIP_TRY(hipMemcpy(d_cls, cls.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_lstd, lstd.data(), sizeof(float) * nc, hipMemcpyHostToDevice));
+
+    // 2. training
+    const auto t1 = std::chrono::steady_clock::now();
+    float losses[3] = {0, 0, 0};
+    const double chaosStd = (chaosThreshold - mean) / sd;
+    RC_TRY(bifhip_mlp_train_device(nCoarse, d_cls, d_lstd, epochs, std::min(batch, nc), lr, 1.0,
+                                   seed, 6.0, chaosStd, 1, losses));
+    st[BIFHIP_SSTAT_MS_TRAIN] = ms_since(t1);
+    st[BIFHIP_SSTAT_LOSS_CE] = losses[0];
+    st[BIFHIP_SSTAT_LOSS_MSE] = losses[1];
+
+    // 3. inference
+    const auto t2 = std::chrono::steady_clock::now();
+    const long long nf = (long long)nFine * nFine;
+    unsigned char *d_pcls, *d_pch, *d_flag;
+    float *d_conf, *d_plle;
+    RC_TRY(pool_get("sg_pcls", (size_t)nf, &d_pcls));
+    RC_TRY(pool_get("sg_flag", (size_t)nf, &d_flag));
+    RC_TRY(pool_get("sg_pch", (size_t)nf, &d_pch));
+    RC_TRY(pool_get("sg_conf", (size_t)nf, &d_conf));
+    RC_TRY(pool_get("sg_plle", (size_t)nf, &d_plle));
+    RC_TRY(bifhip_mlp_infer_device(nFine, mean, sd, chaosThreshold, d_pcls, d_pch, d_conf,
+                                   d_plle));
+    st[BIFHIP_SSTAT_MS_INFER] = ms_since(t2);
+
+    // 4. flag and re-solve
+    const auto t3 = std::chrono::steady_clock::now();
+    int *d_list, *d_count;
+    RC_TRY(pool_get("sg_list", (size_t)nf, &d_list));
+    RC_TRY(pool_get("sg_count", 1, &d_count));
+    HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), s));
+    HIP_TRY(launch_flag(nFine, d_pcls, d_pch, d_conf, (float)confThreshold, d_list, d_count,
+                        d_flag, s));
+    HIP_TRY(launch_fill_pred(nf, d_pcls, d_plle, d_peak, d_lle, s));
+    int nflag = 0;
+    HIP_TRY(hipMemcpyAsync(&nflag, d_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // validation sample of the unflagged cells (seeded)
+    std::vector<int> vlist;
+    if (validationFraction > 0) {
+        std::vector<unsigned char> flag((size_t)nf);
+        HIP_TRY(hipMemcpy(flag.data(), d_flag, (size_t)nf, hipMemcpyDeviceToHost));
+        // draw distinct unflagged cells until the sample holds
+        // fraction * (unflagged cells) of them
+        std::mt19937_64 rv(seed + 17);
+        const long long want =
+            (long long)std::llround(std::min(1.0, validationFraction) * (double)(nf - nflag));
+        long long tries = 0;
+        while ((long long)vlist.size() < want && tries < 8 * want + 64) {
+            ++tries;
+            const long long i = (long long)(rv() % (uint64_t)nf);
+            if (flag[i]) continue;
+            flag[i] = 2;   // taken
+            vlist.push_back((int)i);
+        }
+    }
+    const int nval = (int)vlist.size();
+    const int ntot = nflag + nval;
+    int* d_all;
+    RC_TRY(pool_get("sg_all", (size_t)std::max(1, ntot), &d_all));
+    HIP_TRY(hipMemcpy(d_all, d_list, sizeof(int) * nflag, hipMemcpyDeviceToDevice));
+    if (nval) HIP_TRY(hipMemcpy(d_all + nflag, vlist.data(), sizeof(int) * nval, hipMemcpyHostToDevice));
+    std::vector<int> hall(ntot);
+    HIP_TRY(hipMemcpy(hall.data(), d_all, sizeof(int) * ntot, hipMemcpyDeviceToHost));
+    int* d_rpeak;
+    double* d_rlle;
+    RC_TRY(pool_get("sg_rpeak", (size_t)std::max(1, ntot), &d_rpeak));
+    RC_TRY(pool_get("sg_rlle", (size_t)std::max(1, ntot), &d_rlle));
+    if (ntot > 0)
+        RC_TRY(surrogate_resolve(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
+                                 peakTransient, wv, maxPeaks, dbEps, lleTMax, NT, lleTransient,
+                                 lleEps, hall.data(), d_all, ntot, d_rlle, d_rpeak));
+    // score the surrogate on every re-solved cell before overwriting
+    std::vector<int> rpeak(ntot);
+    std::vector<double> rlle(ntot);
+    std::vector<unsigned char> pc(ntot), ph(ntot);
+    std::vector<float> pl(ntot);
+    if (ntot > 0) {
+        HIP_TRY(hipMemcpy(rpeak.data(), d_rpeak, sizeof(int) * ntot, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(rlle.data(), d_rlle, sizeof(double) * ntot, hipMemcpyDeviceToHost));
+        std::vector<unsigned char> allc((size_t)nf), allh((size_t)nf);
+        HIP_TRY(hipMemcpy(allh.data(), d_pch, (size_t)nf, hipMemcpyDeviceToHost));
+        std::vector<float> alll((size_t)nf);
+        HIP_TRY(hipMemcpy(allc.data(), d_pcls, (size_t)nf, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(alll.data(), d_plle, sizeof(float) * nf, hipMemcpyDeviceToHost));
+        for (int i = 0; i < ntot; ++i) {
+            pc[i] = allc[hall[i]];
+            ph[i] = allh[hall[i]];
+            pl[i] = alll[hall[i]];
+        }
+    }
+    auto score = [&](int lo, int hi, double* clsAcc, double* regAcc, double* mae) {
+        long long n = 0, okc = 0, okr = 0, nl = 0;
+        double err = 0;
+        for (int i = lo; i < hi; ++i) {
+            const int tc = std::min(kClasses, std::max(1, rpeak[i])) - 1;
+            const bool tch = rlle[i] > chaosThreshold;
+            const bool pch = ph[i] != 0;
+            ++n;
+            okc += tc == pc[i];
+            okr += (tc == pc[i]) && (tch == pch);
+            if (std::isfinite(rlle[i])) {
+                err += std::fabs(rlle[i] - pl[i]);
+                ++nl;
+            }
+        }
+        *clsAcc = n ? (double)okc / n : 1.0;
+        *regAcc = n ? (double)okr / n : 1.0;
+        *mae = nl ? err / nl : 0.0;
+    };
+    score(0, nflag, &st[BIFHIP_SSTAT_FLAGGED_CLASS_ACC], &st[BIFHIP_SSTAT_FLAGGED_REGIME_ACC],
+          &st[BIFHIP_SSTAT_FLAGGED_LLE_MAE]);
+    score(nflag, ntot, &st[BIFHIP_SSTAT_VALID_CLASS_ACC], &st[BIFHIP_SSTAT_VALID_REGIME_ACC],
+          &st[BIFHIP_SSTAT_VALID_LLE_MAE]);
+    HIP_TRY(launch_scatter(ntot, d_all, d_rpeak, d_rlle, d_peak, d_lle, s));
+    if (d_resolved) {
+        HIP_TRY(hipMemcpyAsync(d_resolved, d_flag, (size_t)nf, hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    st[BIFHIP_SSTAT_MS_RESOLVE] = ms_since(t3);
+    st[BIFHIP_SSTAT_RESOLVED] = nflag;
+    st[BIFHIP_SSTAT_VALIDATION] = nval;
+    st[BIFHIP_SSTAT_RESOLVED_FRAC] = (double)nflag / (double)nf;
+    st[BIFHIP_SSTAT_MS_TOTAL] = ms_since(t0);
+    st[BIFHIP_SSTAT_LLE_MEAN] = mean;
+    st[BIFHIP_SSTAT_LLE_SD] = sd;
+    if (stats) std::memcpy(stats, st, sizeof(st));
+    resolve_timers();
+    return BIFHIP_OK;
+}
+
+}  // extern "C"

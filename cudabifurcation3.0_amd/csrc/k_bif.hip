@@ -215,8 +215,9 @@ constexpr int kDbR = 4;                  // register sets: P <= 64*kDbR
 constexpr int kDbMaxP = kWave * kDbR;
 
 // One wavefront per cell.  Peak k lives in lane k%64 of register set k/64.
-// Neighbour row i (bits over k) lives in lane i%64 of set i/64, one
-// 64-bit word per k-set.  The label/stack machine of the reference:
+// Neighbour row i (bits over k, one 64-bit word per k-set) is built with a
+// ballot and kept in the wave's LDS slice; the label/stack machine then
+// reads one row per scan as scalar masks.  The reference's machine:
 //   U = unlabelled k (label == 0), S = stack slots holding their own index
 //   (slot k is written with value k; popping slot NumNeibor-1 yields that
 //   slot's index if set, else 0; the popped value becomes the loop
@@ -231,6 +232,7 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
                                                      int* __restrict__ overflow,
                                                      int* __restrict__ nOverflow, int listBase)
 {
+    __shared__ uint64_t s_rows[256 / kWave][kDbMaxP * kDbR];   // 8 KiB per wave
     const int cell = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
     if (cell >= nCells) return;          // wave-uniform
     const int lane = lane_id();
@@ -257,13 +259,10 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
             b[r] = k < P ? rb[k] : 0.0;
         }
         const int nsets = (P + kWave - 1) / kWave;
-        uint64_t nb[kDbR][kDbR];
-#pragma unroll
-        for (int r = 0; r < kDbR; ++r)
-#pragma unroll
-            for (int w = 0; w < kDbR; ++w) nb[r][w] = 0;
+        // this wave's neighbour rows: row i, word w at rows[i * kDbR + w]
+        uint64_t* rows = s_rows[threadIdx.x / kWave];
 
-        // neighbour rows
+        // neighbour rows: one ballot per (row, 64-wide k-set)
 #pragma unroll
         for (int ri = 0; ri < kDbR; ++ri) {
             if (ri >= nsets) break;
@@ -278,7 +277,7 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
                     const int k = w * kWave + lane;
                     const bool c = (k < P - 1) && (k != i) && near(ai, bi, a[w], b[w], T);
                     const uint64_t m = __ballot(c);
-                    if (lane == li) nb[ri][w] = m;
+                    if (lane == 0) rows[i * kDbR + w] = m;
                 }
             }
         }
@@ -323,14 +322,11 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
                 }
             }
             if (scan) {
-                const int ri = i >> 6;
-                const int li = i & 63;
 #pragma unroll
                 for (int w = 0; w < kDbR; ++w) {
-                    uint64_t row = 0;
-#pragma unroll
-                    for (int r = 0; r < kDbR; ++r)
-                        if (r == ri) row = readlane_u64(nb[r][w], li);
+                    if (w >= nsets) break;
+                    // every lane reads the same word; keep it scalar
+                    const uint64_t row = readlane_u64(rows[i * kDbR + w], 0);
                     const uint64_t m = row & U[w];
                     U[w] &= ~m;
                     S[w] |= m;

@@ -306,6 +306,72 @@ int bifhip_get_tile_config(int* blockSize, int* cellsPerLane, int* blocksPerCU);
 /* Launch the tile work through a captured HIP graph (default 1). */
 int bifhip_set_graph(int on);
 
+/* ------------------------------------------ surrogate-refined map */
+
+/* No counterpart in the reference.  A small network (64 Fourier features of
+ * (a, c) -> 3 x 128 ReLU -> LLE + 16 peak-count classes), trained on a
+ * coarse map the plain FP64 method produces and run with BF16 matrix-core
+ * products and FP32 accumulation, predicts the fine map; every cell with
+ * softmax confidence below confThreshold or a 4-neighbour of another class
+ * or LLE regime (chaotic when LLE > chaosThreshold) is re-solved by the
+ * plain FP64 fused kernel (bifhip_lle_peak_map_device semantics), which
+ * also scores the surrogate on those cells and on a validation sample of
+ * validationFraction of the rest.  Outputs: d_peak[nFine^2] (re-solved
+ * cells: DBSCAN count; others: predicted class + 1 in 1..16),
+ * d_lle[nFine^2], d_resolved[nFine^2] (1 where re-solved; may be NULL),
+ * stats[BIFHIP_SSTAT_COUNT]. */
+#define BIFHIP_SSTAT_MS_TOTAL 0
+#define BIFHIP_SSTAT_MS_COARSE 1
+#define BIFHIP_SSTAT_MS_TRAIN 2
+#define BIFHIP_SSTAT_MS_INFER 3
+#define BIFHIP_SSTAT_MS_RESOLVE 4
+#define BIFHIP_SSTAT_RESOLVED 5            /* cells re-solved (flagged)          */
+#define BIFHIP_SSTAT_RESOLVED_FRAC 6
+#define BIFHIP_SSTAT_VALIDATION 7          /* unflagged cells re-solved to score */
+#define BIFHIP_SSTAT_FLAGGED_CLASS_ACC 8   /* surrogate vs FP64 on flagged cells */
+#define BIFHIP_SSTAT_FLAGGED_REGIME_ACC 9
+#define BIFHIP_SSTAT_FLAGGED_LLE_MAE 10
+#define BIFHIP_SSTAT_VALID_CLASS_ACC 11    /* surrogate vs FP64 on the sample    */
+#define BIFHIP_SSTAT_VALID_REGIME_ACC 12
+#define BIFHIP_SSTAT_VALID_LLE_MAE 13
+#define BIFHIP_SSTAT_LOSS_CE 14            /* last-epoch training losses         */
+#define BIFHIP_SSTAT_LOSS_MSE 15
+#define BIFHIP_SSTAT_LLE_MEAN 16
+#define BIFHIP_SSTAT_LLE_SD 17
+#define BIFHIP_SSTAT_COUNT 18
+
+int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int amountOfInitialConditions,
+                                const double* initialConditions, const double* ranges,
+                                const int* indicesOfMutVars, const double* values,
+                                int amountOfValues, double peakTMax, double peakTransient,
+                                int writableVar, int maxAmountOfPeaks, double dbscanEps,
+                                double lleTMax, double NT, double lleTransient, double lleEps,
+                                int epochs, int batch, double lr, unsigned long long seed,
+                                double confThreshold, double chaosThreshold,
+                                double validationFraction, int* d_peak, double* d_lle,
+                                unsigned char* d_resolved, double* stats);
+
+/* The network's pieces, for tests and custom pipelines (device buffers). */
+int bifhip_mlp_param_count(void);
+/* Training on a labelled nLat x nLat lattice: d_cls class 0..15, d_lle
+ * standardised LLE (NaN = no target), chaosStd the chaos threshold in the
+ * same units; loss_out[3] = last-epoch mean CE, MSE, chaos BCE. */
+int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int epochs,
+                            int batch, double lr, double wLle, unsigned long long seed,
+                            double fourierSigma, double chaosStd, int useGraph, float* loss_out);
+/* Inference of an nLat x nLat lattice: class, chaos regime, confidence, LLE. */
+int bifhip_mlp_infer_device(int nLat, double lleMean, double lleSd, double chaosThr,
+                            unsigned char* d_cls, unsigned char* d_chaos, float* d_conf,
+                            float* d_lle);
+int bifhip_mlp_get_params(float* params, int nParams, float* fourier, int nFourier);
+/* One forward + backward pass (host arrays): cells are flattened indices on
+ * an nLat x nLat lattice, cls/lle their targets; grads in parameter
+ * layout W1[128x64] b1 W2[128x128] b2 W3[128x128] b3 Wh[32x128] bh[32];
+ * losses[3]; outputs[rows x 32] head values. */
+int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, const int* cells,
+                        const int* cls, const float* lle, int rows, double wLle, double chaosStd,
+                        float* grads, float* losses, float* outputs);
+
 /* ------------------------------------------------ CSV writers (host) */
 
 /* The reference's file formats, usable without a GPU.

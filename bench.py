@@ -16,6 +16,11 @@ Workloads (one step = one whole map, inputs resident, outputs in HBM):
              flagged cells.  Agreement and speedup are measured against the
              plain FP64 map of the same 4096^2 lattice (computed once).
   plain      the plain FP64 fused LLE + peak-count map on an n x n lattice.
+  tiles      Lyapunov spectrum of 4096 uneven 16x16 refinement tiles (FP32
+             variational state, FP64 Gram-Schmidt), autotuned, one
+             persistent graph-replayed launch per step.
+  copy       a 1 GiB device copy: the reference that scales the HBM byte
+             counters (FETCH_SIZE, WRITE_SIZE) of the profiles.
 Prints one JSON line.  --gpus above 1 is refused: this build uses one GPU.
 """
 import argparse
@@ -36,8 +41,10 @@ PEAK = dict(tMax=500.0, transient=1000.0, eps=0.01)
 LLE = dict(tMax=1000.0, NT=0.5, transient=500.0, eps=1e-8)
 CHAOS = 0.01
 FP64_VALU_PEAK = 78.6e12     # MI355X FP64 vector, FMA counted as 2
+FP32_VALU_PEAK = 157.3e12    # MI355X FP32 vector (packed), FMA counted as 2
 STEP_FLOP = 12               # Rossler sequential Euler step, no contraction
 RENORM_FLOP = 20
+TILE_STEP_FLOP = 48          # state step + 3 tangent-vector steps
 METRIC = "grid points/s, 2D LLE+peak map; surrogate-refined vs plain FP64 @≥99% agree"
 
 
@@ -112,7 +119,7 @@ def cpu_baseline(n, rows):
                       f"DBSCAN peak count + LLE per cell, {dt:.1f} s"}
 
 
-def roofline(bh, cells_total, kernel="lle_peaks"):
+def roofline(bh, cells_total, kernel="lle_peaks", traffic=None):
     """Achieved FP64 FLOP/s of the fused integrator over all its launches:
     algorithmic FLOP (12 per Rossler step, 20 per renormalisation; peak
     detection compares are not counted) / summed HIP-event launch time."""
@@ -122,8 +129,69 @@ def roofline(bh, cells_total, kernel="lle_peaks"):
     ach = flop / (ms / 1000) / 1e12 if ms > 0 else 0.0
     return {"bound": "valu_fp64", "kernel": "k_lle_peaks", "achieved": ach,
             "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s", "frac": ach / (FP64_VALU_PEAK / 1e12),
-            "traffic": None, "flop_per_launch": flop / max(n, 1), "avg_launch_ms": avg,
-            "launches": n, "note": "no-contraction FP64 path: at most 50% of the FMA-counted peak"}
+            "traffic": traffic, "flop_per_launch": flop / max(n, 1), "avg_launch_ms": avg,
+            "launches": n, "cells": cells_total,
+            "note": "FP64 vector ALU (no MFMA on this path); no-contraction FP64 reaches at "
+                    "most 50% of the FMA-counted peak"}
+
+
+def run_tiles(bh, np, args, out):
+    """Lyapunov spectrum of uneven refinement tiles: 4096 tiles of 16x16
+    cells on the 4096^2 lattice, seeded lengths U[200, 2000] time units,
+    FP32 variational state, FP64 Gram-Schmidt every 0.5; one persistent
+    graph-replayed launch per step, configuration from the autotuner."""
+    nt, w, n_pts = 4096, 16, 4096
+    tiles = bh.tiles_generate(nt, w, w, n_pts, 0, 1, 200.0, 2000.0, LLE["NT"])
+    nsteps = int(np.asarray(tiles[2], dtype=np.int64).sum()) * int(LLE["NT"] / H) * w * w
+    d = bh.DeviceArray((nt * w * w * 3,), np.float64)
+    bh.set_tile_config(0)
+    args_ = (d.ptr, tiles, w, w, n_pts, RANGES, [0, 2], IC, VALS, H, LLE["NT"], 0.0)
+    bh.ls_tiles_device(*args_, autotune=True)
+    for _ in range(args.warmup):
+        bh.ls_tiles_device(*args_)
+    bh.synchronize()
+    bh.set_timing(True)
+    bh.reset_kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bh.ls_tiles_device(*args_)
+    bh.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    kms, nl = bh.kernel_time("ls_tiles")
+    flop = TILE_STEP_FLOP * nsteps * args.steps
+    ach = flop / (kms / 1000) / 1e12
+    block, cpl, _ = bh.get_tile_config()
+    cells = nt * w * w
+    out.update(metric="grid points/s, uneven-tile Lyapunov spectrum (FP32 state, FP64 GS)",
+               value=cells / (ms / 1000), unit="grid points/s", ms_per_step=ms, dtype="f32",
+               config={"workload": f"{nt} tiles of {w}x{w} cells on the Rossler {n_pts}x{n_pts} "
+                                   f"(a, c) lattice, lengths U[200, 2000] t.u., NT {LLE['NT']}",
+                       "integration_steps_per_map": nsteps, "tile_block": block,
+                       "cells_per_lane": cpl})
+    out["roofline"] = {"bound": "valu_fp32", "kernel": "k_ls_tiles", "achieved": ach,
+                       "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                       "frac": ach / (FP32_VALU_PEAK / 1e12), "traffic": None,
+                       "avg_launch_ms": kms / max(nl, 1), "launches": nl,
+                       "note": "48 FLOP per step (state + 3 tangent vectors); packed FP32 is "
+                               "disabled, so at most 50% of the published peak"}
+
+
+def run_copy(bh, np, args, out):
+    """Device-to-device copy of a known size: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters into bytes."""
+    mib = 1024
+    a = bh.DeviceArray((mib << 17,), np.float64)
+    b = bh.DeviceArray((mib << 17,), np.float64)
+    for _ in range(args.warmup + args.steps):
+        bh._check(bh.load().bifhip_memcpy_d2d(b.ptr, a.ptr, mib << 20))
+    bh.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bh._check(bh.load().bifhip_memcpy_d2d(b.ptr, a.ptr, mib << 20))
+    bh.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    out.update(metric="device copy GB/s (counter calibration)", value=2 * (mib << 20) / ms / 1e6,
+               unit="GB/s", ms_per_step=ms, config={"workload": f"copy {mib} MiB"})
 
 
 def main():
@@ -131,7 +199,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="surrogate", choices=["surrogate", "plain"])
+    ap.add_argument("--workload", default="surrogate",
+                    choices=["surrogate", "plain", "tiles", "copy"])
     ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
     ap.add_argument("--coarse", type=int, default=256)
     ap.add_argument("--conf", type=float, default=0.95)
@@ -139,6 +208,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-rows", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch tile and training work directly instead of replaying graphs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -153,6 +224,7 @@ def main():
         sys.exit(1)
     bh.reset_context()
     bh.set_perturbation(bh.PERT_AXIS_X)
+    bh.set_graph(not args.no_graph)
     n = args.n or (4096 if args.workload == "surrogate" else 1024)
     cells = n * n
     d_peak = bh.DeviceArray((cells,), np.int32)
@@ -161,6 +233,10 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (a, c) parameter lattice of the Rossler system"}
 
+    if args.workload in ("tiles", "copy"):
+        (run_tiles if args.workload == "tiles" else run_copy)(bh, np, args, out)
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "plain":
         for _ in range(args.warmup):
             plain_map(bh, n, d_peak, d_lle)
@@ -188,15 +264,14 @@ def main():
         plain_map(bh, n, d_peak, d_lle)
         bh.synchronize()
         plain_ms = (time.perf_counter() - t0) * 1000
-        rf = roofline(bh, cells)
         ref_peak, ref_lle = d_peak.to_host(), d_lle.to_host()
         t_cls, t_ch = regimes(np, ref_peak, ref_lle)
-        bh.set_timing(False)
         d_res = bh.DeviceArray((cells,), np.uint8)
         for w in range(args.warmup):
             surrogate_map(bh, n, args.coarse, d_peak, d_lle, d_res, 100 + w, args.conf,
                           args.validation, args.batch)
         bh.synchronize()
+        bh.reset_kernel_times()
         stats = []
         t0 = time.perf_counter()
         for s in range(args.steps):
@@ -204,6 +279,9 @@ def main():
                                        args.validation, args.batch))
         bh.synchronize()
         ms = (time.perf_counter() - t0) * 1000 / args.steps
+        # the integrator's launches inside the timed steps: coarse map + re-solves
+        fused_cells = sum(args.coarse ** 2 + st["resolved"] + st["validation"] for st in stats)
+        rf = roofline(bh, fused_cells)
         s_cls, s_ch = regimes(np, d_peak.to_host(), d_lle.to_host())
         agree = float(((s_cls == t_cls) & (s_ch == t_ch)).mean())
         last = stats[-1]

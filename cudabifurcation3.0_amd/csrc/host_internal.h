@@ -25,6 +25,7 @@ struct Ctx {
     int clusterClamp = 0;
     int verbose = 0;
     int timing = 0;
+    int useGraph = 1;          // replay tile launches and training epochs from HIP graphs
     std::string lastError;
     hipStream_t stream = nullptr;
 };

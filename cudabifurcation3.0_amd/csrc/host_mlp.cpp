@@ -515,7 +515,7 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     float losses[3] = {0, 0, 0};
     const double chaosStd = (chaosThreshold - mean) / sd;
     RC_TRY(bifhip_mlp_train_device(nCoarse, d_cls, d_lstd, epochs, std::min(batch, nc), lr, 1.0,
-                                   seed, 6.0, chaosStd, 1, losses));
+                                   seed, 6.0, chaosStd, g.useGraph, losses));
     st[BIFHIP_SSTAT_MS_TRAIN] = ms_since(t1);
     st[BIFHIP_SSTAT_LOSS_CE] = losses[0];
     st[BIFHIP_SSTAT_LOSS_MSE] = losses[1];

@@ -30,7 +30,6 @@ struct TileConfig {
     int blocksPerCU = 0;  // 0: occupancy maximum
 };
 TileConfig g_cfg;
-bool g_useGraph = true;
 hipStream_t g_tstream = nullptr;
 
 struct GraphCache {
@@ -182,7 +181,7 @@ int bifhip_get_tile_config(int* blockSize, int* cellsPerLane, int* blocksPerCU)
 
 int bifhip_set_graph(int on)
 {
-    g_useGraph = on != 0;
+    g.useGraph = on != 0;
     g_graph.reset();
     return BIFHIP_OK;
 }
@@ -309,7 +308,7 @@ static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, cons
     RC_TRY(job_for(g_cfg.block, g_cfg.cpl, g_cfg.blocksPerCU, J));
     {
         Timed t("ls_tiles", s);
-        RC_TRY(run_job(J, s, g_useGraph));
+        RC_TRY(run_job(J, s, g.useGraph != 0));
     }
     HIP_TRY(hipStreamSynchronize(s));
     resolve_timers();

@@ -291,9 +291,15 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
             U[w] = n >= kWave ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
             S[w] = 0;
         }
+        // The reference visits i = 0, 1, ... and after each stack pop jumps
+        // back to i = slot (or 0), so an empty-stack scan re-walks labelled
+        // indices once per cluster (O(P * clusters) iterations).  Those
+        // iterations change nothing, so the walk jumps straight to the next
+        // unlabelled index >= i (first set bit of U); the visit order of
+        // every scan, and so every label, is unchanged.
         int nn = 0, cluster = 0;
-        for (int i = 0; i < P; ++i) {
-            bool scan = false;
+        int i = 0;
+        while (i < P) {
             if (nn >= 1) {
                 const int slot = nn - 1;
                 const int ws = slot >> 6;
@@ -304,35 +310,36 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
                     if (w == ws) { set = (S[w] & bit) != 0; S[w] &= ~bit; }
                 i = set ? slot : 0;
                 nn = nn - 1;
-                scan = true;
             } else {
-                const int wi = i >> 6;
-                const uint64_t bit = 1ull << (i & 63);
-                bool unl = false;
-#pragma unroll
-                for (int w = 0; w < kDbR; ++w)
-                    if (w == wi) unl = (U[w] & bit) != 0;
-                if (unl) {
-                    nn = 0;
-                    ++cluster;
-#pragma unroll
-                    for (int w = 0; w < kDbR; ++w)
-                        if (w == wi) U[w] &= ~bit;
-                    scan = true;
-                }
-            }
-            if (scan) {
+                int nxt = P;
 #pragma unroll
                 for (int w = 0; w < kDbR; ++w) {
-                    if (w >= nsets) break;
-                    // every lane reads the same word; keep it scalar
-                    const uint64_t row = readlane_u64(rows[i * kDbR + w], 0);
-                    const uint64_t m = row & U[w];
-                    U[w] &= ~m;
-                    S[w] |= m;
-                    nn += __popcll(m);
+                    const int lo = w * kWave;
+                    if (nxt == P && w < nsets && i < lo + kWave) {
+                        const uint64_t m = i <= lo ? U[w] : (U[w] & (~0ull << (i - lo)));
+                        if (m) nxt = lo + __builtin_ctzll(m);
+                    }
                 }
+                if (nxt >= P) break;
+                i = nxt;
+                ++cluster;
+                const int wi = i >> 6;
+                const uint64_t bit = 1ull << (i & 63);
+#pragma unroll
+                for (int w = 0; w < kDbR; ++w)
+                    if (w == wi) U[w] &= ~bit;
             }
+#pragma unroll
+            for (int w = 0; w < kDbR; ++w) {
+                if (w >= nsets) break;
+                // every lane reads the same word; keep it scalar
+                const uint64_t row = readlane_u64(rows[i * kDbR + w], 0);
+                const uint64_t m = row & U[w];
+                U[w] &= ~m;
+                S[w] |= m;
+                nn += __popcll(m);
+            }
+            ++i;
         }
         res = cluster - 1;
     }

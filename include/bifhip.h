@@ -303,7 +303,8 @@ int bifhip_ls_tiles_autotune(int nTiles, int tileW, int tileH, const int* origin
  * cells per lane 1 or 2; blocks per CU 0 = occupancy maximum. */
 int bifhip_set_tile_config(int blockSize, int cellsPerLane, int blocksPerCU);
 int bifhip_get_tile_config(int* blockSize, int* cellsPerLane, int* blocksPerCU);
-/* Launch the tile work through a captured HIP graph (default 1). */
+/* Replay the tile launches and the surrogate's training epochs from captured
+ * HIP graphs (default 1); 0 launches the same kernels directly. */
 int bifhip_set_graph(int on);
 
 /* ------------------------------------------ surrogate-refined map */

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <numeric>
 #include <vector>
 
 #include "bifhip_dev.h"
@@ -38,6 +39,10 @@ hipError_t launch_dbscan_lane(int nList, const int* list, const int* counts, con
                               const double* ivls, long long rowStride, double T, int* scratch,
                               long long scratchStride, int* out, const int* outIndex,
                               hipStream_t st);
+hipError_t launch_dbscan_block(int nList, const int* list, const int* counts, const double* amps,
+                               const double* ivls, long long rowStride, double T, int* out,
+                               const int* outIndex, hipStream_t st);
+int dbscan_block_max_peaks();
 hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, long long rowStride,
                            int cap, int maxPeaks, int kdeSampling, double I1, double I2, double H,
                            int* out, const int* outIndex, int* overflow, int* nOverflow,
@@ -300,6 +305,50 @@ struct ClusterParams {
     double I1, I2, H;
 };
 
+// DBSCAN of cells whose rows hold every peak: entry t reads row
+// r = list ? list[t] : t of counts/amps/ivls (rowStride slots) and writes
+// out[outIndex ? outIndex[t] : r].  Cells with up to dbscan_block_max_peaks()
+// peaks take the workgroup kernel, larger ones the lane kernel.
+int dbscan_full_rows(int n, const int* d_list, int nRows, const int* d_counts,
+                     const double* d_amps, const double* d_ivls, long long rowStride, double T,
+                     int* d_out, const int* d_outIndex)
+{
+    if (n <= 0) return BIFHIP_OK;
+    std::vector<int> cnt(nRows), lst(d_list ? n : 0), oi(d_outIndex ? n : 0);
+    HIP_TRY(hipMemcpy(cnt.data(), d_counts, sizeof(int) * nRows, hipMemcpyDeviceToHost));
+    if (d_list) HIP_TRY(hipMemcpy(lst.data(), d_list, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (d_outIndex)
+        HIP_TRY(hipMemcpy(oi.data(), d_outIndex, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<int> bigRow, bigOut;
+    for (int t = 0; t < n; ++t) {
+        const int r = d_list ? lst[t] : t;
+        if (cnt[r] > dbscan_block_max_peaks()) {
+            bigRow.push_back(r);
+            bigOut.push_back(d_outIndex ? oi[t] : r);
+        }
+    }
+    {
+        Timed tm("dbscan");
+        HIP_TRY(launch_dbscan_block(n, d_list, d_counts, d_amps, d_ivls, rowStride, T, d_out,
+                                    d_outIndex, g.stream));
+    }
+    if (!bigRow.empty()) {
+        const int nb = (int)bigRow.size();
+        DBuf drow, dout, scr;
+        HIP_TRY(drow.alloc(sizeof(int) * nb));
+        HIP_TRY(dout.alloc(sizeof(int) * nb));
+        HIP_TRY(hipMemcpy(drow.p, bigRow.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dout.p, bigOut.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+        HIP_TRY(scr.alloc(sizeof(int) * (size_t)nb * 2 * rowStride));
+        Timed tm("dbscan");
+        HIP_TRY(launch_dbscan_lane(nb, drow.as<int>(), d_counts, d_amps, d_ivls, rowStride, T,
+                                   scr.as<int>(), 2LL * rowStride, d_out, dout.as<int>(),
+                                   g.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return BIFHIP_OK;
+}
+
 int cluster_map_device(const PeakMapSpec& S, const ClusterParams& C, int* d_out)
 {
     const long long cells = S.cells;
@@ -346,18 +395,15 @@ int cluster_map_device(const PeakMapSpec& S, const ClusterParams& C, int* d_out)
         std::vector<int> list(novf);
         HIP_TRY(hipMemcpy(list.data(), d_ovf, sizeof(int) * novf, hipMemcpyDeviceToHost));
         const int cap2 = (int)std::max(2LL, std::min((long long)S.N / 2 + 1, (long long)C.maxPeaks + 1));
-        DBuf cnt, am, iv, scr;
+        DBuf cnt, am, iv;
         HIP_TRY(cnt.alloc(sizeof(int) * novf));
         HIP_TRY(am.alloc(sizeof(double) * (size_t)novf * cap2));
         HIP_TRY(iv.alloc(sizeof(double) * (size_t)novf * cap2));
         RC_TRY(integrate_peaks(S, 0, d_ovf, novf, cap2, cnt.as<int>(), am.as<double>(),
                                iv.as<double>()));
         if (C.kind == 0) {
-            HIP_TRY(scr.alloc(sizeof(int) * (size_t)novf * 2 * cap2));
-            Timed t("dbscan");
-            HIP_TRY(launch_dbscan_lane(novf, nullptr, cnt.as<int>(), am.as<double>(),
-                                       iv.as<double>(), cap2, T, scr.as<int>(), 2LL * cap2, d_out,
-                                       d_ovf, g.stream));
+            RC_TRY(dbscan_full_rows(novf, nullptr, novf, cnt.as<int>(), am.as<double>(),
+                                    iv.as<double>(), cap2, T, d_out, d_ovf));
         } else {
             // rows now hold every peak (cap2 >= P), so this pass cannot overflow
             int *d_o2, *d_n2;
@@ -576,7 +622,7 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
         HIP_TRY(hipMemcpy(pos.data(), d_ovf, sizeof(int) * novf, hipMemcpyDeviceToHost));
         for (int i = 0; i < novf; ++i) flat[i] = h_list ? h_list[pos[i]] : base + pos[i];
         const int cap2 = (int)std::max(2LL, std::min((long long)S.N / 2 + 1, (long long)maxPeaks + 1));
-        DBuf dflat, dpos, cnt, am, iv, scr;
+        DBuf dflat, dpos, cnt, am, iv;
         HIP_TRY(dflat.alloc(sizeof(int) * novf));
         HIP_TRY(dpos.alloc(sizeof(int) * novf));
         HIP_TRY(hipMemcpy(dflat.p, flat.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
@@ -584,14 +630,10 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
         HIP_TRY(cnt.alloc(sizeof(int) * novf));
         HIP_TRY(am.alloc(sizeof(double) * (size_t)novf * cap2));
         HIP_TRY(iv.alloc(sizeof(double) * (size_t)novf * cap2));
-        HIP_TRY(scr.alloc(sizeof(int) * (size_t)novf * 2 * cap2));
         RC_TRY(integrate_peaks(S, 0, dflat.as<int>(), novf, cap2, cnt.as<int>(), am.as<double>(),
                                iv.as<double>()));
-        Timed t("dbscan");
-        HIP_TRY(launch_dbscan_lane(novf, nullptr, cnt.as<int>(), am.as<double>(), iv.as<double>(),
-                                   cap2, T, scr.as<int>(), 2LL * cap2, d_peak, dpos.as<int>(),
-                                   g.stream));
-        HIP_TRY(hipStreamSynchronize(g.stream));
+        RC_TRY(dbscan_full_rows(novf, nullptr, novf, cnt.as<int>(), am.as<double>(),
+                                iv.as<double>(), cap2, T, d_peak, dpos.as<int>()));
     }
     if (g.clusterClamp > 0) HIP_TRY(launch_clamp(n, d_peak, g.clusterClamp, g.stream));
     return BIFHIP_OK;
@@ -1271,14 +1313,9 @@ int bifhip_dbscan_device(const double* d_peaks, const double* d_intervals, int s
     int novf = 0;
     HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
-    if (novf > 0) {
-        DBuf scr;
-        HIP_TRY(scr.alloc(sizeof(int) * (size_t)novf * 2 * sizeOfBlock));
-        Timed t("dbscan");
-        HIP_TRY(launch_dbscan_lane(novf, d_ovf, d_counts, d_peaks, d_intervals, sizeOfBlock, T,
-                                   scr.as<int>(), 2LL * sizeOfBlock, d_out, nullptr, g.stream));
-        HIP_TRY(hipStreamSynchronize(g.stream));
-    }
+    if (novf > 0)
+        RC_TRY(dbscan_full_rows(novf, d_ovf, nBlocks, d_counts, d_peaks, d_intervals,
+                                sizeOfBlock, T, d_out, nullptr));
     return finish();
 }
 

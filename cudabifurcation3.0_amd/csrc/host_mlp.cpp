@@ -29,19 +29,19 @@ typedef __bf16 bf16;
 hipError_t launch_gemm(int epi, int M, int N, int K, const bf16* A, long long sAm, long long sAk,
                        const bf16* B, long long sBk, long long sBn, const float* bias,
                        const bf16* mask, long long sMm, void* C, long long sCm, hipStream_t st,
-                       int splits = 1);
+                       int splits = 1, long long splitStride = -1);
 hipError_t launch_sum_splits(int n, int splits, const float* part, float* out, hipStream_t st);
 hipError_t launch_features(int rows, int n, const int* list, long long base, const float* Bff,
                            bf16* X, hipStream_t st);
 hipError_t launch_loss_grad(int rows, const int* list, const float* out, const int* cls,
                             const float* lle, float wLle, float chaosStd, bf16* dOut, float* loss,
                             hipStream_t st);
-hipError_t launch_colsum(int rows, int N, const bf16* g, float* part, int splits, float* out,
-                         hipStream_t st);
-hipError_t launch_adam(int n, float* p, float* m, float* v, const float* g,
-                       const unsigned char* keep, const int* bfIndex, bf16* wbf, float lr,
-                       float b1, float b2, float eps, const int* step, hipStream_t st);
-hipError_t launch_step_inc(int* step, hipStream_t st);
+hipError_t launch_bias_partials(int rows, int splits, const bf16* const g[4], const int cols[4],
+                                const int off[4], float* gpart, long long stride, int* step,
+                                hipStream_t st);
+hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, int splits,
+                       long long stride, const unsigned char* keep, const int* bfIndex, bf16* wbf,
+                       float lr, float b1, float b2, float eps, const int* step, hipStream_t st);
 hipError_t launch_fill_f32(int n, float* p, float v, hipStream_t st);
 hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, float chaosThr,
                              unsigned char* cls, unsigned char* chaos, float* conf, float* lle,
@@ -87,7 +87,8 @@ constexpr int bW1 = 0, bW2 = bW1 + kHid * kFeat, bW3 = bW2 + kHid * kHid,
               bWh = bW3 + kHid * kHid, kBf = bWh + kHead * kHid;
 
 struct Net {
-    float *p, *m, *v, *g, *bff, *loss, *part;
+    // gpart[z * kParams + i]: split-z partial of the gradient of parameter i
+    float *p, *m, *v, *g, *bff, *loss, *gpart;
     bf16* wbf;
     unsigned char* keep;
     int *bfIndex, *step;
@@ -99,7 +100,7 @@ int net_alloc(Net& N)
     RC_TRY(pool_get("mlp_m", kParams, &N.m));
     RC_TRY(pool_get("mlp_v", kParams, &N.v));
     RC_TRY(pool_get("mlp_g", kParams, &N.g));
-    RC_TRY(pool_get("mlp_part", (size_t)kMaxSplits * kHid * kHid, &N.part));
+    RC_TRY(pool_get("mlp_gpart", (size_t)kMaxSplits * kParams, &N.gpart));
     RC_TRY(pool_get("mlp_bff", kFeat, &N.bff));
     RC_TRY(pool_get("mlp_loss", 3, &N.loss));
     RC_TRY(pool_get("mlp_wbf", kBf, &N.wbf));
@@ -197,45 +198,41 @@ int forward(const Net& N, Acts& A, int rows, int nLat, const int* list, long lon
     return BIFHIP_OK;
 }
 
-// backward of the loss gradient A.dout into N.g (weights and biases)
-// weight gradient dW[M x Nn] = dZ^T X over the batch, split over K = rows
-// into partials (fixed-order sum keeps it deterministic)
+// backward of the loss gradient A.dout into the split partials N.gpart
+// (summed in split order by k_adam, or by launch_sum_splits into N.g).
+// Weight gradient dW[M x Nn] = dZ^T X over the batch: split-K over the rows,
+// split z written at N.gpart + z * kParams + off.
 int weight_grad(const Net& N, int M, int Nn, int rows, const bf16* dz, int ldz, const bf16* x,
-                int ldx, float* out, hipStream_t s)
+                int ldx, int off, hipStream_t s)
 {
-    const int S = splits_for(rows);
     HIP_TRY(launch_gemm(3, M, Nn, rows, dz, 1, ldz, x, ldx, 1, nullptr, nullptr, 0,
-                        S > 1 ? N.part : out, Nn, s, S));
-    if (S > 1) HIP_TRY(launch_sum_splits(M * Nn, S, N.part, out, s));
+                        N.gpart + off, Nn, s, splits_for(rows), kParams));
     return BIFHIP_OK;
 }
 
-int bias_grad(const Net& N, int rows, int Nn, const bf16* dz, float* out, hipStream_t s)
+// step: the optimiser step counter to advance (training), or null
+int backward(const Net& N, Acts& A, int rows, int* step, hipStream_t s)
 {
-    HIP_TRY(launch_colsum(rows, Nn, dz, N.part, splits_for(rows), out, s));
-    return BIFHIP_OK;
-}
-
-int backward(const Net& N, Acts& A, int rows, hipStream_t s)
-{
-    // head: dWh = dOut^T H3, dbh = colsum(dOut), dH3 = (dOut Wh) * (H3 > 0)
-    RC_TRY(weight_grad(N, kHead, kHid, rows, A.dout, kHead, A.h3, kHid, N.g + oWh, s));
-    RC_TRY(bias_grad(N, rows, kHead, A.dout, N.g + oBh, s));
+    // head: dWh = dOut^T H3, dH3 = (dOut Wh) * (H3 > 0)
+    RC_TRY(weight_grad(N, kHead, kHid, rows, A.dout, kHead, A.h3, kHid, oWh, s));
     HIP_TRY(launch_gemm(2, rows, kHid, kHead, A.dout, kHead, 1, N.wbf + bWh, kHid, 1, nullptr,
                         A.h3, kHid, A.d3, kHid, s));
     // layer 3
-    RC_TRY(weight_grad(N, kHid, kHid, rows, A.d3, kHid, A.h2, kHid, N.g + oW3, s));
-    RC_TRY(bias_grad(N, rows, kHid, A.d3, N.g + oB3, s));
+    RC_TRY(weight_grad(N, kHid, kHid, rows, A.d3, kHid, A.h2, kHid, oW3, s));
     HIP_TRY(launch_gemm(2, rows, kHid, kHid, A.d3, kHid, 1, N.wbf + bW3, kHid, 1, nullptr, A.h2,
                         kHid, A.d2, kHid, s));
     // layer 2
-    RC_TRY(weight_grad(N, kHid, kHid, rows, A.d2, kHid, A.h1, kHid, N.g + oW2, s));
-    RC_TRY(bias_grad(N, rows, kHid, A.d2, N.g + oB2, s));
+    RC_TRY(weight_grad(N, kHid, kHid, rows, A.d2, kHid, A.h1, kHid, oW2, s));
     HIP_TRY(launch_gemm(2, rows, kHid, kHid, A.d2, kHid, 1, N.wbf + bW2, kHid, 1, nullptr, A.h1,
                         kHid, A.d1, kHid, s));
     // layer 1 (no input gradient needed)
-    RC_TRY(weight_grad(N, kHid, kFeat, rows, A.d1, kHid, A.x0, kFeat, N.g + oW1, s));
-    RC_TRY(bias_grad(N, rows, kHid, A.d1, N.g + oB1, s));
+    RC_TRY(weight_grad(N, kHid, kFeat, rows, A.d1, kHid, A.x0, kFeat, oW1, s));
+    // the four bias gradients (column sums of dOut, dH3, dH2, dH1), one launch
+    const bf16* const gs[4] = {A.dout, A.d3, A.d2, A.d1};
+    const int cols[4] = {kHead, kHid, kHid, kHid};
+    const int offs[4] = {oBh, oB3, oB2, oB1};
+    HIP_TRY(launch_bias_partials(rows, splits_for(rows), gs, cols, offs, N.gpart, kParams, step,
+                                 s));
     return BIFHIP_OK;
 }
 
@@ -253,10 +250,9 @@ int train_step(const Net& N, Acts& A, const TrainCfg& T, int nLat, const int* li
     RC_TRY(forward(N, A, T.batch, nLat, list, 0, s));
     HIP_TRY(launch_loss_grad(T.batch, list, A.out, cls, lle, T.wLle, T.chaosStd, A.dout, N.loss,
                              s));
-    RC_TRY(backward(N, A, T.batch, s));
-    HIP_TRY(launch_step_inc(N.step, s));
-    HIP_TRY(launch_adam(kParams, N.p, N.m, N.v, N.g, N.keep, N.bfIndex, N.wbf, T.lr, 0.9f,
-                        0.999f, 1e-8f, N.step, s));
+    RC_TRY(backward(N, A, T.batch, N.step, s));
+    HIP_TRY(launch_adam(kParams, N.p, N.m, N.v, N.gpart, splits_for(T.batch), kParams, N.keep,
+                        N.bfIndex, N.wbf, T.lr, 0.9f, 0.999f, 1e-8f, N.step, s));
     return BIFHIP_OK;
 }
 
@@ -443,7 +439,8 @@ int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, con
     RC_TRY(forward(N, A, rows, nLat, dc.as<int>(), 0, s));
     HIP_TRY(launch_loss_grad(rows, seq.as<int>(), A.out, dcls.as<int>(), dl.as<float>(),
                              (float)wLle, (float)chaosStd, A.dout, N.loss, s));
-    RC_TRY(backward(N, A, rows, s));
+    RC_TRY(backward(N, A, rows, nullptr, s));
+    HIP_TRY(launch_sum_splits(kParams, splits_for(rows), N.gpart, N.g, s));
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipMemcpy(grads, N.g, sizeof(float) * kParams, hipMemcpyDeviceToHost));
     if (losses) HIP_TRY(hipMemcpy(losses, N.loss, sizeof(float) * 3, hipMemcpyDeviceToHost));

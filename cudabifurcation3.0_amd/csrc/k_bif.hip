@@ -14,8 +14,10 @@
 //                   per cell: neighbour rows built with __ballot, the
 //                   reference's sequential label/stack machine replayed on
 //                   64-bit masks.  Exact for P <= 256 peaks.
-//  k_dbscan_lane    the same algorithm one lane per cell on HBM scratch, for
-//                   the rare cells with more than 256 peaks.
+//  k_dbscan_block   the same machine, one workgroup per cell, label masks
+//                   in LDS, for the rare cells with 257..16384 peaks.
+//  k_dbscan_lane    the same algorithm one lane per cell on HBM scratch,
+//                   for cells with still more peaks.
 //  k_kde_wave       kde (Library/cudaLibrary.cu:356-422), one wavefront per
 //                   cell: lanes evaluate 64 sample points at a time (each sum
 //                   in the reference's peak order), the local-maximum state
@@ -373,7 +375,10 @@ __global__ __launch_bounds__(64) void k_dbscan_lane(int nList, const int* __rest
     const double* B = ivls + (long long)cell * rowStride;
     int* H = scratch + (long long)t * scratchStride;
     for (int i = 0; i < 2 * P; ++i) H[i] = 0;
-    int nn = 0, cluster = 0;
+    // rem = unlabelled indices below P-1 (the only ones a scan can absorb);
+    // at rem == 0 scans are no-ops and only index P-1 can open a cluster
+    int nn = 0, cluster = 0, rem = P - 1;
+    bool lastFree = true;
     for (int i = 0; i < P; i++) {
         bool scan = false;
         if (nn >= 1) {
@@ -385,20 +390,146 @@ __global__ __launch_bounds__(64) void k_dbscan_lane(int nList, const int* __rest
             nn = 0;
             ++cluster;
             H[i] = cluster;
+            if (i < P - 1) --rem;
+            else lastFree = false;
             scan = true;
+        } else if (rem == 0) {
+            if (lastFree) ++cluster;
+            break;
         }
-        if (scan) {
+        if (scan && rem > 0) {
             const double ai = A[i], bi = B[i];
             for (int k = 0; k < P - 1; k++) {
                 if (i != k && H[k] == 0 && near(ai, bi, A[k], B[k], T)) {
                     H[k] = cluster;
                     H[P + k] = k;
                     ++nn;
+                    --rem;
                 }
             }
         }
     }
     out[o] = cluster - 1;
+}
+
+// Workgroup-per-cell DBSCAN for cells with more peaks than the wave kernel
+// holds (P <= kBlkMaxP).  Same machine as k_dbscan_wave with the label
+// state as bitmasks in LDS: U = unlabelled, S = stack slots holding their
+// own index.  A scan of row i tests every unlabelled k < P-1 in parallel
+// (each k is owned by one thread; its U/S bits change with LDS atomics)
+// and pushes the matches; the order of the pushes within a scan does not
+// matter because slot k only ever holds k.  Peaks are staged in LDS when
+// P <= kBlkLdsP, else read from the rows.  Once no index below P-1 is
+// unlabelled, scans cannot change anything and are skipped; the walk then
+// ends after at most the last index (never absorbed by a scan) opens one
+// more cluster.  Row r = list[t] (or t); result to out[outIndex[t]]
+// (or out[r]).
+constexpr int kBlkLdsP = 3072;
+constexpr int kBlkMaxP = 16384;
+
+__global__ __launch_bounds__(256) void k_dbscan_block(int nList, const int* __restrict__ list,
+                                                      const int* __restrict__ counts,
+                                                      const double* __restrict__ amps,
+                                                      const double* __restrict__ ivls,
+                                                      long long rowStride, double T,
+                                                      int* __restrict__ out,
+                                                      const int* __restrict__ outIndex)
+{
+    __shared__ double s_a[kBlkLdsP], s_b[kBlkLdsP];
+    __shared__ uint64_t s_U[kBlkMaxP / 64], s_S[kBlkMaxP / 64];
+    __shared__ int s_cnt[2], s_next;
+    const int t = blockIdx.x;
+    if (t >= nList) return;                        // block-uniform
+    const int row = list ? list[t] : t;
+    const int o = outIndex ? outIndex[t] : row;
+    const int P = counts[row];
+    if (P <= 0 || P > rowStride || P > kBlkMaxP) { // not handled here (caller filters)
+        if (threadIdx.x == 0) out[o] = P == 1 ? 1 : (P <= 0 ? 0 : -1);
+        return;
+    }
+    if (P == 1) {
+        if (threadIdx.x == 0) out[o] = 1;
+        return;
+    }
+    const double* ra = amps + (long long)row * rowStride;
+    const double* rb = ivls + (long long)row * rowStride;
+    const bool inLds = P <= kBlkLdsP;
+    if (inLds)
+        for (int k = threadIdx.x; k < P; k += blockDim.x) {
+            s_a[k] = ra[k];
+            s_b[k] = rb[k];
+        }
+    const double* A = inLds ? s_a : ra;
+    const double* B = inLds ? s_b : rb;
+    const int nWords = (P + 63) / 64;
+    for (int w = threadIdx.x; w < nWords; w += blockDim.x) {
+        const int n = P - w * 64;
+        s_U[w] = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+        s_S[w] = 0;
+    }
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+
+    // uniform state, computed redundantly by every thread
+    int nn = 0, cluster = 0, i = 0, rem = P - 1, parity = 0;
+    bool lastFree = true;
+    while (i < P) {
+        if (nn >= 1) {
+            const int slot = nn - 1;
+            const uint64_t bit = 1ull << (slot & 63);
+            const bool set = (s_S[slot >> 6] & bit) != 0;
+            __syncthreads();                       // all have read before the clear
+            if (threadIdx.x == 0) s_S[slot >> 6] &= ~bit;
+            i = set ? slot : 0;
+            nn = nn - 1;
+        } else {
+            if (rem == 0) {
+                // only index P-1 can still open a cluster
+                if (lastFree) ++cluster;
+                break;
+            }
+            if (threadIdx.x == 0) s_next = P;
+            __syncthreads();
+            for (int w = (i >> 6) + (int)threadIdx.x; w < nWords; w += blockDim.x) {
+                uint64_t m = s_U[w];
+                if (w == (i >> 6)) m &= ~0ull << (i & 63);
+                if (m) {
+                    atomicMin(&s_next, w * 64 + __builtin_ctzll(m));
+                    break;
+                }
+            }
+            __syncthreads();
+            const int nxt = s_next;
+            if (nxt >= P) break;
+            i = nxt;
+            ++cluster;
+            if (threadIdx.x == 0) s_U[i >> 6] &= ~(1ull << (i & 63));
+            if (i < P - 1) --rem;
+            else lastFree = false;
+        }
+        if (rem > 0) {
+            __syncthreads();                       // U/S updates of thread 0 visible
+            const double ai = A[i], bi = B[i];
+            int found = 0;
+            for (int k = threadIdx.x; k < P - 1; k += blockDim.x) {
+                const uint64_t bit = 1ull << (k & 63);
+                if (k != i && (s_U[k >> 6] & bit) && near(ai, bi, A[k], B[k], T)) {
+                    atomicAnd((unsigned long long*)&s_U[k >> 6], ~(unsigned long long)bit);
+                    atomicOr((unsigned long long*)&s_S[k >> 6], (unsigned long long)bit);
+                    ++found;
+                }
+            }
+            if (found) atomicAdd(&s_cnt[parity], found);
+            if (threadIdx.x == 0) s_cnt[parity ^ 1] = 0;   // next scan's counter
+            __syncthreads();
+            const int c = s_cnt[parity];
+            nn += c;
+            rem -= c;
+            parity ^= 1;
+        }
+        ++i;
+    }
+    if (threadIdx.x == 0) out[o] = cluster - 1;
 }
 
 // ---------------------------------------------------------------- KDE
@@ -667,6 +798,18 @@ hipError_t launch_dbscan_lane(int nList, const int* list, const int* counts, con
                        counts, amps, ivls, rowStride, T, scratch, scratchStride, out, outIndex);
     return hipGetLastError();
 }
+
+hipError_t launch_dbscan_block(int nList, const int* list, const int* counts, const double* amps,
+                               const double* ivls, long long rowStride, double T, int* out,
+                               const int* outIndex, hipStream_t st)
+{
+    if (nList <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dbscan_block, dim3(nList), dim3(256), 0, st, nList, list, counts, amps,
+                       ivls, rowStride, T, out, outIndex);
+    return hipGetLastError();
+}
+
+int dbscan_block_max_peaks() { return kBlkMaxP; }
 
 hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, long long rowStride,
                            int cap, int maxPeaks, int kdeSampling, double I1, double I2, double H,

@@ -194,19 +194,6 @@ __global__ void k_loss_grad(int rows, const int* __restrict__ list, const float*
     atomicAdd(&loss[2], bce * inv);
 }
 
-// Bias gradient, stage 1: block (x, z) sums rows [z*rChunk, ...) of 128
-// columns of a [rows x N] BF16 gradient into part[z][N].
-__global__ void k_colsum(int rows, int N, const bf16* __restrict__ g, int rChunk,
-                         float* __restrict__ part)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    const int r0 = blockIdx.z * rChunk, r1 = min(rows, r0 + rChunk);
-    float s = 0;
-    for (int r = r0; r < r1; ++r) s += (float)g[(long long)r * N + c];
-    part[(long long)blockIdx.z * N + c] = s;
-}
-
 // Sum of split partials in a fixed order: out[i] = sum_z part[z][i].
 __global__ void k_sum_splits(int n, int splits, const float* __restrict__ part,
                              float* __restrict__ out)
@@ -218,12 +205,50 @@ __global__ void k_sum_splits(int n, int splits, const float* __restrict__ part,
     out[i] = s;
 }
 
-// Adam step on the FP32 master parameters; refreshes the BF16 copies of the
-// weight matrices (wmask[i] != 0) and keeps padded head rows at zero.
+// Split partials of the four bias gradients (column sums of the layers'
+// output gradients) in one launch: blockIdx.y picks the layer, blockIdx.x a
+// group of 64 columns, blockIdx.z the row chunk; the partial of split z goes
+// to gpart[z * stride + offset + column], next to the weight-gradient
+// partials, and k_adam adds them in split order.  Block (0,0,0) also
+// advances the optimiser's step counter (read by the following k_adam).
+struct BiasSrc {
+    const bf16* g[4];
+    int cols[4];
+    int off[4];
+};
+
+__global__ __launch_bounds__(256) void k_bias_partials(int rows, int rChunk, BiasSrc src,
+                                                       float* __restrict__ gpart,
+                                                       long long stride, int* __restrict__ step)
+{
+    __shared__ float s_part[4][64];
+    const int layer = blockIdx.y;
+    const int N = src.cols[layer];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int rs = threadIdx.x >> 6;
+    const int r0 = blockIdx.z * rChunk, r1 = min(rows, r0 + rChunk);
+    float s = 0;
+    if (c < N) {
+        const bf16* g = src.g[layer];
+        for (int r = r0 + rs; r < r1; r += 4) s += (float)g[(long long)r * N + c];
+    }
+    s_part[rs][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (rs == 0 && c < N)
+        gpart[(long long)blockIdx.z * stride + src.off[layer] + c] =
+            ((s_part[0][c & 63] + s_part[1][c & 63]) + s_part[2][c & 63]) + s_part[3][c & 63];
+    if (step && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0)
+        *step += 1;
+}
+
+// Adam step on the FP32 master parameters with the gradient summed from its
+// split partials (gpart[z * stride + i], z < splits, fixed order); refreshes
+// the BF16 copies of the weight matrices and keeps padded head rows at zero.
 __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
-                       const float* __restrict__ g, const unsigned char* __restrict__ keep,
-                       const int* __restrict__ bfIndex, bf16* __restrict__ wbf, float lr,
-                       float b1, float b2, float eps, const int* __restrict__ step)
+                       const float* __restrict__ gpart, int splits, long long stride,
+                       const unsigned char* __restrict__ keep, const int* __restrict__ bfIndex,
+                       bf16* __restrict__ wbf, float lr, float b1, float b2, float eps,
+                       const int* __restrict__ step)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -235,7 +260,8 @@ __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, floa
     if (!keep[i]) {
         p[i] = 0;
     } else {
-        const float gi = g[i];
+        float gi = 0;
+        for (int z = 0; z < splits; ++z) gi += gpart[(long long)z * stride + i];
         const float mi = b1 * m[i] + (1 - b1) * gi;
         const float vi = b2 * v[i] + (1 - b2) * gi * gi;
         m[i] = mi;
@@ -245,8 +271,6 @@ __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, floa
     const int bi = bfIndex[i];
     if (bi >= 0) wbf[bi] = (bf16)p[i];
 }
-
-__global__ void k_step_inc(int* step) { *step += 1; }
 
 __global__ void k_fill_f32(int n, float* p, float v)
 {
@@ -339,7 +363,7 @@ __global__ void k_fill_pred(long long n, const unsigned char* __restrict__ cls,
 hipError_t launch_gemm(int epi, int M, int N, int K, const bf16* A, long long sAm, long long sAk,
                        const bf16* B, long long sBk, long long sBn, const float* bias,
                        const bf16* mask, long long sMm, void* C, long long sCm, hipStream_t st,
-                       int splits = 1)
+                       int splits = 1, long long splitStride = -1)
 {
     if (epi != kEpiF32) splits = 1;
     // chunks are whole 32-wide k steps; trailing splits may be empty and
@@ -347,7 +371,7 @@ hipError_t launch_gemm(int epi, int M, int N, int K, const bf16* A, long long sA
     int kChunk = (K + splits - 1) / splits;
     kChunk = (kChunk + 31) / 32 * 32;
     const dim3 grd((N + 63) / 64, (M + 63) / 64, splits), blk(256);
-    const long long splitStride = (long long)M * sCm;
+    if (splitStride < 0) splitStride = (long long)M * sCm;
 #define GEMM_CASE(E)                                                                          \
     hipLaunchKernelGGL((k_gemm<E>), grd, blk, 0, st, M, N, K, A, sAm, sAk, B, sBk, sBn, bias, \
                        mask, sMm, C, sCm, kChunk, splitStride)
@@ -386,36 +410,38 @@ hipError_t launch_sum_splits(int n, int splits, const float* part, float* out, h
     return hipGetLastError();
 }
 
-hipError_t launch_step_inc(int* step, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, st, step);
-    return hipGetLastError();
-}
-
 hipError_t launch_fill_f32(int n, float* p, float v, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_f32, dim3((n + 255) / 256), dim3(256), 0, st, n, p, v);
     return hipGetLastError();
 }
 
-// column sums through `splits` row chunks into part[splits][N], then out
-hipError_t launch_colsum(int rows, int N, const bf16* g, float* part, int splits, float* out,
-                         hipStream_t st)
+// split partials of the four bias gradients (g[l] is [rows x cols[l]]) into
+// gpart[z * stride + off[l] + c]; advances *step when step is not null
+hipError_t launch_bias_partials(int rows, int splits, const bf16* const g[4], const int cols[4],
+                                const int off[4], float* gpart, long long stride, int* step,
+                                hipStream_t st)
 {
+    BiasSrc src;
+    int maxCols = 0;
+    for (int l = 0; l < 4; ++l) {
+        src.g[l] = g[l];
+        src.cols[l] = cols[l];
+        src.off[l] = off[l];
+        maxCols = max(maxCols, cols[l]);
+    }
     const int rChunk = (rows + splits - 1) / splits;
-    hipLaunchKernelGGL(k_colsum, dim3((N + 127) / 128, 1, splits), dim3(128), 0, st, rows, N, g,
-                       rChunk, part);
-    hipLaunchKernelGGL(k_sum_splits, dim3((N + 255) / 256), dim3(256), 0, st, N, splits, part,
-                       out);
+    hipLaunchKernelGGL(k_bias_partials, dim3((maxCols + 63) / 64, 4, splits), dim3(256), 0, st,
+                       rows, rChunk, src, gpart, stride, step);
     return hipGetLastError();
 }
 
-hipError_t launch_adam(int n, float* p, float* m, float* v, const float* g,
-                       const unsigned char* keep, const int* bfIndex, bf16* wbf, float lr,
-                       float b1, float b2, float eps, const int* step, hipStream_t st)
+hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, int splits,
+                       long long stride, const unsigned char* keep, const int* bfIndex, bf16* wbf,
+                       float lr, float b1, float b2, float eps, const int* step, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, n, p, m, v, g, keep,
-                       bfIndex, wbf, lr, b1, b2, eps, step);
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, n, p, m, v, gpart, splits,
+                       stride, keep, bfIndex, wbf, lr, b1, b2, eps, step);
     return hipGetLastError();
 }
 

@@ -186,6 +186,45 @@ def test_dbscan_device_fuzz_bit_exact(orc, bh, dev, max_peaks):
         assert got[b] == want, (b, cnt[b])
 
 
+def test_dbscan_device_large_peak_sets(orc, bh, dev):
+    """Cells beyond the wave kernel's 256 peaks: up to 16384 peaks take the
+    workgroup kernel (peaks staged in LDS up to 3072, read from the rows
+    above), larger sets the lane kernel.  Spread sets (mostly singleton
+    clusters), few-level sets and near-fixed-point sets (one tight cluster,
+    where both kernels stop scanning once every absorbable peak is
+    labelled)."""
+    rng = np.random.default_rng(11)
+    sizes = [257, 900, 3072, 3073, 5000, 6000, 16384, 16390]
+    stride = 16400
+    nb = len(sizes)
+    amps = np.zeros((nb, stride))
+    ivls = np.zeros((nb, stride))
+    cnt = np.array(sizes, dtype=np.int32)
+    for b, P in enumerate(sizes):
+        if P > 10000 or P == 6000:
+            a = 0.3 + rng.normal(size=P) * 0.0005
+            iv = np.full(P, 0.05)
+            if P == 16390:
+                a[-1] += 1.0        # the last peak, never absorbed, apart
+        elif b % 2:
+            a = rng.choice([0.0, 0.02, 0.05], P) + rng.normal(size=P) * 0.002
+            iv = rng.integers(1, 3, size=P) * 0.01
+        else:
+            a = rng.random(P) * 0.5
+            iv = rng.random(P) * 0.2
+        amps[b, :P], ivls[b, :P] = a, iv
+    say("dbscan large peak sets", sizes)
+    out = bh.DeviceArray.full((nb,), -99, np.int32)
+    A, I, Cn = _t(bh, amps.ravel(), np.float64), _t(bh, ivls.ravel(), np.float64), _t(
+        bh, cnt, np.int32)
+    bh.dbscan_device(A.ptr, I.ptr, stride, nb, Cn.ptr, 1 << 30, 0.01, out.ptr)
+    bh.synchronize()
+    got = out.to_host()
+    for b, P in enumerate(sizes):
+        want = orc.dbscan(amps[b, :P], ivls[b, :P], 0.01)
+        assert got[b] == want, (b, P, got[b], want)
+
+
 def test_dbscan_reference_kats_gpu(bh, dev):
     import json, os
     gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))

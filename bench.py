@@ -203,11 +203,15 @@ def main():
                     choices=["surrogate", "plain", "tiles", "copy"])
     ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
     ap.add_argument("--coarse", type=int, default=256)
-    ap.add_argument("--conf", type=float, default=0.95)
+    ap.add_argument("--conf", type=float, default=0.95, help="class softmax confidence threshold")
+    ap.add_argument("--chaos-conf", type=float, default=0.99,
+                    help="chaos-logit confidence threshold")
     ap.add_argument("--validation", type=float, default=0.002)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-rows", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--full-resolve", action="store_true",
+                    help="re-solve every flagged cell in full (no peak-count-only re-solves)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch tile and training work directly instead of replaying graphs")
     args = ap.parse_args()
@@ -225,6 +229,8 @@ def main():
     bh.reset_context()
     bh.set_perturbation(bh.PERT_AXIS_X)
     bh.set_graph(not args.no_graph)
+    bh.set_partial_resolve(not args.full_resolve)
+    bh.set_chaos_confidence(args.chaos_conf)
     n = args.n or (4096 if args.workload == "surrogate" else 1024)
     cells = n * n
     d_peak = bh.DeviceArray((cells,), np.int32)
@@ -280,6 +286,7 @@ def main():
         bh.synchronize()
         ms = (time.perf_counter() - t0) * 1000 / args.steps
         # the integrator's launches inside the timed steps: coarse map + re-solves
+        # peak-count-only re-solves run k_bif_peaks and are not counted here
         fused_cells = sum(args.coarse ** 2 + st["resolved"] + st["validation"] for st in stats)
         rf = roofline(bh, fused_cells)
         s_cls, s_ch = regimes(np, d_peak.to_host(), d_lle.to_host())
@@ -288,8 +295,10 @@ def main():
         out.update(value=cells / (ms / 1000), unit="grid points/s", ms_per_step=ms,
                    config={"workload": f"surrogate-refined 2D LLE + DBSCAN peak-count map, "
                                        f"Rossler {n}x{n} (coarse {args.coarse}x{args.coarse}, "
-                                       f"20 epochs, conf < {args.conf} or class/regime "
-                                       f"boundary re-solved in FP64)",
+                                       f"20 epochs; class confidence < {args.conf} or class "
+                                       f"boundary: peak count re-solved in FP64; chaos "
+                                       f"confidence < {args.chaos_conf} or regime boundary: "
+                                       f"LLE + peak count re-solved in FP64)",
                            "grid": n, "coarse": args.coarse, "ranges": RANGES, "h": H,
                            "peak": PEAK, "lle": LLE, "chaos_threshold": CHAOS})
         out["surrogate"] = {
@@ -300,6 +309,9 @@ def main():
             "plain_fp64_value": cells / (plain_ms / 1000),
             "speedup_vs_plain_fp64": plain_ms / ms,
             "resolved_fraction": last["resolved_frac"],
+            "peak_only_fraction": last["peak_only"] / cells,
+            "surrogate_on_peak_only": {"class_acc": last["peak_only_class_acc"]},
+            "partial_resolve": not args.full_resolve,
             "phase_ms": {k: last[k] for k in ("ms_coarse", "ms_train", "ms_infer", "ms_resolve")},
             "surrogate_on_flagged": {"class_acc": last["flagged_class_acc"],
                                      "regime_acc": last["flagged_regime_acc"]},

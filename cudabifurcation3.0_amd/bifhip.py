@@ -103,6 +103,8 @@ def _declare(L):
         "bifhip_set_tile_config": [_I, _I, _I],
         "bifhip_get_tile_config": [_PI, _PI, _PI],
         "bifhip_set_graph": [_I],
+        "bifhip_set_partial_resolve": [_I],
+        "bifhip_set_chaos_confidence": [_D],
         "bifhip_lle_peak_map_device": [_I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I, _I, _D,
                                        _D, _D, _D, _D, _VP, _VP],
         "bifhip_surrogate_map_device": [_I, _I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I,
@@ -111,7 +113,7 @@ def _declare(L):
         "bifhip_mlp_param_count": [],
         "bifhip_mlp_train_device": [_I, _VP, _VP, _I, _I, _D, _D, C.c_ulonglong, _D, _D, _I,
                                     C.POINTER(C.c_float)],
-        "bifhip_mlp_infer_device": [_I, _D, _D, _D, _VP, _VP, _VP, _VP],
+        "bifhip_mlp_infer_device": [_I, _D, _D, _D, _VP, _VP, _VP, _VP, _VP],
         "bifhip_mlp_get_params": [C.POINTER(C.c_float), _I, C.POINTER(C.c_float), _I],
         "bifhip_mlp_gradient": [_I, C.POINTER(C.c_float), C.POINTER(C.c_float), _PI, _PI,
                                 C.POINTER(C.c_float), _I, _D, _D, C.POINTER(C.c_float),
@@ -262,6 +264,9 @@ def reset_context():
     set_peak_capacity(256)
     set_cluster_clamp(0)
     set_timing(False)
+    set_graph(True)
+    set_partial_resolve(True)
+    set_chaos_confidence(0.99)
 
 
 def device_count():
@@ -452,7 +457,7 @@ def lle_peak_map_device(d_peak, d_lle, nPts, h, initialConditions, ranges, indic
 SSTAT = ["ms_total", "ms_coarse", "ms_train", "ms_infer", "ms_resolve", "resolved",
          "resolved_frac", "validation", "flagged_class_acc", "flagged_regime_acc",
          "flagged_lle_mae", "valid_class_acc", "valid_regime_acc", "valid_lle_mae",
-         "loss_ce", "loss_mse", "lle_mean", "lle_sd"]
+         "loss_ce", "loss_mse", "lle_mean", "lle_sd", "peak_only", "peak_only_class_acc"]
 
 
 def _pf(a):
@@ -487,9 +492,10 @@ def mlp_train_device(n_lat, d_cls, d_lle, epochs, batch, lr, w_lle, seed, sigma,
     return loss
 
 
-def mlp_infer_device(n_lat, mean, sd, chaos_thr, d_cls, d_chaos, d_conf, d_lle):
+def mlp_infer_device(n_lat, mean, sd, chaos_thr, d_cls, d_chaos, d_conf, d_lle,
+                     d_conf_chaos=None):
     _check(load().bifhip_mlp_infer_device(n_lat, mean, sd, chaos_thr, d_cls, d_chaos, d_conf,
-                                          d_lle))
+                                          d_conf_chaos, d_lle))
 
 
 def mlp_get_params():
@@ -547,6 +553,14 @@ def get_tile_config():
 
 def set_graph(on):
     _check(load().bifhip_set_graph(1 if on else 0))
+
+
+def set_partial_resolve(on):
+    _check(load().bifhip_set_partial_resolve(1 if on else 0))
+
+
+def set_chaos_confidence(threshold):
+    _check(load().bifhip_set_chaos_confidence(float(threshold)))
 
 
 # --------------------------------------------------------------- writers

@@ -122,6 +122,38 @@ struct Rossler {
         x2 = x2 + h * (p.b + x2 * (x0 - p.c));
     }
 
+    // Two independent trajectories (x with parameters p, y with q) advanced
+    // one step each, written operation by operation in alternation so the
+    // two dependency chains interleave in the instruction stream; every
+    // operation and its operand order are those of step(), so each result
+    // is bit-identical to step().
+    __device__ __forceinline__ static void step2(double& x0, double& x1, double& x2,
+                                                 const P64& p, double& y0, double& y1,
+                                                 double& y2, const P64& q, double h)
+    {
+        double ta = -x1 - x2, tb = -y1 - y2;
+        ta = h * ta;
+        tb = h * tb;
+        x0 = x0 + ta;
+        y0 = y0 + tb;
+        double ua = p.a * x1, ub = q.a * y1;
+        ua = x0 + ua;
+        ub = y0 + ub;
+        ua = h * ua;
+        ub = h * ub;
+        x1 = x1 + ua;
+        y1 = y1 + ub;
+        double va = x0 - p.c, vb = y0 - q.c;
+        va = x2 * va;
+        vb = y2 * vb;
+        va = p.b + va;
+        vb = q.b + vb;
+        va = h * va;
+        vb = h * vb;
+        x2 = x2 + va;
+        y2 = y2 + vb;
+    }
+
     // FP32 fast form of the same map with fused multiply-adds (7 ops):
     //   x0 <- x0 - h(x1+x2)
     //   x1 <- x1(1+ha) + h x0
@@ -178,6 +210,13 @@ struct Linear {
         x0 = x0 + h * (p.a * x0);
         x1 = x1 + h * (p.b * x1);
         x2 = x2 + h * (p.c * x2);
+    }
+    __device__ __forceinline__ static void step2(double& x0, double& x1, double& x2,
+                                                 const P64& p, double& y0, double& y1,
+                                                 double& y2, const P64& q, double h)
+    {
+        step(x0, x1, x2, p, h);
+        step(y0, y1, y2, q, h);
     }
     __device__ __forceinline__ static P32 params32(const double* v, double h)
     {

@@ -545,6 +545,31 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
 // semantics) of n cells in one trajectory pass.  Cells are the flattened
 // indices h_list[0..n) (or base..base+n when h_list is null); results go
 // to d_lle[i], d_peak[i] for launch position i.
+// Cells whose peaks did not fit their rows (entries pos = d_ovf[0..novf)
+// of a listed or ranged batch): integrate them again with rows that hold
+// every peak and cluster exactly, into d_peak[pos].
+int recompute_overflow(const PeakMapSpec& S, int maxPeaks, double T, const int* h_list, int base,
+                       int novf, const int* d_ovf, int* d_peak)
+{
+    if (novf <= 0) return BIFHIP_OK;
+    std::vector<int> pos(novf), flat(novf);
+    HIP_TRY(hipMemcpy(pos.data(), d_ovf, sizeof(int) * novf, hipMemcpyDeviceToHost));
+    for (int i = 0; i < novf; ++i) flat[i] = h_list ? h_list[pos[i]] : base + pos[i];
+    const int cap2 = (int)std::max(2LL, std::min((long long)S.N / 2 + 1, (long long)maxPeaks + 1));
+    DBuf dflat, dpos, cnt, am, iv;
+    HIP_TRY(dflat.alloc(sizeof(int) * novf));
+    HIP_TRY(dpos.alloc(sizeof(int) * novf));
+    HIP_TRY(hipMemcpy(dflat.p, flat.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dpos.p, pos.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
+    HIP_TRY(cnt.alloc(sizeof(int) * novf));
+    HIP_TRY(am.alloc(sizeof(double) * (size_t)novf * cap2));
+    HIP_TRY(iv.alloc(sizeof(double) * (size_t)novf * cap2));
+    RC_TRY(integrate_peaks(S, 0, dflat.as<int>(), novf, cap2, cnt.as<int>(), am.as<double>(),
+                           iv.as<double>()));
+    return dbscan_full_rows(novf, nullptr, novf, cnt.as<int>(), am.as<double>(), iv.as<double>(),
+                            cap2, T, d_peak, dpos.as<int>());
+}
+
 int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, double epsDb,
                        const int* h_list, const int* d_list, int base, int n, double* d_lle,
                        int* d_peak)
@@ -616,27 +641,62 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
     int novf = 0;
     HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
-    if (novf > 0) {
-        // rare: more peaks than the rows hold -- recompute those cells' peaks
-        std::vector<int> pos(novf), flat(novf);
-        HIP_TRY(hipMemcpy(pos.data(), d_ovf, sizeof(int) * novf, hipMemcpyDeviceToHost));
-        for (int i = 0; i < novf; ++i) flat[i] = h_list ? h_list[pos[i]] : base + pos[i];
-        const int cap2 = (int)std::max(2LL, std::min((long long)S.N / 2 + 1, (long long)maxPeaks + 1));
-        DBuf dflat, dpos, cnt, am, iv;
-        HIP_TRY(dflat.alloc(sizeof(int) * novf));
-        HIP_TRY(dpos.alloc(sizeof(int) * novf));
-        HIP_TRY(hipMemcpy(dflat.p, flat.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(dpos.p, pos.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
-        HIP_TRY(cnt.alloc(sizeof(int) * novf));
-        HIP_TRY(am.alloc(sizeof(double) * (size_t)novf * cap2));
-        HIP_TRY(iv.alloc(sizeof(double) * (size_t)novf * cap2));
-        RC_TRY(integrate_peaks(S, 0, dflat.as<int>(), novf, cap2, cnt.as<int>(), am.as<double>(),
-                               iv.as<double>()));
-        RC_TRY(dbscan_full_rows(novf, nullptr, novf, cnt.as<int>(), am.as<double>(),
-                                iv.as<double>(), cap2, T, d_peak, dpos.as<int>()));
-    }
+    RC_TRY(recompute_overflow(S, maxPeaks, T, h_list, base, novf, d_ovf, d_peak));
     if (g.clusterClamp > 0) HIP_TRY(launch_clamp(n, d_peak, g.clusterClamp, g.stream));
     return BIFHIP_OK;
+}
+
+// DBSCAN peak count (bifurcation2DDBSCAN semantics) of listed cells:
+// cells = h_list/d_list (flattened lattice indices) or base..base+n;
+// d_peak[t] for entry t.
+int peak_cells_device(const PeakMapSpec& S, int maxPeaks, double epsDb, const int* h_list,
+                      const int* d_list, int base, int n, int* d_peak)
+{
+    if (n <= 0) return BIFHIP_OK;
+    const int capMax = std::max(2, std::min(g.peakCap, S.N / 2 + 1));
+    int cap = capMax;
+    if ((long long)maxPeaks + 1 < cap) cap = (int)std::max(2LL, (long long)maxPeaks + 1);
+    const long long rowBytes = (long long)cap * 16 + 8;
+    long long pass = std::max<long long>(1, (8LL << 30) / rowBytes);
+    pass = std::min<long long>(pass, n);
+    if (g.maxCellsPerLaunch > 0) pass = std::min(pass, g.maxCellsPerLaunch);
+    int *d_counts, *d_ovf, *d_novf;
+    double *d_amps, *d_ivls;
+    RC_TRY(pool_get("f_counts", (size_t)pass, &d_counts));
+    RC_TRY(pool_get("f_amps", (size_t)(pass * cap), &d_amps));
+    RC_TRY(pool_get("f_ivls", (size_t)(pass * cap), &d_ivls));
+    RC_TRY(pool_get("f_ovf", (size_t)n, &d_ovf));
+    RC_TRY(pool_get("f_novf", 1, &d_novf));
+    HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
+    const double T = dbscan_threshold(epsDb);
+    for (long long off = 0; off < n; off += pass) {
+        const int m = (int)std::min<long long>(pass, n - off);
+        RC_TRY(integrate_peaks(S, base + (int)off, d_list ? d_list + off : nullptr, m, cap,
+                               d_counts, d_amps, d_ivls));
+        Timed t("dbscan");
+        HIP_TRY(launch_dbscan_wave(m, d_counts, d_amps, d_ivls, cap, cap, maxPeaks, T,
+                                   d_peak + off, d_ovf, d_novf, (int)off, g.stream));
+    }
+    int novf = 0;
+    HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    RC_TRY(recompute_overflow(S, maxPeaks, T, h_list, base, novf, d_ovf, d_peak));
+    if (g.clusterClamp > 0) HIP_TRY(launch_clamp(n, d_peak, g.clusterClamp, g.stream));
+    return BIFHIP_OK;
+}
+
+// Plain FP64 DBSCAN peak count of listed cells (surrogate pipeline, cells
+// whose LLE regime is predicted with confidence): flattened indices on an
+// nPts x nPts lattice.
+int surrogate_resolve_peaks(int nPts, double h, int nIC, const double* ic, const double* ranges,
+                            const int* mut, const double* values, int nVal, double peakTMax,
+                            double peakTransient, int wv, int maxPeaks, double dbEps,
+                            const int* h_list, const int* d_list, int n, int* d_peak)
+{
+    PeakMapSpec S;
+    RC_TRY(peak_spec(S, 2, peakTMax, nPts, h, nIC, ic, ranges, mut, wv, 0.0, peakTransient,
+                     values, nVal, 1));
+    return peak_cells_device(S, maxPeaks, dbEps, h_list, d_list, 0, n, d_peak);
 }
 
 // Plain FP64 fused LLE + peak count of listed cells (surrogate pipeline):

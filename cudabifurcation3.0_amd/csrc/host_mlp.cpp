@@ -44,11 +44,12 @@ hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, 
                        float lr, float b1, float b2, float eps, const int* step, hipStream_t st);
 hipError_t launch_fill_f32(int n, float* p, float v, hipStream_t st);
 hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, float chaosThr,
-                             unsigned char* cls, unsigned char* chaos, float* conf, float* lle,
-                             hipStream_t st);
+                             unsigned char* cls, unsigned char* chaos, float* conf,
+                             float* confChaos, float* lle, hipStream_t st);
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
-                       const float* conf, float confThr, int* list, int* count,
-                       unsigned char* flag, hipStream_t st);
+                       const float* conf, const float* confChaos, float confThr,
+                       float chaosConfThr, bool partial, int* listFull, int* countFull,
+                       int* listPeak, int* countPeak, unsigned char* flag, hipStream_t st);
 hipError_t launch_scatter(int n, const int* list, const int* peak, const double* lle,
                           int* peakMap, double* lleMap, hipStream_t st);
 hipError_t launch_fill_pred(long long n, const unsigned char* cls, const float* lle,
@@ -61,6 +62,10 @@ int surrogate_resolve(int nPts, double h, int nIC, const double* ic, const doubl
                       double peakTransient, int wv, int maxPeaks, double dbEps, double lleTMax,
                       double NT, double lleTransient, double lleEps, const int* h_list,
                       const int* d_list, int n, double* d_lle, int* d_peak);
+int surrogate_resolve_peaks(int nPts, double h, int nIC, const double* ic, const double* ranges,
+                            const int* mut, const double* values, int nVal, double peakTMax,
+                            double peakTransient, int wv, int maxPeaks, double dbEps,
+                            const int* h_list, const int* d_list, int n, int* d_peak);
 }  // namespace host
 }  // namespace bifhip
 
@@ -256,6 +261,32 @@ int train_step(const Net& N, Acts& A, const TrainCfg& T, int nLat, const int* li
     return BIFHIP_OK;
 }
 
+// Inference of the whole nLat x nLat lattice in chunks; confChaos as in
+// k_infer_post (null: conf = min of class and chaos confidence).
+int infer_impl(int nLat, double lleMean, double lleSd, double chaosThr, unsigned char* d_cls,
+               unsigned char* d_chaos, float* d_conf, float* d_confChaos, float* d_lle)
+{
+    RC_TRY(ensure_device());
+    if (nLat < 2 || !d_cls || !d_chaos || !d_conf || !d_lle)
+        return fail(BIFHIP_E_ARG, "bad inference arguments");
+    Net N;
+    RC_TRY(net_alloc(N));
+    const long long n = (long long)nLat * nLat;
+    const int chunk = (int)std::min<long long>(n, 1 << 21);
+    Acts A;
+    RC_TRY(acts_alloc(A, chunk, "inf_"));
+    for (long long off = 0; off < n; off += chunk) {
+        const int rows = (int)std::min<long long>(chunk, n - off);
+        Timed t("mlp_infer");
+        RC_TRY(forward(N, A, rows, nLat, nullptr, off, g.stream));
+        HIP_TRY(launch_infer_post(rows, A.out, (float)lleMean, (float)lleSd, (float)chaosThr,
+                                  d_cls + off, d_chaos + off, d_conf + off,
+                                  d_confChaos ? d_confChaos + off : nullptr, d_lle + off,
+                                  g.stream));
+    }
+    return BIFHIP_OK;
+}
+
 double ms_since(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
@@ -362,23 +393,10 @@ int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int 
 // confidence and LLE (de-standardised with mean/sd) per cell.
 int bifhip_mlp_infer_device(int nLat, double lleMean, double lleSd, double chaosThr,
                             unsigned char* d_cls, unsigned char* d_chaos, float* d_conf,
-                            float* d_lle)
+                            float* d_confChaos, float* d_lle)
 {
-    RC_TRY(ensure_device());
-    Net N;
-    RC_TRY(net_alloc(N));
-    const long long n = (long long)nLat * nLat;
-    const int chunk = (int)std::min<long long>(n, 1 << 21);
-    Acts A;
-    RC_TRY(acts_alloc(A, chunk, "inf_"));
-    for (long long off = 0; off < n; off += chunk) {
-        const int rows = (int)std::min<long long>(chunk, n - off);
-        Timed t("mlp_infer");
-        RC_TRY(forward(N, A, rows, nLat, nullptr, off, g.stream));
-        HIP_TRY(launch_infer_post(rows, A.out, (float)lleMean, (float)lleSd, (float)chaosThr,
-                                  d_cls + off, d_chaos + off, d_conf + off, d_lle + off,
-                                  g.stream));
-    }
+    RC_TRY(infer_impl(nLat, lleMean, lleSd, chaosThr, d_cls, d_chaos, d_conf, d_confChaos,
+                      d_lle));
     return finish();
 }
 
@@ -520,45 +538,51 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     // 3. inference
     const auto t2 = std::chrono::steady_clock::now();
     const long long nf = (long long)nFine * nFine;
+    const bool partial = g.partialResolve != 0;
     unsigned char *d_pcls, *d_pch, *d_flag;
-    float *d_conf, *d_plle;
+    float *d_conf, *d_cconf, *d_plle;
     RC_TRY(pool_get("sg_pcls", (size_t)nf, &d_pcls));
     RC_TRY(pool_get("sg_flag", (size_t)nf, &d_flag));
     RC_TRY(pool_get("sg_pch", (size_t)nf, &d_pch));
     RC_TRY(pool_get("sg_conf", (size_t)nf, &d_conf));
+    RC_TRY(pool_get("sg_cconf", (size_t)nf, &d_cconf));
     RC_TRY(pool_get("sg_plle", (size_t)nf, &d_plle));
-    RC_TRY(bifhip_mlp_infer_device(nFine, mean, sd, chaosThreshold, d_pcls, d_pch, d_conf,
-                                   d_plle));
+    RC_TRY(infer_impl(nFine, mean, sd, chaosThreshold, d_pcls, d_pch, d_conf, d_cconf, d_plle));
+    HIP_TRY(hipStreamSynchronize(s));
     st[BIFHIP_SSTAT_MS_INFER] = ms_since(t2);
 
     // 4. flag and re-solve
     const auto t3 = std::chrono::steady_clock::now();
-    int *d_list, *d_count;
+    int *d_list, *d_plist, *d_count;
     RC_TRY(pool_get("sg_list", (size_t)nf, &d_list));
-    RC_TRY(pool_get("sg_count", 1, &d_count));
-    HIP_TRY(hipMemsetAsync(d_count, 0, sizeof(int), s));
-    HIP_TRY(launch_flag(nFine, d_pcls, d_pch, d_conf, (float)confThreshold, d_list, d_count,
+    RC_TRY(pool_get("sg_plist", (size_t)(partial ? nf : 1), &d_plist));
+    RC_TRY(pool_get("sg_count", 2, &d_count));
+    HIP_TRY(hipMemsetAsync(d_count, 0, 2 * sizeof(int), s));
+    HIP_TRY(launch_flag(nFine, d_pcls, d_pch, d_conf, d_cconf, (float)confThreshold,
+                        (float)g.chaosConfidence, partial, d_list, d_count, d_plist, d_count + 1,
                         d_flag, s));
     HIP_TRY(launch_fill_pred(nf, d_pcls, d_plle, d_peak, d_lle, s));
-    int nflag = 0;
-    HIP_TRY(hipMemcpyAsync(&nflag, d_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    int counts[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(counts, d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    const int nflag = counts[0], npeak = counts[1];
     // validation sample of the unflagged cells (seeded)
     std::vector<int> vlist;
+    std::vector<unsigned char> flag;
     if (validationFraction > 0) {
-        std::vector<unsigned char> flag((size_t)nf);
+        flag.resize((size_t)nf);
         HIP_TRY(hipMemcpy(flag.data(), d_flag, (size_t)nf, hipMemcpyDeviceToHost));
         // draw distinct unflagged cells until the sample holds
         // fraction * (unflagged cells) of them
         std::mt19937_64 rv(seed + 17);
-        const long long want =
-            (long long)std::llround(std::min(1.0, validationFraction) * (double)(nf - nflag));
+        const long long want = (long long)std::llround(std::min(1.0, validationFraction) *
+                                                       (double)(nf - nflag - npeak));
         long long tries = 0;
         while ((long long)vlist.size() < want && tries < 8 * want + 64) {
             ++tries;
             const long long i = (long long)(rv() % (uint64_t)nf);
             if (flag[i]) continue;
-            flag[i] = 2;   // taken
+            flag[i] = 3;   // taken
             vlist.push_back((int)i);
         }
     }
@@ -567,35 +591,46 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     int* d_all;
     RC_TRY(pool_get("sg_all", (size_t)std::max(1, ntot), &d_all));
     HIP_TRY(hipMemcpy(d_all, d_list, sizeof(int) * nflag, hipMemcpyDeviceToDevice));
-    if (nval) HIP_TRY(hipMemcpy(d_all + nflag, vlist.data(), sizeof(int) * nval, hipMemcpyHostToDevice));
-    std::vector<int> hall(ntot);
+    if (nval)
+        HIP_TRY(hipMemcpy(d_all + nflag, vlist.data(), sizeof(int) * nval,
+                          hipMemcpyHostToDevice));
+    std::vector<int> hall(ntot), hpeak(npeak);
     HIP_TRY(hipMemcpy(hall.data(), d_all, sizeof(int) * ntot, hipMemcpyDeviceToHost));
-    int* d_rpeak;
+    if (npeak)
+        HIP_TRY(hipMemcpy(hpeak.data(), d_plist, sizeof(int) * npeak, hipMemcpyDeviceToHost));
+    int *d_rpeak, *d_ppeak;
     double* d_rlle;
     RC_TRY(pool_get("sg_rpeak", (size_t)std::max(1, ntot), &d_rpeak));
     RC_TRY(pool_get("sg_rlle", (size_t)std::max(1, ntot), &d_rlle));
+    RC_TRY(pool_get("sg_ppeak", (size_t)std::max(1, npeak), &d_ppeak));
     if (ntot > 0)
         RC_TRY(surrogate_resolve(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
                                  peakTransient, wv, maxPeaks, dbEps, lleTMax, NT, lleTransient,
                                  lleEps, hall.data(), d_all, ntot, d_rlle, d_rpeak));
+    if (npeak > 0)
+        RC_TRY(surrogate_resolve_peaks(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
+                                       peakTransient, wv, maxPeaks, dbEps, hpeak.data(),
+                                       d_plist, npeak, d_ppeak));
     // score the surrogate on every re-solved cell before overwriting
-    std::vector<int> rpeak(ntot);
+    std::vector<int> rpeak(ntot), ppeak(npeak);
     std::vector<double> rlle(ntot);
-    std::vector<unsigned char> pc(ntot), ph(ntot);
+    std::vector<unsigned char> pc(ntot), ph(ntot), pcp(npeak);
     std::vector<float> pl(ntot);
-    if (ntot > 0) {
-        HIP_TRY(hipMemcpy(rpeak.data(), d_rpeak, sizeof(int) * ntot, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(rlle.data(), d_rlle, sizeof(double) * ntot, hipMemcpyDeviceToHost));
+    if (ntot + npeak > 0) {
         std::vector<unsigned char> allc((size_t)nf), allh((size_t)nf);
-        HIP_TRY(hipMemcpy(allh.data(), d_pch, (size_t)nf, hipMemcpyDeviceToHost));
         std::vector<float> alll((size_t)nf);
         HIP_TRY(hipMemcpy(allc.data(), d_pcls, (size_t)nf, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(allh.data(), d_pch, (size_t)nf, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(alll.data(), d_plle, sizeof(float) * nf, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(rpeak.data(), d_rpeak, sizeof(int) * ntot, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(rlle.data(), d_rlle, sizeof(double) * ntot, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ppeak.data(), d_ppeak, sizeof(int) * npeak, hipMemcpyDeviceToHost));
         for (int i = 0; i < ntot; ++i) {
             pc[i] = allc[hall[i]];
             ph[i] = allh[hall[i]];
             pl[i] = alll[hall[i]];
         }
+        for (int i = 0; i < npeak; ++i) pcp[i] = allc[hpeak[i]];
     }
     auto score = [&](int lo, int hi, double* clsAcc, double* regAcc, double* mae) {
         long long n = 0, okc = 0, okr = 0, nl = 0;
@@ -620,15 +655,27 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
           &st[BIFHIP_SSTAT_FLAGGED_LLE_MAE]);
     score(nflag, ntot, &st[BIFHIP_SSTAT_VALID_CLASS_ACC], &st[BIFHIP_SSTAT_VALID_REGIME_ACC],
           &st[BIFHIP_SSTAT_VALID_LLE_MAE]);
+    {
+        long long okc = 0;
+        for (int i = 0; i < npeak; ++i)
+            okc += (std::min(kClasses, std::max(1, ppeak[i])) - 1) == pcp[i];
+        st[BIFHIP_SSTAT_PEAK_ONLY_CLASS_ACC] = npeak ? (double)okc / npeak : 1.0;
+    }
     HIP_TRY(launch_scatter(ntot, d_all, d_rpeak, d_rlle, d_peak, d_lle, s));
+    HIP_TRY(launch_scatter(npeak, d_plist, d_ppeak, nullptr, d_peak, d_lle, s));
     if (d_resolved) {
-        HIP_TRY(hipMemcpyAsync(d_resolved, d_flag, (size_t)nf, hipMemcpyDeviceToDevice, s));
+        // 1 re-solved in full, 2 peak count only, 3 validation sample, 0 predicted
+        if (!flag.empty())
+            HIP_TRY(hipMemcpyAsync(d_resolved, flag.data(), (size_t)nf, hipMemcpyHostToDevice, s));
+        else
+            HIP_TRY(hipMemcpyAsync(d_resolved, d_flag, (size_t)nf, hipMemcpyDeviceToDevice, s));
     }
     HIP_TRY(hipStreamSynchronize(s));
     st[BIFHIP_SSTAT_MS_RESOLVE] = ms_since(t3);
     st[BIFHIP_SSTAT_RESOLVED] = nflag;
     st[BIFHIP_SSTAT_VALIDATION] = nval;
     st[BIFHIP_SSTAT_RESOLVED_FRAC] = (double)nflag / (double)nf;
+    st[BIFHIP_SSTAT_PEAK_ONLY] = npeak;
     st[BIFHIP_SSTAT_MS_TOTAL] = ms_since(t0);
     st[BIFHIP_SSTAT_LLE_MEAN] = mean;
     st[BIFHIP_SSTAT_LLE_SD] = sd;

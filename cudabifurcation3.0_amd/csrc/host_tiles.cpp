@@ -186,6 +186,20 @@ int bifhip_set_graph(int on)
     return BIFHIP_OK;
 }
 
+int bifhip_set_partial_resolve(int on)
+{
+    g.partialResolve = on != 0;
+    return BIFHIP_OK;
+}
+
+int bifhip_set_chaos_confidence(double threshold)
+{
+    if (!(threshold >= 0.5 && threshold <= 1.0))
+        return fail(BIFHIP_E_ARG, "chaos confidence threshold %g outside [0.5, 1]", threshold);
+    g.chaosConfidence = threshold;
+    return BIFHIP_OK;
+}
+
 static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, const int* originJ,
                       const int* nBlocks, int nPts, const double* ranges, const int* mut,
                       const double* ic, int nIC, const double* values, int nVal, double h,

@@ -141,6 +141,71 @@ __global__ __launch_bounds__(256) void k_bif_peaks(const BifArgs A, int* __restr
     counts[t] = bad ? -1 : ps.count();
 }
 
+// below this many cells a launch keeps one cell per lane (occupancy first)
+constexpr int kPairCells = 1 << 20;
+
+// Same map without the divergence check, two cells per lane (cells 2t and
+// 2t+1 of the launch) so that two independent trajectories interleave in
+// the instruction stream (Rossler::step2); each cell's arithmetic is that
+// of k_bif_peaks.  An odd last lane integrates its single cell twice and
+// stores it once.
+
+template <class M, int WV>
+__global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __restrict__ counts,
+                                                    double* __restrict__ amps,
+                                                    double* __restrict__ ivls)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c0 = 2 * t;
+    if (c0 >= A.nCells) return;
+    const bool two = c0 + 1 < A.nCells;
+    const int c1 = two ? c0 + 1 : c0;
+    const int i0 = A.cellList ? A.cellList[c0] : A.cellBase + c0;
+    const int i1 = A.cellList ? A.cellList[c1] : A.cellBase + c1;
+
+    double v[kMaxVal];
+    cell_values(A.g, i0, v);
+    const typename M::P64 p = M::params(v);
+    cell_values(A.g, i1, v);
+    const typename M::P64 q = M::params(v);
+    const double h = A.h;
+    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
+    double y0 = x0, y1 = x1, y2 = x2;
+
+    int s = 0;
+    for (; s + 2 <= A.skip; s += 2) {
+        M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+        M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+    }
+    for (; s < A.skip; ++s) M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+
+    PeakSink sa, sb;
+    sa.amps = amps ? amps + (long long)c0 * A.rowStride : nullptr;
+    sa.ivls = ivls ? ivls + (long long)c0 * A.rowStride : nullptr;
+    sb.amps = amps && two ? amps + (long long)c1 * A.rowStride : nullptr;
+    sb.ivls = ivls && two ? ivls + (long long)c1 * A.rowStride : nullptr;
+    sa.cap = sb.cap = A.cap;
+    sa.hpf = sb.hpf = A.hpf;
+    PeakStream pa, pb;
+    pa.init();
+    pb.init();
+    const int N = A.N;
+    for (int i = 0; i < N; ++i) {
+        const double da = pick<M, WV>(x0, x1, x2), db = pick<M, WV>(y0, y1, y2);
+        if (i >= 2) {
+            pa.push(da, i, sa);
+            pb.push(db, i, sb);
+        } else {
+            pa.p2 = pa.p1; pa.p1 = da;
+            pb.p2 = pb.p1; pb.p1 = db;
+        }
+        for (int j = 0; j < A.pre - 1; ++j) M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+        M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+    }
+    counts[c0] = pa.count();
+    if (two) counts[c1] = pb.count();
+}
+
 // Raw recorded series (the reference's data[] block) for parity tests.
 template <class M>
 __global__ __launch_bounds__(256) void k_series(const BifArgs A, int wv, double* __restrict__ data,
@@ -669,16 +734,17 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
     for (int b = 0; b < F.nBlocks; ++b) {
         if (k + F.nNT <= w0 || k >= w1) {
             // no recorded state in this block: plain interleaved stepping
-            for (int i = 0; i < F.nNT; ++i) {
-                M::step(x0, x1, x2, p, h);
-                M::step(y0, y1, y2, p, h);
+            int i = 0;
+            for (; i + 2 <= F.nNT; i += 2) {
+                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
+                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
             }
+            for (; i < F.nNT; ++i) M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
             k += F.nNT;
         } else {
             for (int i = 0; i < F.nNT; ++i, ++k) {
                 record(k);
-                M::step(x0, x1, x2, p, h);
-                M::step(y0, y1, y2, p, h);
+                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
             }
         }
         double s = 0;
@@ -737,6 +803,14 @@ static hipError_t launch_bif_m(const BifArgs& A, int wv, bool check, int* counts
 {
     const dim3 blk(256), grd((A.nCells + 255) / 256);
     if (A.nCells <= 0) return hipSuccess;
+    if (!check && A.nCells >= kPairCells) {
+        // enough cells to fill the device at two per lane
+        const dim3 grd2((A.nCells / 2 + 1 + 255) / 256);
+#define BIF2_CASE(W) hipLaunchKernelGGL((k_bif_peaks2<M, W>), grd2, blk, 0, st, A, counts, amps, ivls)
+        if (wv == 0) BIF2_CASE(0); else if (wv == 1) BIF2_CASE(1); else BIF2_CASE(2);
+#undef BIF2_CASE
+        return hipGetLastError();
+    }
 #define BIF_CASE(W, C) hipLaunchKernelGGL((k_bif_peaks<M, W, C>), grd, blk, 0, st, A, counts, amps, ivls)
     if (check) {
         if (wv == 0) BIF_CASE(0, true); else if (wv == 1) BIF_CASE(1, true); else BIF_CASE(2, true);

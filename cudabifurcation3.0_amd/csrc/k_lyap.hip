@@ -40,15 +40,10 @@ __device__ __forceinline__ void run2(double& x0, double& x1, double& x2, double&
     if (!CHECK) {
         int i = 0;
         for (; i + 2 <= nIter; i += 2) {
-            M::step(x0, x1, x2, p, h);
-            M::step(y0, y1, y2, p, h);
-            M::step(x0, x1, x2, p, h);
-            M::step(y0, y1, y2, p, h);
+            M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
+            M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
         }
-        for (; i < nIter; ++i) {
-            M::step(x0, x1, x2, p, h);
-            M::step(y0, y1, y2, p, h);
-        }
+        for (; i < nIter; ++i) M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
         return;
     }
     bool gx = true, gy = true;

@@ -279,14 +279,17 @@ __global__ void k_fill_f32(int n, float* p, float v)
 }
 
 // Inference post-processing of head outputs: class = argmax of the logits,
-// confidence = its softmax probability, LLE = column 0 de-standardised.
-// Chaos regime from the chaos logit; conf = min(class probability, chaos
-// probability), and 0 when the LLE regression disagrees with the chaos
-// classifier about the side of chaosThr (such a cell is always re-solved).
+// chaos regime from the chaos logit, LLE = column 0 de-standardised and
+// kept on the side of chaosThr the chaos logit predicts (the regime of a
+// predicted cell is the classifier's; the regression supplies the value).
+// Class confidence = the class's softmax probability; chaos confidence =
+// the chaos classifier's probability of its answer.  With confChaos null,
+// conf = min of the two; else conf = class confidence and confChaos =
+// chaos confidence.
 __global__ void k_infer_post(int rows, const float* __restrict__ out, float lleMean,
                              float lleStd, float chaosThr, unsigned char* __restrict__ cls,
                              unsigned char* __restrict__ chaos, float* __restrict__ conf,
-                             float* __restrict__ lle)
+                             float* __restrict__ confChaos, float* __restrict__ lle)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
@@ -299,41 +302,79 @@ __global__ void k_infer_post(int rows, const float* __restrict__ out, float lleM
     for (int k = 1; k <= kClasses; ++k) se += __expf(o[k] - mx);
     const float pc = 1.0f / (1.0f + __expf(-o[kChaosCol]));
     const bool ch = pc > 0.5f;
-    const float l = o[0] * lleStd + lleMean;
-    float cf = fminf(1.0f / se, ch ? pc : 1.0f - pc);
-    if ((l > chaosThr) != ch) cf = 0.0f;
+    float l = o[0] * lleStd + lleMean;
+    const float margin = fmaxf(1e-6f, fabsf(chaosThr) * 1e-4f);
+    if (ch && !(l > chaosThr)) l = chaosThr + margin;
+    if (!ch && l > chaosThr) l = chaosThr - margin;
+    const float cc = ch ? pc : 1.0f - pc;
+    const float ck = 1.0f / se;
     cls[r] = (unsigned char)best;
     chaos[r] = ch ? 1 : 0;
-    conf[r] = cf;
+    if (confChaos) {
+        conf[r] = ck;
+        confChaos[r] = cc;
+    } else {
+        conf[r] = fminf(ck, cc);
+    }
     lle[r] = l;
 }
 
-// Cells to re-solve: low confidence, or a 4-neighbour with another class or
-// another LLE regime (chaotic when LLE > thr).  Appended to list[].
+// Wave-aggregated append of the lanes with `take` to list[]: one atomic per
+// wavefront, lanes in lane order after the wave's base.
+__device__ __forceinline__ void wave_append(bool take, int value, int* __restrict__ list,
+                                            int* __restrict__ count)
+{
+    const uint64_t m = __ballot(take);
+    if (!m) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __builtin_ctzll(m);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader);
+    if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+// Cells to re-solve.  A cell needs its peak-count class when the class
+// confidence is below confThr or a 4-neighbour has another class, and its
+// LLE regime when the chaos confidence is below chaosConfThr or a
+// 4-neighbour has another regime.  With confChaos null (conf = min of both,
+// compared with confThr) every such cell is re-solved in full.  Otherwise
+// cells that need their regime are re-solved in full (flag 1, listFull)
+// and cells that only need their class get the peak count alone (flag 2,
+// listPeak); with `partial` false both go to listFull.
 __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
                        const unsigned char* __restrict__ chaos, const float* __restrict__ conf,
-                       float confThr, int* __restrict__ list, int* __restrict__ count,
+                       const float* __restrict__ confChaos, float confThr, float chaosConfThr,
+                       bool partial, int* __restrict__ listFull, int* __restrict__ countFull,
+                       int* __restrict__ listPeak, int* __restrict__ countPeak,
                        unsigned char* __restrict__ flag)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)n * n) return;
-    const int i = (int)(idx % n), j = (int)(idx / n);
-    const int c0 = cls[idx];
-    const int ch0 = chaos[idx];
-    bool f = conf[idx] < confThr;
-    const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+    const bool live = idx < (long long)n * n;   // no early return: the appends are wave-wide
+    bool fc = false, fh = false;
+    if (live) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        const int c0 = cls[idx];
+        const int ch0 = chaos[idx];
+        fc = conf[idx] < confThr;
+        fh = confChaos ? confChaos[idx] < chaosConfThr : fc;
+        const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int ii = i + di[q], jj = j + dj[q];
-        if (ii < 0 || jj < 0 || ii >= n || jj >= n) continue;
-        const long long o = (long long)jj * n + ii;
-        if (cls[o] != c0 || chaos[o] != ch0) f = true;
+        for (int q = 0; q < 4; ++q) {
+            const int ii = i + di[q], jj = j + dj[q];
+            if (ii < 0 || jj < 0 || ii >= n || jj >= n) continue;
+            const long long o = (long long)jj * n + ii;
+            fc |= cls[o] != c0;
+            fh |= chaos[o] != ch0;
+        }
+        if (!confChaos || !partial) fh = fh || fc;
+        flag[idx] = fh ? 1 : (fc ? 2 : 0);
     }
-    flag[idx] = f ? 1 : 0;
-    if (f) list[atomicAdd(count, 1)] = (int)idx;
+    wave_append(live && fh, (int)idx, listFull, countFull);
+    wave_append(live && !fh && fc, (int)idx, listPeak, countPeak);
 }
 
-// Write re-solved cells into the output maps.
+// Write re-solved cells into the output maps (LLE only when lleIn is given).
 __global__ void k_scatter(int n, const int* __restrict__ list, const int* __restrict__ peak,
                           const double* __restrict__ lleIn, int* __restrict__ peakMap,
                           double* __restrict__ lleMap)
@@ -342,7 +383,7 @@ __global__ void k_scatter(int n, const int* __restrict__ list, const int* __rest
     if (t >= n) return;
     const int c = list[t];
     peakMap[c] = peak[t];
-    lleMap[c] = lleIn[t];
+    if (lleIn) lleMap[c] = lleIn[t];
 }
 
 // Surrogate prediction into the output maps (class k -> peak count k+1).
@@ -446,21 +487,23 @@ hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, 
 }
 
 hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, float chaosThr,
-                             unsigned char* cls, unsigned char* chaos, float* conf, float* lle,
-                             hipStream_t st)
+                             unsigned char* cls, unsigned char* chaos, float* conf,
+                             float* confChaos, float* lle, hipStream_t st)
 {
     hipLaunchKernelGGL(k_infer_post, dim3((rows + 255) / 256), dim3(256), 0, st, rows, out, mean,
-                       sd, chaosThr, cls, chaos, conf, lle);
+                       sd, chaosThr, cls, chaos, conf, confChaos, lle);
     return hipGetLastError();
 }
 
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
-                       const float* conf, float confThr, int* list, int* count,
-                       unsigned char* flag, hipStream_t st)
+                       const float* conf, const float* confChaos, float confThr,
+                       float chaosConfThr, bool partial, int* listFull, int* countFull,
+                       int* listPeak, int* countPeak, unsigned char* flag, hipStream_t st)
 {
     const long long cells = (long long)n * n;
     hipLaunchKernelGGL(k_flag, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, n, cls,
-                       chaos, conf, confThr, list, count, flag);
+                       chaos, conf, confChaos, confThr, chaosConfThr, partial, listFull,
+                       countFull, listPeak, countPeak, flag);
     return hipGetLastError();
 }
 

@@ -306,27 +306,45 @@ int bifhip_get_tile_config(int* blockSize, int* cellsPerLane, int* blocksPerCU);
 /* Replay the tile launches and the surrogate's training epochs from captured
  * HIP graphs (default 1); 0 launches the same kernels directly. */
 int bifhip_set_graph(int on);
+/* Surrogate-refined map: cells whose LLE regime is predicted with
+ * confidence and that need only their peak-count class re-solve the peak
+ * count alone (default 1); 0 re-solves every flagged cell in full. */
+int bifhip_set_partial_resolve(int on);
+/* Surrogate-refined map: cells whose chaos-logit confidence is below this
+ * threshold (in [0.5, 1], default 0.99) have their LLE re-solved. */
+int bifhip_set_chaos_confidence(double threshold);
 
 /* ------------------------------------------ surrogate-refined map */
 
 /* No counterpart in the reference.  A small network (64 Fourier features of
- * (a, c) -> 3 x 128 ReLU -> LLE + 16 peak-count classes), trained on a
- * coarse map the plain FP64 method produces and run with BF16 matrix-core
- * products and FP32 accumulation, predicts the fine map; every cell with
- * softmax confidence below confThreshold or a 4-neighbour of another class
- * or LLE regime (chaotic when LLE > chaosThreshold) is re-solved by the
- * plain FP64 fused kernel (bifhip_lle_peak_map_device semantics), which
- * also scores the surrogate on those cells and on a validation sample of
- * validationFraction of the rest.  Outputs: d_peak[nFine^2] (re-solved
- * cells: DBSCAN count; others: predicted class + 1 in 1..16),
- * d_lle[nFine^2], d_resolved[nFine^2] (1 where re-solved; may be NULL),
+ * (a, c) -> 3 x 128 ReLU -> LLE, 16 peak-count classes and a chaos logit),
+ * trained on a coarse map the plain FP64 method produces and run with BF16
+ * matrix-core products and FP32 accumulation, predicts the fine map: the
+ * class from the class logits, the LLE regime (chaotic when LLE >
+ * chaosThreshold) from the chaos logit, the LLE value from the regression
+ * head, kept on the predicted side of chaosThreshold.  A cell needs its
+ * class re-solved when the class softmax confidence is below confThreshold
+ * or a 4-neighbour has another predicted class, and its regime when the
+ * chaos confidence is below bifhip_set_chaos_confidence's threshold (0.99
+ * by default) or a 4-neighbour has another predicted regime.  Cells that
+ * need their regime are re-solved by the plain FP64 fused kernel
+ * (bifhip_lle_peak_map_device semantics).  Cells that need only their class
+ * get the plain FP64 DBSCAN peak count (bifurcation2DDBSCAN semantics) and
+ * keep the predicted LLE when partial re-solves are on
+ * (bifhip_set_partial_resolve, default), else they are re-solved in full.
+ * The re-solves score the surrogate; a validation sample of
+ * validationFraction of the remaining cells is re-solved in full to score
+ * it there.  Outputs: d_peak[nFine^2] (re-solved cells: DBSCAN count;
+ * others: predicted class + 1 in 1..16), d_lle[nFine^2],
+ * d_resolved[nFine^2] (1 re-solved in full, 2 peak count only, 3 validation
+ * sample re-solved in full, 0 predicted; may be NULL),
  * stats[BIFHIP_SSTAT_COUNT]. */
 #define BIFHIP_SSTAT_MS_TOTAL 0
 #define BIFHIP_SSTAT_MS_COARSE 1
 #define BIFHIP_SSTAT_MS_TRAIN 2
 #define BIFHIP_SSTAT_MS_INFER 3
 #define BIFHIP_SSTAT_MS_RESOLVE 4
-#define BIFHIP_SSTAT_RESOLVED 5            /* cells re-solved (flagged)          */
+#define BIFHIP_SSTAT_RESOLVED 5            /* cells re-solved in full            */
 #define BIFHIP_SSTAT_RESOLVED_FRAC 6
 #define BIFHIP_SSTAT_VALIDATION 7          /* unflagged cells re-solved to score */
 #define BIFHIP_SSTAT_FLAGGED_CLASS_ACC 8   /* surrogate vs FP64 on flagged cells */
@@ -339,7 +357,9 @@ int bifhip_set_graph(int on);
 #define BIFHIP_SSTAT_LOSS_MSE 15
 #define BIFHIP_SSTAT_LLE_MEAN 16
 #define BIFHIP_SSTAT_LLE_SD 17
-#define BIFHIP_SSTAT_COUNT 18
+#define BIFHIP_SSTAT_PEAK_ONLY 18          /* cells re-solved for the class only */
+#define BIFHIP_SSTAT_PEAK_ONLY_CLASS_ACC 19
+#define BIFHIP_SSTAT_COUNT 20
 
 int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int amountOfInitialConditions,
                                 const double* initialConditions, const double* ranges,
@@ -360,10 +380,14 @@ int bifhip_mlp_param_count(void);
 int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int epochs,
                             int batch, double lr, double wLle, unsigned long long seed,
                             double fourierSigma, double chaosStd, int useGraph, float* loss_out);
-/* Inference of an nLat x nLat lattice: class, chaos regime, confidence, LLE. */
+/* Inference of an nLat x nLat lattice: class, chaos regime, confidence, LLE.
+ * The regime comes from the chaos logit; the LLE value from the regression
+ * head, kept on that side of chaosThr.  With d_confChaos NULL, d_conf =
+ * min(class confidence, chaos confidence); else d_conf = class confidence
+ * and d_confChaos = chaos confidence. */
 int bifhip_mlp_infer_device(int nLat, double lleMean, double lleSd, double chaosThr,
                             unsigned char* d_cls, unsigned char* d_chaos, float* d_conf,
-                            float* d_lle);
+                            float* d_confChaos, float* d_lle);
 int bifhip_mlp_get_params(float* params, int nParams, float* fourier, int nFourier);
 /* One forward + backward pass (host arrays): cells are flattened indices on
  * an nLat x nLat lattice, cls/lle their targets; grads in parameter

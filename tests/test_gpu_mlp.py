@@ -122,12 +122,14 @@ def test_training_learns_a_synthetic_map(bh):
     assert loss[0] < 0.3
 
 
-def test_surrogate_pipeline_small(bh):
+@pytest.mark.parametrize("partial", [True, False])
+def test_surrogate_pipeline_small(bh, partial):
     nf, nc = 96, 24
     peak = dict(tmax=40.0, trans=20.0)
     lle = dict(tmax=50.0, NT=0.5, trans=10.0, eps=1e-8)
     bh.set_perturbation(bh.PERT_AXIS_X)
-    say("surrogate pipeline", nf, nc)
+    bh.set_partial_resolve(partial)
+    say("surrogate pipeline", nf, nc, "partial", partial)
     d_pk = bh.DeviceArray((nf * nf,), np.int32).zero_()
     d_ll = bh.DeviceArray((nf * nf,), np.float64).zero_()
     d_rs = bh.DeviceArray((nf * nf,), np.uint8).zero_()
@@ -141,14 +143,22 @@ def test_surrogate_pipeline_small(bh):
     bh.lle_peak_map_device(t_pk.ptr, t_ll.ptr, nf, 0.01, IC, R2, [0, 2], VALS, peak["tmax"],
                            peak["trans"], 0, 100000, 0.01, lle["tmax"], lle["NT"], lle["trans"],
                            lle["eps"])
-    pk, ll, rs = d_pk.to_host(), d_ll.to_host(), d_rs.to_host().astype(bool)
+    pk, ll, rs = d_pk.to_host(), d_ll.to_host(), d_rs.to_host()
     tpk, tll = t_pk.to_host(), t_ll.to_host()
-    assert rs.sum() == st["resolved"]
-    np.testing.assert_array_equal(pk[rs], tpk[rs])
-    same = (np.isnan(ll[rs]) & np.isnan(tll[rs])) | (ll[rs] == tll[rs])
+    full = (rs == 1) | (rs == 3)
+    assert (rs == 1).sum() == st["resolved"]
+    assert (rs == 2).sum() == st["peak_only"]
+    assert (rs == 3).sum() == st["validation"] > 0
+    if not partial:
+        assert st["peak_only"] == 0
+    # cells re-solved in full equal the plain FP64 map exactly; cells
+    # re-solved for the class carry the exact peak count
+    np.testing.assert_array_equal(pk[full], tpk[full])
+    same = (np.isnan(ll[full]) & np.isnan(tll[full])) | (ll[full] == tll[full])
     assert same.all()
+    np.testing.assert_array_equal(pk[rs == 2], tpk[rs == 2])
     cls_t = np.clip(tpk, 1, 16)
     cls_s = np.clip(pk, 1, 16)
     reg = (cls_t == cls_s) & ((tll > 0.01) == (ll > 0.01))
-    say("regime agreement", reg.mean(), "resolved fraction", rs.mean())
+    say("regime agreement", reg.mean(), "full", full.mean(), "peak only", (rs == 2).mean())
     assert reg.mean() > 0.9

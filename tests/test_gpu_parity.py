@@ -262,6 +262,19 @@ def test_dbscan_map_config2_rows_bit_exact(orc, bh, max_peaks):
         assert len(np.unique(m)) > 10
 
 
+def test_dbscan_map_two_cells_per_lane(orc, bh):
+    """A launch of at least 2^20 cells integrates two cells per lane; a
+    1024 x 1024 map (short window) against the oracle on whole rows,
+    including the last cell of the launch."""
+    say("dbscan map 1024^2, two cells per lane")
+    m = bh.dbscan_map(30.0, 1024, 0.01, IC, R2, [0, 2], 0, 0.0, 100000, 20.0, VALS, 1, 0.01)
+    m = m.reshape(1024, 1024)
+    for row in (0, 333, 1023):
+        want = orc.dbscan_cells(30.0, 1024, 0.01, IC, R2, [0, 2], 0, 0.0, 100000, 20.0, VALS, 1,
+                                0.01, row * 1024, 1024)
+        np.testing.assert_array_equal(m[row], want, err_msg=f"row {row}")
+
+
 def test_dbscan_overflow_recompute(orc, bh):
     """Rows of 8 slots force every cell through the exact recompute path."""
     say("dbscan overflow recompute")

@@ -64,8 +64,20 @@ struct PeakStream {
         ++found;
     }
 
-    // sample v has index t (t >= 2 is checked by the caller's loop shape)
+    // sample v has index t (t >= 2 is checked by the caller's loop shape).
+    // A lane in kSeek without a candidate (the common case: a peak comes
+    // every few hundred samples) only shifts its window, so the state
+    // machine runs only when some lane of the wavefront needs it.
     __device__ __forceinline__ void push(double v, int t, const PeakSink& s)
+    {
+        // bitwise (not short-circuit) so the test compiles without branches
+        const bool busy = (mode != kSeek) | ((p1 > p2) & (p1 >= v));
+        if (__any(busy)) machine(v, t, s);
+        p2 = p1;
+        p1 = v;
+    }
+
+    __device__ __forceinline__ void machine(double v, int t, const PeakSink& s)
     {
         if (mode == kSeek) {
             if (p1 > p2 && p1 >= v) {
@@ -78,8 +90,6 @@ struct PeakStream {
         } else {
             mode = kSeek;
         }
-        p2 = p1;
-        p1 = v;
     }
 
     __device__ __forceinline__ int count() const { return found > 1 ? found - 1 : 0; }
@@ -130,13 +140,27 @@ __global__ __launch_bounds__(256) void k_bif_peaks(const BifArgs A, int* __restr
     bool bad = false;
     const int N = A.N;
     const int pre = A.pre;
-    for (int i = 0; i < N; ++i) {
+    int i = 0;
+    for (; i < N && i < 2; ++i) {
         const double d = pick<M, WV>(x0, x1, x2);
-        if (i >= 2) ps.push(d, i, sink);
-        else { ps.p2 = ps.p1; ps.p1 = d; }
+        ps.p2 = ps.p1;
+        ps.p1 = d;
         for (int j = 0; j < pre - 1; ++j) M::step(x0, x1, x2, p, h);
         M::step(x0, x1, x2, p, h);
         if (CHECK && diverged(pick<M, WV>(x0, x1, x2), A.maxValue)) { bad = true; break; }
+    }
+    if (!CHECK && pre == 1) {
+        for (; i < N; ++i) {
+            ps.push(pick<M, WV>(x0, x1, x2), i, sink);
+            M::step(x0, x1, x2, p, h);
+        }
+    } else {
+        for (; i < N && !bad; ++i) {
+            ps.push(pick<M, WV>(x0, x1, x2), i, sink);
+            for (int j = 0; j < pre - 1; ++j) M::step(x0, x1, x2, p, h);
+            M::step(x0, x1, x2, p, h);
+            if (CHECK && diverged(pick<M, WV>(x0, x1, x2), A.maxValue)) bad = true;
+        }
     }
     counts[t] = bad ? -1 : ps.count();
 }
@@ -190,17 +214,28 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
     pa.init();
     pb.init();
     const int N = A.N;
-    for (int i = 0; i < N; ++i) {
-        const double da = pick<M, WV>(x0, x1, x2), db = pick<M, WV>(y0, y1, y2);
-        if (i >= 2) {
-            pa.push(da, i, sa);
-            pb.push(db, i, sb);
-        } else {
-            pa.p2 = pa.p1; pa.p1 = da;
-            pb.p2 = pb.p1; pb.p1 = db;
-        }
+    int i = 0;
+    for (; i < N && i < 2; ++i) {
+        pa.p2 = pa.p1;
+        pa.p1 = pick<M, WV>(x0, x1, x2);
+        pb.p2 = pb.p1;
+        pb.p1 = pick<M, WV>(y0, y1, y2);
         for (int j = 0; j < A.pre - 1; ++j) M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
         M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+    }
+    if (A.pre == 1) {
+        for (; i < N; ++i) {
+            pa.push(pick<M, WV>(x0, x1, x2), i, sa);
+            pb.push(pick<M, WV>(y0, y1, y2), i, sb);
+            M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+        }
+    } else {
+        for (; i < N; ++i) {
+            pa.push(pick<M, WV>(x0, x1, x2), i, sa);
+            pb.push(pick<M, WV>(y0, y1, y2), i, sb);
+            for (int j = 0; j < A.pre - 1; ++j) M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+            M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+        }
     }
     counts[c0] = pa.count();
     if (two) counts[c1] = pb.count();
@@ -719,7 +754,16 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
         }
     };
 
+    // transient of the LLE: x alone; states inside the window are recorded
     int k = 0;
+    const int preEnd = min(F.skipL, max(0, w0));
+    for (; k + 4 <= preEnd; k += 4) {
+        M::step(x0, x1, x2, p, h);
+        M::step(x0, x1, x2, p, h);
+        M::step(x0, x1, x2, p, h);
+        M::step(x0, x1, x2, p, h);
+    }
+    for (; k < preEnd; ++k) M::step(x0, x1, x2, p, h);
     for (; k < F.skipL; ++k) {
         record(k);
         M::step(x0, x1, x2, p, h);
@@ -741,6 +785,12 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
             }
             for (; i < F.nNT; ++i) M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
             k += F.nNT;
+        } else if (k >= w0 + 2 && k + F.nNT <= w1) {
+            // every state of this block is a recorded sample past the first two
+            for (int i = 0; i < F.nNT; ++i, ++k) {
+                ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k - w0, sink);
+                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
+            }
         } else {
             for (int i = 0; i < F.nNT; ++i, ++k) {
                 record(k);

@@ -68,6 +68,11 @@ def _declare(L):
         "bifhip_bifurcation1D": drv_bif1,
         "bifhip_bifurcation2DKDE": drv_kde,
         "bifhip_bifurcation2DDBSCAN": drv_db,
+        "bifhip_bifurcation1DIC": drv_bif1,
+        "bifhip_bifurcation2DKDEIC": drv_kde,
+        "bifhip_bifurcation2DDBSCANIC": drv_db,
+        "bifhip_LLE1DIC": drv_ly, "bifhip_LLE2DIC": drv_ly,
+        "bifhip_set_sweep": [_I],
         "bifhip_LLE1D": drv_ly, "bifhip_LLE2D": drv_ly,
         "bifhip_LS1D": drv_ly, "bifhip_LS2D": drv_ly,
         "bifhip_bifurcation1D_to_buffer": drv_bif1 + [_PI, _PD, _I],
@@ -264,6 +269,7 @@ def reset_context():
     set_peak_capacity(256)
     set_cluster_clamp(0)
     set_timing(False)
+    set_sweep(SWEEP_PARAMETERS)
     set_graph(True)
     set_partial_resolve(True)
     set_chaos_confidence(0.99)
@@ -279,36 +285,62 @@ def value_by_idx(idx, nPts, s, f, k):
 
 # ---------------------------------------- reference-named drivers (files)
 
-def bifurcation1D(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
-                  indicesOfMutVars, writableVar, maxValue, transientTime, values,
-                  amountOfValues, preScaller):
+def _bif1(fn, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+          indicesOfMutVars, writableVar, maxValue, transientTime, values, amountOfValues,
+          preScaller):
     ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
-    _check(load().bifhip_bifurcation1D(tMax, nPts, h, amountOfInitialConditions, _pd(ic), _pd(r),
-                                       _pi(m), writableVar, maxValue, transientTime, _pd(v),
-                                       amountOfValues, preScaller))
+    _check(fn(tMax, nPts, h, amountOfInitialConditions, _pd(ic), _pd(r), _pi(m), writableVar,
+              maxValue, transientTime, _pd(v), amountOfValues, preScaller))
 
 
-def bifurcation2DKDE(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
-                     indicesOfMutVars, writableVar, maxValue, maxAmountOfPeaks, transientTime,
-                     values, amountOfValues, preScaller, kdeSampling, kdeSamplesInterval1,
-                     kdeSamplesInterval2, kdeSamplesSmooth):
+def _kde2(fn, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+          indicesOfMutVars, writableVar, maxValue, maxAmountOfPeaks, transientTime, values,
+          amountOfValues, preScaller, kdeSampling, kdeSamplesInterval1, kdeSamplesInterval2,
+          kdeSamplesSmooth):
     ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
-    _check(load().bifhip_bifurcation2DKDE(tMax, nPts, h, amountOfInitialConditions, _pd(ic),
-                                          _pd(r), _pi(m), writableVar, maxValue,
-                                          maxAmountOfPeaks, transientTime, _pd(v),
-                                          amountOfValues, preScaller, kdeSampling,
-                                          kdeSamplesInterval1, kdeSamplesInterval2,
-                                          kdeSamplesSmooth))
+    _check(fn(tMax, nPts, h, amountOfInitialConditions, _pd(ic), _pd(r), _pi(m), writableVar,
+              maxValue, maxAmountOfPeaks, transientTime, _pd(v), amountOfValues, preScaller,
+              kdeSampling, kdeSamplesInterval1, kdeSamplesInterval2, kdeSamplesSmooth))
 
 
-def bifurcation2DDBSCAN(tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
-                        indicesOfMutVars, writableVar, maxValue, maxAmountOfPeaks,
-                        transientTime, values, amountOfValues, preScaller, eps):
+def _db2(fn, tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges,
+         indicesOfMutVars, writableVar, maxValue, maxAmountOfPeaks, transientTime, values,
+         amountOfValues, preScaller, eps):
     ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
-    _check(load().bifhip_bifurcation2DDBSCAN(tMax, nPts, h, amountOfInitialConditions, _pd(ic),
-                                             _pd(r), _pi(m), writableVar, maxValue,
-                                             maxAmountOfPeaks, transientTime, _pd(v),
-                                             amountOfValues, preScaller, eps))
+    _check(fn(tMax, nPts, h, amountOfInitialConditions, _pd(ic), _pd(r), _pi(m), writableVar,
+              maxValue, maxAmountOfPeaks, transientTime, _pd(v), amountOfValues, preScaller, eps))
+
+
+def bifurcation1D(*a):
+    _bif1(load().bifhip_bifurcation1D, *a)
+
+
+def bifurcation2DKDE(*a):
+    _kde2(load().bifhip_bifurcation2DKDE, *a)
+
+
+def bifurcation2DDBSCAN(*a):
+    _db2(load().bifhip_bifurcation2DDBSCAN, *a)
+
+
+def bifurcation1DIC(*a):
+    _bif1(load().bifhip_bifurcation1DIC, *a)
+
+
+def bifurcation2DKDEIC(*a):
+    _kde2(load().bifhip_bifurcation2DKDEIC, *a)
+
+
+def bifurcation2DDBSCANIC(*a):
+    _db2(load().bifhip_bifurcation2DDBSCANIC, *a)
+
+
+SWEEP_PARAMETERS, SWEEP_INITIAL_CONDITIONS = 0, 1
+
+
+def set_sweep(target):
+    """Sweep target of the buffer and device entry points."""
+    _check(load().bifhip_set_sweep(int(target)))
 
 
 def _lyap_file(fn, tMax, NT, nPts, h, eps, initialConditions, amountOfInitialConditions, ranges,
@@ -332,6 +364,14 @@ def LS1D(*a):
 
 def LS2D(*a):
     _lyap_file(load().bifhip_LS2D, *a)
+
+
+def LLE1DIC(*a):
+    _lyap_file(load().bifhip_LLE1DIC, *a)
+
+
+def LLE2DIC(*a):
+    _lyap_file(load().bifhip_LLE2DIC, *a)
 
 
 # ------------------------------------------------ in-memory (host) twins

@@ -26,6 +26,7 @@ enum ModelId : int { kRossler = 0, kLinear = 1 };
 struct Grid {
     int nPts;
     int dimension;
+    int icSweep;      // 1: swept entries index the initial state, 0: the parameters
     int mut[2];
     float pk[2];
     double start[2];
@@ -241,18 +242,31 @@ struct Linear {
     }
 };
 
-// Load the cell's parameter vector with the swept entries substituted
-// (calculateDiscreteModelCUDA, Library/cudaLibrary.cu:96-105).
-__device__ __forceinline__ void cell_values(const Grid& g, int idx, double* v)
+// Parameter vector and initial state of a cell, the swept entries
+// substituted into the parameters (calculateDiscreteModelCUDA,
+// Library/cudaLibrary.cu:96-105) or, for a sweep over initial conditions,
+// into the state (calculateDiscreteModelICCUDA / LLEKernelICCUDA,
+// Library/cudaLibrary.cu:190-192, 837-839).
+__device__ __forceinline__ void cell_setup(const Grid& g, int idx, double* v, double& x0,
+                                           double& x1, double& x2)
 {
 #pragma unroll
     for (int i = 0; i < kMaxVal; ++i) v[i] = g.values[i];
+    x0 = g.ic[0];
+    x1 = g.ic[1];
+    x2 = g.ic[2];
     for (int k = 0; k < g.dimension; ++k) {
         const double val = grid_value(g, idx, k);
         // dynamic index into a small register array: keep it a select chain
+        if (g.icSweep) {
+            if (g.mut[k] == 0) x0 = val;
+            else if (g.mut[k] == 1) x1 = val;
+            else if (g.mut[k] == 2) x2 = val;
+        } else {
 #pragma unroll
-        for (int i = 0; i < kMaxVal; ++i)
-            if (i == g.mut[k]) v[i] = val;
+            for (int i = 0; i < kMaxVal; ++i)
+                if (i == g.mut[k]) v[i] = val;
+        }
     }
 }
 

@@ -183,8 +183,10 @@ int make_grid(Grid& G, int nPts, int dimension, const double* ranges, const int*
     G.nVal = nVal;
     for (int i = 0; i < nIC; ++i) G.ic[i] = ic[i];
     for (int i = 0; i < nVal; ++i) G.values[i] = values[i];
+    G.icSweep = g.sweepIC;
+    const int nMut = g.sweepIC ? nIC : nVal;   // what indicesOfMutVars indexes
     for (int k = 0; k < dimension; ++k) {
-        if (mut[k] < 0 || mut[k] >= nVal)
+        if (mut[k] < 0 || mut[k] >= nMut)
             return fail(BIFHIP_E_ARG, "indicesOfMutVars[%d] = %d out of range", k, mut[k]);
         G.mut[k] = mut[k];
         G.pk[k] = powf((float)nPts, (float)k);
@@ -459,7 +461,9 @@ int lyap_spec(LyapSpec& S, int dimension, double tMax, double NT, int nPts, doub
 }
 
 // Host table of reference perturbation directions for cells [off, off+n):
-// LLE: 0.5*sinf((int)(idx*(i*idx+1)+1)) normalised (Library/cudaLibrary.cu:720-732)
+// LLE: 0.5*sinf((int)(idx*(i*idx+1)+1)) normalised (Library/cudaLibrary.cu:720-732);
+//      sweeps over initial conditions: 0.5*sinf(0.2171828(i+1) + idx +
+//      (0.2171828 + i*idx)) normalised (Library/cudaLibrary.cu:844-855)
 // LS:  0.5*sinf(0.2171828(i+1)(j+1) + idx + (0.2171828 + i*j*idx)) rows
 //      normalised (Library/cudaLibrary.cu:1025-1040)
 void directions(bool ls, long long off, int n, long long chunk, std::vector<double>& out)
@@ -474,8 +478,15 @@ void directions(bool ls, long long off, int n, long long chunk, std::vector<doub
             const uint32_t id = (uint32_t)local;
             double zp = 0;
             for (int i = 0; i < 3; ++i) {
-                const int32_t arg = (int32_t)(id * ((uint32_t)i * id + 1u) + 1u);
-                z[i] = 0.5 * (double)sinf((float)arg);
+                if (g.sweepIC) {
+                    // LLEKernelICCUDA, Library/cudaLibrary.cu:846
+                    const int idx = (int)local;
+                    const double arg = 0.2171828 * (i + 1) + idx + (0.2171828 + i * idx);
+                    z[i] = sinf((float)arg) * 0.5;
+                } else {
+                    const int32_t arg = (int32_t)(id * ((uint32_t)i * id + 1u) + 1u);
+                    z[i] = 0.5 * (double)sinf((float)arg);
+                }
                 zp += z[i] * z[i];
             }
             zp = std::sqrt(zp);
@@ -944,7 +955,10 @@ int lyap_impl(bool ls, int dimension, double tMax, double NT, int nPts, double h
                      transientTime, values, nVal));
     progress(ls ? (dimension == 1 ? "LS1D" : "LS2D") : (dimension == 1 ? "LLE1D" : "LLE2D"),
              S.cells);
-    const long long defChunk = (!ls && dimension == 2) ? 200000 : 0;
+    // LLE2D seeds directions from a 200000-cell chunk-local index
+    // (Library/hostLibrary.cu:1552); LLE2DIC's chunk is free-memory sized,
+    // here one chunk
+    const long long defChunk = (!ls && dimension == 2 && !g.sweepIC) ? 200000 : 0;
     RC_TRY(lyap_map_device(S, ls, defChunk, d_out));
     return finish();
 }
@@ -1131,10 +1145,18 @@ double bifhip_value_by_idx(int idx, int nPts, double s, double f, int k)
 
 // ----------------------------------------------------------- file drivers
 
+// Sweep target for the duration of one named driver call.
+struct SweepGuard {
+    int saved;
+    explicit SweepGuard(bool ic) : saved(g.sweepIC) { g.sweepIC = ic ? 1 : 0; }
+    ~SweepGuard() { g.sweepIC = saved; }
+};
+
 int bifhip_bifurcation1D(double tMax, int nPts, double h, int nIC, const double* ic,
                          const double* ranges, const int* mut, int wv, double maxValue,
                          double transientTime, const double* values, int nVal, int pre)
 {
+    SweepGuard sweep(false);
     std::vector<int> counts;
     std::vector<double> peaks;
     int stride = 0;
@@ -1148,6 +1170,7 @@ int bifhip_bifurcation2DKDE(double tMax, int nPts, double h, int nIC, const doub
                             int maxPeaks, double transientTime, const double* values, int nVal,
                             int pre, int kdeSampling, double I1, double I2, double H)
 {
+    SweepGuard sweep(false);
     if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
     std::vector<int> out((size_t)nPts * nPts);
     RC_TRY(cluster_to_host(1, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
@@ -1162,6 +1185,7 @@ int bifhip_bifurcation2DDBSCAN(double tMax, int nPts, double h, int nIC, const d
                                int maxPeaks, double transientTime, const double* values,
                                int nVal, int pre, double eps)
 {
+    SweepGuard sweep(false);
     if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
     std::vector<int> out((size_t)nPts * nPts);
     RC_TRY(cluster_to_host(0, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
@@ -1173,6 +1197,7 @@ int bifhip_LLE1D(double tMax, double NT, int nPts, double h, double eps, const d
                  int nIC, const double* ranges, const int* mut, int wv, double maxValue,
                  double transientTime, const double* values, int nVal)
 {
+    SweepGuard sweep(false);
     if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
     std::vector<double> out((size_t)nPts);
     RC_TRY(lyap_to_host(false, 1, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
@@ -1185,6 +1210,7 @@ int bifhip_LLE2D(double tMax, double NT, int nPts, double h, double eps, const d
                  int nIC, const double* ranges, const int* mut, int wv, double maxValue,
                  double transientTime, const double* values, int nVal)
 {
+    SweepGuard sweep(false);
     if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
     std::vector<double> out((size_t)nPts * nPts);
     RC_TRY(lyap_to_host(false, 2, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
@@ -1198,6 +1224,7 @@ int bifhip_LS1D(double tMax, double NT, int nPts, double h, double eps, const do
                 int nIC, const double* ranges, const int* mut, int wv, double maxValue,
                 double transientTime, const double* values, int nVal)
 {
+    SweepGuard sweep(false);
     if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
     std::vector<double> out((size_t)nPts * nIC);
     RC_TRY(lyap_to_host(true, 1, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
@@ -1210,6 +1237,7 @@ int bifhip_LS2D(double tMax, double NT, int nPts, double h, double eps, const do
                 int nIC, const double* ranges, const int* mut, int wv, double maxValue,
                 double transientTime, const double* values, int nVal)
 {
+    SweepGuard sweep(false);
     if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
     const long long cells = (long long)nPts * nPts;
     std::vector<double> out((size_t)cells * nIC);
@@ -1221,6 +1249,93 @@ int bifhip_LS2D(double tMax, double NT, int nPts, double h, double eps, const do
         const std::string path = g.outPath + std::to_string(k + 1) + ".csv";
         RC_TRY(write_map_double(path, nPts, ranges, 1, 6, out.data(), cells, nIC, k));
     }
+    return BIFHIP_OK;
+}
+
+// ------------------------------------- drivers sweeping initial conditions
+//
+// Same computations with the swept values written into the initial state
+// (indicesOfMutVars index initialConditions), as the reference's *IC
+// drivers; output formats as theirs.
+
+// Library/hostLibrary.cu:243-393 (bifurcation1DIC): the bifurcation1D format.
+int bifhip_bifurcation1DIC(double tMax, int nPts, double h, int nIC, const double* ic,
+                           const double* ranges, const int* mut, int wv, double maxValue,
+                           double transientTime, const double* values, int nVal, int pre)
+{
+    SweepGuard sweep(true);
+    std::vector<int> counts;
+    std::vector<double> peaks;
+    int stride = 0;
+    RC_TRY(bif1d_impl(tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime, values,
+                      nVal, pre, counts, peaks, stride));
+    return write_bif1d(g.outPath, nPts, ranges, counts.data(), peaks.data(), stride);
+}
+
+// Library/hostLibrary.cu:929-1090 (bifurcation2DKDEIC): no range header.
+int bifhip_bifurcation2DKDEIC(double tMax, int nPts, double h, int nIC, const double* ic,
+                              const double* ranges, const int* mut, int wv, double maxValue,
+                              int maxPeaks, double transientTime, const double* values, int nVal,
+                              int pre, int kdeSampling, double I1, double I2, double H)
+{
+    SweepGuard sweep(true);
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<int> out((size_t)nPts * nPts);
+    RC_TRY(cluster_to_host(1, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                           transientTime, values, nVal, pre, 0.0, kdeSampling, I1, I2, H,
+                           out.data()));
+    return write_map_int(g.outPath, nPts, ranges, 0, 6, out.data(), (long long)out.size());
+}
+
+// Library/hostLibrary.cu:1094-1266 (bifurcation2DDBSCANIC): with the header.
+int bifhip_bifurcation2DDBSCANIC(double tMax, int nPts, double h, int nIC, const double* ic,
+                                 const double* ranges, const int* mut, int wv, double maxValue,
+                                 int maxPeaks, double transientTime, const double* values,
+                                 int nVal, int pre, double eps)
+{
+    SweepGuard sweep(true);
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<int> out((size_t)nPts * nPts);
+    RC_TRY(cluster_to_host(0, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                           transientTime, values, nVal, pre, eps, 0, 0, 0, 0, out.data()));
+    return write_map_int(g.outPath, nPts, ranges, 1, 6, out.data(), (long long)out.size());
+}
+
+// Library/hostLibrary.cu:1397-1520 (LLE1DIC): the LLE1D format.
+int bifhip_LLE1DIC(double tMax, double NT, int nPts, double h, double eps, const double* ic,
+                   int nIC, const double* ranges, const int* mut, int wv, double maxValue,
+                   double transientTime, const double* values, int nVal)
+{
+    SweepGuard sweep(true);
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<double> out((size_t)nPts);
+    RC_TRY(lyap_to_host(false, 1, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                        transientTime, values, nVal, out.data()));
+    (void)wv;
+    return write_1d_values(g.outPath, nPts, ranges, out.data(), 1);
+}
+
+// Library/hostLibrary.cu:1666-1798 (LLE2DIC): no range header.
+int bifhip_LLE2DIC(double tMax, double NT, int nPts, double h, double eps, const double* ic,
+                   int nIC, const double* ranges, const int* mut, int wv, double maxValue,
+                   double transientTime, const double* values, int nVal)
+{
+    SweepGuard sweep(true);
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    std::vector<double> out((size_t)nPts * nPts);
+    RC_TRY(lyap_to_host(false, 2, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
+                        transientTime, values, nVal, out.data()));
+    (void)wv;
+    return write_map_double(g.outPath, nPts, ranges, 0, 6, out.data(), (long long)out.size(), 1,
+                            0);
+}
+
+int bifhip_set_sweep(int target)
+{
+    if (target != BIFHIP_SWEEP_PARAMETERS && target != BIFHIP_SWEEP_INITIAL_CONDITIONS)
+        return fail(BIFHIP_E_ARG, "sweep target must be BIFHIP_SWEEP_PARAMETERS or "
+                                  "BIFHIP_SWEEP_INITIAL_CONDITIONS");
+    g.sweepIC = target == BIFHIP_SWEEP_INITIAL_CONDITIONS;
     return BIFHIP_OK;
 }
 

@@ -114,10 +114,11 @@ __global__ __launch_bounds__(256) void k_bif_peaks(const BifArgs A, int* __restr
     const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
 
     double v[kMaxVal];
-    cell_values(A.g, idx, v);
+    double x0, x1, x2;
+    cell_setup(A.g, idx, v, x0, x1, x2);
     const typename M::P64 p = M::params(v);
     const double h = A.h;
-    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
+
 
     // transient, no divergence check (Library/cudaLibrary.cu:107-108)
     int s = 0;
@@ -188,13 +189,12 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
     const int i1 = A.cellList ? A.cellList[c1] : A.cellBase + c1;
 
     double v[kMaxVal];
-    cell_values(A.g, i0, v);
+    double x0, x1, x2, y0, y1, y2;
+    cell_setup(A.g, i0, v, x0, x1, x2);
     const typename M::P64 p = M::params(v);
-    cell_values(A.g, i1, v);
+    cell_setup(A.g, i1, v, y0, y1, y2);
     const typename M::P64 q = M::params(v);
     const double h = A.h;
-    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
-    double y0 = x0, y1 = x1, y2 = x2;
 
     int s = 0;
     for (; s + 2 <= A.skip; s += 2) {
@@ -250,10 +250,11 @@ __global__ __launch_bounds__(256) void k_series(const BifArgs A, int wv, double*
     if (t >= A.nCells) return;
     const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
     double v[kMaxVal];
-    cell_values(A.g, idx, v);
+    double x0, x1, x2;
+    cell_setup(A.g, idx, v, x0, x1, x2);
     const typename M::P64 p = M::params(v);
     const double h = A.h;
-    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
+
     for (int s = 0; s < A.skip; ++s) M::step(x0, x1, x2, p, h);
     double* row = data + (long long)t * A.rowStride;
     int flag = 0;
@@ -730,10 +731,11 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
     if (t >= A.nCells) return;
     const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
     double v[kMaxVal];
-    cell_values(A.g, idx, v);
+    double x0, x1, x2;
+    cell_setup(A.g, idx, v, x0, x1, x2);
     const typename M::P64 p = M::params(v);
     const double h = A.h, eps = F.eps;
-    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
+
 
     PeakSink sink;
     sink.amps = amps + (long long)t * A.rowStride;

@@ -66,10 +66,10 @@ __global__ __launch_bounds__(256) void k_lle64(const LyapArgs A, double* __restr
     if (t >= A.nCells) return;
     const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
     double v[kMaxVal];
-    cell_values(A.g, idx, v);
+    double x0, x1, x2;
+    cell_setup(A.g, idx, v, x0, x1, x2);
     const typename M::P64 p = M::params(v);
     const double h = A.h, eps = A.eps;
-    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
     double z0 = 1, z1 = 0, z2 = 0;
     if (A.dirs) {
         const double* d = A.dirs + (long long)t * 3;
@@ -101,10 +101,11 @@ __global__ __launch_bounds__(256) void k_lle32(const LyapArgs A, double* __restr
     if (t >= A.nCells) return;
     const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
     double v[kMaxVal];
-    cell_values(A.g, idx, v);
+    double s0, s1, s2;
+    cell_setup(A.g, idx, v, s0, s1, s2);
     const typename M::P32 q = M::params32(v, A.h);
     const float mv = (float)A.maxValue;
-    float x0 = (float)A.g.ic[0], x1 = (float)A.g.ic[1], x2 = (float)A.g.ic[2];
+    float x0 = (float)s0, x1 = (float)s1, x2 = (float)s2;
     float u0 = 1, u1 = 0, u2 = 0;
     if (A.dirs) {
         const double* d = A.dirs + (long long)t * 3;
@@ -179,10 +180,10 @@ __global__ __launch_bounds__(256) void k_ls64(const LyapArgs A, double* __restri
     if (t >= A.nCells) return;
     const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
     double v[kMaxVal];
-    cell_values(A.g, idx, v);
+    double x0, x1, x2;
+    cell_setup(A.g, idx, v, x0, x1, x2);
     const typename M::P64 p = M::params(v);
     const double h = A.h, eps = A.eps;
-    double x0 = A.g.ic[0], x1 = A.g.ic[1], x2 = A.g.ic[2];
     double z[9], y[9], den[3], res[3] = {0, 0, 0};
     if (A.dirs) {
         const double* d = A.dirs + (long long)t * 9;

@@ -76,12 +76,12 @@ __global__ void k_ls_tiles(const TileArgs A)
             const int ci = live[c] ? cell % A.tileW : 0;
             const int cj = live[c] ? cell / A.tileW : 0;
             const int idx = (A.originJ[tile] + cj) * A.g.nPts + (A.originI[tile] + ci);
-            double vals[kMaxVal];
-            cell_values(A.g, idx, vals);
+            double vals[kMaxVal], s0, s1, s2;
+            cell_setup(A.g, idx, vals, s0, s1, s2);
             q[c] = M::params32(vals, A.h);
-            x0[c] = (float)A.g.ic[0];
-            x1[c] = (float)A.g.ic[1];
-            x2[c] = (float)A.g.ic[2];
+            x0[c] = (float)s0;
+            x1[c] = (float)s1;
+            x2[c] = (float)s2;
 #pragma unroll
             for (int k = 0; k < 9; ++k) v[c][k] = (k % 4 == 0) ? 1.0f : 0.0f;
             acc[c][0] = acc[c][1] = acc[c][2] = 0;

@@ -35,6 +35,9 @@ extern "C" {
 #define BIFHIP_PREC_FP64 0     /* reference operation order, no contraction     */
 #define BIFHIP_PREC_FP32 1     /* FP32 bulk integration, FP64 renorm/log        */
 
+#define BIFHIP_SWEEP_PARAMETERS 0         /* indicesOfMutVars index values[]        */
+#define BIFHIP_SWEEP_INITIAL_CONDITIONS 1 /* ... index initialConditions[]         */
+
 #define BIFHIP_PERT_REFERENCE 0 /* 0.5*sinf(...) of the chunk-local index      */
 #define BIFHIP_PERT_AXIS_X 1    /* unit perturbation along x[0]                */
 
@@ -144,6 +147,55 @@ int bifhip_LS2D(double tMax, double NT, int nPts, double h, double eps,
                 const double* ranges, const int* indicesOfMutVars, int writableVar,
                 double maxValue, double transientTime, const double* values,
                 int amountOfValues);
+
+/* Drivers sweeping initial conditions: the same computations with the
+ * swept values written into the initial state (indicesOfMutVars index
+ * initialConditions), as calculateDiscreteModelICCUDA and LLEKernelICCUDA
+ * (Library/cudaLibrary.cu:165-207, 789-908); LLE directions use
+ * LLEKernelICCUDA's formula of the global index. */
+
+/* Library/hostLibrary.cuh:89-102 (bifurcation1DIC), bifurcation1D's format. */
+int bifhip_bifurcation1DIC(double tMax, int nPts, double h, int amountOfInitialConditions,
+                           const double* initialConditions, const double* ranges,
+                           const int* indicesOfMutVars, int writableVar, double maxValue,
+                           double transientTime, const double* values, int amountOfValues,
+                           int preScaller);
+
+/* Library/hostLibrary.cuh:254-272 (bifurcation2DKDEIC): no range header. */
+int bifhip_bifurcation2DKDEIC(double tMax, int nPts, double h, int amountOfInitialConditions,
+                              const double* initialConditions, const double* ranges,
+                              const int* indicesOfMutVars, int writableVar, double maxValue,
+                              int maxAmountOfPeaks, double transientTime, const double* values,
+                              int amountOfValues, int preScaller, int kdeSampling,
+                              double kdeSamplesInterval1, double kdeSamplesInterval2,
+                              double kdeSamplesSmooth);
+
+/* Library/hostLibrary.cuh:296-311 (bifurcation2DDBSCANIC). */
+int bifhip_bifurcation2DDBSCANIC(double tMax, int nPts, double h, int amountOfInitialConditions,
+                                 const double* initialConditions, const double* ranges,
+                                 const int* indicesOfMutVars, int writableVar, double maxValue,
+                                 int maxAmountOfPeaks, double transientTime,
+                                 const double* values, int amountOfValues, int preScaller,
+                                 double eps);
+
+/* Library/hostLibrary.cuh:371-385 (LLE1DIC), LLE1D's format. */
+int bifhip_LLE1DIC(double tMax, double NT, int nPts, double h, double eps,
+                   const double* initialConditions, int amountOfInitialConditions,
+                   const double* ranges, const int* indicesOfMutVars, int writableVar,
+                   double maxValue, double transientTime, const double* values,
+                   int amountOfValues);
+
+/* Library/hostLibrary.cuh:445-459 (LLE2DIC): no range header. */
+int bifhip_LLE2DIC(double tMax, double NT, int nPts, double h, double eps,
+                   const double* initialConditions, int amountOfInitialConditions,
+                   const double* ranges, const int* indicesOfMutVars, int writableVar,
+                   double maxValue, double transientTime, const double* values,
+                   int amountOfValues);
+
+/* Sweep target of the buffer and device entry points below
+ * (BIFHIP_SWEEP_PARAMETERS by default); the named file drivers fix their
+ * own target. */
+int bifhip_set_sweep(int target);
 
 /* ------------------------------------------- drivers (host buffers) */
 

@@ -76,6 +76,56 @@ inline void bifurcation2DDBSCAN(const double tMax, const int nPts, const double 
                                             eps));
 }
 
+// Library/hostLibrary.cuh:89-102
+inline void bifurcation1DIC(const double tMax, const int nPts, const double h,
+                            const int amountOfInitialConditions, const double* initialConditions,
+                            const double* ranges, const int* indicesOfMutVars,
+                            const int writableVar, const double maxValue,
+                            const double transientTime, const double* values,
+                            const int amountOfValues, const int preScaller)
+{
+    BIFHIP_CHECK(bifhip_bifurcation1DIC(tMax, nPts, h, amountOfInitialConditions,
+                                        initialConditions, ranges, indicesOfMutVars, writableVar,
+                                        maxValue, transientTime, values, amountOfValues,
+                                        preScaller));
+}
+
+// Library/hostLibrary.cuh:254-272
+inline void bifurcation2DKDEIC(const double tMax, const int nPts, const double h,
+                               const int amountOfInitialConditions,
+                               const double* initialConditions, const double* ranges,
+                               const int* indicesOfMutVars, const int writableVar,
+                               const double maxValue, const int maxAmountOfPeaks,
+                               const double transientTime, const double* values,
+                               const int amountOfValues, const int preScaller,
+                               const int kdeSampling, const double kdeSamplesInterval1,
+                               const double kdeSamplesInterval2, const double kdeSamplesSmooth)
+{
+    BIFHIP_CHECK(bifhip_bifurcation2DKDEIC(tMax, nPts, h, amountOfInitialConditions,
+                                           initialConditions, ranges, indicesOfMutVars,
+                                           writableVar, maxValue, maxAmountOfPeaks,
+                                           transientTime, values, amountOfValues, preScaller,
+                                           kdeSampling, kdeSamplesInterval1,
+                                           kdeSamplesInterval2, kdeSamplesSmooth));
+}
+
+// Library/hostLibrary.cuh:296-311
+inline void bifurcation2DDBSCANIC(const double tMax, const int nPts, const double h,
+                                  const int amountOfInitialConditions,
+                                  const double* initialConditions, const double* ranges,
+                                  const int* indicesOfMutVars, const int writableVar,
+                                  const double maxValue, const int maxAmountOfPeaks,
+                                  const double transientTime, const double* values,
+                                  const int amountOfValues, const int preScaller,
+                                  const double eps)
+{
+    BIFHIP_CHECK(bifhip_bifurcation2DDBSCANIC(tMax, nPts, h, amountOfInitialConditions,
+                                              initialConditions, ranges, indicesOfMutVars,
+                                              writableVar, maxValue, maxAmountOfPeaks,
+                                              transientTime, values, amountOfValues, preScaller,
+                                              eps));
+}
+
 #define BIFHIP_LYAP_WRAPPER(NAME)                                                              \
     inline void NAME(const double tMax, const double NT, const int nPts, const double h,       \
                      const double eps, const double* initialConditions,                        \
@@ -94,5 +144,7 @@ BIFHIP_LYAP_WRAPPER(LLE1D)  // Library/hostLibrary.cuh:334-348
 BIFHIP_LYAP_WRAPPER(LLE2D)  // Library/hostLibrary.cuh:408-422
 BIFHIP_LYAP_WRAPPER(LS1D)   // Library/hostLibrary.cuh:482-496
 BIFHIP_LYAP_WRAPPER(LS2D)   // Library/hostLibrary.cuh:520-534
+BIFHIP_LYAP_WRAPPER(LLE1DIC)  // Library/hostLibrary.cuh:371-385
+BIFHIP_LYAP_WRAPPER(LLE2DIC)  // Library/hostLibrary.cuh:445-459
 
 #undef BIFHIP_LYAP_WRAPPER

@@ -286,6 +286,14 @@ void orc_gram_schmidt(const double *a, double *b, int n, double *denoms)
     }
 }
 
+/* Sweep target: parameters (0, calculateDiscreteModelCUDA,
+ * Library/cudaLibrary.cu:96-105) or initial conditions (1,
+ * calculateDiscreteModelICCUDA and LLEKernelICCUDA, Library/cudaLibrary.cu:
+ * 190-192, 837-839).  Set before a batch; read-only inside it. */
+static int g_sweep_ic = 0;
+
+void orc_set_sweep_ic(int on) { g_sweep_ic = on != 0; }
+
 static void setup_cell(int model, int nPts, long cellIdx, int dimension,
                        const double *ranges, const int *mutIdx,
                        const double *ic, int nIC, const double *values,
@@ -296,9 +304,14 @@ static void setup_cell(int model, int nPts, long cellIdx, int dimension,
         x[i] = ic[i];
     for (int i = 0; i < nVal; ++i)
         v[i] = values[i];
-    for (int i = 0; i < dimension; ++i)
-        v[mutIdx[i]] = orc_value_by_idx((int)cellIdx, nPts, ranges[2 * i],
-                                        ranges[2 * i + 1], i);
+    for (int i = 0; i < dimension; ++i) {
+        double val = orc_value_by_idx((int)cellIdx, nPts, ranges[2 * i],
+                                      ranges[2 * i + 1], i);
+        if (g_sweep_ic)
+            x[mutIdx[i]] = val;
+        else
+            v[mutIdx[i]] = val;
+    }
 }
 
 /* calculateDiscreteModelCUDA, Library/cudaLibrary.cu:78-117: set the swept
@@ -320,7 +333,8 @@ int orc_bif_cell(int model, int nPts, long cellIdx, int dimension,
 }
 
 /* LLE direction, Library/cudaLibrary.cu:720-732: 0.5*sinf of the wrapped
- * 32-bit integer idx*(i*idx+1)+1, normalised in double. */
+ * 32-bit integer idx*(i*idx+1)+1, normalised in double; for a sweep over
+ * initial conditions the LLEKernelICCUDA form (Library/cudaLibrary.cu:846). */
 static void lle_direction(long localIdx, int nIC, int pertMode, double *z)
 {
     if (pertMode == ORC_PERT_AXIS_X) {
@@ -331,8 +345,15 @@ static void lle_direction(long localIdx, int nIC, int pertMode, double *z)
     double zp = 0;
     uint32_t id = (uint32_t)localIdx;
     for (int i = 0; i < nIC; ++i) {
-        int32_t arg = (int32_t)(id * ((uint32_t)i * id + 1u) + 1u);
-        z[i] = 0.5 * (double)sinf((float)arg);
+        if (g_sweep_ic) {
+            /* LLEKernelICCUDA, Library/cudaLibrary.cu:846 */
+            int idx = (int)localIdx;
+            double arg = 0.2171828 * (i + 1) + idx + (0.2171828 + i * idx);
+            z[i] = sinf((float)arg) * 0.5;
+        } else {
+            int32_t arg = (int32_t)(id * ((uint32_t)i * id + 1u) + 1u);
+            z[i] = 0.5 * (double)sinf((float)arg);
+        }
         zp += z[i] * z[i];
     }
     zp = sqrt(zp);

@@ -41,6 +41,8 @@ extern "C" {
 #define ORC_MODEL_LINEAR  1   /* x_i += h*(a_i*x_i): closed-form test model     */
 
 int  orc_model_dim(int model);
+/* swept entries go into the initial conditions (1) or the parameters (0) */
+void orc_set_sweep_ic(int on);
 void orc_step(int model, double *x, const double *a, double h);
 
 int  orc_loop(int model, double *x, const double *values, double h,

@@ -41,6 +41,8 @@ def lib():
 
 
 def _declare(L):
+    L.orc_set_sweep_ic.argtypes = [_I]
+    L.orc_set_sweep_ic.restype = None
     L.orc_step.argtypes = [_I, _PD, _PD, _D]
     L.orc_loop.argtypes = [_I, _PD, _PD, _D, _I, _I, _I, _D, _PD, _L, _I]
     L.orc_loop.restype = _I
@@ -98,6 +100,11 @@ def i32(x):
 
 
 # ---------------- small wrappers ----------------
+
+def set_sweep_ic(on):
+    """Swept entries go into the initial conditions (True) or the parameters."""
+    lib().orc_set_sweep_ic(1 if on else 0)
+
 
 def step(x, a, h, model=MODEL_ROSSLER):
     x = f64(x).copy()

@@ -219,6 +219,59 @@ def test_lle_linear_model_closed_form(orc):
         assert abs(out[idx] - want) < 1e-8, (idx, out[idx], want)
 
 
+def _ic_lle_dir(idx):
+    """LLEKernelICCUDA's direction: 0.5*sinf(0.2171828(i+1) + idx +
+    (0.2171828 + i*idx)), normalised (float32 sin as in _ref_lle_dir)."""
+    z = np.array([0.5 * float(np.sin(np.float32(0.2171828 * (i + 1) + idx + (0.2171828 + i * idx))))
+                  for i in range(3)])
+    return z / np.linalg.norm(z)
+
+
+def test_lle_ic_sweep_directions_closed_form(orc):
+    """A sweep over initial conditions seeds the perturbation with
+    LLEKernelICCUDA's formula: on the linear model at the zero state (the
+    swept initial value is 0 in every cell) the LLE is log|M^N z|/T for that
+    z, and differs from the parameter-sweep direction's value."""
+    h, T, NT = 0.01, 200, 0.5
+    orc.set_sweep_ic(True)
+    try:
+        out = orc.lle_cells(1, T, NT, 4, h, 1e-8, [0, 0, 0], [0.0, 0.0], [0], 0.0, 20, LAM,
+                            orc.PERT_REFERENCE, 0, 0, 4, model=orc.MODEL_LINEAR)
+    finally:
+        orc.set_sweep_ic(False)
+    N = int(T / NT) * int(NT / h)
+    lg = np.log1p(h * np.array(LAM)) * N
+    m = lg.max()
+    for idx in range(4):
+        want = (m + 0.5 * np.log(np.sum((_ic_lle_dir(idx) * np.exp(lg - m)) ** 2))) / T
+        other = (m + 0.5 * np.log(np.sum((_ref_lle_dir(idx) * np.exp(lg - m)) ** 2))) / T
+        assert abs(out[idx] - want) < 1e-8, (idx, out[idx], want)
+        assert abs(out[idx] - other) > 1e-6
+
+
+def test_ic_sweep_substitutes_initial_state(orc):
+    """Sweeping initial conditions writes the lattice values into the state
+    (calculateDiscreteModelICCUDA): each cell's series equals a plain run
+    from the initial conditions with those entries replaced by the closed-form
+    lattice values, parameters untouched."""
+    n, h, skip, N = 6, 0.01, 300, 400
+    rng = [-1.0, 1.0, 0.0, 2.0]
+    vals = [0.2, 0.2, 5.7]
+    orc.set_sweep_ic(True)
+    try:
+        got = [orc.bif_cell(c, n, rng, [0, 2], IC, vals, h, skip, N)[1] for c in range(n * n)]
+    finally:
+        orc.set_sweep_ic(False)
+    for c in range(n * n):
+        i, j = c % n, c // n
+        x = [rng[0] + i * ((rng[1] - rng[0]) / (n - 1)), IC[1],
+             rng[2] + j * ((rng[3] - rng[2]) / (n - 1))]
+        # a one-point parameter sweep [a, a] leaves the parameters as they are
+        _, want = orc.bif_cell(0, n, [vals[0], vals[0]], [0], x, vals, h, skip, N)
+        np.testing.assert_array_equal(got[c], want, err_msg=str(c))
+    assert not np.array_equal(got[0], got[n * n - 1])
+
+
 def test_ls_linear_model_closed_form(orc):
     """Axis directions stay aligned under a diagonal map: the spectrum is
     exactly log(1+h*a_i)/h; reference directions converge to the same set."""

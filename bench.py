@@ -204,7 +204,7 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
     ap.add_argument("--coarse", type=int, default=256)
     ap.add_argument("--conf", type=float, default=0.95, help="class softmax confidence threshold")
-    ap.add_argument("--chaos-conf", type=float, default=0.99,
+    ap.add_argument("--chaos-conf", type=float, default=0.98,
                     help="chaos-logit confidence threshold")
     ap.add_argument("--validation", type=float, default=0.002)
     ap.add_argument("--batch", type=int, default=4096)

@@ -18,6 +18,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "bifhip.h")
 
 MODEL_ROSSLER = 0
 MODEL_LINEAR = 1
+MODEL_MEMRISTIVE = 2
 PREC_FP64 = 0
 PREC_FP32 = 1
 PERT_REFERENCE = 0
@@ -272,7 +273,7 @@ def reset_context():
     set_sweep(SWEEP_PARAMETERS)
     set_graph(True)
     set_partial_resolve(True)
-    set_chaos_confidence(0.99)
+    set_chaos_confidence(0.98)
 
 
 def device_count():

@@ -15,7 +15,7 @@ constexpr int kMaxVal = 16;   // parameter vector capacity
 constexpr int kWave = 64;     // CDNA wavefront width
 
 // Model ids (the public numbering, see include/bifhip.h).
-enum ModelId : int { kRossler = 0, kLinear = 1 };
+enum ModelId : int { kRossler = 0, kLinear = 1, kMemristive = 2 };
 
 // Grid description handed to every map kernel by value.
 //   value of swept parameter k for flattened cell idx (getValueByIdx,
@@ -241,6 +241,69 @@ struct Linear {
         v0 *= p.a; v1 *= p.b; v2 *= p.c;
     }
 };
+
+// Five-variable memristive system, the active body of the reference's
+// calculateDiscreteModel (Library/cudaLibrary.cu:19-43): a composition
+// step of an explicit sequential pass of length k = h*a7 (k held in a
+// float, Library/cudaLibrary.cu:19-27), the state then rounded to float
+// (y[5], :29-33), and a semi-implicit pass of length h*(1-a7) whose linear
+// terms are solved in closed form (:36-43).  Parameters a0..a7; |x4|
+// (double absolute value) enters the dissipation term.  Every operation in
+// the reference's order, no contraction.
+struct Memristive {
+    static constexpr int D = 5;
+    struct P64 { double a[8]; };
+
+    __device__ __forceinline__ static P64 params(const double* v)
+    {
+        P64 p;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) p.a[i] = v[i];
+        return p;
+    }
+
+    __device__ __forceinline__ static void stepn(double (&x)[5], const P64& p, double h)
+    {
+        const double* a = p.a;
+        const float k = (float)(h * a[7]);
+        x[4] = x[4] + k * (x[0]);
+        x[3] = x[3] + k * (-a[2] * x[1] - a[3] * x[3]);
+        x[2] = x[2] + k * (a[1] * (x[1] - x[0] - x[2]));
+        x[1] = x[1] + k * (-x[2] + x[3]);
+        x[0] = x[0] + k * (a[0] * (x[2] - (a[4] + a[5] + a[6] * fabs(x[4])) * x[0]));
+        const float y0 = (float)x[0], y1 = (float)x[1], y2 = (float)x[2], y3 = (float)x[3],
+                    y4 = (float)x[4];
+        const double h1 = h * (1 - a[7]);
+        x[0] = (y0 + h1 * (a[0] * x[2])) / (1 + h1 * a[0] * (a[4] + a[5] + a[6] * fabs(x[4])));
+        x[1] = y1 + h1 * (-x[2] + x[3]);
+        x[2] = (y2 + h1 * (a[1] * (x[1] - x[0]))) / (1 + h1 * a[1]);
+        x[3] = (y3 + h1 * (-a[2] * x[1])) / (1 + h1 * a[3]);
+        x[4] = y4 + h1 * (x[0]);
+    }
+};
+
+// Parameter vector and initial state (D variables) of a cell, swept
+// entries substituted as in cell_setup.
+template <int D>
+__device__ __forceinline__ void cell_setup_n(const Grid& g, int idx, double* v, double (&x)[D])
+{
+#pragma unroll
+    for (int i = 0; i < kMaxVal; ++i) v[i] = g.values[i];
+#pragma unroll
+    for (int i = 0; i < D; ++i) x[i] = g.ic[i];
+    for (int k = 0; k < g.dimension; ++k) {
+        const double val = grid_value(g, idx, k);
+        if (g.icSweep) {
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+                if (i == g.mut[k]) x[i] = val;
+        } else {
+#pragma unroll
+            for (int i = 0; i < kMaxVal; ++i)
+                if (i == g.mut[k]) v[i] = val;
+        }
+    }
+}
 
 // Parameter vector and initial state of a cell, the swept entries
 // substituted into the parameters (calculateDiscreteModelCUDA,

@@ -161,7 +161,7 @@ double value_by_idx(int idx, int nPts, double s, double f, int k)
     return s + ((double)(t % nPts) * ((double)(f - s) / (double)(nPts - 1)));
 }
 
-int model_dim(int model) { return model == BIFHIP_MODEL_LINEAR ? 3 : 3; }
+int model_dim(int model) { return model == BIFHIP_MODEL_MEMRISTIVE ? 5 : 3; }
 
 int make_grid(Grid& G, int nPts, int dimension, const double* ranges, const int* mut,
               const double* ic, int nIC, const double* values, int nVal)
@@ -173,8 +173,10 @@ int make_grid(Grid& G, int nPts, int dimension, const double* ranges, const int*
     if (nIC != model_dim(g.model))
         return fail(BIFHIP_E_ARG, "model %d has %d state variables, got %d initial conditions",
                     g.model, model_dim(g.model), nIC);
-    if (nVal < 3 || nVal > kMaxVal)
-        return fail(BIFHIP_E_ARG, "amountOfValues must be in [3, %d] (got %d)", kMaxVal, nVal);
+    const int minVal = g.model == BIFHIP_MODEL_MEMRISTIVE ? 8 : 3;
+    if (nVal < minVal || nVal > kMaxVal)
+        return fail(BIFHIP_E_ARG, "amountOfValues must be in [%d, %d] for model %d (got %d)",
+                    minVal, kMaxVal, g.model, nVal);
     const long long cells = dimension == 1 ? (long long)nPts : (long long)nPts * nPts;
     if (cells > (1LL << 30)) return fail(BIFHIP_E_LIMIT, "grid of %lld cells is too large", cells);
     G.nPts = nPts;
@@ -468,7 +470,8 @@ int lyap_spec(LyapSpec& S, int dimension, double tMax, double NT, int nPts, doub
 //      normalised (Library/cudaLibrary.cu:1025-1040)
 void directions(bool ls, long long off, int n, long long chunk, std::vector<double>& out)
 {
-    const int per = ls ? 9 : 3;
+    const int D = model_dim(g.model);
+    const int per = ls ? D * D : D;
     out.resize((size_t)n * per);
     for (int t = 0; t < n; ++t) {
         const long long gi = off + t;
@@ -477,7 +480,7 @@ void directions(bool ls, long long off, int n, long long chunk, std::vector<doub
         if (!ls) {
             const uint32_t id = (uint32_t)local;
             double zp = 0;
-            for (int i = 0; i < 3; ++i) {
+            for (int i = 0; i < D; ++i) {
                 if (g.sweepIC) {
                     // LLEKernelICCUDA, Library/cudaLibrary.cu:846
                     const int idx = (int)local;
@@ -490,18 +493,18 @@ void directions(bool ls, long long off, int n, long long chunk, std::vector<doub
                 zp += z[i] * z[i];
             }
             zp = std::sqrt(zp);
-            for (int i = 0; i < 3; ++i) z[i] /= zp;
+            for (int i = 0; i < D; ++i) z[i] /= zp;
         } else {
             const int idx = (int)local;
-            for (int j = 0; j < 3; ++j) {
+            for (int j = 0; j < D; ++j) {
                 double zp = 0;
-                for (int i = 0; i < 3; ++i) {
+                for (int i = 0; i < D; ++i) {
                     const double arg = 0.2171828 * (i + 1) * (j + 1) + idx + (0.2171828 + i * j * idx);
-                    z[j * 3 + i] = sinf((float)arg) * 0.5;
-                    zp += z[j * 3 + i] * z[j * 3 + i];
+                    z[j * D + i] = sinf((float)arg) * 0.5;
+                    zp += z[j * D + i] * z[j * D + i];
                 }
                 zp = std::sqrt(zp);
-                for (int i = 0; i < 3; ++i) z[j * 3 + i] /= zp;
+                for (int i = 0; i < D; ++i) z[j * D + i] /= zp;
             }
         }
     }
@@ -513,7 +516,10 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
     const long long chunk = launch_chunk(cells);
     const long long dirChunk = g.dirChunk >= 0 ? g.dirChunk : defaultChunk;
     const bool refDirs = g.pert == BIFHIP_PERT_REFERENCE;
-    const int per = ls ? 9 : 3;
+    const int D = model_dim(g.model);
+    const int per = ls ? D * D : D;
+    if (!ls && g.precision == BIFHIP_PREC_FP32 && g.model == BIFHIP_MODEL_MEMRISTIVE)
+        return fail(BIFHIP_E_ARG, "the FP32 LLE path supports the Rossler and linear models");
     double* d_dirs = nullptr;
     std::vector<double> hdirs;
     if (refDirs) RC_TRY(pool_get("dirs", (size_t)std::min(chunk, cells) * per, &d_dirs));
@@ -540,7 +546,7 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
         }
         if (ls) {
             Timed t("ls");
-            HIP_TRY(launch_ls(g.model, A, S.maxValue != 0, d_out + off * 3, g.stream));
+            HIP_TRY(launch_ls(g.model, A, S.maxValue != 0, d_out + off * D, g.stream));
         } else {
             Timed t(g.precision == BIFHIP_PREC_FP32 ? "lle32" : "lle");
             HIP_TRY(launch_lle(g.model, g.precision, A, S.maxValue != 0, d_out + off, g.stream));
@@ -585,6 +591,8 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
                        const int* h_list, const int* d_list, int base, int n, double* d_lle,
                        int* d_peak)
 {
+    if (g.model == BIFHIP_MODEL_MEMRISTIVE)
+        return fail(BIFHIP_E_ARG, "the fused map supports the Rossler and linear models");
     if (S.pre != 1) return fail(BIFHIP_E_ARG, "the fused map needs preScaller 1");
     if (S.maxValue != 0 || L.maxValue != 0)
         return fail(BIFHIP_E_ARG, "the fused map runs without divergence checks (maxValue 0)");
@@ -972,7 +980,9 @@ int lyap_to_host(bool ls, int dimension, double tMax, double NT, int nPts, doubl
     if (nPts < 1 || dimension < 1 || dimension > 2) return fail(BIFHIP_E_ARG, "bad grid");
     const long long cells = cells_of(nPts, dimension);
     const size_t n = (size_t)cells * (ls ? nIC : 1);
-    if (ls && nIC != 3) return fail(BIFHIP_E_ARG, "LS supports 3 state variables");
+    if (ls && nIC != model_dim(g.model))
+        return fail(BIFHIP_E_ARG, "LS of model %d needs %d state variables (got %d)", g.model,
+                    model_dim(g.model), nIC);
     DBuf d;
     HIP_TRY(d.alloc(sizeof(double) * n));
     RC_TRY(lyap_impl(ls, dimension, tMax, NT, nPts, h, eps, ic, nIC, ranges, mut, maxValue,
@@ -1001,7 +1011,8 @@ const char* bifhip_get_output_path(void) { return g.outPath.c_str(); }
 
 int bifhip_set_model(int model)
 {
-    if (model != BIFHIP_MODEL_ROSSLER && model != BIFHIP_MODEL_LINEAR)
+    if (model != BIFHIP_MODEL_ROSSLER && model != BIFHIP_MODEL_LINEAR &&
+        model != BIFHIP_MODEL_MEMRISTIVE)
         return fail(BIFHIP_E_ARG, "unknown model %d", model);
     g.model = model;
     return BIFHIP_OK;

@@ -28,7 +28,7 @@ struct Ctx {
     int useGraph = 1;          // replay tile launches and training epochs from HIP graphs
     int sweepIC = 0;           // swept entries index the initial conditions (IC drivers)
     int partialResolve = 1;    // surrogate: class-only cells re-solve the peak count alone
-    double chaosConfidence = 0.99;  // surrogate: chaos confidence below this re-solves the LLE
+    double chaosConfidence = 0.98;  // surrogate: chaos confidence below this re-solves the LLE
     std::string lastError;
     hipStream_t stream = nullptr;
 };

@@ -208,6 +208,8 @@ static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, cons
     RC_TRY(ensure_device());
     if (nTiles < 1 || tileW < 1 || tileH < 1 || !originI || !originJ || !nBlocks || !d_out)
         return fail(BIFHIP_E_ARG, "bad tile arguments");
+    if (g.model == BIFHIP_MODEL_MEMRISTIVE)
+        return fail(BIFHIP_E_ARG, "the tile path supports the Rossler and linear models");
     Grid G;
     RC_TRY(make_grid(G, nPts, 2, ranges, mut, ic, nIC, values, nVal));
     for (int t = 0; t < nTiles; ++t) {

@@ -1,0 +1,271 @@
+// k_generic.hip -- kernels for models with a state of D variables held as a
+// register array (the five-variable memristive system of the reference's
+// calculateDiscreteModel, Library/cudaLibrary.cu:19-43).  Same algorithms
+// and operation order as the three-variable kernels in k_bif.hip and
+// k_lyap.hip, one trajectory per lane:
+//
+//  k_gen_bif_peaks  transient, recorded window into the streaming peak
+//                   finder (calculateDiscreteModelCUDA + peakFinder,
+//                   Library/cudaLibrary.cu:78-117, 220-266);
+//  k_gen_series     the raw recorded series (parity tests);
+//  k_gen_lle        Benettin two-trajectory LLE (LLEKernelCUDA,
+//                   Library/cudaLibrary.cu:666-785);
+//  k_gen_ls         finite-difference spectrum with classical Gram-Schmidt
+//                   (LSKernelCUDA / gramSchmidtProcess,
+//                   Library/cudaLibrary.cu:913-1100).
+#include "bifhip_dev.h"
+#include "peak_stream.h"
+
+namespace bifhip {
+
+template <int D>
+__device__ __forceinline__ double component(const double (&x)[D], int i)
+{
+    double r = x[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k)
+        if (k == i) r = x[k];
+    return r;
+}
+
+// nIter steps with the reference's early stop on x[wv] (writableVar 0 in
+// the Lyapunov kernels).
+template <class M, bool CHECK>
+__device__ __forceinline__ void run_n(double (&x)[M::D], const typename M::P64& p, double h,
+                                      int nIter, int wv, double maxValue)
+{
+    for (int i = 0; i < nIter; ++i) {
+        M::stepn(x, p, h);
+        if (CHECK && diverged(component<M::D>(x, wv), maxValue)) return;
+    }
+}
+
+template <class M, bool CHECK>
+__global__ __launch_bounds__(256) void k_gen_bif_peaks(const BifArgs A, int wv,
+                                                       int* __restrict__ counts,
+                                                       double* __restrict__ amps,
+                                                       double* __restrict__ ivls)
+{
+    constexpr int D = M::D;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nCells) return;
+    const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
+    double v[kMaxVal], x[D];
+    cell_setup_n<D>(A.g, idx, v, x);
+    const typename M::P64 p = M::params(v);
+    const double h = A.h;
+    // transient, no divergence check (Library/cudaLibrary.cu:107-108)
+    for (int s = 0; s < A.skip; ++s) M::stepn(x, p, h);
+
+    PeakSink sink;
+    sink.amps = amps ? amps + (long long)t * A.rowStride : nullptr;
+    sink.ivls = ivls ? ivls + (long long)t * A.rowStride : nullptr;
+    sink.cap = A.cap;
+    sink.hpf = A.hpf;
+    PeakStream ps;
+    ps.init();
+    bool bad = false;
+    for (int i = 0; i < A.N && !bad; ++i) {
+        const double d = component<D>(x, wv);
+        if (i >= 2) ps.push(d, i, sink);
+        else { ps.p2 = ps.p1; ps.p1 = d; }
+        for (int j = 0; j < A.pre - 1; ++j) M::stepn(x, p, h);
+        M::stepn(x, p, h);
+        if (CHECK && diverged(component<D>(x, wv), A.maxValue)) bad = true;
+    }
+    counts[t] = bad ? -1 : ps.count();
+}
+
+template <class M>
+__global__ __launch_bounds__(256) void k_gen_series(const BifArgs A, int wv,
+                                                    double* __restrict__ data,
+                                                    int* __restrict__ flags)
+{
+    constexpr int D = M::D;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nCells) return;
+    const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
+    double v[kMaxVal], x[D];
+    cell_setup_n<D>(A.g, idx, v, x);
+    const typename M::P64 p = M::params(v);
+    const double h = A.h;
+    for (int s = 0; s < A.skip; ++s) M::stepn(x, p, h);
+    double* row = data + (long long)t * A.rowStride;
+    int flag = 0;
+    for (int i = 0; i < A.N; ++i) {
+        row[i] = component<D>(x, wv);
+        for (int j = 0; j < A.pre - 1; ++j) M::stepn(x, p, h);
+        M::stepn(x, p, h);
+        if (A.maxValue != 0 && diverged(component<D>(x, wv), A.maxValue)) {
+            flag = -1;
+            break;
+        }
+    }
+    flags[t] = flag;
+}
+
+template <class M, bool CHECK>
+__global__ __launch_bounds__(256) void k_gen_lle(const LyapArgs A, double* __restrict__ out)
+{
+    constexpr int D = M::D;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nCells) return;
+    const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
+    double v[kMaxVal], x[D], y[D], z[D];
+    cell_setup_n<D>(A.g, idx, v, x);
+    const typename M::P64 p = M::params(v);
+    const double h = A.h, eps = A.eps;
+#pragma unroll
+    for (int i = 0; i < D; ++i) z[i] = A.dirs ? A.dirs[(long long)t * D + i] : (i == 0 ? 1.0 : 0.0);
+    run_n<M, CHECK>(x, p, h, A.skip, 0, A.maxValue);
+#pragma unroll
+    for (int i = 0; i < D; ++i) y[i] = z[i] * eps + x[i];
+    double result = 0;
+    for (int b = 0; b < A.nBlocks; ++b) {
+        run_n<M, CHECK>(x, p, h, A.nNT, 0, A.maxValue);
+        run_n<M, CHECK>(y, p, h, A.nNT, 0, A.maxValue);
+        double s = 0;
+#pragma unroll
+        for (int l = 0; l < D; ++l) s += (x[l] - y[l]) * (x[l] - y[l]);
+        s = sqrt(s) / eps;
+        result += log(s);
+        if (s != 0) s = (1 / s);
+#pragma unroll
+        for (int j = 0; j < D; ++j) y[j] = x[j] - ((x[j] - y[j]) * s);
+    }
+    out[t] = result / A.tMax;
+}
+
+// Classical Gram-Schmidt of D row vectors a -> b with norms.
+template <int D>
+__device__ __forceinline__ void gs_n(const double (&a)[D * D], double (&b)[D * D],
+                                     double (&den)[D])
+{
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) b[i * D + j] = a[i * D + j];
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            double num = 0, dd = 0;
+#pragma unroll
+            for (int m = 0; m < D; ++m) {
+                num += a[i * D + m] * b[j * D + m];
+                dd += b[j * D + m] * b[j * D + m];
+            }
+            const double f = dd == 0 ? 0 : num / dd;
+#pragma unroll
+            for (int m = 0; m < D; ++m) b[i * D + m] -= f * b[j * D + m];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double d = 0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) d += b[i * D + j] * b[i * D + j];
+        d = sqrt(d);
+#pragma unroll
+        for (int j = 0; j < D; ++j) b[i * D + j] = d == 0 ? 0 : b[i * D + j] / d;
+        den[i] = d;
+    }
+}
+
+template <class M, bool CHECK>
+__global__ __launch_bounds__(256) void k_gen_ls(const LyapArgs A, double* __restrict__ out)
+{
+    constexpr int D = M::D;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nCells) return;
+    const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
+    double v[kMaxVal], x[D];
+    cell_setup_n<D>(A.g, idx, v, x);
+    const typename M::P64 p = M::params(v);
+    const double h = A.h, eps = A.eps;
+    double z[D * D], y[D * D], den[D], res[D];
+#pragma unroll
+    for (int i = 0; i < D * D; ++i)
+        z[i] = A.dirs ? A.dirs[(long long)t * D * D + i] : ((i % (D + 1)) == 0 ? 1.0 : 0.0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) res[i] = 0;
+    run_n<M, CHECK>(x, p, h, A.skip, 0, A.maxValue);
+    gs_n<D>(z, y, den);
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+#pragma unroll
+        for (int i = 0; i < D; ++i) y[j * D + i] = y[j * D + i] * eps + x[i];
+    for (int b = 0; b < A.nBlocks; ++b) {
+        run_n<M, CHECK>(x, p, h, A.nNT, 0, A.maxValue);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double yk[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) yk[i] = y[k * D + i];
+            run_n<M, CHECK>(yk, p, h, A.nNT, 0, A.maxValue);
+#pragma unroll
+            for (int i = 0; i < D; ++i) y[k * D + i] = yk[i] - x[i];
+        }
+        gs_n<D>(y, z, den);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            res[k] += log(den[k] / eps);
+#pragma unroll
+            for (int j = 0; j < D; ++j) y[k * D + j] = (double)(x[j] + z[k * D + j] * eps);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) out[(long long)t * D + i] = res[i] / A.tMax;
+}
+
+// ------------------------------------------------------------ launchers
+
+hipError_t launch_gen_bif(int model, const BifArgs& A, int wv, bool check, int* counts,
+                          double* amps, double* ivls, hipStream_t st)
+{
+    if (A.nCells <= 0) return hipSuccess;
+    if (model != kMemristive) return hipErrorInvalidValue;
+    const dim3 blk(256), grd((A.nCells + 255) / 256);
+    if (check)
+        hipLaunchKernelGGL((k_gen_bif_peaks<Memristive, true>), grd, blk, 0, st, A, wv, counts,
+                           amps, ivls);
+    else
+        hipLaunchKernelGGL((k_gen_bif_peaks<Memristive, false>), grd, blk, 0, st, A, wv, counts,
+                           amps, ivls);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_series(int model, const BifArgs& A, int wv, double* data, int* flags,
+                             hipStream_t st)
+{
+    if (A.nCells <= 0) return hipSuccess;
+    if (model != kMemristive) return hipErrorInvalidValue;
+    const dim3 blk(256), grd((A.nCells + 255) / 256);
+    hipLaunchKernelGGL((k_gen_series<Memristive>), grd, blk, 0, st, A, wv, data, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_lle(int model, const LyapArgs& A, bool check, double* out,
+                          hipStream_t st)
+{
+    if (A.nCells <= 0) return hipSuccess;
+    if (model != kMemristive) return hipErrorInvalidValue;
+    const dim3 blk(256), grd((A.nCells + 255) / 256);
+    if (check)
+        hipLaunchKernelGGL((k_gen_lle<Memristive, true>), grd, blk, 0, st, A, out);
+    else
+        hipLaunchKernelGGL((k_gen_lle<Memristive, false>), grd, blk, 0, st, A, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_ls(int model, const LyapArgs& A, bool check, double* out, hipStream_t st)
+{
+    if (A.nCells <= 0) return hipSuccess;
+    if (model != kMemristive) return hipErrorInvalidValue;
+    const dim3 blk(256), grd((A.nCells + 255) / 256);
+    if (check)
+        hipLaunchKernelGGL((k_gen_ls<Memristive, true>), grd, blk, 0, st, A, out);
+    else
+        hipLaunchKernelGGL((k_gen_ls<Memristive, false>), grd, blk, 0, st, A, out);
+    return hipGetLastError();
+}
+
+}  // namespace bifhip

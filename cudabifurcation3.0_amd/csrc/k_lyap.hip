@@ -263,10 +263,16 @@ __global__ __launch_bounds__(64) void k_gs(const double* __restrict__ a, double*
 
 // --------------------------------------------------------- launchers
 
+hipError_t launch_gen_lle(int model, const LyapArgs& A, bool check, double* out,
+                          hipStream_t st);
+hipError_t launch_gen_ls(int model, const LyapArgs& A, bool check, double* out, hipStream_t st);
+
 hipError_t launch_lle(int model, int precision, const LyapArgs& A, bool check, double* out,
                       hipStream_t st)
 {
     if (A.nCells <= 0) return hipSuccess;
+    if (model == kMemristive)
+        return precision == 1 ? hipErrorInvalidValue : launch_gen_lle(model, A, check, out, st);
     const dim3 blk(256), grd((A.nCells + 255) / 256);
 #define LLE_CASE(KER, MOD, C) hipLaunchKernelGGL((KER<MOD, C>), grd, blk, 0, st, A, out)
     if (precision == 1) {
@@ -283,6 +289,7 @@ hipError_t launch_lle(int model, int precision, const LyapArgs& A, bool check, d
 hipError_t launch_ls(int model, const LyapArgs& A, bool check, double* out, hipStream_t st)
 {
     if (A.nCells <= 0) return hipSuccess;
+    if (model == kMemristive) return launch_gen_ls(model, A, check, out, st);
     const dim3 blk(256), grd((A.nCells + 255) / 256);
 #define LS_CASE(MOD, C) hipLaunchKernelGGL((k_ls64<MOD, C>), grd, blk, 0, st, A, out)
     if (model == kLinear) { if (check) LS_CASE(Linear, true); else LS_CASE(Linear, false); }

@@ -31,6 +31,9 @@ extern "C" {
 
 #define BIFHIP_MODEL_ROSSLER 0 /* Library/cudaLibrary.cu:15-17 (values {a,b,c}) */
 #define BIFHIP_MODEL_LINEAR 1  /* x_i += h*a_i*x_i: closed-form test model      */
+#define BIFHIP_MODEL_MEMRISTIVE 2 /* Library/cudaLibrary.cu:19-43: five variables,
+                                     values a0..a7 (a7 = explicit share of the step);
+                                     FP64 drivers only (no fused map, tiles, FP32) */
 
 #define BIFHIP_PREC_FP64 0     /* reference operation order, no contraction     */
 #define BIFHIP_PREC_FP32 1     /* FP32 bulk integration, FP64 renorm/log        */
@@ -363,7 +366,7 @@ int bifhip_set_graph(int on);
  * count alone (default 1); 0 re-solves every flagged cell in full. */
 int bifhip_set_partial_resolve(int on);
 /* Surrogate-refined map: cells whose chaos-logit confidence is below this
- * threshold (in [0.5, 1], default 0.99) have their LLE re-solved. */
+ * threshold (in [0.5, 1], default 0.98) have their LLE re-solved. */
 int bifhip_set_chaos_confidence(double threshold);
 
 /* ------------------------------------------ surrogate-refined map */
@@ -377,7 +380,7 @@ int bifhip_set_chaos_confidence(double threshold);
  * head, kept on the predicted side of chaosThreshold.  A cell needs its
  * class re-solved when the class softmax confidence is below confThreshold
  * or a 4-neighbour has another predicted class, and its regime when the
- * chaos confidence is below bifhip_set_chaos_confidence's threshold (0.99
+ * chaos confidence is below bifhip_set_chaos_confidence's threshold (0.98
  * by default) or a 4-neighbour has another predicted regime.  Cells that
  * need their regime are re-solved by the plain FP64 fused kernel
  * (bifhip_lle_peak_map_device semantics).  Cells that need only their class

@@ -30,7 +30,7 @@ int orc_threads(void)
 
 int orc_model_dim(int model)
 {
-    return model == ORC_MODEL_ROSSLER ? 3 : ORC_MAXD;
+    return model == ORC_MODEL_ROSSLER ? 3 : (model == ORC_MODEL_MEMRISTIVE ? 5 : ORC_MAXD);
 }
 
 /* calculateDiscreteModel, Library/cudaLibrary.cu:15-17 (the Rossler body):
@@ -50,12 +50,39 @@ static inline void linear_step(double *x, const double *a, double h, int n)
         x[i] = x[i] + h * (a[i] * x[i]);
 }
 
+/* calculateDiscreteModel, Library/cudaLibrary.cu:19-43 (the active body):
+ * five-variable memristive system, composition step -- an explicit
+ * sequential pass of length k = h*a7 with k held in a float (:19-27), the
+ * state rounded to float (:29-33), then a semi-implicit pass of length
+ * h*(1-a7) with the linear terms solved in closed form (:36-43).  abs() of
+ * the double x4 is taken as the double absolute value. */
+static inline void memristive_step(double *x, const double *a, double h)
+{
+    float y[5], k;
+    k = (float)(h * a[7]);
+    x[4] = x[4] + k * (x[0]);
+    x[3] = x[3] + k * (-a[2] * x[1] - a[3] * x[3]);
+    x[2] = x[2] + k * (a[1] * (x[1] - x[0] - x[2]));
+    x[1] = x[1] + k * (-x[2] + x[3]);
+    x[0] = x[0] + k * (a[0] * (x[2] - (a[4] + a[5] + a[6] * fabs(x[4])) * x[0]));
+    for (int i = 0; i < 5; ++i)
+        y[i] = (float)x[i];
+    double h1 = h * (1 - a[7]);
+    x[0] = (y[0] + h1 * (a[0] * x[2])) / (1 + h1 * a[0] * (a[4] + a[5] + a[6] * fabs(x[4])));
+    x[1] = y[1] + h1 * (-x[2] + x[3]);
+    x[2] = (y[2] + h1 * (a[1] * (x[1] - x[0]))) / (1 + h1 * a[1]);
+    x[3] = (y[3] + h1 * (-a[2] * x[1])) / (1 + h1 * a[3]);
+    x[4] = y[4] + h1 * (x[0]);
+}
+
 static _Thread_local int g_linear_dim = 3;
 
 void orc_step(int model, double *x, const double *a, double h)
 {
     if (model == ORC_MODEL_ROSSLER)
         rossler_step(x, a, h);
+    else if (model == ORC_MODEL_MEMRISTIVE)
+        memristive_step(x, a, h);
     else
         linear_step(x, a, h, g_linear_dim);
 }

@@ -39,6 +39,7 @@ extern "C" {
 /* Model ids shared with the product (same numbering). */
 #define ORC_MODEL_ROSSLER 0   /* sequential explicit Euler, cudaLibrary.cu:15-17 */
 #define ORC_MODEL_LINEAR  1   /* x_i += h*(a_i*x_i): closed-form test model     */
+#define ORC_MODEL_MEMRISTIVE 2 /* five-variable system, cudaLibrary.cu:19-43    */
 
 int  orc_model_dim(int model);
 /* swept entries go into the initial conditions (1) or the parameters (0) */

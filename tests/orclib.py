@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liborc.so")
 
 MODEL_ROSSLER = 0
 MODEL_LINEAR = 1
+MODEL_MEMRISTIVE = 2
 PERT_REFERENCE = 0
 PERT_AXIS_X = 1
 

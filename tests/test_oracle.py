@@ -334,3 +334,55 @@ def test_variational_ls_matches_finite_difference(orc):
                           orc.PERT_AXIS_X, 0, 0, 1)[0]
         var = orc.ls_var_cells([0.2], [0.2], [c], IC, h, int(trans / h), 50, [int(T / 0.5)])[0]
         np.testing.assert_allclose(var, fd, atol=2e-4)
+
+
+# --------------------------------------------- five-variable memristive model
+
+# parameter vector of the reference's active main.cpp call (main.cpp:127)
+MEM = [8.0, 35.0, 15.0, 0.01, -1.6676, 0.0676, 0.3682, 0.5]
+
+
+def _mem_rhs(x, a):
+    """The ODE both passes of the composition step discretise."""
+    return np.array([a[0] * (x[2] - (a[4] + a[5] + a[6] * abs(x[4])) * x[0]),
+                     -x[2] + x[3],
+                     a[1] * (x[1] - x[0] - x[2]),
+                     -a[2] * x[1] - a[3] * x[3],
+                     x[0]])
+
+
+def test_memristive_step_converges_to_its_ode(orc):
+    """For every split a7 (explicit share 1, 1/2, 0) repeated steps converge
+    to the ODE's solution at first order: halving h halves the error.  A
+    dropped term, a wrong sign or a swapped variable changes the limit and
+    fails this."""
+    from scipy.integrate import solve_ivp
+    x0 = [0.5, -0.2, 0.3, 0.1, -0.4]
+    T = 0.5
+    for a7 in (1.0, 0.5, 0.0):
+        a = MEM[:7] + [a7]
+        ref = solve_ivp(lambda t, x: _mem_rhs(x, a), (0, T), x0, method="DOP853", rtol=1e-12,
+                        atol=1e-14).y[:, -1]
+        errs = []
+        for h in (2e-3, 1e-3, 5e-4):
+            x = np.array(x0)
+            for _ in range(int(round(T / h))):
+                x = orc.step(x, a, h, model=orc.MODEL_MEMRISTIVE)
+            errs.append(float(np.abs(x - ref).max()))
+        assert errs[0] < 0.1 * np.abs(ref).max(), (a7, errs)
+        assert errs[1] < 0.6 * errs[0] and errs[2] < 0.6 * errs[1], (a7, errs)
+
+
+def test_memristive_step_rounds_the_state_to_float(orc):
+    """With a7 = 1 the semi-implicit pass has zero length, so a step returns
+    the explicit pass rounded to float (the reference's float y[5]); with
+    a7 = 0 the explicit pass has zero length."""
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        x = rng.normal(size=5)
+        y = orc.step(x, MEM[:7] + [1.0], 0.005, model=orc.MODEL_MEMRISTIVE)
+        assert (y.astype(np.float32).astype(np.float64) == y).all()
+        assert not (x.astype(np.float32).astype(np.float64) == x).all()
+    # fixed point of the ODE: the origin
+    z = orc.step(np.zeros(5), MEM, 0.005, model=orc.MODEL_MEMRISTIVE)
+    assert (z == 0).all()

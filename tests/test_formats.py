@@ -1,0 +1,93 @@
+"""CPU tests of the file readers (cudabifurcation3.0_amd/formats.py): files
+written by the library's writers, and text in the reference's layout,
+read back to the values and ranges that went in."""
+import numpy as np
+
+import bifhip as B
+import formats as F
+from test_abi import _ref_map_text
+
+
+def test_map2d_round_trip_int_and_double(tmp_path):
+    rng = np.random.default_rng(1)
+    ranges = [0.05, 0.45, 2.0, 10.0]
+    n = 7
+    ints = rng.integers(0, 40, size=n * n).astype(np.int32)
+    p = tmp_path / "m.csv"
+    B.write_map_int(p, n, ranges, ints)
+    r, m = F.read_map2d(p)
+    np.testing.assert_array_equal(r, [[0.05, 0.45], [2, 10]])
+    np.testing.assert_array_equal(m, ints.reshape(n, n))
+    # doubles at precision 17 round-trip exactly
+    vals = rng.normal(size=n * n) * 1e-3
+    B.write_map_double(p, n, ranges, vals, precision=17)
+    _, m = F.read_map2d(p)
+    np.testing.assert_array_equal(m, vals.reshape(n, n))
+
+
+def test_map2d_reads_reference_layout_and_matlab_orientation(tmp_path):
+    """Row j holds the second-axis index j: the cell of flat index j*n + i
+    is values[j, i], what imagesc(x, y, a) shows at (x_i, y_j)."""
+    n = 4
+    flat = np.arange(n * n)
+    p = tmp_path / "m.csv"
+    p.write_text(_ref_map_text(n, [1, 2, 3, 4], flat, 6))
+    r, m = F.read_map2d(p)
+    assert m[2, 1] == 2 * n + 1
+    assert m.shape == (n, n)
+    # header-less files (bifurcation2DKDEIC, LLE2DIC)
+    p.write_text(_ref_map_text(n, [1, 2, 3, 4], flat, 6, header=False))
+    r, m = F.read_map2d(p, header=False)
+    assert r is None and m[3, 0] == 3 * n
+
+
+def test_map2d_nan_cells(tmp_path):
+    p = tmp_path / "m.csv"
+    p.write_text("0 1\n0 1\n1, nan, \n-nan, 2")
+    _, m = F.read_map2d(p)
+    assert m[0, 0] == 1 and np.isnan(m[0, 1]) and np.isnan(m[1, 0]) and m[1, 1] == 2
+
+
+def test_ls2d_files(tmp_path):
+    n = 3
+    vals = np.arange(n * n * 3, dtype=float).reshape(n * n, 3)
+    for k in range(3):
+        B.write_map_double(tmp_path / f"mat.csv{k + 1}.csv", n, [0, 1, 0, 1], vals.ravel(),
+                           precision=12, stride=3, offset=k)
+    r, maps = F.read_ls2d(str(tmp_path / "mat.csv"), 3)
+    for k in range(3):
+        np.testing.assert_array_equal(maps[k], vals[:, k].reshape(n, n))
+
+
+def test_one_dimensional_files(tmp_path):
+    p = tmp_path / "b.csv"
+    counts = np.array([2, 0, 1], dtype=np.int32)
+    peaks = np.array([[1.5, 2.5], [0, 0], [3.25, 0]])
+    B.write_bif1d(p, 3, [2.0, 7.0], counts, peaks)
+    par, pk = F.read_bif1d(p)
+    np.testing.assert_array_equal(par, [2.0, 2.0, 7.0])
+    np.testing.assert_array_equal(pk, [1.5, 2.5, 3.25])
+    B.write_1d_values(p, 3, [2.0, 7.0], np.arange(9, dtype=float), per_cell=3)
+    a = F.read_lines(p)
+    assert a.shape == (3, 4)
+    np.testing.assert_array_equal(a[:, 0], F.lattice(2.0, 7.0, 3))
+
+
+def test_lattice_is_value_by_idx():
+    for n, s, e in ((1000, 2.0, 7.0), (512, 0.05, 0.45), (17, -6.0, 6.0)):
+        want = [B.value_by_idx(i, n, s, e, 0) for i in range(n)]
+        np.testing.assert_array_equal(F.lattice(s, e, n), want)
+
+
+def test_map3d_layout(tmp_path):
+    n = 3
+    vol = np.arange(n ** 3).reshape(n, n, n)
+    lines = ["0 1", "2 3", "4 5"]
+    for k in range(n):
+        for j in range(n):
+            lines.append(", ".join(str(v) for v in vol[k, j]) + ", ")
+    p = tmp_path / "v.csv"
+    p.write_text("\n".join(lines))
+    r, v = F.read_map3d(p)
+    np.testing.assert_array_equal(r, [[0, 1], [2, 3], [4, 5]])
+    np.testing.assert_array_equal(v, vol)

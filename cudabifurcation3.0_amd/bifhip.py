@@ -74,6 +74,8 @@ def _declare(L):
         "bifhip_bifurcation2DDBSCANIC": drv_db,
         "bifhip_LLE1DIC": drv_ly, "bifhip_LLE2DIC": drv_ly,
         "bifhip_set_sweep": [_I],
+        "bifhip_calculationOfPeriodicityByOstrovsky": drv_db + [_D],
+        "bifhip_ostrovsky_to_buffer": drv_db + [_D, _PD],
         "bifhip_LLE1D": drv_ly, "bifhip_LLE2D": drv_ly,
         "bifhip_LS1D": drv_ly, "bifhip_LS2D": drv_ly,
         "bifhip_bifurcation1D_to_buffer": drv_bif1 + [_PI, _PD, _I],
@@ -334,6 +336,28 @@ def bifurcation2DKDEIC(*a):
 
 def bifurcation2DDBSCANIC(*a):
     _db2(load().bifhip_bifurcation2DDBSCANIC, *a)
+
+
+def calculationOfPeriodicityByOstrovsky(tMax, nPts, h, amountOfInitialConditions,
+                                       initialConditions, ranges, indicesOfMutVars, writableVar,
+                                       maxValue, maxAmountOfPeaks, transientTime, values,
+                                       amountOfValues, preScaller, eps, ostrovskyThreshold):
+    ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(load().bifhip_calculationOfPeriodicityByOstrovsky(
+        tMax, nPts, h, amountOfInitialConditions, _pd(ic), _pd(r), _pi(m), writableVar, maxValue,
+        maxAmountOfPeaks, transientTime, _pd(v), amountOfValues, preScaller, eps,
+        ostrovskyThreshold))
+
+
+def ostrovsky_map(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, writableVar,
+                  maxValue, maxAmountOfPeaks, transientTime, values, preScaller, eps, threshold):
+    ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    out = np.zeros(nPts * nPts)
+    _check(load().bifhip_ostrovsky_to_buffer(tMax, nPts, h, len(ic), _pd(ic), _pd(r), _pi(m),
+                                             writableVar, maxValue, maxAmountOfPeaks,
+                                             transientTime, _pd(v), len(v), preScaller, eps,
+                                             threshold, _pd(out)))
+    return out
 
 
 SWEEP_PARAMETERS, SWEEP_INITIAL_CONDITIONS = 0, 1

@@ -155,6 +155,11 @@ struct Rossler {
         y2 = y2 + vb;
     }
 
+    __device__ __forceinline__ static void stepn(double (&x)[3], const P64& p, double h)
+    {
+        step(x[0], x[1], x[2], p, h);
+    }
+
     // FP32 fast form of the same map with fused multiply-adds (7 ops):
     //   x0 <- x0 - h(x1+x2)
     //   x1 <- x1(1+ha) + h x0
@@ -218,6 +223,10 @@ struct Linear {
     {
         step(x0, x1, x2, p, h);
         step(y0, y1, y2, q, h);
+    }
+    __device__ __forceinline__ static void stepn(double (&x)[3], const P64& p, double h)
+    {
+        step(x[0], x[1], x[2], p, h);
     }
     __device__ __forceinline__ static P32 params32(const double* v, double h)
     {

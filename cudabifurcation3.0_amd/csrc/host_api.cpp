@@ -43,6 +43,12 @@ hipError_t launch_dbscan_block(int nList, const int* list, const int* counts, co
                                const double* ivls, long long rowStride, double T, int* out,
                                const int* outIndex, hipStream_t st);
 int dbscan_block_max_peaks();
+hipError_t launch_gen_bif(int model, const BifArgs& A, int wv, bool check, int* counts,
+                          double* amps, double* ivls, double* last, hipStream_t st);
+hipError_t launch_osc_flags(int n, const int* counts, const double* amps, long long rowStride,
+                            const double* last, double thr, unsigned char* flags, hipStream_t st);
+hipError_t launch_osc_combine(int n, const int* clusters, const unsigned char* flags,
+                              double* out, hipStream_t st);
 hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, long long rowStride,
                            int cap, int maxPeaks, int kdeSampling, double I1, double I2, double H,
                            int* out, const int* outIndex, int* overflow, int* nOverflow,
@@ -423,6 +429,74 @@ int cluster_map_device(const PeakMapSpec& S, const ClusterParams& C, int* d_out)
     }
     if (C.kind == 0 && g.clusterClamp > 0) HIP_TRY(launch_clamp((int)cells, d_out, g.clusterClamp, g.stream));
     return BIFHIP_OK;
+}
+
+void progress(const char* name, long long cells);
+
+// Ostrovsky periodicity map (calculationOfPeriodicityByOstrovsky,
+// Library/hostLibrary.cu:748-925) of a 2D lattice swept over initial
+// conditions: rows hold every peak of a cell, so the five flags and the
+// DBSCAN count see the reference's full peak list.
+int ostrovsky_map_device(const PeakMapSpec& S, int maxPeaks, double eps, double threshold,
+                         double* d_out)
+{
+    const long long cells = S.cells;
+    const int cap = std::max(2, S.N / 2 + 1);
+    const long long rowBytes = (long long)cap * 16 + 16;
+    long long pass = std::max<long long>(1, (8LL << 30) / rowBytes);
+    pass = std::min(pass, cells);
+    if (g.maxCellsPerLaunch > 0) pass = std::min(pass, g.maxCellsPerLaunch);
+    int *d_counts, *d_clusters, *d_ovf, *d_novf;
+    double *d_amps, *d_ivls, *d_last;
+    unsigned char* d_flags;
+    RC_TRY(pool_get("o_counts", (size_t)pass, &d_counts));
+    RC_TRY(pool_get("o_clusters", (size_t)pass, &d_clusters));
+    RC_TRY(pool_get("o_amps", (size_t)(pass * cap), &d_amps));
+    RC_TRY(pool_get("o_ivls", (size_t)(pass * cap), &d_ivls));
+    RC_TRY(pool_get("o_last", (size_t)pass, &d_last));
+    RC_TRY(pool_get("o_flags", (size_t)pass, &d_flags));
+    RC_TRY(pool_get("o_ovf", (size_t)pass, &d_ovf));
+    RC_TRY(pool_get("o_novf", 1, &d_novf));
+    const double T = dbscan_threshold(eps);
+    for (long long off = 0; off < cells; off += pass) {
+        const int m = (int)std::min(pass, cells - off);
+        BifArgs A = bif_args(S, (int)off, nullptr, m, cap);
+        {
+            Timed t("bif_peaks");
+            HIP_TRY(launch_gen_bif(g.model, A, S.wv, S.maxValue != 0, d_counts, d_amps, d_ivls,
+                                   d_last, g.stream));
+        }
+        HIP_TRY(launch_osc_flags(m, d_counts, d_amps, cap, d_last, threshold, d_flags,
+                                 g.stream));
+        HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
+        {
+            Timed t("dbscan");
+            HIP_TRY(launch_dbscan_wave(m, d_counts, d_amps, d_ivls, cap, cap, maxPeaks, T,
+                                       d_clusters, d_ovf, d_novf, 0, g.stream));
+        }
+        int novf = 0;
+        HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        if (novf > 0)   // rows already hold every peak: cluster them in place
+            RC_TRY(dbscan_full_rows(novf, d_ovf, m, d_counts, d_amps, d_ivls, cap, T, d_clusters,
+                                    nullptr));
+        HIP_TRY(launch_osc_combine(m, d_clusters, d_flags, d_out + off, g.stream));
+    }
+    return BIFHIP_OK;
+}
+
+int ostrovsky_impl(double tMax, int nPts, double h, int nIC, const double* ic,
+                   const double* ranges, const int* mut, int wv, double maxValue, int maxPeaks,
+                   double transientTime, const double* values, int nVal, int pre, double eps,
+                   double threshold, double* d_out)
+{
+    RC_TRY(ensure_device());
+    PeakMapSpec S;
+    RC_TRY(peak_spec(S, 2, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime,
+                     values, nVal, pre));
+    progress("OSTROVSKY", S.cells);
+    RC_TRY(ostrovsky_map_device(S, maxPeaks, eps, threshold, d_out));
+    return finish();
 }
 
 // ------------------------------------------------------- Lyapunov core
@@ -1339,6 +1413,49 @@ int bifhip_LLE2DIC(double tMax, double NT, int nPts, double h, double eps, const
     (void)wv;
     return write_map_double(g.outPath, nPts, ranges, 0, 6, out.data(), (long long)out.size(), 1,
                             0);
+}
+
+// Library/hostLibrary.cu:748-925 (calculationOfPeriodicityByOstrovsky):
+// the 2D map layout with the header, doubles at the stream's precision.
+int bifhip_calculationOfPeriodicityByOstrovsky(double tMax, int nPts, double h, int nIC,
+                                               const double* ic, const double* ranges,
+                                               const int* mut, int wv, double maxValue,
+                                               int maxPeaks, double transientTime,
+                                               const double* values, int nVal, int pre,
+                                               double eps, double ostrovskyThreshold)
+{
+    SweepGuard sweep(true);
+    if (nPts < 1) return fail(BIFHIP_E_ARG, "nPts must be >= 1");
+    const long long cells = (long long)nPts * nPts;
+    std::vector<double> out((size_t)cells);
+    {
+        DBuf d;
+        HIP_TRY(d.alloc(sizeof(double) * (size_t)cells));
+        RC_TRY(ostrovsky_impl(tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                              transientTime, values, nVal, pre, eps, ostrovskyThreshold,
+                              d.as<double>()));
+        HIP_TRY(hipMemcpy(out.data(), d.p, sizeof(double) * (size_t)cells,
+                          hipMemcpyDeviceToHost));
+    }
+    return write_map_double(g.outPath, nPts, ranges, 1, 6, out.data(), cells, 1, 0);
+}
+
+int bifhip_ostrovsky_to_buffer(double tMax, int nPts, double h, int nIC, const double* ic,
+                               const double* ranges, const int* mut, int wv, double maxValue,
+                               int maxPeaks, double transientTime, const double* values,
+                               int nVal, int pre, double eps, double ostrovskyThreshold,
+                               double* out)
+{
+    SweepGuard sweep(true);
+    if (nPts < 1 || !out) return fail(BIFHIP_E_ARG, "bad arguments");
+    const long long cells = (long long)nPts * nPts;
+    DBuf d;
+    HIP_TRY(d.alloc(sizeof(double) * (size_t)cells));
+    RC_TRY(ostrovsky_impl(tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                          transientTime, values, nVal, pre, eps, ostrovskyThreshold,
+                          d.as<double>()));
+    HIP_TRY(hipMemcpy(out, d.p, sizeof(double) * (size_t)cells, hipMemcpyDeviceToHost));
+    return BIFHIP_OK;
 }
 
 int bifhip_set_sweep(int target)

@@ -805,14 +805,15 @@ static hipError_t launch_bif_m(const BifArgs& A, int wv, bool check, int* counts
 }
 
 hipError_t launch_gen_bif(int model, const BifArgs& A, int wv, bool check, int* counts,
-                          double* amps, double* ivls, hipStream_t st);
+                          double* amps, double* ivls, double* last, hipStream_t st);
 hipError_t launch_gen_series(int model, const BifArgs& A, int wv, double* data, int* flags,
                              hipStream_t st);
 
 hipError_t launch_bif(int model, const BifArgs& A, int wv, bool check, int* counts, double* amps,
                       double* ivls, hipStream_t st)
 {
-    if (model == kMemristive) return launch_gen_bif(model, A, wv, check, counts, amps, ivls, st);
+    if (model == kMemristive)
+        return launch_gen_bif(model, A, wv, check, counts, amps, ivls, nullptr, st);
     if (model == kLinear) return launch_bif_m<Linear>(A, wv, check, counts, amps, ivls, st);
     return launch_bif_m<Rossler>(A, wv, check, counts, amps, ivls, st);
 }

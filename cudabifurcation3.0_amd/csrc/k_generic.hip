@@ -12,7 +12,12 @@
 //                   Library/cudaLibrary.cu:666-785);
 //  k_gen_ls         finite-difference spectrum with classical Gram-Schmidt
 //                   (LSKernelCUDA / gramSchmidtProcess,
-//                   Library/cudaLibrary.cu:913-1100).
+//                   Library/cudaLibrary.cu:913-1100);
+//  k_osc_flags,     Ostrovsky's periodicity classifier on top of the peak
+//  k_osc_combine    rows and DBSCAN counts (Library/cudaLibrary.cu:286-352,
+//                   628-662).
+// k_gen_bif_peaks also serves the three-variable models when the last
+// recorded sample is wanted (the classifier's fifth flag).
 #include "bifhip_dev.h"
 #include "peak_stream.h"
 
@@ -40,11 +45,13 @@ __device__ __forceinline__ void run_n(double (&x)[M::D], const typename M::P64& 
     }
 }
 
+// last[t] (when given): the last recorded sample of cell t.
 template <class M, bool CHECK>
 __global__ __launch_bounds__(256) void k_gen_bif_peaks(const BifArgs A, int wv,
                                                        int* __restrict__ counts,
                                                        double* __restrict__ amps,
-                                                       double* __restrict__ ivls)
+                                                       double* __restrict__ ivls,
+                                                       double* __restrict__ last)
 {
     constexpr int D = M::D;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -65,8 +72,10 @@ __global__ __launch_bounds__(256) void k_gen_bif_peaks(const BifArgs A, int wv,
     PeakStream ps;
     ps.init();
     bool bad = false;
+    double rec = 0;
     for (int i = 0; i < A.N && !bad; ++i) {
         const double d = component<D>(x, wv);
+        rec = d;
         if (i >= 2) ps.push(d, i, sink);
         else { ps.p2 = ps.p1; ps.p1 = d; }
         for (int j = 0; j < A.pre - 1; ++j) M::stepn(x, p, h);
@@ -74,6 +83,63 @@ __global__ __launch_bounds__(256) void k_gen_bif_peaks(const BifArgs A, int wv,
         if (CHECK && diverged(component<D>(x, wv), A.maxValue)) bad = true;
     }
     counts[t] = bad ? -1 : ps.count();
+    if (last) last[t] = rec;
+}
+
+// Ostrovsky's five flags of a cell from its full peak row
+// (peakFinderCUDAForCalculationOfPeriodicityByOstrovsky,
+// Library/cudaLibrary.cu:293-351), bit k = flag k:
+//   0 no kept peak rises above its predecessor
+//   1 kept peaks after the first lie on both sides of the threshold
+//   2 not (all of them above it)
+//   3 at most one kept peak (or a diverged trajectory)
+//   4 the last recorded sample is above the threshold
+// A diverged cell (count -1) sets only flag 3.
+__global__ __launch_bounds__(256) void k_osc_flags(int n, const int* __restrict__ counts,
+                                                   const double* __restrict__ amps,
+                                                   long long rowStride,
+                                                   const double* __restrict__ last, double thr,
+                                                   unsigned char* __restrict__ flags)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int P = counts[c];
+    if (P == -1) {
+        flags[c] = 1u << 3;
+        return;
+    }
+    const double* pk = amps + (long long)c * rowStride;
+    bool f0 = true, one = false, zero = false;
+    for (int i = 1; i < P; ++i) {
+        const double a = pk[i];
+        if (a - pk[i - 1] > 0) f0 = false;
+        if (a > thr) one = true;
+        else zero = true;
+    }
+    const bool f1 = one && zero;
+    const bool f2 = !(one && !zero);
+    const bool f3 = P == 0 || P == 1;
+    const bool f4 = last[c] > thr;
+    flags[c] = (unsigned char)((f0 ? 1 : 0) | (f1 ? 2 : 0) | (f2 ? 4 : 0) | (f3 ? 8 : 0) |
+                               (f4 ? 16 : 0));
+}
+
+// The classifier's value from the DBSCAN count and the flags
+// (dbscanCUDAForCalculationOfPeriodicityByOstrovsky,
+// Library/cudaLibrary.cu:641-661), rules applied in the reference's order.
+__global__ __launch_bounds__(256) void k_osc_combine(int n, const int* __restrict__ clusters,
+                                                     const unsigned char* __restrict__ flags,
+                                                     double* __restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const unsigned f = flags[c];
+    double o = (double)clusters[c];
+    if (f & 4) o = -1 * o;
+    if (f & 8) o = (f & 16) ? 1 : -1;
+    if (f & 2) o = 0;
+    if (f & 1) o = o > 0 ? 0.5 : (o == 0 ? 0 : -0.5);
+    out[c] = o;
 }
 
 template <class M>
@@ -218,18 +284,44 @@ __global__ __launch_bounds__(256) void k_gen_ls(const LyapArgs A, double* __rest
 
 // ------------------------------------------------------------ launchers
 
-hipError_t launch_gen_bif(int model, const BifArgs& A, int wv, bool check, int* counts,
-                          double* amps, double* ivls, hipStream_t st)
+template <class M>
+static void gen_bif_m(const BifArgs& A, int wv, bool check, int* counts, double* amps,
+                      double* ivls, double* last, hipStream_t st)
 {
-    if (A.nCells <= 0) return hipSuccess;
-    if (model != kMemristive) return hipErrorInvalidValue;
     const dim3 blk(256), grd((A.nCells + 255) / 256);
     if (check)
-        hipLaunchKernelGGL((k_gen_bif_peaks<Memristive, true>), grd, blk, 0, st, A, wv, counts,
-                           amps, ivls);
+        hipLaunchKernelGGL((k_gen_bif_peaks<M, true>), grd, blk, 0, st, A, wv, counts, amps, ivls,
+                           last);
     else
-        hipLaunchKernelGGL((k_gen_bif_peaks<Memristive, false>), grd, blk, 0, st, A, wv, counts,
-                           amps, ivls);
+        hipLaunchKernelGGL((k_gen_bif_peaks<M, false>), grd, blk, 0, st, A, wv, counts, amps,
+                           ivls, last);
+}
+
+hipError_t launch_gen_bif(int model, const BifArgs& A, int wv, bool check, int* counts,
+                          double* amps, double* ivls, double* last, hipStream_t st)
+{
+    if (A.nCells <= 0) return hipSuccess;
+    if (model == kMemristive) gen_bif_m<Memristive>(A, wv, check, counts, amps, ivls, last, st);
+    else if (model == kLinear) gen_bif_m<Linear>(A, wv, check, counts, amps, ivls, last, st);
+    else gen_bif_m<Rossler>(A, wv, check, counts, amps, ivls, last, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_osc_flags(int n, const int* counts, const double* amps, long long rowStride,
+                            const double* last, double thr, unsigned char* flags, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_osc_flags, dim3((n + 255) / 256), dim3(256), 0, st, n, counts, amps,
+                       rowStride, last, thr, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_osc_combine(int n, const int* clusters, const unsigned char* flags,
+                              double* out, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_osc_combine, dim3((n + 255) / 256), dim3(256), 0, st, n, clusters,
+                       flags, out);
     return hipGetLastError();
 }
 

@@ -195,6 +195,33 @@ int bifhip_LLE2DIC(double tMax, double NT, int nPts, double h, double eps,
                    double maxValue, double transientTime, const double* values,
                    int amountOfValues);
 
+/* Library/hostLibrary.cuh:211-227 (calculationOfPeriodicityByOstrovsky): the
+ * periodicity classifier over a sweep of initial conditions (the reference
+ * uses calculateDiscreteModelICCUDA).  Per cell: the DBSCAN count of the
+ * peaks, negated unless every kept peak after the first exceeds
+ * ostrovskyThreshold; +-1 by the last sample's side of the threshold when
+ * at most one peak is kept; 0 when the peaks lie on both sides; halved to
+ * +-0.5 when no peak rises above its predecessor
+ * (Library/cudaLibrary.cu:286-352, 628-662).  A diverged trajectory sets
+ * only the few-peaks flag (the reference leaves the others unwritten).
+ * Writes the 2D map layout with the range header. */
+int bifhip_calculationOfPeriodicityByOstrovsky(double tMax, int nPts, double h,
+                                               int amountOfInitialConditions,
+                                               const double* initialConditions,
+                                               const double* ranges,
+                                               const int* indicesOfMutVars, int writableVar,
+                                               double maxValue, int maxAmountOfPeaks,
+                                               double transientTime, const double* values,
+                                               int amountOfValues, int preScaller, double eps,
+                                               double ostrovskyThreshold);
+/* The same map into out[nPts * nPts] (host memory). */
+int bifhip_ostrovsky_to_buffer(double tMax, int nPts, double h, int amountOfInitialConditions,
+                               const double* initialConditions, const double* ranges,
+                               const int* indicesOfMutVars, int writableVar, double maxValue,
+                               int maxAmountOfPeaks, double transientTime, const double* values,
+                               int amountOfValues, int preScaller, double eps,
+                               double ostrovskyThreshold, double* out);
+
 /* Sweep target of the buffer and device entry points below
  * (BIFHIP_SWEEP_PARAMETERS by default); the named file drivers fix their
  * own target. */

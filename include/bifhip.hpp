@@ -126,6 +126,20 @@ inline void bifurcation2DDBSCANIC(const double tMax, const int nPts, const doubl
                                               eps));
 }
 
+// Library/hostLibrary.cuh:211-227
+inline void calculationOfPeriodicityByOstrovsky(
+    const double tMax, const int nPts, const double h, const int amountOfInitialConditions,
+    const double* initialConditions, const double* ranges, const int* indicesOfMutVars,
+    const int writableVar, const double maxValue, const int maxAmountOfPeaks,
+    const double transientTime, const double* values, const int amountOfValues,
+    const int preScaller, const double eps, const double ostrovskyThreshold)
+{
+    BIFHIP_CHECK(bifhip_calculationOfPeriodicityByOstrovsky(
+        tMax, nPts, h, amountOfInitialConditions, initialConditions, ranges, indicesOfMutVars,
+        writableVar, maxValue, maxAmountOfPeaks, transientTime, values, amountOfValues,
+        preScaller, eps, ostrovskyThreshold));
+}
+
 #define BIFHIP_LYAP_WRAPPER(NAME)                                                              \
     inline void NAME(const double tMax, const double NT, const int nPts, const double h,       \
                      const double eps, const double* initialConditions,                        \

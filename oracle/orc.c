@@ -627,6 +627,82 @@ void orc_dbscan_cells(int model, double tMax, int nPts, double h, int nIC,
     }
 }
 
+/* calculationOfPeriodicityByOstrovsky, Library/hostLibrary.cu:748-925:
+ * initial-condition sweep (calculateDiscreteModelICCUDA), peaks with five
+ * flags (peakFinderCUDAForCalculationOfPeriodicityByOstrovsky,
+ * Library/cudaLibrary.cu:286-352), DBSCAN (dbscanDouble, the dbscan
+ * algorithm, :533-605) and the flag rules of
+ * dbscanCUDAForCalculationOfPeriodicityByOstrovsky (:628-662).  A diverged
+ * cell sets only flag 3 in the reference (the others are unwritten device
+ * memory); here they are false. */
+void orc_ostrovsky_cells(int model, double tMax, int nPts, double h, int nIC,
+                         const double *ic, const double *ranges,
+                         const int *mutIdx, int writableVar, double maxValue,
+                         int maxAmountOfPeaks, double transientTime,
+                         const double *values, int nVal, int preScaller,
+                         double eps, double threshold, long c0, long nc,
+                         double *out)
+{
+    int N = (int)(tMax / h / preScaller);
+    int skip = (int)(transientTime / h);
+    int saved = g_sweep_ic;
+    g_sweep_ic = 1;
+#pragma omp parallel
+    {
+        double *s = (double *)malloc(sizeof(double) * (size_t)N);
+        double *iv = (double *)malloc(sizeof(double) * (size_t)N);
+        double *hp = (double *)malloc(sizeof(double) * (size_t)N);
+#pragma omp for schedule(dynamic, 64)
+        for (long c = 0; c < nc; ++c) {
+            int ok = orc_bif_cell(model, nPts, c0 + c, 2, ranges, mutIdx, ic,
+                                  nIC, values, nVal, h, skip, N, preScaller,
+                                  writableVar, maxValue, s);
+            int f[5] = {0, 0, 0, 0, 0};
+            double o;
+            if (!ok) {
+                f[3] = 1;
+                o = 0;
+            } else {
+                double last = s[N - 1];
+                int P = orc_peak_finder(s, 0, N, s, iv, 0.01);
+                f[0] = 1;
+                for (int i = 1; i < P; ++i)
+                    if (s[i] - s[i - 1] > 0) {
+                        f[0] = 0;
+                        break;
+                    }
+                int one = 0, zero = 0;
+                for (int i = 1; i < P; ++i) {
+                    if (s[i] > threshold)
+                        one = 1;
+                    else
+                        zero = 1;
+                    if (one && zero)
+                        break;
+                }
+                f[1] = one && zero;
+                f[2] = !(one && !zero);
+                f[3] = P == 0 || P == 1;
+                f[4] = last > threshold;
+                o = orc_dbscan(s, iv, hp, 0, P, N, maxAmountOfPeaks, eps);
+            }
+            if (f[2])
+                o = -1 * o;
+            if (f[3])
+                o = f[4] ? 1 : -1;
+            if (f[1])
+                o = 0;
+            if (f[0])
+                o = o > 0 ? 0.5 : (o == 0 ? 0 : -0.5);
+            out[c] = o;
+        }
+        free(s);
+        free(iv);
+        free(hp);
+    }
+    g_sweep_ic = saved;
+}
+
 void orc_kde_cells(int model, double tMax, int nPts, double h, int nIC,
                    const double *ic, const double *ranges, const int *mutIdx,
                    int writableVar, double maxValue, int maxAmountOfPeaks,

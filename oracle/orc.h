@@ -129,6 +129,14 @@ void orc_dbscan_cells(int model, double tMax, int nPts, double h, int nIC,
                       double eps, long c0, long nc, int *out);
 
 /* bifurcation2DKDE (hostLibrary.cu:397-567) over cells [c0, c0+nc). */
+/* calculationOfPeriodicityByOstrovsky cells (initial-condition sweep) */
+void orc_ostrovsky_cells(int model, double tMax, int nPts, double h, int nIC,
+                         const double *ic, const double *ranges,
+                         const int *mutIdx, int writableVar, double maxValue,
+                         int maxAmountOfPeaks, double transientTime,
+                         const double *values, int nVal, int preScaller,
+                         double eps, double threshold, long c0, long nc,
+                         double *out);
 void orc_kde_cells(int model, double tMax, int nPts, double h, int nIC,
                    const double *ic, const double *ranges, const int *mutIdx,
                    int writableVar, double maxValue, int maxAmountOfPeaks,

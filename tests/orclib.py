@@ -44,6 +44,9 @@ def lib():
 def _declare(L):
     L.orc_set_sweep_ic.argtypes = [_I]
     L.orc_set_sweep_ic.restype = None
+    L.orc_ostrovsky_cells.argtypes = [_I, _D, _I, _D, _I, _PD, _PD, _PI, _I, _D, _I, _D, _PD, _I,
+                                      _I, _D, _D, _L, _L, _PD]
+    L.orc_ostrovsky_cells.restype = None
     L.orc_step.argtypes = [_I, _PD, _PD, _D]
     L.orc_loop.argtypes = [_I, _PD, _PD, _D, _I, _I, _I, _D, _PD, _L, _I]
     L.orc_loop.restype = _I
@@ -187,6 +190,16 @@ def dbscan_cells(tMax, nPts, h, ic, ranges, mut, wv, max_value, max_peaks,
     lib().orc_dbscan_cells(model, tMax, nPts, h, len(ic), pd(ic), pd(ranges),
                            pi(mut), wv, max_value, max_peaks, transient,
                            pd(values), len(values), pre, eps, c0, nc, pi(out))
+    return out
+
+
+def ostrovsky_cells(tMax, nPts, h, ic, ranges, mut, wv, max_value, max_peaks, transient,
+                    values, pre, eps, threshold, c0, nc, model=MODEL_ROSSLER):
+    ic = f64(ic); ranges = f64(ranges); mut = i32(mut); values = f64(values)
+    out = np.zeros(nc)
+    lib().orc_ostrovsky_cells(model, tMax, nPts, h, len(ic), pd(ic), pd(ranges), pi(mut), wv,
+                              max_value, max_peaks, transient, pd(values), len(values), pre,
+                              eps, threshold, c0, nc, pd(out))
     return out
 
 

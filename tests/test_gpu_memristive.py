@@ -111,3 +111,37 @@ def test_unsupported_paths_refuse_the_model(bh):
         bh.set_precision(bh.PREC_FP64)
     with pytest.raises(bh.BifhipError):
         bh.dbscan_map(2.0, 4, 0.005, IC5[:3], RPAR, MPAR, 0, 0.0, 100, 1.0, MEM, 1, 0.02)
+
+
+def test_ostrovsky_classifier_matches_oracle(orc, bh, tmp_path):
+    """calculationOfPeriodicityByOstrovsky on the reference's own setting
+    (shortened): every cell's value, a count, its negation, +-1, 0 or
+    +-0.5, equals the oracle's exactly; the file has the 2D layout."""
+    from test_abi import _ref_map_text
+    n = 12
+    args = (30.0, n, 0.005, 5, IC5, RIC, MUT, 4, 1e7, 10000000, 20.0, MEM, 8, 1, 0.02, 2.0)
+    say("Ostrovsky map", n)
+    got = bh.ostrovsky_map(30.0, n, 0.005, IC5, RIC, MUT, 4, 1e7, 10000000, 20.0, MEM, 1, 0.02,
+                           2.0)
+    want = orc.ostrovsky_cells(30.0, n, 0.005, IC5, RIC, MUT, 4, 1e7, 10000000, 20.0, MEM, 1,
+                               0.02, 2.0, 0, n * n, model=orc.MODEL_MEMRISTIVE)
+    np.testing.assert_array_equal(got, want)
+    say("values", sorted(set(want.tolist()))[:12])
+    out = tmp_path / "mat.csv"
+    bh.set_output_path(out)
+    bh.calculationOfPeriodicityByOstrovsky(*args)
+    assert out.read_text() == _ref_map_text(n, RIC, want, 6, fmt=lambda v: f"{v:g}")
+
+
+def test_ostrovsky_rossler(orc, bh):
+    """The classifier on the Rossler system (initial conditions x0, z0
+    swept), with divergence checks on."""
+    bh.set_model(bh.MODEL_ROSSLER)
+    n = 10
+    ic = [0.1, 0.1, 0.1]
+    rng = [-2.0, 2.0, 0.0, 1.0]
+    got = bh.ostrovsky_map(100.0, n, 0.01, ic, rng, [0, 2], 0, 1e3, 100000, 100.0,
+                           [0.2, 0.2, 5.7], 1, 0.01, 1.0)
+    want = orc.ostrovsky_cells(100.0, n, 0.01, ic, rng, [0, 2], 0, 1e3, 100000, 100.0,
+                               [0.2, 0.2, 5.7], 1, 0.01, 1.0, 0, n * n)
+    np.testing.assert_array_equal(got, want)

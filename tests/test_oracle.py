@@ -386,3 +386,43 @@ def test_memristive_step_rounds_the_state_to_float(orc):
     # fixed point of the ODE: the origin
     z = orc.step(np.zeros(5), MEM, 0.005, model=orc.MODEL_MEMRISTIVE)
     assert (z == 0).all()
+
+
+# -------------------------------------------------- Ostrovsky classifier
+
+@pytest.mark.parametrize("case", GOLD["rossler_dbscan_cells"]["maxAmountOfPeaks_1e5"][:4],
+                         ids=lambda c: f"c={c['c']}")
+def test_ostrovsky_signs_over_reference_counts(orc, case):
+    """The classifier's output is the DBSCAN count (pinned by the
+    reference's values for these cells) with the reference's flag rules:
+    every kept peak above the threshold -> +count, every one below ->
+    -count, peaks on both sides -> 0."""
+    c = case["c"]
+    want = case["result"]
+    args = (500.0, 2, 0.01, IC, [0.1, 0.1, 0.1, 0.1], [0, 2], 0, 0.0, 100000, 1000.0,
+            [0.2, 0.2, c], 1, 0.01)
+    assert orc.ostrovsky_cells(*args, -100.0, 0, 1)[0] == want
+    assert orc.ostrovsky_cells(*args, 100.0, 0, 1)[0] == -want
+    ok, s = orc.bif_cell(0, 2, [0.2, 0.2], [0], IC, [0.2, 0.2, c], 0.01, 100000, 50000)
+    P, pk, _ = orc.peak_finder(s, 0.01)
+    mid = 0.5 * (pk[1:P].min() + pk[1:P].max())
+    if pk[1:P].max() - pk[1:P].min() > 1e-3:
+        assert orc.ostrovsky_cells(*args, mid, 0, 1)[0] == 0
+
+
+def test_ostrovsky_peakless_cells_follow_the_last_sample(orc):
+    """A monotonically decaying linear trajectory has no peaks: flag 3 makes
+    the output +1 when the last recorded sample is above the threshold, else
+    -1, and flag 0 (no increasing pair of peaks, vacuously true) halves it.
+    The initial value is swept (x0 in [0.5, 2]) and the last sample has the
+    closed form x0 * (1 + h*a)^(skip + N - 1)."""
+    h, a, skip, N = 0.01, -0.1, 100, 400
+    lam = [a, a, a]
+    n = 6
+    x0 = np.array([orc.value_by_idx(i, n, 0.5, 2.0, 0) for i in range(n)])
+    last = x0 * (1 + h * a) ** (skip + N - 1)
+    thr = float(np.median(last))
+    out = orc.ostrovsky_cells(N * h, n, h, [1.0, 0.0, 0.0], [0.5, 2.0, 0.0, 0.0], [0, 1], 0,
+                              0.0, 100000, skip * h, lam, 1, 0.01, thr, 0, n,
+                              model=orc.MODEL_LINEAR)
+    np.testing.assert_array_equal(out, np.where(last > thr, 0.5, -0.5))

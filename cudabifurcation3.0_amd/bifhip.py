@@ -76,6 +76,8 @@ def _declare(L):
         "bifhip_set_sweep": [_I],
         "bifhip_calculationOfPeriodicityByOstrovsky": drv_db + [_D],
         "bifhip_ostrovsky_to_buffer": drv_db + [_D, _PD],
+        "bifhip_modelingOneSystemDenis": [_D, _D, _D, _I, _PD, _I, _PD, _I],
+        "bifhip_denis_to_buffer": [_D, _D, _D, _I, _PD, _I, _PD, _I, _PD, _LL],
         "bifhip_LLE1D": drv_ly, "bifhip_LLE2D": drv_ly,
         "bifhip_LS1D": drv_ly, "bifhip_LS2D": drv_ly,
         "bifhip_bifurcation1D_to_buffer": drv_bif1 + [_PI, _PD, _I],
@@ -358,6 +360,21 @@ def ostrovsky_map(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars, wr
                                              transientTime, _pd(v), len(v), preScaller, eps,
                                              threshold, _pd(out)))
     return out
+
+
+def modelingOneSystemDenis(tMax, h, hSpecial, amountOfInitialConditions, initialConditions,
+                           writableVar, values, amountOfValues):
+    ic, v = _f64(initialConditions), _f64(values)
+    _check(load().bifhip_modelingOneSystemDenis(tMax, h, hSpecial, amountOfInitialConditions,
+                                                _pd(ic), writableVar, _pd(v), amountOfValues))
+
+
+def denis_series(tMax, h, hSpecial, initialConditions, writableVar, values):
+    ic, v = _f64(initialConditions), _f64(values)
+    out = np.zeros(max(int(tMax / h), 1))
+    _check(load().bifhip_denis_to_buffer(tMax, h, hSpecial, len(ic), _pd(ic), writableVar,
+                                         _pd(v), len(v), _pd(out), len(out)))
+    return out[:int(tMax / h)]
 
 
 SWEEP_PARAMETERS, SWEEP_INITIAL_CONDITIONS = 0, 1

@@ -49,6 +49,8 @@ hipError_t launch_osc_flags(int n, const int* counts, const double* amps, long l
                             const double* last, double thr, unsigned char* flags, hipStream_t st);
 hipError_t launch_osc_combine(int n, const int* clusters, const unsigned char* flags,
                               double* out, hipStream_t st);
+hipError_t launch_denis(int model, const Grid& g, int T, double h, double hS, int iters, int wv,
+                        long long nPoints, double* data, hipStream_t st);
 hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, long long rowStride,
                            int cap, int maxPeaks, int kdeSampling, double I1, double I2, double H,
                            int* out, const int* outIndex, int* overflow, int* nOverflow,
@@ -1455,6 +1457,57 @@ int bifhip_ostrovsky_to_buffer(double tMax, int nPts, double h, int nIC, const d
                           transientTime, values, nVal, pre, eps, ostrovskyThreshold,
                           d.as<double>()));
     HIP_TRY(hipMemcpy(out, d.p, sizeof(double) * (size_t)cells, hipMemcpyDeviceToHost));
+    return BIFHIP_OK;
+}
+
+// modelingOneSystemDenis (Library/hostLibrary.cu:162-239): derived counts
+// (int)(tMax/h) output points, (int)(hSpecial/h) trajectories and
+// (int)(tMax/hSpecial) coarse steps each; unwritten points are 0.
+int bifhip_denis_to_buffer(double tMax, double h, double hSpecial, int nIC, const double* ic,
+                           int wv, const double* values, int nVal, double* out, long long nOut)
+{
+    RC_TRY(ensure_device());
+    if (!ic || !values || !out || !(h > 0) || !(hSpecial > 0))
+        return fail(BIFHIP_E_ARG, "bad arguments");
+    const long long nPoints = (long long)(int)(tMax / h);
+    const int T = (int)(hSpecial / h);
+    const int iters = (int)(tMax / hSpecial);
+    if (nPoints < 0 || nOut < nPoints)
+        return fail(BIFHIP_E_ARG, "output holds %lld points, %lld needed", nOut, nPoints);
+    if (wv < 0 || wv >= nIC) return fail(BIFHIP_E_ARG, "writableVar %d out of range", wv);
+    const double dummyRange[2] = {0, 0};
+    const int dummyMut[1] = {0};
+    Grid G;
+    RC_TRY(make_grid(G, 1, 1, dummyRange, dummyMut, ic, nIC, values, nVal));
+    G.dimension = 0;   // nothing swept
+    DBuf d;
+    HIP_TRY(d.alloc(sizeof(double) * (size_t)std::max(1LL, nPoints)));
+    HIP_TRY(hipMemsetAsync(d.p, 0, sizeof(double) * (size_t)std::max(1LL, nPoints), g.stream));
+    {
+        Timed t("denis");
+        HIP_TRY(launch_denis(g.model, G, T, h, hSpecial, iters, wv, nPoints, d.as<double>(),
+                             g.stream));
+    }
+    HIP_TRY(hipMemcpyAsync(out, d.p, sizeof(double) * (size_t)nPoints, hipMemcpyDeviceToHost,
+                           g.stream));
+    return finish();
+}
+
+// Library/hostLibrary.cuh:59-67: "h*j, value" lines, 12 significant digits.
+int bifhip_modelingOneSystemDenis(double tMax, double h, double hSpecial, int nIC,
+                                  const double* ic, int wv, const double* values, int nVal)
+{
+    SweepGuard sweep(false);
+    if (!(h > 0)) return fail(BIFHIP_E_ARG, "h must be positive");
+    const long long nPoints = std::max(0LL, (long long)(int)(tMax / h));
+    std::vector<double> out((size_t)std::max(1LL, nPoints));
+    RC_TRY(bifhip_denis_to_buffer(tMax, h, hSpecial, nIC, ic, wv, values, nVal, out.data(),
+                                  nPoints));
+    FILE* f = std::fopen(g.outPath.c_str(), "w");
+    if (!f) return file_error(g.outPath);
+    for (long long j = 0; j < nPoints; ++j)
+        std::fprintf(f, "%.12g, %.12g\n", h * (double)j, out[(size_t)j]);
+    if (std::fclose(f) != 0) return file_error(g.outPath);
     return BIFHIP_OK;
 }
 

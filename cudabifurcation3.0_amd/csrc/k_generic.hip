@@ -13,6 +13,8 @@
 //  k_gen_ls         finite-difference spectrum with classical Gram-Schmidt
 //                   (LSKernelCUDA / gramSchmidtProcess,
 //                   Library/cudaLibrary.cu:913-1100);
+//  k_denis          one system sampled from offset coarse-step trajectories
+//                   (calculateDiscreteModelDenisCUDA, :121-158);
 //  k_osc_flags,     Ostrovsky's periodicity classifier on top of the peak
 //  k_osc_combine    rows and DBSCAN counts (Library/cudaLibrary.cu:286-352,
 //                   628-662).
@@ -282,6 +284,29 @@ __global__ __launch_bounds__(256) void k_gen_ls(const LyapArgs A, double* __rest
     for (int i = 0; i < D; ++i) out[(long long)t * D + i] = res[i] / A.tMax;
 }
 
+// One system sampled finely from many coarse trajectories
+// (calculateDiscreteModelDenisCUDA, Library/cudaLibrary.cu:121-158): lane
+// idx < T takes idx steps of h, then `iters` steps of hS, recording x[wv]
+// before each into data[idx + i*T] (only indices below nPoints exist).
+template <class M>
+__global__ __launch_bounds__(256) void k_denis(const Grid g, int T, double h, double hS,
+                                               int iters, int wv, long long nPoints,
+                                               double* __restrict__ data)
+{
+    constexpr int D = M::D;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= T) return;
+    double v[kMaxVal], x[D];
+    cell_setup_n<D>(g, 0, v, x);
+    const typename M::P64 p = M::params(v);
+    for (int s = 0; s < idx; ++s) M::stepn(x, p, h);
+    for (int i = 0; i < iters; ++i) {
+        const long long j = idx + (long long)i * T;
+        if (j < nPoints) data[j] = component<D>(x, wv);
+        M::stepn(x, p, hS);
+    }
+}
+
 // ------------------------------------------------------------ launchers
 
 template <class M>
@@ -360,4 +385,23 @@ hipError_t launch_gen_ls(int model, const LyapArgs& A, bool check, double* out, 
     return hipGetLastError();
 }
 
+}  // namespace bifhip
+
+namespace bifhip {
+hipError_t launch_denis(int model, const Grid& g, int T, double h, double hS, int iters, int wv,
+                        long long nPoints, double* data, hipStream_t st)
+{
+    if (T <= 0) return hipSuccess;
+    const dim3 blk(256), grd((T + 255) / 256);
+    if (model == kMemristive)
+        hipLaunchKernelGGL((k_denis<Memristive>), grd, blk, 0, st, g, T, h, hS, iters, wv,
+                           nPoints, data);
+    else if (model == kLinear)
+        hipLaunchKernelGGL((k_denis<Linear>), grd, blk, 0, st, g, T, h, hS, iters, wv, nPoints,
+                           data);
+    else
+        hipLaunchKernelGGL((k_denis<Rossler>), grd, blk, 0, st, g, T, h, hS, iters, wv, nPoints,
+                           data);
+    return hipGetLastError();
+}
 }  // namespace bifhip

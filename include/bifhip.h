@@ -222,6 +222,22 @@ int bifhip_ostrovsky_to_buffer(double tMax, int nPts, double h, int amountOfInit
                                int amountOfValues, int preScaller, double eps,
                                double ostrovskyThreshold, double* out);
 
+/* Library/hostLibrary.cuh:59-67 (modelingOneSystemDenis): one system from
+ * initialConditions sampled every h over tMax by (int)(hSpecial/h)
+ * trajectories of step hSpecial, trajectory k started k steps of h in
+ * (calculateDiscreteModelDenisCUDA, Library/cudaLibrary.cu:121-158).
+ * Writes "t, x[writableVar]" lines, 12 significant digits. */
+int bifhip_modelingOneSystemDenis(double tMax, double h, double hSpecial,
+                                  int amountOfInitialConditions,
+                                  const double* initialConditions, int writableVar,
+                                  const double* values, int amountOfValues);
+/* The same series into out[nOut], nOut >= (int)(tMax/h); samples no
+ * trajectory reaches are 0. */
+int bifhip_denis_to_buffer(double tMax, double h, double hSpecial, int amountOfInitialConditions,
+                           const double* initialConditions, int writableVar,
+                           const double* values, int amountOfValues, double* out,
+                           long long nOut);
+
 /* Sweep target of the buffer and device entry points below
  * (BIFHIP_SWEEP_PARAMETERS by default); the named file drivers fix their
  * own target. */

@@ -126,6 +126,17 @@ inline void bifurcation2DDBSCANIC(const double tMax, const int nPts, const doubl
                                               eps));
 }
 
+// Library/hostLibrary.cuh:59-67
+inline void modelingOneSystemDenis(const double tMax, const double h, const double hSpecial,
+                                   const int amountOfInitialConditions,
+                                   const double* initialConditions, const int writableVar,
+                                   const double* values, const int amountOfValues)
+{
+    BIFHIP_CHECK(bifhip_modelingOneSystemDenis(tMax, h, hSpecial, amountOfInitialConditions,
+                                               initialConditions, writableVar, values,
+                                               amountOfValues));
+}
+
 // Library/hostLibrary.cuh:211-227
 inline void calculationOfPeriodicityByOstrovsky(
     const double tMax, const int nPts, const double h, const int amountOfInitialConditions,

@@ -703,6 +703,38 @@ void orc_ostrovsky_cells(int model, double tMax, int nPts, double h, int nIC,
     g_sweep_ic = saved;
 }
 
+/* modelingOneSystemDenis + calculateDiscreteModelDenisCUDA,
+ * Library/hostLibrary.cu:162-239, Library/cudaLibrary.cu:121-158:
+ * trajectory idx < (int)(hS/h) takes idx steps of h, then (int)(tMax/hS)
+ * steps of hS recording x[wv] before each into out[idx + i*T]; out has
+ * (int)(tMax/h) points, the unwritten ones stay 0. */
+void orc_denis(int model, double tMax, double h, double hS, int nIC,
+               const double *ic, int writableVar, const double *values,
+               int nVal, double *out, long nOut)
+{
+    int T = (int)(hS / h);
+    int iters = (int)(tMax / hS);
+    for (long j = 0; j < nOut; ++j)
+        out[j] = 0;
+    if (model == ORC_MODEL_LINEAR)
+        g_linear_dim = nIC;
+    for (int idx = 0; idx < T; ++idx) {
+        double x[ORC_MAXD], v[ORC_MAXV];
+        for (int i = 0; i < nIC; ++i)
+            x[i] = ic[i];
+        for (int i = 0; i < nVal; ++i)
+            v[i] = values[i];
+        for (int s = 0; s < idx; ++s)
+            orc_step(model, x, v, h);
+        for (int i = 0; i < iters; ++i) {
+            long j = idx + (long)i * T;
+            if (j < nOut)
+                out[j] = x[writableVar];
+            orc_step(model, x, v, hS);
+        }
+    }
+}
+
 void orc_kde_cells(int model, double tMax, int nPts, double h, int nIC,
                    const double *ic, const double *ranges, const int *mutIdx,
                    int writableVar, double maxValue, int maxAmountOfPeaks,

@@ -137,6 +137,10 @@ void orc_ostrovsky_cells(int model, double tMax, int nPts, double h, int nIC,
                          const double *values, int nVal, int preScaller,
                          double eps, double threshold, long c0, long nc,
                          double *out);
+/* modelingOneSystemDenis series, out[nOut] (nOut = (int)(tMax/h)) */
+void orc_denis(int model, double tMax, double h, double hS, int nIC,
+               const double *ic, int writableVar, const double *values,
+               int nVal, double *out, long nOut);
 void orc_kde_cells(int model, double tMax, int nPts, double h, int nIC,
                    const double *ic, const double *ranges, const int *mutIdx,
                    int writableVar, double maxValue, int maxAmountOfPeaks,

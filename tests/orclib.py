@@ -47,6 +47,8 @@ def _declare(L):
     L.orc_ostrovsky_cells.argtypes = [_I, _D, _I, _D, _I, _PD, _PD, _PI, _I, _D, _I, _D, _PD, _I,
                                       _I, _D, _D, _L, _L, _PD]
     L.orc_ostrovsky_cells.restype = None
+    L.orc_denis.argtypes = [_I, _D, _D, _D, _I, _PD, _I, _PD, _I, _PD, _L]
+    L.orc_denis.restype = None
     L.orc_step.argtypes = [_I, _PD, _PD, _D]
     L.orc_loop.argtypes = [_I, _PD, _PD, _D, _I, _I, _I, _D, _PD, _L, _I]
     L.orc_loop.restype = _I
@@ -200,6 +202,15 @@ def ostrovsky_cells(tMax, nPts, h, ic, ranges, mut, wv, max_value, max_peaks, tr
     lib().orc_ostrovsky_cells(model, tMax, nPts, h, len(ic), pd(ic), pd(ranges), pi(mut), wv,
                               max_value, max_peaks, transient, pd(values), len(values), pre,
                               eps, threshold, c0, nc, pd(out))
+    return out
+
+
+def denis(tMax, h, hS, ic, wv, values, model=MODEL_ROSSLER):
+    ic = f64(ic); values = f64(values)
+    n = int(tMax / h)
+    out = np.zeros(max(n, 0))
+    lib().orc_denis(model, tMax, h, hS, len(ic), pd(ic), wv, pd(values), len(values), pd(out),
+                    len(out))
     return out
 
 

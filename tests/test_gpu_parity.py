@@ -469,3 +469,18 @@ def test_driver_files_match_reference_format(orc, bh, tmp_path):
     bh.bifurcation2DKDE(80.0, n, 0.01, 3, IC, R2, [0, 2], 0, 0.0, 1000, 50.0, VALS, 3, 1, 10,
                         -15.0, 15.0, 0.05)
     assert out.read_text().split("\n")[:2] == ["0.05 0.45", "2 10"]
+
+
+def test_denis_series_and_file(orc, bh, tmp_path):
+    """modelingOneSystemDenis: the interleaved single-system series equals
+    the oracle's bit for bit, and the file is "t, x" at 12 digits."""
+    say("denis series")
+    got = bh.denis_series(30.0, 0.01, 0.2, IC, 1, VALS)
+    want = orc.denis(30.0, 0.01, 0.2, IC, 1, VALS)
+    np.testing.assert_array_equal(got, want)
+    out = tmp_path / "mat.csv"
+    bh.set_output_path(out)
+    bh.modelingOneSystemDenis(30.0, 0.01, 0.2, 3, IC, 1, VALS, 3)
+    lines = out.read_text().splitlines()
+    assert len(lines) == len(want)
+    assert lines[7] == f"{0.01 * 7:.12g}, {want[7]:.12g}"

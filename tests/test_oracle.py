@@ -426,3 +426,22 @@ def test_ostrovsky_peakless_cells_follow_the_last_sample(orc):
                               0.0, 100000, skip * h, lam, 1, 0.01, thr, 0, n,
                               model=orc.MODEL_LINEAR)
     np.testing.assert_array_equal(out, np.where(last > thr, 0.5, -0.5))
+
+
+# ---------------------------------------------------- single-system series
+
+def test_denis_series_linear_closed_form(orc):
+    """Sample j = idx + i*T comes from trajectory idx after idx steps of h
+    and i steps of hS: x0 (1+h a)^idx (1+hS a)^i on the linear model; the
+    samples past T*iters stay 0."""
+    h, hS, tMax, a = 0.01, 0.05, 1.03, -0.3
+    T, iters, n = int(hS / h), int(tMax / hS), int(tMax / h)
+    out = orc.denis(tMax, h, hS, [2.0, 0.0, 0.0], 0, [a, a, a], model=orc.MODEL_LINEAR)
+    assert len(out) == n
+    for j in range(n):
+        idx, i = j % T, j // T
+        if i < iters:
+            want = 2.0 * (1 + h * a) ** idx * (1 + hS * a) ** i
+            assert abs(out[j] - want) <= 1e-13 * abs(want), j
+        else:
+            assert out[j] == 0

@@ -208,7 +208,8 @@ def main():
                     help="chaos-logit confidence threshold")
     ap.add_argument("--validation", type=float, default=0.002)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--cpu-rows", type=int, default=4)
+    ap.add_argument("--cpu-cells", type=int, default=65536,
+                    help="cells of the CPU baseline sample (whole lattice rows, ~10 s on 16 cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--full-resolve", action="store_true",
                     help="re-solve every flagged cell in full (no peak-count-only re-solves)")
@@ -322,7 +323,7 @@ def main():
         }
         out["roofline"] = rf
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_rows if args.workload == "plain" else 1)
+        out["cpu_baseline"] = cpu_baseline(n, max(1, args.cpu_cells // n))
     print(json.dumps(out), flush=True)
 
 

@@ -313,7 +313,8 @@ def main():
             "peak_only_fraction": last["peak_only"] / cells,
             "surrogate_on_peak_only": {"class_acc": last["peak_only_class_acc"]},
             "partial_resolve": not args.full_resolve,
-            "phase_ms": {k: last[k] for k in ("ms_coarse", "ms_train", "ms_infer", "ms_resolve")},
+            "phase_ms": {k: last[k] for k in ("ms_coarse", "ms_train", "ms_infer", "ms_resolve",
+                                              "ms_flag", "ms_solve", "ms_score")},
             "surrogate_on_flagged": {"class_acc": last["flagged_class_acc"],
                                      "regime_acc": last["flagged_regime_acc"]},
             "surrogate_on_validation_sample": {"cells": last["validation"],

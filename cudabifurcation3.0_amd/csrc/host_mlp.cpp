@@ -43,6 +43,7 @@ hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, 
                        long long stride, const unsigned char* keep, const int* bfIndex, bf16* wbf,
                        float lr, float b1, float b2, float eps, const int* step, hipStream_t st);
 hipError_t launch_fill_f32(int n, float* p, float v, hipStream_t st);
+hipError_t launch_fill_f64(int n, double* p, double v, hipStream_t st);
 hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, float chaosThr,
                              unsigned char* cls, unsigned char* chaos, float* conf,
                              float* confChaos, float* lle, hipStream_t st);
@@ -54,6 +55,10 @@ hipError_t launch_scatter(int n, const int* list, const int* peak, const double*
                           int* peakMap, double* lleMap, hipStream_t st);
 hipError_t launch_fill_pred(long long n, const unsigned char* cls, const float* lle,
                             int* peakMap, double* lleMap, hipStream_t st);
+hipError_t launch_score(int nFull, int nTot, const int* list, const int* rpeak, const double* rlle,
+                        int nPeak, const int* plist, const int* ppeak, const unsigned char* pcls,
+                        const unsigned char* pch, const float* plle, double chaosThr,
+                        double* acc, hipStream_t st);
 namespace host {
 struct PeakMapSpec;
 struct LyapSpec;
@@ -603,6 +608,9 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     RC_TRY(pool_get("sg_rpeak", (size_t)std::max(1, ntot), &d_rpeak));
     RC_TRY(pool_get("sg_rlle", (size_t)std::max(1, ntot), &d_rlle));
     RC_TRY(pool_get("sg_ppeak", (size_t)std::max(1, npeak), &d_ppeak));
+    HIP_TRY(hipStreamSynchronize(s));
+    st[BIFHIP_SSTAT_MS_FLAG] = ms_since(t3);
+    const auto t4 = std::chrono::steady_clock::now();
     if (ntot > 0)
         RC_TRY(surrogate_resolve(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
                                  peakTransient, wv, maxPeaks, dbEps, lleTMax, NT, lleTransient,
@@ -611,56 +619,30 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
         RC_TRY(surrogate_resolve_peaks(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
                                        peakTransient, wv, maxPeaks, dbEps, hpeak.data(),
                                        d_plist, npeak, d_ppeak));
-    // score the surrogate on every re-solved cell before overwriting
-    std::vector<int> rpeak(ntot), ppeak(npeak);
-    std::vector<double> rlle(ntot);
-    std::vector<unsigned char> pc(ntot), ph(ntot), pcp(npeak);
-    std::vector<float> pl(ntot);
-    if (ntot + npeak > 0) {
-        std::vector<unsigned char> allc((size_t)nf), allh((size_t)nf);
-        std::vector<float> alll((size_t)nf);
-        HIP_TRY(hipMemcpy(allc.data(), d_pcls, (size_t)nf, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(allh.data(), d_pch, (size_t)nf, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(alll.data(), d_plle, sizeof(float) * nf, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(rpeak.data(), d_rpeak, sizeof(int) * ntot, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(rlle.data(), d_rlle, sizeof(double) * ntot, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(ppeak.data(), d_ppeak, sizeof(int) * npeak, hipMemcpyDeviceToHost));
-        for (int i = 0; i < ntot; ++i) {
-            pc[i] = allc[hall[i]];
-            ph[i] = allh[hall[i]];
-            pl[i] = alll[hall[i]];
-        }
-        for (int i = 0; i < npeak; ++i) pcp[i] = allc[hpeak[i]];
-    }
-    auto score = [&](int lo, int hi, double* clsAcc, double* regAcc, double* mae) {
-        long long n = 0, okc = 0, okr = 0, nl = 0;
-        double err = 0;
-        for (int i = lo; i < hi; ++i) {
-            const int tc = std::min(kClasses, std::max(1, rpeak[i])) - 1;
-            const bool tch = rlle[i] > chaosThreshold;
-            const bool pch = ph[i] != 0;
-            ++n;
-            okc += tc == pc[i];
-            okr += (tc == pc[i]) && (tch == pch);
-            if (std::isfinite(rlle[i])) {
-                err += std::fabs(rlle[i] - pl[i]);
-                ++nl;
-            }
-        }
-        *clsAcc = n ? (double)okc / n : 1.0;
-        *regAcc = n ? (double)okr / n : 1.0;
-        *mae = nl ? err / nl : 0.0;
+    HIP_TRY(hipStreamSynchronize(s));
+    st[BIFHIP_SSTAT_MS_SOLVE] = ms_since(t4);
+    const auto t5 = std::chrono::steady_clock::now();
+    // score the surrogate on every re-solved cell (on the device) before
+    // the re-solved values overwrite the predictions
+    double* d_acc;
+    RC_TRY(pool_get("sg_acc", 15, &d_acc));
+    HIP_TRY(launch_fill_f64(15, d_acc, 0.0, s));
+    HIP_TRY(launch_score(nflag, ntot, d_all, d_rpeak, d_rlle, npeak, d_plist, d_ppeak, d_pcls,
+                         d_pch, d_plle, chaosThreshold, d_acc, s));
+    double acc[15];
+    HIP_TRY(hipMemcpyAsync(acc, d_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    auto score = [&](int gi, double* clsAcc, double* regAcc, double* mae) {
+        const double* a = acc + gi * 5;
+        *clsAcc = a[0] > 0 ? a[1] / a[0] : 1.0;
+        *regAcc = a[0] > 0 ? a[2] / a[0] : 1.0;
+        *mae = a[3] > 0 ? a[4] / a[3] : 0.0;
     };
-    score(0, nflag, &st[BIFHIP_SSTAT_FLAGGED_CLASS_ACC], &st[BIFHIP_SSTAT_FLAGGED_REGIME_ACC],
+    score(0, &st[BIFHIP_SSTAT_FLAGGED_CLASS_ACC], &st[BIFHIP_SSTAT_FLAGGED_REGIME_ACC],
           &st[BIFHIP_SSTAT_FLAGGED_LLE_MAE]);
-    score(nflag, ntot, &st[BIFHIP_SSTAT_VALID_CLASS_ACC], &st[BIFHIP_SSTAT_VALID_REGIME_ACC],
+    score(1, &st[BIFHIP_SSTAT_VALID_CLASS_ACC], &st[BIFHIP_SSTAT_VALID_REGIME_ACC],
           &st[BIFHIP_SSTAT_VALID_LLE_MAE]);
-    {
-        long long okc = 0;
-        for (int i = 0; i < npeak; ++i)
-            okc += (std::min(kClasses, std::max(1, ppeak[i])) - 1) == pcp[i];
-        st[BIFHIP_SSTAT_PEAK_ONLY_CLASS_ACC] = npeak ? (double)okc / npeak : 1.0;
-    }
+    st[BIFHIP_SSTAT_PEAK_ONLY_CLASS_ACC] = acc[10] > 0 ? acc[11] / acc[10] : 1.0;
     HIP_TRY(launch_scatter(ntot, d_all, d_rpeak, d_rlle, d_peak, d_lle, s));
     HIP_TRY(launch_scatter(npeak, d_plist, d_ppeak, nullptr, d_peak, d_lle, s));
     if (d_resolved) {
@@ -672,6 +654,7 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     }
     HIP_TRY(hipStreamSynchronize(s));
     st[BIFHIP_SSTAT_MS_RESOLVE] = ms_since(t3);
+    st[BIFHIP_SSTAT_MS_SCORE] = ms_since(t5);
     st[BIFHIP_SSTAT_RESOLVED] = nflag;
     st[BIFHIP_SSTAT_VALIDATION] = nval;
     st[BIFHIP_SSTAT_RESOLVED_FRAC] = (double)nflag / (double)nf;

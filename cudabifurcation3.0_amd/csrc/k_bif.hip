@@ -248,6 +248,110 @@ __device__ __forceinline__ bool near(double xa, double ya, double xb, double yb,
 constexpr int kDbR = 4;                  // register sets: P <= 64*kDbR
 constexpr int kDbMaxP = kWave * kDbR;
 
+// DBSCAN of one cell by one wavefront, specialised on the number NS of
+// 64-peak words (P <= 64*NS) so the scalar mask logic has no idle words.
+template <int NS>
+__device__ __forceinline__ int dbscan_wave_cell(const double* ra, const double* rb, int P,
+                                                double T, uint64_t* rows, int lane)
+{
+    double a[NS], b[NS];
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        const int k = r * kWave + lane;
+        a[r] = k < P ? ra[k] : 0.0;
+        b[r] = k < P ? rb[k] : 0.0;
+    }
+    // neighbour rows: one ballot per (row, 64-wide k-set), kept in LDS
+#pragma unroll
+    for (int ri = 0; ri < NS; ++ri) {
+        const int iend = min(kWave, P - ri * kWave);
+        for (int li = 0; li < iend; ++li) {
+            const int i = ri * kWave + li;
+            const double ai = readlane_d(a[ri], li);
+            const double bi = readlane_d(b[ri], li);
+#pragma unroll
+            for (int w = 0; w < NS; ++w) {
+                const int k = w * kWave + lane;
+                const bool c = (k < P - 1) && (k != i) && near(ai, bi, a[w], b[w], T);
+                const uint64_t m = __ballot(c);
+                if (lane == 0) rows[i * kDbR + w] = m;
+            }
+        }
+    }
+
+    // label / stack machine, wave-uniform (scalar registers)
+    uint64_t U[NS], S[NS];
+#pragma unroll
+    for (int w = 0; w < NS; ++w) {
+        const int n = P - w * kWave;
+        U[w] = n >= kWave ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
+        S[w] = 0;
+    }
+    // The reference visits i = 0, 1, ... and after each stack pop jumps
+    // back to i = slot (or 0), so an empty-stack scan re-walks labelled
+    // indices once per cluster.  Those iterations change nothing, so the
+    // walk jumps straight to the next unlabelled index >= i.  rem counts
+    // the unlabelled indices below P-1 (the only ones a scan can absorb);
+    // at rem == 0 pops and scans change no label, P-1 is never absorbed
+    // (scans test k < P-1) and the walk still reaches it (a popped slot is
+    // at most P-2), so it opens one more cluster exactly when unlabelled.
+    int nn = 0, cluster = 0, i = 0, rem = P - 1;
+    bool lastFree = true;
+    while (i < P) {
+        if (rem == 0) {
+            if (lastFree) ++cluster;
+            break;
+        }
+        if (nn >= 1) {
+            const int slot = nn - 1;
+            const int ws = slot >> 6;
+            const uint64_t bit = 1ull << (slot & 63);
+            bool set = false;
+#pragma unroll
+            for (int w = 0; w < NS; ++w)
+                if (w == ws) { set = (S[w] & bit) != 0; S[w] &= ~bit; }
+            i = set ? slot : 0;
+            nn = nn - 1;
+        } else {
+            int nxt = P;
+#pragma unroll
+            for (int w = 0; w < NS; ++w) {
+                const int lo = w * kWave;
+                if (nxt == P && i < lo + kWave) {
+                    const uint64_t m = i <= lo ? U[w] : (U[w] & (~0ull << (i - lo)));
+                    if (m) nxt = lo + __builtin_ctzll(m);
+                }
+            }
+            if (nxt >= P) break;
+            i = nxt;
+            ++cluster;
+            const int wi = i >> 6;
+            const uint64_t bit = 1ull << (i & 63);
+#pragma unroll
+            for (int w = 0; w < NS; ++w)
+                if (w == wi) U[w] &= ~bit;
+            if (i < P - 1) --rem;
+            else lastFree = false;
+        }
+        {
+            int c = 0;
+#pragma unroll
+            for (int w = 0; w < NS; ++w) {
+                // every lane reads the same word; keep it scalar
+                const uint64_t row = readlane_u64(rows[i * kDbR + w], 0);
+                const uint64_t m = row & U[w];
+                U[w] &= ~m;
+                S[w] |= m;
+                c += __popcll(m);
+            }
+            nn += c;
+            rem -= c;
+        }
+        ++i;
+    }
+    return cluster - 1;
+}
+
 // One wavefront per cell.  Peak k lives in lane k%64 of register set k/64.
 // Neighbour row i (bits over k, one 64-bit word per k-set) is built with a
 // ballot and kept in the wave's LDS slice; the label/stack machine then
@@ -276,7 +380,7 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
     else if (P == 1) res = 1;
     else if (P > maxPeaks) res = 0;
     else if (P > cap || P > kDbMaxP) {
-        // handled by k_dbscan_lane
+        // handled by the workgroup / lane kernels
         if (lane == 0) {
             const int k = atomicAdd(nOverflow, 1);
             overflow[k] = listBase + cell;
@@ -285,97 +389,12 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
     } else {
         const double* ra = amps + (long long)cell * rowStride;
         const double* rb = ivls + (long long)cell * rowStride;
-        double a[kDbR], b[kDbR];
-#pragma unroll
-        for (int r = 0; r < kDbR; ++r) {
-            const int k = r * kWave + lane;
-            a[r] = k < P ? ra[k] : 0.0;
-            b[r] = k < P ? rb[k] : 0.0;
-        }
-        const int nsets = (P + kWave - 1) / kWave;
-        // this wave's neighbour rows: row i, word w at rows[i * kDbR + w]
         uint64_t* rows = s_rows[threadIdx.x / kWave];
-
-        // neighbour rows: one ballot per (row, 64-wide k-set)
-#pragma unroll
-        for (int ri = 0; ri < kDbR; ++ri) {
-            if (ri >= nsets) break;
-            const int iend = min(kWave, P - ri * kWave);
-            for (int li = 0; li < iend; ++li) {
-                const int i = ri * kWave + li;
-                const double ai = readlane_d(a[ri], li);
-                const double bi = readlane_d(b[ri], li);
-#pragma unroll
-                for (int w = 0; w < kDbR; ++w) {
-                    if (w >= nsets) break;
-                    const int k = w * kWave + lane;
-                    const bool c = (k < P - 1) && (k != i) && near(ai, bi, a[w], b[w], T);
-                    const uint64_t m = __ballot(c);
-                    if (lane == 0) rows[i * kDbR + w] = m;
-                }
-            }
-        }
-
-        // label / stack machine, wave-uniform
-        uint64_t U[kDbR], S[kDbR];
-#pragma unroll
-        for (int w = 0; w < kDbR; ++w) {
-            const int lo = w * kWave;
-            const int n = P - lo;
-            U[w] = n >= kWave ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
-            S[w] = 0;
-        }
-        // The reference visits i = 0, 1, ... and after each stack pop jumps
-        // back to i = slot (or 0), so an empty-stack scan re-walks labelled
-        // indices once per cluster (O(P * clusters) iterations).  Those
-        // iterations change nothing, so the walk jumps straight to the next
-        // unlabelled index >= i (first set bit of U); the visit order of
-        // every scan, and so every label, is unchanged.
-        int nn = 0, cluster = 0;
-        int i = 0;
-        while (i < P) {
-            if (nn >= 1) {
-                const int slot = nn - 1;
-                const int ws = slot >> 6;
-                const uint64_t bit = 1ull << (slot & 63);
-                bool set = false;
-#pragma unroll
-                for (int w = 0; w < kDbR; ++w)
-                    if (w == ws) { set = (S[w] & bit) != 0; S[w] &= ~bit; }
-                i = set ? slot : 0;
-                nn = nn - 1;
-            } else {
-                int nxt = P;
-#pragma unroll
-                for (int w = 0; w < kDbR; ++w) {
-                    const int lo = w * kWave;
-                    if (nxt == P && w < nsets && i < lo + kWave) {
-                        const uint64_t m = i <= lo ? U[w] : (U[w] & (~0ull << (i - lo)));
-                        if (m) nxt = lo + __builtin_ctzll(m);
-                    }
-                }
-                if (nxt >= P) break;
-                i = nxt;
-                ++cluster;
-                const int wi = i >> 6;
-                const uint64_t bit = 1ull << (i & 63);
-#pragma unroll
-                for (int w = 0; w < kDbR; ++w)
-                    if (w == wi) U[w] &= ~bit;
-            }
-#pragma unroll
-            for (int w = 0; w < kDbR; ++w) {
-                if (w >= nsets) break;
-                // every lane reads the same word; keep it scalar
-                const uint64_t row = readlane_u64(rows[i * kDbR + w], 0);
-                const uint64_t m = row & U[w];
-                U[w] &= ~m;
-                S[w] |= m;
-                nn += __popcll(m);
-            }
-            ++i;
-        }
-        res = cluster - 1;
+        const int nsets = (P + kWave - 1) / kWave;
+        if (nsets == 1) res = dbscan_wave_cell<1>(ra, rb, P, T, rows, lane);
+        else if (nsets == 2) res = dbscan_wave_cell<2>(ra, rb, P, T, rows, lane);
+        else if (nsets == 3) res = dbscan_wave_cell<3>(ra, rb, P, T, rows, lane);
+        else res = dbscan_wave_cell<4>(ra, rb, P, T, rows, lane);
     }
     if (lane == 0) out[cell] = res;
 }
@@ -506,6 +525,12 @@ __global__ __launch_bounds__(256) void k_dbscan_block(int nList, const int* __re
     int nn = 0, cluster = 0, i = 0, rem = P - 1, parity = 0;
     bool lastFree = true;
     while (i < P) {
+        if (rem == 0) {
+            // pending pops change no label; only index P-1 can still open
+            // a cluster (block-uniform exit: rem is the same in every thread)
+            if (lastFree) ++cluster;
+            break;
+        }
         if (nn >= 1) {
             const int slot = nn - 1;
             const uint64_t bit = 1ull << (slot & 63);
@@ -515,11 +540,6 @@ __global__ __launch_bounds__(256) void k_dbscan_block(int nList, const int* __re
             i = set ? slot : 0;
             nn = nn - 1;
         } else {
-            if (rem == 0) {
-                // only index P-1 can still open a cluster
-                if (lastFree) ++cluster;
-                break;
-            }
             if (threadIdx.x == 0) s_next = P;
             __syncthreads();
             for (int w = (i >> 6) + (int)threadIdx.x; w < nWords; w += blockDim.x) {

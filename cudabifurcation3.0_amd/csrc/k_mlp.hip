@@ -272,6 +272,12 @@ __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, floa
     if (bi >= 0) wbf[bi] = (bf16)p[i];
 }
 
+__global__ void k_fill_f64(int n, double* p, double v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_fill_f32(int n, float* p, float v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -397,6 +403,51 @@ __global__ void k_fill_pred(long long n, const unsigned char* __restrict__ cls,
     lleMap[i] = (double)lle[i];
 }
 
+// Scores of the surrogate on re-solved cells, accumulated per group:
+// 0 = flagged and re-solved in full (entries [0, nFull) of list),
+// 1 = validation sample (entries [nFull, nTot)), 2 = peak count only
+// (plist).  acc[g*5 + 0..4] = cells, class right, regime right, finite
+// LLE cells, sum |LLE error| (as doubles).
+__global__ __launch_bounds__(256) void k_score(int nFull, int nTot, const int* __restrict__ list,
+                                               const int* __restrict__ rpeak,
+                                               const double* __restrict__ rlle, int nPeak,
+                                               const int* __restrict__ plist,
+                                               const int* __restrict__ ppeak,
+                                               const unsigned char* __restrict__ pcls,
+                                               const unsigned char* __restrict__ pch,
+                                               const float* __restrict__ plle, double chaosThr,
+                                               double* __restrict__ acc)
+{
+    __shared__ double s_acc[15];
+    if (threadIdx.x < 15) s_acc[threadIdx.x] = 0;
+    __syncthreads();
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nTot) {
+        const int g = t < nFull ? 0 : 1;
+        const int cell = list[t];
+        const int tc = min(kClasses, max(1, rpeak[t])) - 1;
+        const double tl = rlle[t];
+        const bool tch = tl > chaosThr;
+        const bool okc = tc == (int)pcls[cell];
+        const bool okr = okc && (tch == (pch[cell] != 0));
+        atomicAdd(&s_acc[g * 5 + 0], 1.0);
+        if (okc) atomicAdd(&s_acc[g * 5 + 1], 1.0);
+        if (okr) atomicAdd(&s_acc[g * 5 + 2], 1.0);
+        if (isfinite(tl)) {
+            atomicAdd(&s_acc[g * 5 + 3], 1.0);
+            atomicAdd(&s_acc[g * 5 + 4], fabs(tl - (double)plle[cell]));
+        }
+    }
+    if (t < nPeak) {
+        const int cell = plist[t];
+        const int tc = min(kClasses, max(1, ppeak[t])) - 1;
+        atomicAdd(&s_acc[10], 1.0);
+        if (tc == (int)pcls[cell]) atomicAdd(&s_acc[11], 1.0);
+    }
+    __syncthreads();
+    if (threadIdx.x < 15 && s_acc[threadIdx.x] != 0) atomicAdd(&acc[threadIdx.x], s_acc[threadIdx.x]);
+}
+
 // ------------------------------------------------------------ launchers
 
 // splits > 1 (kEpiF32 only): K is cut into `splits` chunks written to
@@ -448,6 +499,12 @@ hipError_t launch_sum_splits(int n, int splits, const float* part, float* out, h
 {
     hipLaunchKernelGGL(k_sum_splits, dim3((n + 255) / 256), dim3(256), 0, st, n, splits, part,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(int n, double* p, double v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_f64, dim3((n + 255) / 256), dim3(256), 0, st, n, p, v);
     return hipGetLastError();
 }
 
@@ -524,4 +581,18 @@ hipError_t launch_fill_pred(long long n, const unsigned char* cls, const float* 
     return hipGetLastError();
 }
 
+}  // namespace bifhip
+
+namespace bifhip {
+hipError_t launch_score(int nFull, int nTot, const int* list, const int* rpeak, const double* rlle,
+                        int nPeak, const int* plist, const int* ppeak, const unsigned char* pcls,
+                        const unsigned char* pch, const float* plle, double chaosThr,
+                        double* acc, hipStream_t st)
+{
+    const int n = max(nTot, nPeak);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_score, dim3((n + 255) / 256), dim3(256), 0, st, nFull, nTot, list, rpeak,
+                       rlle, nPeak, plist, ppeak, pcls, pch, plle, chaosThr, acc);
+    return hipGetLastError();
+}
 }  // namespace bifhip

@@ -457,7 +457,10 @@ int bifhip_set_chaos_confidence(double threshold);
 #define BIFHIP_SSTAT_LLE_SD 17
 #define BIFHIP_SSTAT_PEAK_ONLY 18          /* cells re-solved for the class only */
 #define BIFHIP_SSTAT_PEAK_ONLY_CLASS_ACC 19
-#define BIFHIP_SSTAT_COUNT 20
+#define BIFHIP_SSTAT_MS_FLAG 20            /* parts of MS_RESOLVE: flagging and   */
+#define BIFHIP_SSTAT_MS_SOLVE 21           /* sampling, the FP64 re-solves, and   */
+#define BIFHIP_SSTAT_MS_SCORE 22           /* scoring + writing the results       */
+#define BIFHIP_SSTAT_COUNT 23
 
 int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int amountOfInitialConditions,
                                 const double* initialConditions, const double* ranges,

@@ -155,6 +155,19 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
         M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
     }
     if (A.pre == 1) {
+        // three samples per iteration: the (p2, p1, current) windows rotate
+        // through registers without copies
+        for (; i + 3 <= N; i += 3) {
+            pa.push(pick<M, WV>(x0, x1, x2), i, sa);
+            pb.push(pick<M, WV>(y0, y1, y2), i, sb);
+            M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+            pa.push(pick<M, WV>(x0, x1, x2), i + 1, sa);
+            pb.push(pick<M, WV>(y0, y1, y2), i + 1, sb);
+            M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+            pa.push(pick<M, WV>(x0, x1, x2), i + 2, sa);
+            pb.push(pick<M, WV>(y0, y1, y2), i + 2, sb);
+            M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+        }
         for (; i < N; ++i) {
             pa.push(pick<M, WV>(x0, x1, x2), i, sa);
             pb.push(pick<M, WV>(y0, y1, y2), i, sb);
@@ -739,8 +752,19 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
             for (; i < F.nNT; ++i) M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
             k += F.nNT;
         } else if (k >= w0 + 2 && k + F.nNT <= w1) {
-            // every state of this block is a recorded sample past the first two
-            for (int i = 0; i < F.nNT; ++i, ++k) {
+            // every state of this block is a recorded sample past the first
+            // two; three samples per iteration so the sample window
+            // (p2, p1, current) rotates through registers without copies
+            int i = 0;
+            for (; i + 3 <= F.nNT; i += 3, k += 3) {
+                ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k - w0, sink);
+                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
+                ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k + 1 - w0, sink);
+                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
+                ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k + 2 - w0, sink);
+                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
+            }
+            for (; i < F.nNT; ++i, ++k) {
                 ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k - w0, sink);
                 M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
             }

@@ -27,8 +27,19 @@ struct PeakSink {
 struct PeakStream {
     double p1, p2;
     int mode, found, lastj;
+    // Wavefront-uniform: bit l set when lane l's machine is not in kSeek.
+    // Kept as a scalar mask so the per-sample test needs only the two
+    // candidate compares; refreshed after every machine run.
+    uint64_t pend;
 
-    __device__ __forceinline__ void init() { p1 = p2 = 0; mode = kSeek; found = 0; lastj = 0; }
+    __device__ __forceinline__ void init()
+    {
+        p1 = p2 = 0;
+        mode = kSeek;
+        found = 0;
+        lastj = 0;
+        pend = 0;
+    }
 
     __device__ __forceinline__ void emit(double amp, int j, const PeakSink& s)
     {
@@ -49,9 +60,15 @@ struct PeakStream {
     // machine runs only when some lane of the wavefront needs it.
     __device__ __forceinline__ void push(double v, int t, const PeakSink& s)
     {
-        // bitwise (not short-circuit) so the test compiles without branches
-        const bool busy = (mode != kSeek) | ((p1 > p2) & (p1 >= v));
-        if (__any(busy)) machine(v, t, s);
+        // each compare feeds a ballot directly as a lane mask
+        const uint64_t cand = __builtin_amdgcn_ballot_w64(p1 > p2) &
+                              __builtin_amdgcn_ballot_w64(p1 >= v);
+        if ((cand | pend) != 0) {
+            machine(v, t, s);
+            // lanes not active here (left a divergent loop) keep their bits
+            const bool busy = mode != kSeek;
+            pend = (pend & ~__builtin_amdgcn_ballot_w64(true)) | __builtin_amdgcn_ballot_w64(busy);
+        }
         p2 = p1;
         p1 = v;
     }

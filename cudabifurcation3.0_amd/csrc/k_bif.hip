@@ -261,34 +261,62 @@ __device__ __forceinline__ bool near(double xa, double ya, double xb, double yb,
 constexpr int kDbR = 4;                  // register sets: P <= 64*kDbR
 constexpr int kDbMaxP = kWave * kDbR;
 
+// Bits of 64-bit word w (global indices 64w .. 64w+63) with index >= i,
+// and with index < q.
+__device__ __forceinline__ uint64_t word_ge(int w, int i)
+{
+    const int d = i - w * kWave;
+    return d <= 0 ? ~0ull : (d >= kWave ? 0ull : (~0ull << d));
+}
+__device__ __forceinline__ uint64_t word_lt(int w, int q) { return ~word_ge(w, q); }
+
 // DBSCAN of one cell by one wavefront, specialised on the number NS of
 // 64-peak words (P <= 64*NS) so the scalar mask logic has no idle words.
+// Peak k lives in lane k%64 of register set k/64.  Neighbour row i (bits
+// over k, one 64-bit word per k-set) is one ballot per word; word w of
+// row i is kept in lane i%64 of the register pair R[i/64][w], so the
+// label machine reads a row with readlanes (no LDS round trip).  The
+// anchor peak of row i is read with scalar loads (ra, rb and i are
+// wave-uniform).
 template <int NS>
-__device__ __forceinline__ int dbscan_wave_cell(const double* ra, const double* rb, int P,
-                                                double T, uint64_t* rows, int lane)
+__device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
+                                                const double* __restrict__ rb, int P, double T,
+                                                int lane)
 {
     double a[NS], b[NS];
+    uint64_t valid[NS];               // k < P-1: the reference never absorbs the last peak
 #pragma unroll
     for (int r = 0; r < NS; ++r) {
         const int k = r * kWave + lane;
         a[r] = k < P ? ra[k] : 0.0;
         b[r] = k < P ? rb[k] : 0.0;
+        valid[r] = __builtin_amdgcn_ballot_w64(k < P - 1);
     }
-    // neighbour rows: one ballot per (row, 64-wide k-set), kept in LDS
+    uint64_t R[NS][NS];
+    uint64_t Z[NS];                   // isolated peaks: empty neighbour row
+#pragma unroll
+    for (int ri = 0; ri < NS; ++ri) {
+        Z[ri] = 0;
+#pragma unroll
+        for (int w = 0; w < NS; ++w) R[ri][w] = 0;
+    }
 #pragma unroll
     for (int ri = 0; ri < NS; ++ri) {
         const int iend = min(kWave, P - ri * kWave);
         for (int li = 0; li < iend; ++li) {
             const int i = ri * kWave + li;
-            const double ai = readlane_d(a[ri], li);
-            const double bi = readlane_d(b[ri], li);
+            const double ai = ra[i];
+            const double bi = rb[i];
+            const bool mine = lane == li;             // the lane that keeps row i
+            uint64_t any = 0;
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
-                const int k = w * kWave + lane;
-                const bool c = (k < P - 1) && (k != i) && near(ai, bi, a[w], b[w], T);
-                const uint64_t m = __ballot(c);
-                if (lane == 0) rows[i * kDbR + w] = m;
+                uint64_t m = __builtin_amdgcn_ballot_w64(near(ai, bi, a[w], b[w], T)) & valid[w];
+                if (w == ri) m &= ~(1ull << li);      // k != i
+                R[ri][w] = mine ? m : R[ri][w];
+                any |= m;
             }
+            if (any == 0) Z[ri] |= 1ull << li;
         }
     }
 
@@ -296,18 +324,26 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* ra, const double* 
     uint64_t U[NS], S[NS];
 #pragma unroll
     for (int w = 0; w < NS; ++w) {
-        const int n = P - w * kWave;
-        U[w] = n >= kWave ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
+        U[w] = word_lt(w, P);
         S[w] = 0;
     }
-    // The reference visits i = 0, 1, ... and after each stack pop jumps
-    // back to i = slot (or 0), so an empty-stack scan re-walks labelled
-    // indices once per cluster.  Those iterations change nothing, so the
-    // walk jumps straight to the next unlabelled index >= i.  rem counts
-    // the unlabelled indices below P-1 (the only ones a scan can absorb);
-    // at rem == 0 pops and scans change no label, P-1 is never absorbed
-    // (scans test k < P-1) and the walk still reaches it (a popped slot is
-    // at most P-2), so it opens one more cluster exactly when unlabelled.
+    // The reference visits i = 0, 1, ... and after each stack pop jumps to
+    // i = slot (or 0); an empty-stack step at a labelled index changes
+    // nothing.  Steps that provably change nothing but i and nn are taken in
+    // bulk, with the state the reference reaches after them:
+    //  - rem counts the unlabelled indices below P-1, the only ones a scan
+    //    can absorb.  At rem == 0 pops and scans change no label, P-1 is
+    //    never absorbed (scans test k < P-1) and the walk still reaches it
+    //    (a popped slot is at most P-2), so it opens one more cluster
+    //    exactly when unlabelled.
+    //  - A pop of an unset slot sets i = 0 and scans row 0.  Row 0 was
+    //    scanned first (the walk starts at 0), so it has no unlabelled
+    //    neighbour left: the run of unset slots above the highest set slot
+    //    below nn is popped at once, leaving i = 1.
+    //  - With an empty stack the walk visits the unlabelled indices >= i in
+    //    order.  An isolated index (empty row) only opens a cluster of its
+    //    own, so every isolated unlabelled index before the next
+    //    non-isolated one is labelled at once.
     int nn = 0, cluster = 0, i = 0, rem = P - 1;
     bool lastFree = true;
     while (i < P) {
@@ -316,43 +352,72 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* ra, const double* 
             break;
         }
         if (nn >= 1) {
-            const int slot = nn - 1;
-            const int ws = slot >> 6;
-            const uint64_t bit = 1ull << (slot & 63);
-            bool set = false;
+            int hs = -1;                       // highest set slot below nn
+#pragma unroll
+            for (int w = NS - 1; w >= 0; --w) {
+                const uint64_t m = S[w] & word_lt(w, nn);
+                if (hs < 0 && m != 0) hs = w * kWave + 63 - __builtin_clzll(m);
+            }
+            if (hs != nn - 1) {
+                nn = hs + 1;
+                i = 1;
+                continue;
+            }
 #pragma unroll
             for (int w = 0; w < NS; ++w)
-                if (w == ws) { set = (S[w] & bit) != 0; S[w] &= ~bit; }
-            i = set ? slot : 0;
-            nn = nn - 1;
+                if (w == (hs >> 6)) S[w] &= ~(1ull << (hs & 63));
+            i = hs;
+            nn = hs;
         } else {
-            int nxt = P;
+            int q = P;                         // next non-isolated unlabelled >= i
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
-                const int lo = w * kWave;
-                if (nxt == P && i < lo + kWave) {
-                    const uint64_t m = i <= lo ? U[w] : (U[w] & (~0ull << (i - lo)));
-                    if (m) nxt = lo + __builtin_ctzll(m);
-                }
+                const uint64_t m = U[w] & ~Z[w] & word_ge(w, i);
+                if (q == P && m != 0) q = w * kWave + __builtin_ctzll(m);
             }
-            if (nxt >= P) break;
-            i = nxt;
+            int c = 0;
+#pragma unroll
+            for (int w = 0; w < NS; ++w) {
+                const uint64_t m = U[w] & Z[w] & word_ge(w, i) & word_lt(w, q);
+                U[w] &= ~m;
+                c += __popcll(m);
+            }
+            if (c != 0) {
+                cluster += c;
+                const int wl = (P - 1) >> 6;
+                bool lastNow = false;
+#pragma unroll
+                for (int w = 0; w < NS; ++w)
+                    if (w == wl) lastNow = ((U[w] >> ((P - 1) & 63)) & 1ull) != 0;
+                if (lastFree && !lastNow) {
+                    lastFree = false;
+                    --c;
+                }
+                rem -= c;
+                if (rem == 0) continue;
+            }
+            if (q >= P) break;
+            i = q;
             ++cluster;
-            const int wi = i >> 6;
-            const uint64_t bit = 1ull << (i & 63);
 #pragma unroll
             for (int w = 0; w < NS; ++w)
-                if (w == wi) U[w] &= ~bit;
+                if (w == (i >> 6)) U[w] &= ~(1ull << (i & 63));
             if (i < P - 1) --rem;
             else lastFree = false;
         }
         {
+            const int ri = i >> 6, li = i & 63;
+            uint64_t row[NS];
+#pragma unroll
+            for (int r = 0; r < NS; ++r)
+                if (r == ri) {
+#pragma unroll
+                    for (int w = 0; w < NS; ++w) row[w] = readlane_u64(R[r][w], li);
+                }
             int c = 0;
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
-                // every lane reads the same word; keep it scalar
-                const uint64_t row = readlane_u64(rows[i * kDbR + w], 0);
-                const uint64_t m = row & U[w];
+                const uint64_t m = row[w] & U[w];
                 U[w] &= ~m;
                 S[w] |= m;
                 c += __popcll(m);
@@ -365,16 +430,15 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* ra, const double* 
     return cluster - 1;
 }
 
-// One wavefront per cell.  Peak k lives in lane k%64 of register set k/64.
-// Neighbour row i (bits over k, one 64-bit word per k-set) is built with a
-// ballot and kept in the wave's LDS slice; the label/stack machine then
-// reads one row per scan as scalar masks.  The reference's machine:
+// One wavefront per cell (dbscan_wave_cell holds the peaks and neighbour
+// rows in registers).  The reference's machine:
 //   U = unlabelled k (label == 0), S = stack slots holding their own index
 //   (slot k is written with value k; popping slot NumNeibor-1 yields that
 //   slot's index if set, else 0; the popped value becomes the loop
 //   variable, then ++).  Result cluster-1.
 // Cells whose peaks do not fit (P > min(cap, 256), P <= maxPeaks) are
-// appended to overflow[] as listBase + cell and left to k_dbscan_lane.
+// appended to overflow[] as listBase + cell and left to the workgroup and
+// lane kernels.
 __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __restrict__ counts,
                                                      const double* __restrict__ amps,
                                                      const double* __restrict__ ivls,
@@ -383,8 +447,8 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
                                                      int* __restrict__ overflow,
                                                      int* __restrict__ nOverflow, int listBase)
 {
-    __shared__ uint64_t s_rows[256 / kWave][kDbMaxP * kDbR];   // 8 KiB per wave
-    const int cell = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    const int cell = __builtin_amdgcn_readfirstlane(
+        (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
     if (cell >= nCells) return;          // wave-uniform
     const int lane = lane_id();
     const int P = __builtin_amdgcn_readfirstlane(counts[cell]);
@@ -402,12 +466,11 @@ __global__ __launch_bounds__(256) void k_dbscan_wave(int nCells, const int* __re
     } else {
         const double* ra = amps + (long long)cell * rowStride;
         const double* rb = ivls + (long long)cell * rowStride;
-        uint64_t* rows = s_rows[threadIdx.x / kWave];
         const int nsets = (P + kWave - 1) / kWave;
-        if (nsets == 1) res = dbscan_wave_cell<1>(ra, rb, P, T, rows, lane);
-        else if (nsets == 2) res = dbscan_wave_cell<2>(ra, rb, P, T, rows, lane);
-        else if (nsets == 3) res = dbscan_wave_cell<3>(ra, rb, P, T, rows, lane);
-        else res = dbscan_wave_cell<4>(ra, rb, P, T, rows, lane);
+        if (nsets == 1) res = dbscan_wave_cell<1>(ra, rb, P, T, lane);
+        else if (nsets == 2) res = dbscan_wave_cell<2>(ra, rb, P, T, lane);
+        else if (nsets == 3) res = dbscan_wave_cell<3>(ra, rb, P, T, lane);
+        else res = dbscan_wave_cell<4>(ra, rb, P, T, lane);
     }
     if (lane == 0) out[cell] = res;
 }

@@ -186,6 +186,35 @@ def test_dbscan_device_fuzz_bit_exact(orc, bh, dev, max_peaks):
         assert got[b] == want, (b, cnt[b])
 
 
+def test_dbscan_device_sparse_sets_bit_exact(orc, bh, dev):
+    """Chaotic-like peak sets: most peaks isolated, a few small clumps and
+    repeats, every peak count from 2 to 256 (one to four 64-peak words)."""
+    rng = np.random.default_rng(23)
+    sizes = np.arange(2, 257)
+    nb, stride = len(sizes), 256
+    amps = np.zeros((nb, stride))
+    ivls = np.zeros((nb, stride))
+    for b, P in enumerate(sizes):
+        a = rng.random(P) * rng.choice([0.1, 0.3, 1.0])
+        iv = rng.random(P) * 0.05
+        k = rng.integers(0, P, size=max(1, P // 8))
+        a[k] = a[rng.integers(0, P, size=len(k))] + rng.normal(size=len(k)) * 0.003
+        if b % 3 == 0:
+            a[-1] = a[0]                     # the last peak is never absorbed
+            iv[-1] = iv[0]
+        amps[b, :P], ivls[b, :P] = a, iv
+    cnt = sizes.astype(np.int32)
+    say("dbscan sparse sets", nb)
+    out = bh.DeviceArray.full((nb,), -99, np.int32)
+    A, I, Cn = _t(bh, amps.ravel(), np.float64), _t(bh, ivls.ravel(), np.float64), _t(
+        bh, cnt, np.int32)
+    bh.dbscan_device(A.ptr, I.ptr, stride, nb, Cn.ptr, 100000, 0.01, out.ptr)
+    bh.synchronize()
+    got = out.to_host()
+    for b, P in enumerate(sizes):
+        assert got[b] == orc.dbscan(amps[b, :P], ivls[b, :P], 0.01), (b, P)
+
+
 def test_dbscan_device_large_peak_sets(orc, bh, dev):
     """Cells beyond the wave kernel's 256 peaks: up to 16384 peaks take the
     workgroup kernel (peaks staged in LDS up to 3072, read from the rows

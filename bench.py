@@ -16,6 +16,8 @@ Workloads (one step = one whole map, inputs resident, outputs in HBM):
              flagged cells.  Agreement and speedup are measured against the
              plain FP64 map of the same 4096^2 lattice (computed once).
   plain      the plain FP64 fused LLE + peak-count map on an n x n lattice.
+  lle        the 1024^2 LLE map with FP32 bulk integration, timed against
+             the plain FP64 LLE map, with its error and sign agreement.
   tiles      Lyapunov spectrum of 4096 uneven 16x16 refinement tiles (FP32
              variational state, FP64 Gram-Schmidt), autotuned, one
              persistent graph-replayed launch per step.
@@ -176,6 +178,75 @@ def run_tiles(bh, np, args, out):
                                "disabled, so at most 50% of the published peak"}
 
 
+LLE32_STEP_FLOP = 11         # FP32 fused-multiply-add form of the Rossler step (FMA = 2)
+LLE32_DEV_FLOP = 13          # FP32 deviation map of the perturbed trajectory (FMA = 2)
+
+
+def run_lle(bh, np, args, out):
+    """2D LLE map of the Rossler box on an n x n lattice: FP32 bulk
+    integration (FP64 norm, log accumulation and renormalisation) timed
+    against the plain FP64 map of the same lattice, with the FP32 map's
+    error and sign / regime agreement against the FP64 run.  Perturbation
+    1e-8 along x."""
+    n = args.n or 1024
+    cells = n * n
+    d64 = bh.DeviceArray((cells,), np.float64)
+    d32 = bh.DeviceArray((cells,), np.float64)
+    lle_args = (2, LLE["tMax"], LLE["NT"], n, H, LLE["eps"], IC, RANGES, [0, 2], 0.0,
+                LLE["transient"], VALS)
+
+    def timed(dst, prec, steps, kernel):
+        bh.set_precision(prec)
+        for _ in range(args.warmup):
+            bh.lle_map_device(dst.ptr, *lle_args)
+        bh.synchronize()
+        bh.set_timing(True)
+        bh.reset_kernel_times()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bh.lle_map_device(dst.ptr, *lle_args)
+        bh.synchronize()
+        ms = (time.perf_counter() - t0) * 1000 / steps
+        kms, nl = bh.kernel_time(kernel)
+        return ms, kms, nl
+
+    ms64, k64, n64 = timed(d64, bh.PREC_FP64, 1, "lle")
+    ms32, k32, n32 = timed(d32, bh.PREC_FP32, args.steps, "lle32")
+    bh.set_precision(bh.PREC_FP64)
+    a, b = d64.to_host(), d32.to_host()
+    fin = np.isfinite(a) & np.isfinite(b)
+    err = np.abs(a[fin] - b[fin])
+    with np.errstate(invalid="ignore"):
+        sign = float((np.sign(a[fin]) == np.sign(b[fin])).mean())
+        chaos = float(((a > CHAOS) == (b > CHAOS)).mean())
+    skip = int(LLE["transient"] / H)
+    nNT = int(LLE["NT"] / H)
+    nb = int(LLE["tMax"] / LLE["NT"])
+    flop32 = (LLE32_STEP_FLOP * (skip + nb * nNT) + LLE32_DEV_FLOP * nb * nNT) * cells
+    ach = flop32 * n32 / (k32 / 1000) / 1e12
+    out.update(metric="grid points/s, 2D LLE map, FP32 bulk vs FP64", value=cells / (ms32 / 1000),
+               unit="grid points/s", ms_per_step=ms32, dtype="f32",
+               config={"workload": f"2D LLE map, Rossler {n}x{n} (a, c) lattice, FP32 bulk "
+                                   f"integration with FP64 norm / log / renormalisation, "
+                                   f"perturbation 1e-8 along x",
+                       "grid": n, "ranges": RANGES, "h": H, "lle": LLE})
+    out["fp64_reference"] = {"ms": ms64, "value": cells / (ms64 / 1000),
+                             "kernel_ms": k64 / max(n64, 1), "speedup_fp32_vs_fp64": ms64 / ms32}
+    out["error_vs_fp64"] = {"cells_compared": int(fin.sum()),
+                            "nan_cells_agree": bool(np.array_equal(np.isnan(a), np.isnan(b))),
+                            "max_abs": float(err.max()), "mean_abs": float(err.mean()),
+                            "p99_abs": float(np.percentile(err, 99)),
+                            "sign_agreement": sign, "chaos_agreement": chaos,
+                            "chaos_threshold": CHAOS}
+    out["roofline"] = {"bound": "valu_fp32", "kernel": "k_lle32", "achieved": ach,
+                       "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                       "frac": ach / (FP32_VALU_PEAK / 1e12), "traffic": None,
+                       "avg_launch_ms": k32 / max(n32, 1), "launches": n32,
+                       "note": f"{LLE32_STEP_FLOP} FLOP per state step, {LLE32_DEV_FLOP} per "
+                               f"deviation step (FMA = 2); packed FP32 is disabled, so at most "
+                               f"50% of the published peak"}
+
+
 def run_copy(bh, np, args, out):
     """Device-to-device copy of a known size: the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters into bytes."""
@@ -200,7 +271,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="surrogate",
-                    choices=["surrogate", "plain", "tiles", "copy"])
+                    choices=["surrogate", "plain", "lle", "tiles", "copy"])
     ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
     ap.add_argument("--coarse", type=int, default=256)
     ap.add_argument("--conf", type=float, default=0.95, help="class softmax confidence threshold")
@@ -240,8 +311,8 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (a, c) parameter lattice of the Rossler system"}
 
-    if args.workload in ("tiles", "copy"):
-        (run_tiles if args.workload == "tiles" else run_copy)(bh, np, args, out)
+    if args.workload in ("tiles", "copy", "lle"):
+        {"tiles": run_tiles, "copy": run_copy, "lle": run_lle}[args.workload](bh, np, args, out)
         print(json.dumps(out), flush=True)
         return
     if args.workload == "plain":

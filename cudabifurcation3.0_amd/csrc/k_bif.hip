@@ -27,6 +27,12 @@
 
 namespace bifhip {
 
+// samples tested together by the trajectory peak stream's batched path
+#ifndef BIFHIP_PEAK_BATCH
+#define BIFHIP_PEAK_BATCH 4
+#endif
+constexpr int kPeakBatch = BIFHIP_PEAK_BATCH;
+
 template <class M, int WV>
 __device__ __forceinline__ double pick(double x0, double x1, double x2)
 {
@@ -67,21 +73,29 @@ __global__ __launch_bounds__(256) void k_bif_peaks(const BifArgs A, int* __restr
     sink.cap = A.cap;
     sink.hpf = A.hpf;
 
-    PeakStream ps;
+    TrajPeakStream ps;
     ps.init();
     bool bad = false;
     const int N = A.N;
     const int pre = A.pre;
     int i = 0;
     for (; i < N && i < 2; ++i) {
-        const double d = pick<M, WV>(x0, x1, x2);
-        ps.p2 = ps.p1;
-        ps.p1 = d;
+        ps.prime(pick<M, WV>(x0, x1, x2), i);
         for (int j = 0; j < pre - 1; ++j) M::step(x0, x1, x2, p, h);
         M::step(x0, x1, x2, p, h);
         if (CHECK && diverged(pick<M, WV>(x0, x1, x2), A.maxValue)) { bad = true; break; }
     }
     if (!CHECK && pre == 1) {
+        constexpr int K = kPeakBatch;
+        for (; i + K <= N; i += K) {
+            double va[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                va[k] = pick<M, WV>(x0, x1, x2);
+                M::step(x0, x1, x2, p, h);
+            }
+            ps.push_batch(va, i, sink);
+        }
         for (; i < N; ++i) {
             ps.push(pick<M, WV>(x0, x1, x2), i, sink);
             M::step(x0, x1, x2, p, h);
@@ -141,32 +155,30 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
     sb.ivls = ivls && two ? ivls + (long long)c1 * A.rowStride : nullptr;
     sa.cap = sb.cap = A.cap;
     sa.hpf = sb.hpf = A.hpf;
-    PeakStream pa, pb;
+    TrajPeakStream pa, pb;
     pa.init();
     pb.init();
     const int N = A.N;
     int i = 0;
     for (; i < N && i < 2; ++i) {
-        pa.p2 = pa.p1;
-        pa.p1 = pick<M, WV>(x0, x1, x2);
-        pb.p2 = pb.p1;
-        pb.p1 = pick<M, WV>(y0, y1, y2);
+        pa.prime(pick<M, WV>(x0, x1, x2), i);
+        pb.prime(pick<M, WV>(y0, y1, y2), i);
         for (int j = 0; j < A.pre - 1; ++j) M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
         M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
     }
     if (A.pre == 1) {
-        // three samples per iteration: the (p2, p1, current) windows rotate
-        // through registers without copies
-        for (; i + 3 <= N; i += 3) {
-            pa.push(pick<M, WV>(x0, x1, x2), i, sa);
-            pb.push(pick<M, WV>(y0, y1, y2), i, sb);
-            M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
-            pa.push(pick<M, WV>(x0, x1, x2), i + 1, sa);
-            pb.push(pick<M, WV>(y0, y1, y2), i + 1, sb);
-            M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
-            pa.push(pick<M, WV>(x0, x1, x2), i + 2, sa);
-            pb.push(pick<M, WV>(y0, y1, y2), i + 2, sb);
-            M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+        // K samples per iteration, tested together (push_batch)
+        constexpr int K = kPeakBatch;
+        for (; i + K <= N; i += K) {
+            double va[K], vb[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                va[k] = pick<M, WV>(x0, x1, x2);
+                vb[k] = pick<M, WV>(y0, y1, y2);
+                M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
+            }
+            pa.push_batch(va, i, sa);
+            pb.push_batch(vb, i, sb);
         }
         for (; i < N; ++i) {
             pa.push(pick<M, WV>(x0, x1, x2), i, sa);
@@ -769,7 +781,7 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
     sink.ivls = ivls + (long long)t * A.rowStride;
     sink.cap = A.cap;
     sink.hpf = A.hpf;
-    PeakStream ps;
+    TrajPeakStream ps;
     ps.init();
     const int w0 = A.skip, w1 = A.skip + A.N;   // window of recorded states [w0, w1)
 
@@ -779,7 +791,7 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
             const double d = WV == 0 ? x0 : (WV == 1 ? x1 : x2);
             const int i = k - w0;
             if (i >= 2) ps.push(d, i, sink);
-            else { ps.p2 = ps.p1; ps.p1 = d; }
+            else ps.prime(d, i);
         }
     };
 
@@ -816,16 +828,17 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
             k += F.nNT;
         } else if (k >= w0 + 2 && k + F.nNT <= w1) {
             // every state of this block is a recorded sample past the first
-            // two; three samples per iteration so the sample window
-            // (p2, p1, current) rotates through registers without copies
+            // two; K samples per iteration, tested together (push_batch)
+            constexpr int K = kPeakBatch;
             int i = 0;
-            for (; i + 3 <= F.nNT; i += 3, k += 3) {
-                ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k - w0, sink);
-                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
-                ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k + 1 - w0, sink);
-                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
-                ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k + 2 - w0, sink);
-                M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
+            for (; i + K <= F.nNT; i += K, k += K) {
+                double va[K];
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) {
+                    va[kk] = WV == 0 ? x0 : (WV == 1 ? x1 : x2);
+                    M::step2(x0, x1, x2, p, y0, y1, y2, p, h);
+                }
+                ps.push_batch(va, k - w0, sink);
             }
             for (; i < F.nNT; ++i, ++k) {
                 ps.push(WV == 0 ? x0 : (WV == 1 ? x1 : x2), k - w0, sink);

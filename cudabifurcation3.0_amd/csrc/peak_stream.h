@@ -7,15 +7,35 @@ namespace bifhip {
 
 // ------------------------------------------------------------ peak stream
 //
-// Streaming form of peakFinder.  The batch scan evaluates candidate i when
-// d[i+1] is known; a plateau scan compares d[j] with d[j+1]; after a peak
-// or a rising plateau exit at j the next candidate is j+2.  Modes:
-//   kSeek  evaluate candidate t-1 on sample t
-//   kPlat  inside a plateau: compare d[t-1] with d[t]
-//   kSkip  candidate t-1 is skipped (the loop's ++i after i = j+1)
+// Streaming form of peakFinder (Library/cudaLibrary.cu:220-266).  The batch
+// scan evaluates candidate i (d[i] > d[i-1] && d[i] >= d[i+1]) when d[i+1]
+// is known; on a plateau it compares d[j] with d[j+1]; after a peak or a
+// rising plateau exit at j the next candidate is j+2.  Per lane the machine
+// is in one of three states:
+//   seek      evaluate candidate t-1 on sample t
+//   plateau   compare d[t-1] with d[t]
+//   skip      candidate t-1 is skipped (the loop's ++i after i = j+1)
 // The first peak found only provides the time origin of the first interval
 // (peakFinder drops it); intervals are (j_k - j_{k-1}) * hpf.
-enum : int { kSeek = 0, kPlat = 1, kSkip = 2 };
+//
+// The plateau and skip states are wavefront-uniform lane masks (SGPRs), so
+// a sample costs one FP64 compare (p1 >= v) and a few scalar ops; the
+// per-lane part runs only on a sample where some lane of the wavefront has
+// a candidate or sits on a plateau.
+//
+// TRAJ: samples come from an integrated trajectory of a model whose every
+// component update contains the component itself (Rossler, the linear test
+// model), so a NaN sample is never followed by a non-NaN one.  Then
+// d[i] > d[i-1] equals !(d[i-1] >= d[i]) -- the previous sample's compare --
+// for every pair that can occur (with d[i-1] NaN, d[i] is NaN and the
+// candidate fails on d[i] >= d[i+1] in both forms).  Two consequences:
+//   * a peak at j has d[j] > d[j+1], so the rising test of candidate j+1
+//     fails by itself: only a rising plateau exit needs the skip mask;
+//   * push_batch() tests K samples with K compares and one branch, and runs
+//     the per-sample machine only for a batch holding a candidate or a
+//     pending plateau / skip.
+// Without TRAJ (series from memory, general models) the rising test is an
+// explicit compare.
 
 struct PeakSink {
     double* amps;     // row base (nullptr: count only)
@@ -24,21 +44,29 @@ struct PeakSink {
     double hpf;       // peakFinder's h (0.01 by default)
 };
 
-struct PeakStream {
-    double p1, p2;
-    int mode, found, lastj;
-    // Wavefront-uniform: bit l set when lane l's machine is not in kSeek.
-    // Kept as a scalar mask so the per-sample test needs only the two
-    // candidate compares; refreshed after every machine run.
-    uint64_t pend;
+template <bool TRAJ>
+struct PeakStreamT {
+    double p1, p2;    // d[t-1], d[t-2] (p2 unused with TRAJ)
+    int found, lastj;
+    // wavefront-uniform lane masks
+    uint64_t plat;    // lanes on a plateau
+    uint64_t skip;    // lanes whose next candidate is skipped
+    uint64_t geprev;  // TRAJ: lanes with d[t-2] >= d[t-1]
 
     __device__ __forceinline__ void init()
     {
         p1 = p2 = 0;
-        mode = kSeek;
         found = 0;
         lastj = 0;
-        pend = 0;
+        plat = skip = geprev = 0;
+    }
+
+    // samples 0 and 1 fill the window
+    __device__ __forceinline__ void prime(double v, int i)
+    {
+        if (TRAJ && i == 1) geprev = __builtin_amdgcn_ballot_w64(p1 >= v);
+        p2 = p1;
+        p1 = v;
     }
 
     __device__ __forceinline__ void emit(double amp, int j, const PeakSink& s)
@@ -54,41 +82,69 @@ struct PeakStream {
         ++found;
     }
 
-    // sample v has index t (t >= 2 is checked by the caller's loop shape).
-    // A lane in kSeek without a candidate (the common case: a peak comes
-    // every few hundred samples) only shifts its window, so the state
-    // machine runs only when some lane of the wavefront needs it.
+    // One sample of the machine: d[t-1] = prev, d[t] = v, rising candidates
+    // (before the skip / plateau masks) in c.
+    __device__ __forceinline__ void machine(double prev, double v, uint64_t c, int t,
+                                            const PeakSink& s)
+    {
+        const uint64_t cand = c & ~skip & ~plat;
+        uint64_t nskip = 0;
+        if ((cand | plat) != 0) {
+            const uint64_t gt = __builtin_amdgcn_ballot_w64(prev > v);
+            const uint64_t lt = __builtin_amdgcn_ballot_w64(prev < v);
+            const uint64_t em = (cand | plat) & gt;      // strict fall: a peak at t-1
+            nskip = plat & lt;                           // rising plateau exit
+            if (!TRAJ) nskip |= em;
+            plat = (cand & ~gt) | (plat & ~gt & ~lt);    // equal: (still) on a plateau
+            if ((em >> __lane_id()) & 1) emit(prev, t - 1, s);
+        }
+        skip = nskip;
+    }
+
+    // sample v has index t >= 2
     __device__ __forceinline__ void push(double v, int t, const PeakSink& s)
     {
-        // each compare feeds a ballot directly as a lane mask
-        const uint64_t cand = __builtin_amdgcn_ballot_w64(p1 > p2) &
-                              __builtin_amdgcn_ballot_w64(p1 >= v);
-        if ((cand | pend) != 0) {
-            machine(v, t, s);
-            // lanes not active here (left a divergent loop) keep their bits
-            const bool busy = mode != kSeek;
-            pend = (pend & ~__builtin_amdgcn_ballot_w64(true)) | __builtin_amdgcn_ballot_w64(busy);
-        }
-        p2 = p1;
+        const uint64_t ge = __builtin_amdgcn_ballot_w64(p1 >= v);
+        const uint64_t rise = TRAJ ? ~geprev : __builtin_amdgcn_ballot_w64(p1 > p2);
+        // lanes that left a divergent loop hold no pending state
+        plat &= __builtin_amdgcn_read_exec();
+        const uint64_t c = ge & rise;
+        if ((c | plat | skip) != 0) machine(p1, v, c, t, s);
+        if (TRAJ) geprev = ge;
+        else p2 = p1;
         p1 = v;
     }
 
-    __device__ __forceinline__ void machine(double v, int t, const PeakSink& s)
+    // K consecutive samples v[0..K-1] with indices t..t+K-1 (t >= 2), all
+    // lanes of the wavefront active (TRAJ only).
+    template <int K>
+    __device__ __forceinline__ void push_batch(const double (&v)[K], int t, const PeakSink& s)
     {
-        if (mode == kSeek) {
-            if (p1 > p2 && p1 >= v) {
-                if (p1 > v) { emit(p1, t - 1, s); mode = kSkip; }
-                else mode = kPlat;
-            }
-        } else if (mode == kPlat) {
-            if (p1 < v) mode = kSkip;
-            else if (p1 > v) { emit(p1, t - 1, s); mode = kSkip; }
-        } else {
-            mode = kSeek;
+        static_assert(TRAJ, "batched samples need the trajectory form");
+        uint64_t ge[K], c[K];
+        ge[0] = __builtin_amdgcn_ballot_w64(p1 >= v[0]);
+#pragma unroll
+        for (int k = 1; k < K; ++k) ge[k] = __builtin_amdgcn_ballot_w64(v[k - 1] >= v[k]);
+        c[0] = ge[0] & ~geprev;
+        uint64_t any = c[0];
+#pragma unroll
+        for (int k = 1; k < K; ++k) {
+            c[k] = ge[k] & ~ge[k - 1];
+            any |= c[k];
         }
+        if ((any | plat | skip) != 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if ((c[k] | plat | skip) != 0) machine(k == 0 ? p1 : v[k - 1], v[k], c[k], t + k, s);
+        }
+        geprev = ge[K - 1];
+        p1 = v[K - 1];
     }
 
     __device__ __forceinline__ int count() const { return found > 1 ? found - 1 : 0; }
 };
+
+using PeakStream = PeakStreamT<false>;
+using TrajPeakStream = PeakStreamT<true>;
 
 }  // namespace bifhip

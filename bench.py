@@ -43,7 +43,7 @@ PEAK = dict(tMax=500.0, transient=1000.0, eps=0.01)
 LLE = dict(tMax=1000.0, NT=0.5, transient=500.0, eps=1e-8)
 CHAOS = 0.01
 FP64_VALU_PEAK = 78.6e12     # MI355X FP64 vector, FMA counted as 2
-FP32_VALU_PEAK = 157.3e12    # MI355X FP32 vector (packed), FMA counted as 2
+FP32_VALU_PEAK = 157.3e12    # MI355X FP32 vector (published), FMA counted as 2
 STEP_FLOP = 12               # Rossler sequential Euler step, no contraction
 RENORM_FLOP = 20
 TILE_STEP_FLOP = 48          # state step + 3 tangent-vector steps
@@ -121,7 +121,16 @@ def cpu_baseline(n, rows):
                       f"DBSCAN peak count + LLE per cell, {dt:.1f} s"}
 
 
-def roofline(bh, cells_total, kernel="lle_peaks", traffic=None):
+# HBM bytes per cell of k_lle_peaks at the benchmark's settings, from the
+# FETCH_SIZE / WRITE_SIZE counter runs of the plain 1024^2 map scaled by the
+# same counters on a 1 GiB copy (FETCH x 2.0, WRITE x 1.0): 396 KiB fetched
+# and 5 386 628 KiB written per 1 048 576-cell launch.  Algorithmic bytes are
+# about 1 372 per cell (85 peaks x 16 B, plus 12 B of results); the rest is
+# the 32-byte write granule charged for each isolated 8-byte peak store.
+LLE_PEAKS_HBM_BYTES_PER_CELL = (396 + 5386628) * 1024 / 1048576
+
+
+def roofline(bh, cells_total, kernel="lle_peaks"):
     """Achieved FP64 FLOP/s of the fused integrator over all its launches:
     algorithmic FLOP (12 per Rossler step, 20 per renormalisation; peak
     detection compares are not counted) / summed HIP-event launch time."""
@@ -131,7 +140,10 @@ def roofline(bh, cells_total, kernel="lle_peaks", traffic=None):
     ach = flop / (ms / 1000) / 1e12 if ms > 0 else 0.0
     return {"bound": "valu_fp64", "kernel": "k_lle_peaks", "achieved": ach,
             "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s", "frac": ach / (FP64_VALU_PEAK / 1e12),
-            "traffic": traffic, "flop_per_launch": flop / max(n, 1), "avg_launch_ms": avg,
+            "traffic": LLE_PEAKS_HBM_BYTES_PER_CELL * cells_total / max(n, 1),
+            "traffic_note": "HBM bytes per launch from the counter profile (per-cell figure x "
+                            "cells per launch)",
+            "flop_per_launch": flop / max(n, 1), "avg_launch_ms": avg,
             "launches": n, "cells": cells_total,
             "note": "FP64 vector ALU (no MFMA on this path); no-contraction FP64 reaches at "
                     "most 50% of the FMA-counted peak"}
@@ -174,8 +186,8 @@ def run_tiles(bh, np, args, out):
                        "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s",
                        "frac": ach / (FP32_VALU_PEAK / 1e12), "traffic": None,
                        "avg_launch_ms": kms / max(nl, 1), "launches": nl,
-                       "note": "48 FLOP per step (state + 3 tangent vectors); packed FP32 is "
-                               "disabled, so at most 50% of the published peak"}
+                       "note": "48 FLOP per step (state + 3 tangent vectors, FMA = 2); built "
+                               "without packed FP32 instructions"}
 
 
 LLE32_STEP_FLOP = 11         # FP32 fused-multiply-add form of the Rossler step (FMA = 2)
@@ -243,8 +255,8 @@ def run_lle(bh, np, args, out):
                        "frac": ach / (FP32_VALU_PEAK / 1e12), "traffic": None,
                        "avg_launch_ms": k32 / max(n32, 1), "launches": n32,
                        "note": f"{LLE32_STEP_FLOP} FLOP per state step, {LLE32_DEV_FLOP} per "
-                               f"deviation step (FMA = 2); packed FP32 is disabled, so at most "
-                               f"50% of the published peak"}
+                               f"deviation step (FMA = 2); built without packed FP32 "
+                               f"instructions"}
 
 
 def run_copy(bh, np, args, out):
@@ -334,7 +346,9 @@ def main():
         out["kernels_ms_per_step"] = {k: bh.kernel_time(k)[0] / args.steps
                                       for k in ("lle_peaks", "dbscan")}
     else:
-        # reference: the plain FP64 map of the same lattice, timed once
+        # reference: the plain FP64 map of the same lattice, timed once after
+        # a small map has loaded its kernels
+        plain_map(bh, args.coarse, d_peak, d_lle)
         bh.set_timing(True)
         bh.reset_kernel_times()
         bh.synchronize()

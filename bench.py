@@ -290,7 +290,8 @@ def main():
     ap.add_argument("--chaos-conf", type=float, default=0.98,
                     help="chaos-logit confidence threshold")
     ap.add_argument("--validation", type=float, default=0.002)
-    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--batch", type=int, default=2048,
+                    help="training batch rows (20 epochs of the 256^2 coarse map)")
     ap.add_argument("--cpu-cells", type=int, default=65536,
                     help="cells of the CPU baseline sample (whole lattice rows, ~10 s on 16 cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

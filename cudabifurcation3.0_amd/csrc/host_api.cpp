@@ -59,6 +59,7 @@ hipError_t launch_lle_peaks(int model, int wv, const FusedArgs& F, double* lle, 
                             double* amps, double* ivls, hipStream_t st);
 hipError_t launch_clamp(int n, int* v, int clamp, hipStream_t st);
 hipError_t launch_fill(long long n, int* v, int value, hipStream_t st);
+hipError_t launch_gather(int n, const int* idx, const int* src, int* dst, hipStream_t st);
 hipError_t launch_lle(int model, int precision, const LyapArgs& A, bool check, double* out,
                       hipStream_t st);
 hipError_t launch_ls(int model, const LyapArgs& A, bool check, double* out, hipStream_t st);
@@ -641,19 +642,24 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
 // Cells whose peaks did not fit their rows (entries pos = d_ovf[0..novf)
 // of a listed or ranged batch): integrate them again with rows that hold
 // every peak and cluster exactly, into d_peak[pos].
-int recompute_overflow(const PeakMapSpec& S, int maxPeaks, double T, const int* h_list, int base,
-                       int novf, const int* d_ovf, int* d_peak)
+int recompute_overflow(const PeakMapSpec& S, int maxPeaks, double T, const int* h_list,
+                       const int* d_list, int base, int novf, const int* d_ovf, int* d_peak)
 {
     if (novf <= 0) return BIFHIP_OK;
     std::vector<int> pos(novf), flat(novf);
     HIP_TRY(hipMemcpy(pos.data(), d_ovf, sizeof(int) * novf, hipMemcpyDeviceToHost));
-    for (int i = 0; i < novf; ++i) flat[i] = h_list ? h_list[pos[i]] : base + pos[i];
     const int cap2 = (int)std::max(2LL, std::min((long long)S.N / 2 + 1, (long long)maxPeaks + 1));
     DBuf dflat, dpos, cnt, am, iv;
     HIP_TRY(dflat.alloc(sizeof(int) * novf));
     HIP_TRY(dpos.alloc(sizeof(int) * novf));
-    HIP_TRY(hipMemcpy(dflat.p, flat.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dpos.p, pos.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
+    if (!h_list && d_list) {
+        // cell indices of the overflowing entries, gathered from the device list
+        HIP_TRY(launch_gather(novf, dpos.as<int>(), d_list, dflat.as<int>(), g.stream));
+    } else {
+        for (int i = 0; i < novf; ++i) flat[i] = h_list ? h_list[pos[i]] : base + pos[i];
+        HIP_TRY(hipMemcpy(dflat.p, flat.data(), sizeof(int) * novf, hipMemcpyHostToDevice));
+    }
     HIP_TRY(cnt.alloc(sizeof(int) * novf));
     HIP_TRY(am.alloc(sizeof(double) * (size_t)novf * cap2));
     HIP_TRY(iv.alloc(sizeof(double) * (size_t)novf * cap2));
@@ -712,8 +718,13 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
             // directions of the listed cells (their flattened indices)
             hdirs.resize((size_t)m * 3);
             std::vector<double> one;
+            std::vector<int> seg;
+            if (!h_list && d_list) {
+                seg.resize((size_t)m);
+                HIP_TRY(hipMemcpy(seg.data(), d_list + off, sizeof(int) * m, hipMemcpyDeviceToHost));
+            }
             for (int i = 0; i < m; ++i) {
-                const long long gi = h_list ? h_list[off + i] : base + off + i;
+                const long long gi = h_list ? h_list[off + i] : (d_list ? seg[i] : base + off + i);
                 directions(false, gi, 1, dirChunk, one);
                 std::memcpy(hdirs.data() + (size_t)i * 3, one.data(), sizeof(double) * 3);
             }
@@ -736,7 +747,7 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
     int novf = 0;
     HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
-    RC_TRY(recompute_overflow(S, maxPeaks, T, h_list, base, novf, d_ovf, d_peak));
+    RC_TRY(recompute_overflow(S, maxPeaks, T, h_list, d_list, base, novf, d_ovf, d_peak));
     if (g.clusterClamp > 0) HIP_TRY(launch_clamp(n, d_peak, g.clusterClamp, g.stream));
     return BIFHIP_OK;
 }
@@ -775,7 +786,7 @@ int peak_cells_device(const PeakMapSpec& S, int maxPeaks, double epsDb, const in
     int novf = 0;
     HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
-    RC_TRY(recompute_overflow(S, maxPeaks, T, h_list, base, novf, d_ovf, d_peak));
+    RC_TRY(recompute_overflow(S, maxPeaks, T, h_list, d_list, base, novf, d_ovf, d_peak));
     if (g.clusterClamp > 0) HIP_TRY(launch_clamp(n, d_peak, g.clusterClamp, g.stream));
     return BIFHIP_OK;
 }

@@ -2,9 +2,10 @@
 //
 // 1. coarse map: the fused FP64 kernel on an nCoarse x nCoarse lattice;
 // 2. training: 64 Fourier features -> 3 x 128 ReLU -> head (LLE + 16
-//    peak-count classes), BF16 MFMA products with FP32 accumulation
-//    (k_mlp.hip), Adam on FP32 master weights; one epoch is captured as a
-//    HIP graph and replayed;
+//    peak-count classes + chaos logit), BF16 MFMA products with FP32
+//    accumulation; a step is one fused forward/backward launch
+//    (k_train_fused, gradient partials per 64 rows) and one Adam launch on
+//    FP32 master weights; one epoch is captured as a HIP graph and replayed;
 // 3. inference of the nFine x nFine lattice in chunks;
 // 4. cells with low confidence or a neighbour of another class / LLE regime
 //    are re-solved with the plain FP64 fused kernel, which also scores the
@@ -27,22 +28,23 @@
 namespace bifhip {
 typedef __bf16 bf16;
 hipError_t launch_gemm(int epi, int M, int N, int K, const bf16* A, long long sAm, long long sAk,
-                       const bf16* B, long long sBk, long long sBn, const float* bias,
-                       const bf16* mask, long long sMm, void* C, long long sCm, hipStream_t st,
-                       int splits = 1, long long splitStride = -1);
+                       const bf16* B, long long sBk, long long sBn, const float* bias, void* C,
+                       long long sCm, hipStream_t st);
 hipError_t launch_sum_splits(int n, int splits, const float* part, float* out, hipStream_t st);
 hipError_t launch_features(int rows, int n, const int* list, long long base, const float* Bff,
                            bf16* X, hipStream_t st);
-hipError_t launch_loss_grad(int rows, const int* list, const float* out, const int* cls,
-                            const float* lle, float wLle, float chaosStd, bf16* dOut, float* loss,
-                            hipStream_t st);
-hipError_t launch_bias_partials(int rows, int splits, const bf16* const g[4], const int cols[4],
-                                const int off[4], float* gpart, long long stride, int* step,
-                                hipStream_t st);
+int train_fused_blocks(int rows);
+hipError_t launch_train_fused(int rows, int nLat, const int* list, const int* tlist,
+                              const float* Bff, const bf16* wbf, const bf16* wbfT, const float* p,
+                              const int* cls, const float* lle, float wLle, float chaosStd,
+                              float* gpart, long long stride, float* loss, int* step,
+                              hipStream_t st);
 hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, int splits,
                        long long stride, const unsigned char* keep, const int* bfIndex, bf16* wbf,
-                       float lr, float b1, float b2, float eps, const int* step, hipStream_t st);
+                       const int* bfIndexT, bf16* wbfT, float lr, float b1, float b2, float eps,
+                       const int* step, hipStream_t st);
 hipError_t launch_fill_f32(int n, float* p, float v, hipStream_t st);
+hipError_t launch_fill(long long n, int* v, int value, hipStream_t st);
 hipError_t launch_fill_f64(int n, double* p, double v, hipStream_t st);
 hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, float chaosThr,
                              unsigned char* cls, unsigned char* chaos, float* conf,
@@ -50,7 +52,9 @@ hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, f
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
                        float chaosConfThr, bool partial, int* listFull, int* countFull,
-                       int* listPeak, int* countPeak, unsigned char* flag, hipStream_t st);
+                       int* listPeak, int* countPeak, unsigned long long vseed,
+                       unsigned int vthr, int* listVal, int* countVal, unsigned char* flag,
+                       hipStream_t st);
 hipError_t launch_scatter(int n, const int* list, const int* peak, const double* lle,
                           int* peakMap, double* lleMap, hipStream_t st);
 hipError_t launch_fill_pred(long long n, const unsigned char* cls, const float* lle,
@@ -81,10 +85,9 @@ namespace {
 
 constexpr int kFeat = 64, kHid = 128, kHead = 32, kClasses = 16;
 constexpr int kLiveHead = 2 + kClasses;   // LLE, 16 classes, chaos logit
-constexpr int kMaxSplits = 32;
-
-// K-splits of the weight-gradient products over a batch of `rows`
-int splits_for(int rows) { return std::max(1, std::min(kMaxSplits, rows / 256)); }
+// gradient partials: one per 64 rows of a batch (k_train_fused), so batches
+// of up to 64 * kMaxSplits rows
+constexpr int kMaxSplits = 64;
 
 // flat FP32 parameter layout (row-major [out][in] matrices)
 constexpr int oW1 = 0, oB1 = oW1 + kHid * kFeat;
@@ -92,16 +95,19 @@ constexpr int oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid;
 constexpr int oW3 = oB2 + kHid, oB3 = oW3 + kHid * kHid;
 constexpr int oWh = oB3 + kHid, oBh = oWh + kHead * kHid;
 constexpr int kParams = oBh + kHead;
-// BF16 weight copies
+// BF16 weight copies, and transposed copies of W2, W3 and Wh for the
+// backward products
 constexpr int bW1 = 0, bW2 = bW1 + kHid * kFeat, bW3 = bW2 + kHid * kHid,
               bWh = bW3 + kHid * kHid, kBf = bWh + kHead * kHid;
+constexpr int tW2 = 0, tW3 = tW2 + kHid * kHid, tWh = tW3 + kHid * kHid,
+              kBfT = tWh + kHid * kHead;
 
 struct Net {
-    // gpart[z * kParams + i]: split-z partial of the gradient of parameter i
+    // gpart[z * kParams + i]: partial z of the gradient of parameter i
     float *p, *m, *v, *g, *bff, *loss, *gpart;
-    bf16* wbf;
+    bf16 *wbf, *wbfT;
     unsigned char* keep;
-    int *bfIndex, *step;
+    int *bfIndex, *bfIndexT, *step;
 };
 
 int net_alloc(Net& N)
@@ -114,9 +120,48 @@ int net_alloc(Net& N)
     RC_TRY(pool_get("mlp_bff", kFeat, &N.bff));
     RC_TRY(pool_get("mlp_loss", 3, &N.loss));
     RC_TRY(pool_get("mlp_wbf", kBf, &N.wbf));
+    RC_TRY(pool_get("mlp_wbft", kBfT, &N.wbfT));
     RC_TRY(pool_get("mlp_keep", kParams, &N.keep));
     RC_TRY(pool_get("mlp_bfidx", kParams, &N.bfIndex));
+    RC_TRY(pool_get("mlp_bfidxt", kParams, &N.bfIndexT));
     RC_TRY(pool_get("mlp_step", 1, &N.step));
+    return BIFHIP_OK;
+}
+
+// Position of each parameter in the BF16 weight copies (-1: a bias) and in
+// the transposed copies (-1: none).
+void bf16_maps(std::vector<int>& bfi, std::vector<int>& bfiT)
+{
+    bfi.assign(kParams, -1);
+    bfiT.assign(kParams, -1);
+    auto map = [&](int off, int rows, int cols, int bfoff, int toff) {
+        for (int r = 0; r < rows; ++r)
+            for (int c = 0; c < cols; ++c) {
+                bfi[off + r * cols + c] = bfoff + r * cols + c;
+                if (toff >= 0) bfiT[off + r * cols + c] = toff + c * rows + r;
+            }
+    };
+    map(oW1, kHid, kFeat, bW1, -1);
+    map(oW2, kHid, kHid, bW2, tW2);
+    map(oW3, kHid, kHid, bW3, tW3);
+    map(oWh, kHead, kHid, bWh, tWh);
+}
+
+// BF16 copies (plain and transposed) of the FP32 parameters p, with the
+// index maps that k_adam refreshes them through
+int upload_weights(const Net& N, const float* p)
+{
+    std::vector<int> bfi, bfiT;
+    bf16_maps(bfi, bfiT);
+    std::vector<bf16> wbf(kBf), wbfT(kBfT);
+    for (int i = 0; i < kParams; ++i) {
+        if (bfi[i] >= 0) wbf[bfi[i]] = (bf16)p[i];
+        if (bfiT[i] >= 0) wbfT[bfiT[i]] = (bf16)p[i];
+    }
+    HIP_TRY(hipMemcpy(N.wbf, wbf.data(), sizeof(bf16) * kBf, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(N.wbfT, wbfT.data(), sizeof(bf16) * kBfT, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(N.bfIndex, bfi.data(), sizeof(int) * kParams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(N.bfIndexT, bfiT.data(), sizeof(int) * kParams, hipMemcpyHostToDevice));
     return BIFHIP_OK;
 }
 
@@ -128,21 +173,19 @@ int net_init(Net& N, unsigned long long seed, float sigma)
     auto u01 = [&]() { return (double)(rng() >> 11) * (1.0 / 9007199254740992.0); };
     std::vector<float> p(kParams, 0.0f);
     std::vector<unsigned char> keep(kParams, 1);
-    std::vector<int> bfi(kParams, -1);
-    auto fill = [&](int off, int rows, int cols, int bfoff, double scale, int liveRows) {
+    auto fill = [&](int off, int rows, int cols, double scale, int liveRows) {
         const double lim = scale * std::sqrt(6.0 / cols);
         for (int r = 0; r < rows; ++r)
             for (int c = 0; c < cols; ++c) {
                 const int i = off + r * cols + c;
                 p[i] = r < liveRows ? (float)((2 * u01() - 1) * lim) : 0.0f;
                 keep[i] = r < liveRows;
-                bfi[i] = bfoff + r * cols + c;
             }
     };
-    fill(oW1, kHid, kFeat, bW1, 1.0, kHid);
-    fill(oW2, kHid, kHid, bW2, 1.0, kHid);
-    fill(oW3, kHid, kHid, bW3, 1.0, kHid);
-    fill(oWh, kHead, kHid, bWh, 0.5, kLiveHead);
+    fill(oW1, kHid, kFeat, 1.0, kHid);
+    fill(oW2, kHid, kHid, 1.0, kHid);
+    fill(oW3, kHid, kHid, 1.0, kHid);
+    fill(oWh, kHead, kHid, 0.5, kLiveHead);
     for (int i = kLiveHead; i < kHead; ++i) keep[oBh + i] = 0;
     std::vector<float> bff(kFeat);
     for (int i = 0; i < kFeat; i += 2) {
@@ -151,23 +194,19 @@ int net_init(Net& N, unsigned long long seed, float sigma)
         bff[i] = (float)(sigma * r * std::cos(th));
         bff[i + 1] = (float)(sigma * r * std::sin(th));
     }
-    std::vector<bf16> wbf(kBf);
-    for (int i = 0; i < kParams; ++i)
-        if (bfi[i] >= 0) wbf[bfi[i]] = (bf16)p[i];
     HIP_TRY(hipMemcpy(N.p, p.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(N.m, 0, sizeof(float) * kParams));
     HIP_TRY(hipMemset(N.v, 0, sizeof(float) * kParams));
     HIP_TRY(hipMemcpy(N.bff, bff.data(), sizeof(float) * kFeat, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(N.wbf, wbf.data(), sizeof(bf16) * kBf, hipMemcpyHostToDevice));
+    RC_TRY(upload_weights(N, p.data()));
     HIP_TRY(hipMemcpy(N.keep, keep.data(), kParams, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(N.bfIndex, bfi.data(), sizeof(int) * kParams, hipMemcpyHostToDevice));
     const int zero = 0;
     HIP_TRY(hipMemcpy(N.step, &zero, sizeof(int), hipMemcpyHostToDevice));
     return BIFHIP_OK;
 }
 
 struct Acts {
-    bf16 *x0, *h1, *h2, *h3, *dout, *d3, *d2, *d1;
+    bf16 *x0, *h1, *h2, *h3;
     float* out;
 };
 
@@ -179,16 +218,6 @@ int acts_alloc(Acts& A, int rows, const char* tag)
     RC_TRY(pool_get((t + "h2").c_str(), (size_t)rows * kHid, &A.h2));
     RC_TRY(pool_get((t + "h3").c_str(), (size_t)rows * kHid, &A.h3));
     RC_TRY(pool_get((t + "out").c_str(), (size_t)rows * kHead, &A.out));
-    A.dout = A.d3 = A.d2 = A.d1 = nullptr;
-    return BIFHIP_OK;
-}
-
-int acts_alloc_grad(Acts& A, int rows)
-{
-    RC_TRY(pool_get("tr_dout", (size_t)rows * kHead, &A.dout));
-    RC_TRY(pool_get("tr_d3", (size_t)rows * kHid, &A.d3));
-    RC_TRY(pool_get("tr_d2", (size_t)rows * kHid, &A.d2));
-    RC_TRY(pool_get("tr_d1", (size_t)rows * kHid, &A.d1));
     return BIFHIP_OK;
 }
 
@@ -198,51 +227,13 @@ int forward(const Net& N, Acts& A, int rows, int nLat, const int* list, long lon
 {
     HIP_TRY(launch_features(rows, nLat, list, base, N.bff, A.x0, s));
     HIP_TRY(launch_gemm(0, rows, kHid, kFeat, A.x0, kFeat, 1, N.wbf + bW1, 1, kFeat, N.p + oB1,
-                        nullptr, 0, A.h1, kHid, s));
+                        A.h1, kHid, s));
     HIP_TRY(launch_gemm(0, rows, kHid, kHid, A.h1, kHid, 1, N.wbf + bW2, 1, kHid, N.p + oB2,
-                        nullptr, 0, A.h2, kHid, s));
+                        A.h2, kHid, s));
     HIP_TRY(launch_gemm(0, rows, kHid, kHid, A.h2, kHid, 1, N.wbf + bW3, 1, kHid, N.p + oB3,
-                        nullptr, 0, A.h3, kHid, s));
+                        A.h3, kHid, s));
     HIP_TRY(launch_gemm(1, rows, kHead, kHid, A.h3, kHid, 1, N.wbf + bWh, 1, kHid, N.p + oBh,
-                        nullptr, 0, A.out, kHead, s));
-    return BIFHIP_OK;
-}
-
-// backward of the loss gradient A.dout into the split partials N.gpart
-// (summed in split order by k_adam, or by launch_sum_splits into N.g).
-// Weight gradient dW[M x Nn] = dZ^T X over the batch: split-K over the rows,
-// split z written at N.gpart + z * kParams + off.
-int weight_grad(const Net& N, int M, int Nn, int rows, const bf16* dz, int ldz, const bf16* x,
-                int ldx, int off, hipStream_t s)
-{
-    HIP_TRY(launch_gemm(3, M, Nn, rows, dz, 1, ldz, x, ldx, 1, nullptr, nullptr, 0,
-                        N.gpart + off, Nn, s, splits_for(rows), kParams));
-    return BIFHIP_OK;
-}
-
-// step: the optimiser step counter to advance (training), or null
-int backward(const Net& N, Acts& A, int rows, int* step, hipStream_t s)
-{
-    // head: dWh = dOut^T H3, dH3 = (dOut Wh) * (H3 > 0)
-    RC_TRY(weight_grad(N, kHead, kHid, rows, A.dout, kHead, A.h3, kHid, oWh, s));
-    HIP_TRY(launch_gemm(2, rows, kHid, kHead, A.dout, kHead, 1, N.wbf + bWh, kHid, 1, nullptr,
-                        A.h3, kHid, A.d3, kHid, s));
-    // layer 3
-    RC_TRY(weight_grad(N, kHid, kHid, rows, A.d3, kHid, A.h2, kHid, oW3, s));
-    HIP_TRY(launch_gemm(2, rows, kHid, kHid, A.d3, kHid, 1, N.wbf + bW3, kHid, 1, nullptr, A.h2,
-                        kHid, A.d2, kHid, s));
-    // layer 2
-    RC_TRY(weight_grad(N, kHid, kHid, rows, A.d2, kHid, A.h1, kHid, oW2, s));
-    HIP_TRY(launch_gemm(2, rows, kHid, kHid, A.d2, kHid, 1, N.wbf + bW2, kHid, 1, nullptr, A.h1,
-                        kHid, A.d1, kHid, s));
-    // layer 1 (no input gradient needed)
-    RC_TRY(weight_grad(N, kHid, kFeat, rows, A.d1, kHid, A.x0, kFeat, oW1, s));
-    // the four bias gradients (column sums of dOut, dH3, dH2, dH1), one launch
-    const bf16* const gs[4] = {A.dout, A.d3, A.d2, A.d1};
-    const int cols[4] = {kHead, kHid, kHid, kHid};
-    const int offs[4] = {oBh, oB3, oB2, oB1};
-    HIP_TRY(launch_bias_partials(rows, splits_for(rows), gs, cols, offs, N.gpart, kParams, step,
-                                 s));
+                        A.out, kHead, s));
     return BIFHIP_OK;
 }
 
@@ -253,16 +244,17 @@ struct TrainCfg {
     bool graph = true;
 };
 
-// one optimisation step on rows of the permuted sample list
-int train_step(const Net& N, Acts& A, const TrainCfg& T, int nLat, const int* list,
-               const int* cls, const float* lle, hipStream_t s)
+// one optimisation step on rows of the permuted sample list: the fused
+// forward/backward launch (gradient partial per 64 rows, step counter
+// advanced) and Adam over the partials in block order
+int train_step(const Net& N, const TrainCfg& T, int nLat, const int* list, const int* cls,
+               const float* lle, hipStream_t s)
 {
-    RC_TRY(forward(N, A, T.batch, nLat, list, 0, s));
-    HIP_TRY(launch_loss_grad(T.batch, list, A.out, cls, lle, T.wLle, T.chaosStd, A.dout, N.loss,
-                             s));
-    RC_TRY(backward(N, A, T.batch, N.step, s));
-    HIP_TRY(launch_adam(kParams, N.p, N.m, N.v, N.gpart, splits_for(T.batch), kParams, N.keep,
-                        N.bfIndex, N.wbf, T.lr, 0.9f, 0.999f, 1e-8f, N.step, s));
+    HIP_TRY(launch_train_fused(T.batch, nLat, list, nullptr, N.bff, N.wbf, N.wbfT, N.p, cls, lle,
+                               T.wLle, T.chaosStd, N.gpart, kParams, N.loss, N.step, s));
+    HIP_TRY(launch_adam(kParams, N.p, N.m, N.v, N.gpart, train_fused_blocks(T.batch), kParams,
+                        N.keep, N.bfIndex, N.wbf, N.bfIndexT, N.wbfT, T.lr, 0.9f, 0.999f, 1e-8f,
+                        N.step, s));
     return BIFHIP_OK;
 }
 
@@ -314,12 +306,11 @@ int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int 
     const int n = nLat * nLat;
     if (nLat < 2 || batch < 16 || batch > n || epochs < 0 || !d_cls || !d_lle)
         return fail(BIFHIP_E_ARG, "bad training arguments");
+    if (train_fused_blocks(batch) > kMaxSplits)
+        return fail(BIFHIP_E_ARG, "batch %d above %d rows", batch, 64 * kMaxSplits);
     Net N;
     RC_TRY(net_alloc(N));
     RC_TRY(net_init(N, seed, (float)fourierSigma));
-    Acts A;
-    RC_TRY(acts_alloc(A, batch, "tr_"));
-    RC_TRY(acts_alloc_grad(A, batch));
     int* d_perm;
     const int steps = n / batch;
     RC_TRY(pool_get("tr_perm", (size_t)steps * batch, &d_perm));
@@ -354,7 +345,7 @@ int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int 
                 int crc = BIFHIP_OK;
                 if (launch_fill_f32(3, N.loss, 0.0f, cs) != hipSuccess) crc = fail(1, "fill");
                 for (int k = 0; k < steps && crc == BIFHIP_OK; ++k)
-                    crc = train_step(N, A, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, cs);
+                    crc = train_step(N, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, cs);
                 hipError_t ce = hipStreamEndCapture(cs, &graph);
                 if (crc != BIFHIP_OK) { rc = crc; break; }
                 if (ce != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
@@ -371,7 +362,7 @@ int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int 
             Timed t("mlp_train", s);
             HIP_TRY(launch_fill_f32(3, N.loss, 0.0f, s));
             for (int k = 0; k < steps && rc == BIFHIP_OK; ++k)
-                rc = train_step(N, A, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, s);
+                rc = train_step(N, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, s);
         }
     }
     if (cs) {
@@ -419,7 +410,9 @@ int bifhip_mlp_get_params(float* params, int nParams, float* fourier, int nFouri
 
 // One forward + backward pass of given parameters on given lattice cells;
 // gradients (same layout as the parameters) and the three losses back to
-// the host.  Used to check the MFMA kernels against a CPU reference.
+// the host.  Used to check the MFMA kernels against a CPU reference: the
+// gradient and losses come from the fused training kernel (targets by row),
+// the head outputs from the inference forward pass.
 int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, const int* cells,
                         const int* cls, const float* lle, int rows, double wLle, double chaosStd,
                         float* grads, float* losses, float* outputs)
@@ -427,25 +420,13 @@ int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, con
     RC_TRY(ensure_device());
     if (rows < 1 || !params || !fourier || !cells || !cls || !lle || !grads)
         return fail(BIFHIP_E_ARG, "bad gradient arguments");
+    if (train_fused_blocks(rows) > kMaxSplits)
+        return fail(BIFHIP_E_ARG, "rows %d above %d", rows, 64 * kMaxSplits);
     Net N;
     RC_TRY(net_alloc(N));
-    std::vector<int> bfi(kParams, -1);
-    std::vector<bf16> wbf(kBf);
-    auto idx = [&](int off, int rows_, int cols, int bfoff) {
-        for (int i = 0; i < rows_ * cols; ++i) bfi[off + i] = bfoff + i;
-    };
-    idx(oW1, kHid, kFeat, bW1);
-    idx(oW2, kHid, kHid, bW2);
-    idx(oW3, kHid, kHid, bW3);
-    idx(oWh, kHead, kHid, bWh);
-    for (int i = 0; i < kParams; ++i)
-        if (bfi[i] >= 0) wbf[bfi[i]] = (bf16)params[i];
     HIP_TRY(hipMemcpy(N.p, params, sizeof(float) * kParams, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(N.bff, fourier, sizeof(float) * kFeat, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(N.wbf, wbf.data(), sizeof(bf16) * kBf, hipMemcpyHostToDevice));
-    Acts A;
-    RC_TRY(acts_alloc(A, rows, "gr_"));
-    RC_TRY(acts_alloc_grad(A, rows));
+    RC_TRY(upload_weights(N, params));
     DBuf dc, dcls, dl, seq;
     HIP_TRY(dc.alloc(sizeof(int) * rows));
     HIP_TRY(dcls.alloc(sizeof(int) * rows));
@@ -459,16 +440,20 @@ int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, con
     HIP_TRY(hipMemcpy(seq.p, iota.data(), sizeof(int) * rows, hipMemcpyHostToDevice));
     hipStream_t s = g.stream;
     HIP_TRY(launch_fill_f32(3, N.loss, 0.0f, s));
-    RC_TRY(forward(N, A, rows, nLat, dc.as<int>(), 0, s));
-    HIP_TRY(launch_loss_grad(rows, seq.as<int>(), A.out, dcls.as<int>(), dl.as<float>(),
-                             (float)wLle, (float)chaosStd, A.dout, N.loss, s));
-    RC_TRY(backward(N, A, rows, nullptr, s));
-    HIP_TRY(launch_sum_splits(kParams, splits_for(rows), N.gpart, N.g, s));
+    HIP_TRY(launch_train_fused(rows, nLat, dc.as<int>(), seq.as<int>(), N.bff, N.wbf, N.wbfT, N.p,
+                               dcls.as<int>(), dl.as<float>(), (float)wLle, (float)chaosStd,
+                               N.gpart, kParams, N.loss, nullptr, s));
+    HIP_TRY(launch_sum_splits(kParams, train_fused_blocks(rows), N.gpart, N.g, s));
+    if (outputs) {
+        Acts A;
+        RC_TRY(acts_alloc(A, rows, "gr_"));
+        RC_TRY(forward(N, A, rows, nLat, dc.as<int>(), 0, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(outputs, A.out, sizeof(float) * rows * kHead, hipMemcpyDeviceToHost));
+    }
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipMemcpy(grads, N.g, sizeof(float) * kParams, hipMemcpyDeviceToHost));
     if (losses) HIP_TRY(hipMemcpy(losses, N.loss, sizeof(float) * 3, hipMemcpyDeviceToHost));
-    if (outputs)
-        HIP_TRY(hipMemcpy(outputs, A.out, sizeof(float) * rows * kHead, hipMemcpyDeviceToHost));
     return BIFHIP_OK;
 }
 
@@ -558,51 +543,29 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
 
     // 4. flag and re-solve
     const auto t3 = std::chrono::steady_clock::now();
-    int *d_list, *d_plist, *d_count;
-    RC_TRY(pool_get("sg_list", (size_t)nf, &d_list));
+    // full re-solves first, then the validation sample (cells drawn on the
+    // device among the unflagged ones, a seeded hash per cell), in one list
+    int *d_all, *d_vlist, *d_plist, *d_count;
+    RC_TRY(pool_get("sg_all", (size_t)nf, &d_all));
+    RC_TRY(pool_get("sg_vlist", (size_t)(validationFraction > 0 ? nf : 1), &d_vlist));
     RC_TRY(pool_get("sg_plist", (size_t)(partial ? nf : 1), &d_plist));
-    RC_TRY(pool_get("sg_count", 2, &d_count));
-    HIP_TRY(hipMemsetAsync(d_count, 0, 2 * sizeof(int), s));
+    RC_TRY(pool_get("sg_count", 3, &d_count));
+    HIP_TRY(launch_fill(3, d_count, 0, s));
+    const double vf = std::min(1.0, std::max(0.0, validationFraction));
+    const unsigned int vthr =
+        vf > 0 ? (unsigned int)std::min(4294967295.0, std::floor(vf * 4294967296.0)) : 0u;
     HIP_TRY(launch_flag(nFine, d_pcls, d_pch, d_conf, d_cconf, (float)confThreshold,
-                        (float)g.chaosConfidence, partial, d_list, d_count, d_plist, d_count + 1,
-                        d_flag, s));
+                        (float)g.chaosConfidence, partial, d_all, d_count, d_plist, d_count + 1,
+                        seed + 17, vthr, d_vlist, d_count + 2, d_flag, s));
     HIP_TRY(launch_fill_pred(nf, d_pcls, d_plle, d_peak, d_lle, s));
-    int counts[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(counts, d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    int counts[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(counts, d_count, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    const int nflag = counts[0], npeak = counts[1];
-    // validation sample of the unflagged cells (seeded)
-    std::vector<int> vlist;
-    std::vector<unsigned char> flag;
-    if (validationFraction > 0) {
-        flag.resize((size_t)nf);
-        HIP_TRY(hipMemcpy(flag.data(), d_flag, (size_t)nf, hipMemcpyDeviceToHost));
-        // draw distinct unflagged cells until the sample holds
-        // fraction * (unflagged cells) of them
-        std::mt19937_64 rv(seed + 17);
-        const long long want = (long long)std::llround(std::min(1.0, validationFraction) *
-                                                       (double)(nf - nflag - npeak));
-        long long tries = 0;
-        while ((long long)vlist.size() < want && tries < 8 * want + 64) {
-            ++tries;
-            const long long i = (long long)(rv() % (uint64_t)nf);
-            if (flag[i]) continue;
-            flag[i] = 3;   // taken
-            vlist.push_back((int)i);
-        }
-    }
-    const int nval = (int)vlist.size();
+    const int nflag = counts[0], npeak = counts[1], nval = counts[2];
     const int ntot = nflag + nval;
-    int* d_all;
-    RC_TRY(pool_get("sg_all", (size_t)std::max(1, ntot), &d_all));
-    HIP_TRY(hipMemcpy(d_all, d_list, sizeof(int) * nflag, hipMemcpyDeviceToDevice));
     if (nval)
-        HIP_TRY(hipMemcpy(d_all + nflag, vlist.data(), sizeof(int) * nval,
-                          hipMemcpyHostToDevice));
-    std::vector<int> hall(ntot), hpeak(npeak);
-    HIP_TRY(hipMemcpy(hall.data(), d_all, sizeof(int) * ntot, hipMemcpyDeviceToHost));
-    if (npeak)
-        HIP_TRY(hipMemcpy(hpeak.data(), d_plist, sizeof(int) * npeak, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpyAsync(d_all + nflag, d_vlist, sizeof(int) * nval,
+                               hipMemcpyDeviceToDevice, s));
     int *d_rpeak, *d_ppeak;
     double* d_rlle;
     RC_TRY(pool_get("sg_rpeak", (size_t)std::max(1, ntot), &d_rpeak));
@@ -614,11 +577,11 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     if (ntot > 0)
         RC_TRY(surrogate_resolve(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
                                  peakTransient, wv, maxPeaks, dbEps, lleTMax, NT, lleTransient,
-                                 lleEps, hall.data(), d_all, ntot, d_rlle, d_rpeak));
+                                 lleEps, nullptr, d_all, ntot, d_rlle, d_rpeak));
     if (npeak > 0)
         RC_TRY(surrogate_resolve_peaks(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
-                                       peakTransient, wv, maxPeaks, dbEps, hpeak.data(),
-                                       d_plist, npeak, d_ppeak));
+                                       peakTransient, wv, maxPeaks, dbEps, nullptr, d_plist,
+                                       npeak, d_ppeak));
     HIP_TRY(hipStreamSynchronize(s));
     st[BIFHIP_SSTAT_MS_SOLVE] = ms_since(t4);
     const auto t5 = std::chrono::steady_clock::now();
@@ -645,13 +608,9 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     st[BIFHIP_SSTAT_PEAK_ONLY_CLASS_ACC] = acc[10] > 0 ? acc[11] / acc[10] : 1.0;
     HIP_TRY(launch_scatter(ntot, d_all, d_rpeak, d_rlle, d_peak, d_lle, s));
     HIP_TRY(launch_scatter(npeak, d_plist, d_ppeak, nullptr, d_peak, d_lle, s));
-    if (d_resolved) {
-        // 1 re-solved in full, 2 peak count only, 3 validation sample, 0 predicted
-        if (!flag.empty())
-            HIP_TRY(hipMemcpyAsync(d_resolved, flag.data(), (size_t)nf, hipMemcpyHostToDevice, s));
-        else
-            HIP_TRY(hipMemcpyAsync(d_resolved, d_flag, (size_t)nf, hipMemcpyDeviceToDevice, s));
-    }
+    // 1 re-solved in full, 2 peak count only, 3 validation sample, 0 predicted
+    if (d_resolved)
+        HIP_TRY(hipMemcpyAsync(d_resolved, d_flag, (size_t)nf, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     st[BIFHIP_SSTAT_MS_RESOLVE] = ms_since(t3);
     st[BIFHIP_SSTAT_MS_SCORE] = ms_since(t5);

@@ -1016,6 +1016,22 @@ hipError_t launch_clamp(int n, int* v, int clamp, hipStream_t st)
     return hipGetLastError();
 }
 
+// dst[i] = src[idx[i]] (cell indices of listed entries)
+__global__ void k_gather_i32(int n, const int* __restrict__ idx, const int* __restrict__ src,
+                             int* __restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+hipError_t launch_gather(int n, const int* idx, const int* src, int* dst, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, idx,
+                       src, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(long long n, int* v, int value, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;

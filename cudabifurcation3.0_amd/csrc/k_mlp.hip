@@ -3,11 +3,12 @@
 // The surrogate maps a parameter point (a, c) to the LLE and the peak-count
 // class of the cell.  Network: 64 Fourier features -> 3 x 128 ReLU -> one
 // 32-wide head (column 0: standardised LLE, columns 1..16: class logits,
-// the rest zero).  Every product -- forward, input gradient and weight
-// gradient -- runs through k_gemm, a v_mfma_f32_16x16x32_bf16 kernel whose
-// operands are addressed with (row, k) strides, so transposed operands need
-// no copies.  Weights are kept in FP32 (Adam state) with BF16 copies for
-// the matrix cores.
+// column 17: chaos logit, the rest zero).  Inference runs layer by layer
+// through k_gemm, a v_mfma_f32_16x16x32_bf16 kernel with strided operands;
+// a training step is one k_train_fused launch (forward, loss gradient,
+// backward, gradient partials per workgroup) and one k_adam launch.
+// Weights are kept in FP32 (Adam state) with BF16 copies, and transposed
+// BF16 copies for the backward products, for the matrix cores.
 #include "bifhip_dev.h"
 
 namespace bifhip {
@@ -25,8 +26,6 @@ constexpr int kChaosCol = 1 + kClasses;   // chaos logit column
 enum Epi : int {
     kEpiReluBf16 = 0,   // out(bf16) = relu(acc + bias)
     kEpiBiasF32 = 1,    // out(f32)  = acc + bias
-    kEpiMaskBf16 = 2,   // out(bf16) = acc * (mask > 0)
-    kEpiF32 = 3,        // out(f32)  = acc
 };
 
 // C[M x N] = sum_k A(m, k) B(k, n) with A(m,k) = A[m*sAm + k*sAk],
@@ -36,16 +35,12 @@ enum Epi : int {
 //   A: row (lane & 15), k = 8 * (lane >> 4) + e
 //   B: col (lane & 15), k = 8 * (lane >> 4) + e
 //   D: col (lane & 15), row = 4 * (lane >> 4) + e   (e < 4)
-// Split-K: block z sums k in [z*kChunk, min(K, (z+1)*kChunk)) and writes its
-// partial to C + z * splitStride (kEpiF32 only); k_sum_splits adds them.
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* __restrict__ A,
                                               long long sAm, long long sAk,
                                               const bf16* __restrict__ B, long long sBk,
                                               long long sBn, const float* __restrict__ bias,
-                                              const bf16* __restrict__ mask, long long sMm,
-                                              void* __restrict__ Cbase, long long sCm,
-                                              int kChunk, long long splitStride)
+                                              void* __restrict__ C, long long sCm)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -53,10 +48,6 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* _
     const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
     const int li = lane & 15;
     const int kq = (lane >> 4) * 8;
-    const int kBegin = blockIdx.z * kChunk;
-    const int kEnd = min(K, kBegin + kChunk);
-    void* C = EPI == kEpiF32 ? (void*)(static_cast<float*>(Cbase) + blockIdx.z * splitStride)
-                             : Cbase;
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -65,31 +56,31 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* _
 
     const bool aVec = (sAk == 1) && (K % 8 == 0) && (sAm % 8 == 0);
     const bool bVec = (sBk == 1) && (K % 8 == 0) && (sBn % 8 == 0);
-    for (int k0 = kBegin; k0 < kEnd; k0 += 32) {
+    for (int k0 = 0; k0 < K; k0 += 32) {
         bf16x8 af[2], bfr[2];
         const int kb = k0 + kq;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const int r = m0 + 16 * t + li;
-            if (aVec && r < M && kb + 8 <= kEnd) {
+            if (aVec && r < M && kb + 8 <= K) {
                 af[t] = *reinterpret_cast<const bf16x8*>(A + (long long)r * sAm + kb);
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int k = kb + e;
-                    af[t][e] = (r < M && k < kEnd) ? A[(long long)r * sAm + (long long)k * sAk]
-                                                   : (bf16)0.0f;
+                    af[t][e] = (r < M && k < K) ? A[(long long)r * sAm + (long long)k * sAk]
+                                                : (bf16)0.0f;
                 }
             }
             const int c = n0 + 16 * t + li;
-            if (bVec && c < N && kb + 8 <= kEnd) {
+            if (bVec && c < N && kb + 8 <= K) {
                 bfr[t] = *reinterpret_cast<const bf16x8*>(B + (long long)c * sBn + kb);
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int k = kb + e;
-                    bfr[t][e] = (c < N && k < kEnd) ? B[(long long)k * sBk + (long long)c * sBn]
-                                                    : (bf16)0.0f;
+                    bfr[t][e] = (c < N && k < K) ? B[(long long)k * sBk + (long long)c * sBn]
+                                                 : (bf16)0.0f;
                 }
             }
         }
@@ -110,18 +101,11 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* _
                 const int r = m0 + 16 * i + (lane >> 4) * 4 + e;
                 if (r >= M) continue;
                 float v = acc[i][j][e];
-                if (EPI == kEpiReluBf16) {
-                    v += bias ? bias[c] : 0.0f;
+                v += bias ? bias[c] : 0.0f;
+                if (EPI == kEpiReluBf16)
                     static_cast<bf16*>(C)[(long long)r * sCm + c] = (bf16)(v > 0 ? v : 0.0f);
-                } else if (EPI == kEpiBiasF32) {
-                    v += bias ? bias[c] : 0.0f;
+                else
                     static_cast<float*>(C)[(long long)r * sCm + c] = v;
-                } else if (EPI == kEpiMaskBf16) {
-                    const float mk = (float)mask[(long long)r * sMm + c];
-                    static_cast<bf16*>(C)[(long long)r * sCm + c] = (bf16)(mk > 0 ? v : 0.0f);
-                } else {
-                    static_cast<float*>(C)[(long long)r * sCm + c] = v;
-                }
             }
         }
     }
@@ -154,29 +138,21 @@ __global__ void k_features(int rows, int n, const int* __restrict__ list, long l
 // Column 17 is the chaos logit: binary cross-entropy against
 // (standardised LLE > chaosStd), a non-finite LLE counting as not chaotic;
 // its loss goes to loss[2].
-__global__ void k_loss_grad(int rows, const int* __restrict__ list, const float* __restrict__ out,
-                            const int* __restrict__ clsAll, const float* __restrict__ lleAll,
-                            float wLle, float chaosStd, bf16* __restrict__ dOut,
-                            float* __restrict__ loss)
+__device__ __forceinline__ void loss_row(const float* o, int y, float target, float wLle,
+                                         float chaosStd, float inv, bf16 (&d)[kHead], float& ce,
+                                         float& mse, float& bce)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    const int src = list[r];
-    const int y = clsAll[src];
-    const float target = lleAll[src];
-    const float* o = out + (long long)r * kHead;
     float mx = o[1];
     for (int k = 2; k <= kClasses; ++k) mx = fmaxf(mx, o[k]);
     float se = 0;
     for (int k = 1; k <= kClasses; ++k) se += __expf(o[k] - mx);
-    const float inv = 1.0f / (float)rows;
-    bf16* d = dOut + (long long)r * kHead;
     for (int k = 1; k <= kClasses; ++k) {
         const float p = __expf(o[k] - mx) / se;
         d[k] = (bf16)((p - (k - 1 == y ? 1.0f : 0.0f)) * inv);
     }
-    float ce = -(o[1 + y] - mx - __logf(se));
-    float mse = 0, g0 = 0;
+    ce = -(o[1 + y] - mx - __logf(se));
+    mse = 0;
+    float g0 = 0;
     if (isfinite(target)) {
         const float e = o[0] - target;
         mse = e * e;
@@ -187,11 +163,8 @@ __global__ void k_loss_grad(int rows, const int* __restrict__ list, const float*
     const float tc = (target > chaosStd) ? 1.0f : 0.0f;     // NaN -> 0
     const float pc = 1.0f / (1.0f + __expf(-z));
     d[kChaosCol] = (bf16)((pc - tc) * inv);
-    const float bce = fmaxf(z, 0.0f) - z * tc + __logf(1.0f + __expf(-fabsf(z)));
+    bce = fmaxf(z, 0.0f) - z * tc + __logf(1.0f + __expf(-fabsf(z)));
     for (int k = kChaosCol + 1; k < kHead; ++k) d[k] = (bf16)0.0f;
-    atomicAdd(&loss[0], ce * inv);
-    atomicAdd(&loss[1], mse * inv);
-    atomicAdd(&loss[2], bce * inv);
 }
 
 // Sum of split partials in a fixed order: out[i] = sum_z part[z][i].
@@ -205,49 +178,14 @@ __global__ void k_sum_splits(int n, int splits, const float* __restrict__ part,
     out[i] = s;
 }
 
-// Split partials of the four bias gradients (column sums of the layers'
-// output gradients) in one launch: blockIdx.y picks the layer, blockIdx.x a
-// group of 64 columns, blockIdx.z the row chunk; the partial of split z goes
-// to gpart[z * stride + offset + column], next to the weight-gradient
-// partials, and k_adam adds them in split order.  Block (0,0,0) also
-// advances the optimiser's step counter (read by the following k_adam).
-struct BiasSrc {
-    const bf16* g[4];
-    int cols[4];
-    int off[4];
-};
-
-__global__ __launch_bounds__(256) void k_bias_partials(int rows, int rChunk, BiasSrc src,
-                                                       float* __restrict__ gpart,
-                                                       long long stride, int* __restrict__ step)
-{
-    __shared__ float s_part[4][64];
-    const int layer = blockIdx.y;
-    const int N = src.cols[layer];
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int rs = threadIdx.x >> 6;
-    const int r0 = blockIdx.z * rChunk, r1 = min(rows, r0 + rChunk);
-    float s = 0;
-    if (c < N) {
-        const bf16* g = src.g[layer];
-        for (int r = r0 + rs; r < r1; r += 4) s += (float)g[(long long)r * N + c];
-    }
-    s_part[rs][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (rs == 0 && c < N)
-        gpart[(long long)blockIdx.z * stride + src.off[layer] + c] =
-            ((s_part[0][c & 63] + s_part[1][c & 63]) + s_part[2][c & 63]) + s_part[3][c & 63];
-    if (step && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0)
-        *step += 1;
-}
-
 // Adam step on the FP32 master parameters with the gradient summed from its
 // split partials (gpart[z * stride + i], z < splits, fixed order); refreshes
 // the BF16 copies of the weight matrices and keeps padded head rows at zero.
 __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
                        const float* __restrict__ gpart, int splits, long long stride,
                        const unsigned char* __restrict__ keep, const int* __restrict__ bfIndex,
-                       bf16* __restrict__ wbf, float lr, float b1, float b2, float eps,
+                       bf16* __restrict__ wbf, const int* __restrict__ bfIndexT,
+                       bf16* __restrict__ wbfT, float lr, float b1, float b2, float eps,
                        const int* __restrict__ step)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,6 +208,318 @@ __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, floa
     }
     const int bi = bfIndex[i];
     if (bi >= 0) wbf[bi] = (bf16)p[i];
+    const int ti = bfIndexT ? bfIndexT[i] : -1;
+    if (ti >= 0) wbfT[ti] = (bf16)p[i];
+}
+
+// ------------------------------------------------------ fused training step
+//
+// One workgroup (4 waves) takes kRows rows of a batch through a whole
+// training step: Fourier features, the three ReLU layers and the head, the
+// loss gradient (loss_row), the backward pass, and the weight- and
+// bias-gradient sums over its rows, written as partial kb of the gradient
+// to gpart[kb * stride + parameter]; k_adam adds the partials in block
+// order.  Activations stay in LDS in two layouts, row-major [row][unit] and
+// transposed [unit][row] (rows padded by 8 against bank conflicts), so every
+// MFMA operand is one 16-byte read:
+//   forward   H^T  = W   X^T     A = W (BF16, global)    B = X   row-major
+//   backward  dX^T = W^T dY^T    A = W^T (BF16 copy)     B = dY  row-major
+//   weights   dW   = dY^T X      A = dY^T transposed     B = X^T transposed
+// Each D tile (4 consecutive units of one row per lane) is stored to both
+// layouts.  The forward products run the same MFMA sequence as k_gemm
+// (K ascending in steps of 32), so the activations equal the unfused path's.
+// Rows past `rows` enter with zero features and a zero loss gradient and
+// add nothing.
+constexpr int kRows = 64;
+constexpr int kTr = kRows + 8;          // transposed row length
+constexpr int kLdX = kFeat + 8;
+constexpr int kLdH = kHid + 8;
+constexpr int kLdO = kHead + 8;
+// FP32 parameter offsets (row-major [out][in] matrices, then the bias)
+constexpr int pW1 = 0, pB1 = pW1 + kHid * kFeat, pW2 = pB1 + kHid, pB2 = pW2 + kHid * kHid;
+constexpr int pW3 = pB2 + kHid, pB3 = pW3 + kHid * kHid, pWh = pB3 + kHid, pBh = pWh + kHead * kHid;
+// BF16 copies: wbf = W1 W2 W3 Wh; wbfT = W2^T W3^T Wh^T
+constexpr int qW1 = 0, qW2 = qW1 + kHid * kFeat, qW3 = qW2 + kHid * kHid, qWh = qW3 + kHid * kHid;
+constexpr int tW2 = 0, tW3 = tW2 + kHid * kHid, tWh = tW3 + kHid * kHid;
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 ld8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// D tile (units u0..u0+3 of row r, one lane) to both layouts
+__device__ __forceinline__ void store_tile(bf16* rm, int ld, bf16* tr, int u0, int r,
+                                           const float (&v)[4])
+{
+    bf16x4 q;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q[e] = (bf16)v[e];
+    *reinterpret_cast<bf16x4*>(rm + r * ld + u0) = q;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) tr[(u0 + e) * kTr + r] = q[e];
+}
+
+// hidden layer forward: out = relu(W in^T + b), W [kHid][K] (global BF16)
+template <int K>
+__device__ __forceinline__ void fwd_hidden(const bf16* __restrict__ W, const float* __restrict__ b,
+                                           const bf16* in, int ldIn, bf16* outRm, bf16* outTr,
+                                           int wave, int li, int kq, int rq)
+{
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8 a[2], x[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = ld8(W + ((2 * wave + i) * 16 + li) * K + k0 + kq);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = ld8(in + (j * 16 + li) * ldIn + k0 + kq);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], x[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int u0 = (2 * wave + i) * 16 + rq;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float t = acc[i][j][e] + b[u0 + e];
+                v[e] = t > 0 ? t : 0.0f;
+            }
+            store_tile(outRm, kLdH, outTr, u0, j * 16 + li, v);
+        }
+    }
+}
+
+// input gradient of a hidden layer, masked by the layer input's ReLU:
+// acc(k, r) = sum_m WT[k][m] dY[r][m] for this wave's 2 x 4 tiles
+template <int M>
+__device__ __forceinline__ void bwd_input(const bf16* __restrict__ WT, const bf16* dY, int ldY,
+                                          f32x4 (&acc)[2][4], int wave, int li, int kq)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int m0 = 0; m0 < M; m0 += 32) {
+        bf16x8 a[2], y[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = ld8(WT + ((2 * wave + i) * 16 + li) * M + m0 + kq);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) y[j] = ld8(dY + (j * 16 + li) * ldY + m0 + kq);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], y[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+// mask by (h > 0) from the row-major activation, then store both layouts
+__device__ __forceinline__ void store_masked(const f32x4 (&acc)[2][4], const bf16* hRm,
+                                             bf16* outRm, bf16* outTr, int wave, int li, int rq)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int u0 = (2 * wave + i) * 16 + rq;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = j * 16 + li;
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                v[e] = (float)hRm[r * kLdH + u0 + e] > 0 ? acc[i][j][e] : 0.0f;
+            store_tile(outRm, kLdH, outTr, u0, r, v);
+        }
+    }
+}
+
+// weight gradient dW[M x N] (+)= sum over the kRows rows of dY^T X from the
+// transposed layouts; this wave's rows of tiles are {2 wave, 2 wave + 1}
+// (TI = 2) or all of them with columns {2 wave, 2 wave + 1} (head, TI = M/16)
+template <int TI, int TJ>
+__device__ __forceinline__ void wgrad(const bf16* dYT, const bf16* XT, int i0, int j0,
+                                      float* __restrict__ g, int ldG, int li, int kq, int rq)
+{
+    f32x4 acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int r0 = 0; r0 < kRows; r0 += 32) {
+        bf16x8 a[TI], x[TJ];
+#pragma unroll
+        for (int i = 0; i < TI; ++i) a[i] = ld8(dYT + ((i0 + i) * 16 + li) * kTr + r0 + kq);
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) x[j] = ld8(XT + ((j0 + j) * 16 + li) * kTr + r0 + kq);
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+            for (int j = 0; j < TJ; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], x[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                g[((i0 + i) * 16 + rq + e) * ldG + (j0 + j) * 16 + li] = acc[i][j][e];
+}
+
+// list[row]: lattice cell of each batch row (features); targets are read at
+// clsAll/lleAll[tlist ? tlist[row] : list[row]].
+__global__ __launch_bounds__(256) void k_train_fused(
+    int rows, int nLat, const int* __restrict__ list, const int* __restrict__ tlist,
+    const float* __restrict__ Bff,
+    const bf16* __restrict__ wbf, const bf16* __restrict__ wbfT, const float* __restrict__ p,
+    const int* __restrict__ clsAll, const float* __restrict__ lleAll, float wLle, float chaosStd,
+    float* __restrict__ gpart, long long stride, float* __restrict__ loss, int* __restrict__ step)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sX[kRows * kLdX];
+    __shared__ __attribute__((aligned(16))) bf16 sXT[kFeat * kTr];
+    __shared__ __attribute__((aligned(16))) bf16 sH[3][kRows * kLdH];
+    __shared__ __attribute__((aligned(16))) bf16 sHT[3][kHid * kTr];
+    __shared__ __attribute__((aligned(16))) bf16 sDO[kRows * kLdO];
+    __shared__ __attribute__((aligned(16))) bf16 sDOT[kHead * kTr];
+    __shared__ float sOut[kRows * (kHead + 1)];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, kq = (lane >> 4) * 8, rq = (lane >> 4) * 4;
+    const int row0 = blockIdx.x * kRows;
+    const int nrows = min(kRows, rows - row0);
+    float* gp = gpart + (long long)blockIdx.x * stride;
+    if (step && blockIdx.x == 0 && tid == 0) *step += 1;
+
+    // Fourier features (k_features' arithmetic); rows past the batch are zero
+    const float inv = nLat > 1 ? 1.0f / (float)(nLat - 1) : 0.0f;
+    for (int t = tid; t < kRows * (kFeat / 2); t += 256) {
+        const int r = t / (kFeat / 2), f = t % (kFeat / 2);
+        float sv = 0.0f, cv = 0.0f;
+        if (r < nrows) {
+            const long long cell = list[row0 + r];
+            const float u = (float)(cell % nLat) * inv;
+            const float v = (float)(cell / nLat) * inv;
+            const float ph = 6.283185307179586f * (Bff[2 * f] * u + Bff[2 * f + 1] * v);
+            sincosf(ph, &sv, &cv);
+        }
+        const bf16 bs = (bf16)sv, bc = (bf16)cv;
+        sX[r * kLdX + f] = bs;
+        sX[r * kLdX + kFeat / 2 + f] = bc;
+        sXT[f * kTr + r] = bs;
+        sXT[(kFeat / 2 + f) * kTr + r] = bc;
+    }
+    __syncthreads();
+
+    // forward
+    fwd_hidden<kFeat>(wbf + qW1, p + pB1, sX, kLdX, sH[0], sHT[0], wave, li, kq, rq);
+    __syncthreads();
+    fwd_hidden<kHid>(wbf + qW2, p + pB2, sH[0], kLdH, sH[1], sHT[1], wave, li, kq, rq);
+    __syncthreads();
+    fwd_hidden<kHid>(wbf + qW3, p + pB3, sH[1], kLdH, sH[2], sHT[2], wave, li, kq, rq);
+    __syncthreads();
+    {
+        // head: this wave's row tile, both unit tiles, FP32 + bias
+        f32x4 acc[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+#pragma unroll
+        for (int k0 = 0; k0 < kHid; k0 += 32) {
+            const bf16x8 x = ld8(sH[2] + (wave * 16 + li) * kLdH + k0 + kq);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    ld8(wbf + qWh + (i * 16 + li) * kHid + k0 + kq), x, acc[i], 0, 0, 0);
+        }
+        const int r = wave * 16 + li;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int u = i * 16 + rq + e;
+                sOut[r * (kHead + 1) + u] = acc[i][e] + p[pBh + u];
+            }
+    }
+    __syncthreads();
+
+    // loss gradient, one row per lane of wave 0
+    if (wave == 0) {
+        const int r = lane;
+        bf16 d[kHead];
+        float ce = 0, mse = 0, bce = 0;
+        if (r < nrows) {
+            const int src = tlist ? tlist[row0 + r] : list[row0 + r];
+            loss_row(sOut + r * (kHead + 1), clsAll[src], lleAll[src], wLle, chaosStd,
+                     1.0f / (float)rows, d, ce, mse, bce);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kHead; ++k) d[k] = (bf16)0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kHead; ++k) {
+            sDO[r * kLdO + k] = d[k];
+            sDOT[k * kTr + r] = d[k];
+        }
+        const float w = 1.0f / (float)rows;
+        ce *= w;
+        mse *= w;
+        bce *= w;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            ce += __shfl_xor(ce, o);
+            mse += __shfl_xor(mse, o);
+            bce += __shfl_xor(bce, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&loss[0], ce);
+            atomicAdd(&loss[1], mse);
+            atomicAdd(&loss[2], bce);
+        }
+    }
+    __syncthreads();
+
+    // head: dWh = dOut^T H3; dH3 = (dOut Wh) masked by H3
+    wgrad<2, 2>(sDOT, sHT[2], 0, 2 * wave, gp + pWh, kHid, li, kq, rq);
+    f32x4 acc[2][4];
+    bwd_input<kHead>(wbfT + tWh, sDO, kLdO, acc, wave, li, kq);
+    __syncthreads();   // H3 (both layouts) is read for the last time above
+    store_masked(acc, sH[2], sH[2], sHT[2], wave, li, rq);
+    __syncthreads();
+    // layer 3: dW3 = dH3^T H2; dH2 = (dH3 W3) masked by H2
+    wgrad<2, 8>(sHT[2], sHT[1], 2 * wave, 0, gp + pW3, kHid, li, kq, rq);
+    bwd_input<kHid>(wbfT + tW3, sH[2], kLdH, acc, wave, li, kq);
+    __syncthreads();
+    store_masked(acc, sH[1], sH[1], sHT[1], wave, li, rq);
+    __syncthreads();
+    // layer 2: dW2 = dH2^T H1; dH1 = (dH2 W2) masked by H1
+    wgrad<2, 8>(sHT[1], sHT[0], 2 * wave, 0, gp + pW2, kHid, li, kq, rq);
+    bwd_input<kHid>(wbfT + tW2, sH[1], kLdH, acc, wave, li, kq);
+    __syncthreads();
+    store_masked(acc, sH[0], sH[0], sHT[0], wave, li, rq);
+    __syncthreads();
+    // layer 1: dW1 = dH1^T X
+    wgrad<2, 4>(sHT[0], sXT, 2 * wave, 0, gp + pW1, kFeat, li, kq, rq);
+
+    // bias gradients: column sums over the rows of dOut, dH3, dH2, dH1
+    for (int c = tid; c < kHead + 3 * kHid; c += 256) {
+        const bf16* src;
+        int off;
+        if (c < kHead) { src = sDOT + c * kTr; off = pBh + c; }
+        else if (c < kHead + kHid) { src = sHT[2] + (c - kHead) * kTr; off = pB3 + c - kHead; }
+        else if (c < kHead + 2 * kHid) { src = sHT[1] + (c - kHead - kHid) * kTr; off = pB2 + c - kHead - kHid; }
+        else { src = sHT[0] + (c - kHead - 2 * kHid) * kTr; off = pB1 + c - kHead - 2 * kHid; }
+        float sum = 0;
+        for (int r = 0; r < kRows; ++r) sum += (float)src[r];
+        gp[off] = sum;
+    }
 }
 
 __global__ void k_fill_f64(int n, double* p, double v)
@@ -348,16 +598,30 @@ __device__ __forceinline__ void wave_append(bool take, int value, int* __restric
 // cells that need their regime are re-solved in full (flag 1, listFull)
 // and cells that only need their class get the peak count alone (flag 2,
 // listPeak); with `partial` false both go to listFull.
+// Validation draw of an unflagged cell: a seeded 64-bit mix of the cell
+// index (splitmix64 finaliser), kept when its top 32 bits fall below vthr
+// (vthr = fraction * 2^32; 0 draws nothing).
+__device__ __forceinline__ bool validation_pick(long long idx, unsigned long long seed,
+                                                unsigned int vthr)
+{
+    unsigned long long z = (unsigned long long)idx + seed * 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (unsigned int)(z >> 32) < vthr;
+}
+
 __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
                        const unsigned char* __restrict__ chaos, const float* __restrict__ conf,
                        const float* __restrict__ confChaos, float confThr, float chaosConfThr,
                        bool partial, int* __restrict__ listFull, int* __restrict__ countFull,
                        int* __restrict__ listPeak, int* __restrict__ countPeak,
-                       unsigned char* __restrict__ flag)
+                       unsigned long long vseed, unsigned int vthr, int* __restrict__ listVal,
+                       int* __restrict__ countVal, unsigned char* __restrict__ flag)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = idx < (long long)n * n;   // no early return: the appends are wave-wide
-    bool fc = false, fh = false;
+    bool fc = false, fh = false, fv = false;
     if (live) {
         const int i = (int)(idx % n), j = (int)(idx / n);
         const int c0 = cls[idx];
@@ -374,10 +638,12 @@ __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
             fh |= chaos[o] != ch0;
         }
         if (!confChaos || !partial) fh = fh || fc;
-        flag[idx] = fh ? 1 : (fc ? 2 : 0);
+        fv = !fh && !fc && vthr != 0 && validation_pick(idx, vseed, vthr);
+        flag[idx] = fh ? 1 : (fc ? 2 : (fv ? 3 : 0));
     }
     wave_append(live && fh, (int)idx, listFull, countFull);
     wave_append(live && !fh && fc, (int)idx, listPeak, countPeak);
+    if (vthr != 0) wave_append(live && fv, (int)idx, listVal, countVal);
 }
 
 // Write re-solved cells into the output maps (LLE only when lleIn is given).
@@ -450,30 +716,17 @@ __global__ __launch_bounds__(256) void k_score(int nFull, int nTot, const int* _
 
 // ------------------------------------------------------------ launchers
 
-// splits > 1 (kEpiF32 only): K is cut into `splits` chunks written to
-// C + z*M*sCm, to be added by launch_sum_splits.
 hipError_t launch_gemm(int epi, int M, int N, int K, const bf16* A, long long sAm, long long sAk,
-                       const bf16* B, long long sBk, long long sBn, const float* bias,
-                       const bf16* mask, long long sMm, void* C, long long sCm, hipStream_t st,
-                       int splits = 1, long long splitStride = -1)
+                       const bf16* B, long long sBk, long long sBn, const float* bias, void* C,
+                       long long sCm, hipStream_t st)
 {
-    if (epi != kEpiF32) splits = 1;
-    // chunks are whole 32-wide k steps; trailing splits may be empty and
-    // then write zero partials, so the caller always sums `splits` of them
-    int kChunk = (K + splits - 1) / splits;
-    kChunk = (kChunk + 31) / 32 * 32;
-    const dim3 grd((N + 63) / 64, (M + 63) / 64, splits), blk(256);
-    if (splitStride < 0) splitStride = (long long)M * sCm;
-#define GEMM_CASE(E)                                                                          \
-    hipLaunchKernelGGL((k_gemm<E>), grd, blk, 0, st, M, N, K, A, sAm, sAk, B, sBk, sBn, bias, \
-                       mask, sMm, C, sCm, kChunk, splitStride)
-    switch (epi) {
-        case kEpiReluBf16: GEMM_CASE(kEpiReluBf16); break;
-        case kEpiBiasF32: GEMM_CASE(kEpiBiasF32); break;
-        case kEpiMaskBf16: GEMM_CASE(kEpiMaskBf16); break;
-        default: GEMM_CASE(kEpiF32); break;
-    }
-#undef GEMM_CASE
+    const dim3 grd((N + 63) / 64, (M + 63) / 64), blk(256);
+    if (epi == kEpiReluBf16)
+        hipLaunchKernelGGL((k_gemm<kEpiReluBf16>), grd, blk, 0, st, M, N, K, A, sAm, sAk, B, sBk,
+                           sBn, bias, C, sCm);
+    else
+        hipLaunchKernelGGL((k_gemm<kEpiBiasF32>), grd, blk, 0, st, M, N, K, A, sAm, sAk, B, sBk,
+                           sBn, bias, C, sCm);
     return hipGetLastError();
 }
 
@@ -483,15 +736,6 @@ hipError_t launch_features(int rows, int n, const int* list, long long base, con
     const long long th = (long long)rows * (kFeat / 2);
     hipLaunchKernelGGL(k_features, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, st, rows, n,
                        list, base, Bff, X);
-    return hipGetLastError();
-}
-
-hipError_t launch_loss_grad(int rows, const int* list, const float* out, const int* cls,
-                            const float* lle, float wLle, float chaosStd, bf16* dOut, float* loss,
-                            hipStream_t st)
-{
-    hipLaunchKernelGGL(k_loss_grad, dim3((rows + 255) / 256), dim3(256), 0, st, rows, list, out,
-                       cls, lle, wLle, chaosStd, dOut, loss);
     return hipGetLastError();
 }
 
@@ -514,32 +758,28 @@ hipError_t launch_fill_f32(int n, float* p, float v, hipStream_t st)
     return hipGetLastError();
 }
 
-// split partials of the four bias gradients (g[l] is [rows x cols[l]]) into
-// gpart[z * stride + off[l] + c]; advances *step when step is not null
-hipError_t launch_bias_partials(int rows, int splits, const bf16* const g[4], const int cols[4],
-                                const int off[4], float* gpart, long long stride, int* step,
-                                hipStream_t st)
+hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, int splits,
+                       long long stride, const unsigned char* keep, const int* bfIndex, bf16* wbf,
+                       const int* bfIndexT, bf16* wbfT, float lr, float b1, float b2, float eps,
+                       const int* step, hipStream_t st)
 {
-    BiasSrc src;
-    int maxCols = 0;
-    for (int l = 0; l < 4; ++l) {
-        src.g[l] = g[l];
-        src.cols[l] = cols[l];
-        src.off[l] = off[l];
-        maxCols = max(maxCols, cols[l]);
-    }
-    const int rChunk = (rows + splits - 1) / splits;
-    hipLaunchKernelGGL(k_bias_partials, dim3((maxCols + 63) / 64, 4, splits), dim3(256), 0, st,
-                       rows, rChunk, src, gpart, stride, step);
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, n, p, m, v, gpart, splits,
+                       stride, keep, bfIndex, wbf, bfIndexT, wbfT, lr, b1, b2, eps, step);
     return hipGetLastError();
 }
 
-hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, int splits,
-                       long long stride, const unsigned char* keep, const int* bfIndex, bf16* wbf,
-                       float lr, float b1, float b2, float eps, const int* step, hipStream_t st)
+int train_fused_blocks(int rows) { return (rows + kRows - 1) / kRows; }
+
+hipError_t launch_train_fused(int rows, int nLat, const int* list, const int* tlist,
+                              const float* Bff, const bf16* wbf, const bf16* wbfT, const float* p,
+                              const int* cls, const float* lle, float wLle, float chaosStd,
+                              float* gpart, long long stride, float* loss, int* step,
+                              hipStream_t st)
 {
-    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, n, p, m, v, gpart, splits,
-                       stride, keep, bfIndex, wbf, lr, b1, b2, eps, step);
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_train_fused, dim3(train_fused_blocks(rows)), dim3(256), 0, st, rows,
+                       nLat, list, tlist, Bff, wbf, wbfT, p, cls, lle, wLle, chaosStd, gpart,
+                       stride, loss, step);
     return hipGetLastError();
 }
 
@@ -555,12 +795,14 @@ hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, f
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
                        float chaosConfThr, bool partial, int* listFull, int* countFull,
-                       int* listPeak, int* countPeak, unsigned char* flag, hipStream_t st)
+                       int* listPeak, int* countPeak, unsigned long long vseed,
+                       unsigned int vthr, int* listVal, int* countVal, unsigned char* flag,
+                       hipStream_t st)
 {
     const long long cells = (long long)n * n;
     hipLaunchKernelGGL(k_flag, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, n, cls,
                        chaos, conf, confChaos, confThr, chaosConfThr, partial, listFull,
-                       countFull, listPeak, countPeak, flag);
+                       countFull, listPeak, countPeak, vseed, vthr, listVal, countVal, flag);
     return hipGetLastError();
 }
 

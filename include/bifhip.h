@@ -430,9 +430,9 @@ int bifhip_set_chaos_confidence(double threshold);
  * get the plain FP64 DBSCAN peak count (bifurcation2DDBSCAN semantics) and
  * keep the predicted LLE when partial re-solves are on
  * (bifhip_set_partial_resolve, default), else they are re-solved in full.
- * The re-solves score the surrogate; a validation sample of
- * validationFraction of the remaining cells is re-solved in full to score
- * it there.  Outputs: d_peak[nFine^2] (re-solved cells: DBSCAN count;
+ * The re-solves score the surrogate; a validation sample -- each remaining
+ * cell drawn with probability validationFraction by a hash of its index and
+ * the seed -- is re-solved in full to score it there.  Outputs: d_peak[nFine^2] (re-solved cells: DBSCAN count;
  * others: predicted class + 1 in 1..16), d_lle[nFine^2],
  * d_resolved[nFine^2] (1 re-solved in full, 2 peak count only, 3 validation
  * sample re-solved in full, 0 predicted; may be NULL),

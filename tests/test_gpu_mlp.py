@@ -122,14 +122,20 @@ def test_training_learns_a_synthetic_map(bh):
     assert loss[0] < 0.3
 
 
-@pytest.mark.parametrize("partial", [True, False])
-def test_surrogate_pipeline_small(bh, partial):
+# cap: peak-row capacity (8 sends every re-solved cell through the overflow
+# recompute, which gathers cell indices from the device lists); pert: the
+# reference perturbation directions read the listed cells' indices back
+@pytest.mark.parametrize("partial,cap,pert", [(True, 0, 1), (False, 0, 1), (True, 8, 1),
+                                              (True, 0, 0)])
+def test_surrogate_pipeline_small(bh, partial, cap, pert):
     nf, nc = 96, 24
     peak = dict(tmax=40.0, trans=20.0)
     lle = dict(tmax=50.0, NT=0.5, trans=10.0, eps=1e-8)
-    bh.set_perturbation(bh.PERT_AXIS_X)
+    bh.set_perturbation(pert)
     bh.set_partial_resolve(partial)
-    say("surrogate pipeline", nf, nc, "partial", partial)
+    if cap:
+        bh.set_peak_capacity(cap)
+    say("surrogate pipeline", nf, nc, "partial", partial, "cap", cap, "pert", pert)
     d_pk = bh.DeviceArray((nf * nf,), np.int32).zero_()
     d_ll = bh.DeviceArray((nf * nf,), np.float64).zero_()
     d_rs = bh.DeviceArray((nf * nf,), np.uint8).zero_()

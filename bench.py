@@ -385,7 +385,8 @@ def main():
                                        f"20 epochs; class confidence < {args.conf} or class "
                                        f"boundary: peak count re-solved in FP64; chaos "
                                        f"confidence < {args.chaos_conf} or regime boundary: "
-                                       f"LLE + peak count re-solved in FP64)",
+                                       f"LLE re-solved in FP64, with the peak count when the "
+                                       f"class is flagged too)",
                            "grid": n, "coarse": args.coarse, "ranges": RANGES, "h": H,
                            "peak": PEAK, "lle": LLE, "chaos_threshold": CHAOS})
         out["surrogate"] = {
@@ -397,6 +398,9 @@ def main():
             "speedup_vs_plain_fp64": plain_ms / ms,
             "resolved_fraction": last["resolved_frac"],
             "peak_only_fraction": last["peak_only"] / cells,
+            "lle_only_fraction": last["lle_only"] / cells,
+            "lle_only_recounted_fraction": last["lle_only_recounted"] / cells,
+            "surrogate_on_lle_only": {"regime_acc": last["lle_only_regime_acc"]},
             "surrogate_on_peak_only": {"class_acc": last["peak_only_class_acc"]},
             "partial_resolve": not args.full_resolve,
             "phase_ms": {k: last[k] for k in ("ms_coarse", "ms_train", "ms_infer", "ms_resolve",

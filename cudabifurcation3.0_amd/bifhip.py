@@ -539,7 +539,8 @@ def lle_peak_map_device(d_peak, d_lle, nPts, h, initialConditions, ranges, indic
 SSTAT = ["ms_total", "ms_coarse", "ms_train", "ms_infer", "ms_resolve", "resolved",
          "resolved_frac", "validation", "flagged_class_acc", "flagged_regime_acc",
          "flagged_lle_mae", "valid_class_acc", "valid_regime_acc", "valid_lle_mae",
-         "loss_ce", "loss_mse", "lle_mean", "lle_sd", "peak_only", "peak_only_class_acc", "ms_flag", "ms_solve", "ms_score"]
+         "loss_ce", "loss_mse", "lle_mean", "lle_sd", "peak_only", "peak_only_class_acc", "ms_flag",
+         "ms_solve", "ms_score", "lle_only", "lle_only_regime_acc", "lle_only_recounted"]
 
 
 def _pf(a):

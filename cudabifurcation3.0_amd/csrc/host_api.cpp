@@ -805,6 +805,64 @@ int surrogate_resolve_peaks(int nPts, double h, int nIC, const double* ic, const
     return peak_cells_device(S, maxPeaks, dbEps, h_list, d_list, 0, n, d_peak);
 }
 
+// Plain FP64 LLE (LLE2D semantics) of n listed cells (flattened indices
+// d_list[0..n) on an nPts x nPts lattice) into d_lle[0..n): the surrogate's
+// re-solve of cells whose class is predicted with confidence.
+int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const double* ranges,
+                          const int* mut, const double* values, int nVal, double lleTMax,
+                          double NT, double lleTransient, double lleEps, const int* d_list, int n,
+                          double* d_lle)
+{
+    if (n <= 0) return BIFHIP_OK;
+    if (g.model == BIFHIP_MODEL_MEMRISTIVE)
+        return fail(BIFHIP_E_ARG, "the LLE re-solve supports the Rossler and linear models");
+    LyapSpec L;
+    RC_TRY(lyap_spec(L, 2, lleTMax, NT, nPts, h, lleEps, ic, nIC, ranges, mut, 0.0, lleTransient,
+                     values, nVal));
+    const long long chunk = launch_chunk(n);
+    const bool refDirs = g.pert == BIFHIP_PERT_REFERENCE;
+    const long long dirChunk = g.dirChunk >= 0 ? g.dirChunk : 200000;
+    double* d_dirs = nullptr;
+    std::vector<double> hdirs, one;
+    std::vector<int> seg;
+    if (refDirs) RC_TRY(pool_get("lr_dirs", (size_t)std::min<long long>(chunk, n) * 3, &d_dirs));
+    for (long long off = 0; off < n; off += chunk) {
+        const int m = (int)std::min<long long>(chunk, n - off);
+        LyapArgs A;
+        A.g = L.G;
+        A.h = L.h;
+        A.eps = L.eps;
+        A.tMax = L.tMax;
+        A.maxValue = 0.0;
+        A.cellBase = 0;
+        A.cellList = d_list + off;
+        A.nCells = m;
+        A.skip = L.skip;
+        A.nNT = L.nNT;
+        A.nBlocks = L.nBlocks;
+        A.dirs = nullptr;
+        if (refDirs) {
+            // directions of the listed cells (their flattened indices)
+            seg.resize((size_t)m);
+            HIP_TRY(hipMemcpy(seg.data(), d_list + off, sizeof(int) * m, hipMemcpyDeviceToHost));
+            hdirs.resize((size_t)m * 3);
+            for (int i = 0; i < m; ++i) {
+                directions(false, seg[i], 1, dirChunk, one);
+                std::memcpy(hdirs.data() + (size_t)i * 3, one.data(), sizeof(double) * 3);
+            }
+            HIP_TRY(hipMemcpyAsync(d_dirs, hdirs.data(), sizeof(double) * hdirs.size(),
+                                   hipMemcpyHostToDevice, g.stream));
+            A.dirs = d_dirs;
+        }
+        {
+            Timed t("lle");
+            HIP_TRY(launch_lle(g.model, BIFHIP_PREC_FP64, A, false, d_lle + off, g.stream));
+        }
+        if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
+    }
+    return BIFHIP_OK;
+}
+
 // Plain FP64 fused LLE + peak count of listed cells (surrogate pipeline):
 // cells = h_list/d_list (flattened indices on an nPts x nPts lattice), or
 // base..base+n when both lists are null.

@@ -52,15 +52,19 @@ hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, f
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
                        float chaosConfThr, bool partial, int* listFull, int* countFull,
-                       int* listPeak, int* countPeak, unsigned long long vseed,
-                       unsigned int vthr, int* listVal, int* countVal, unsigned char* flag,
-                       hipStream_t st);
+                       int* listPeak, int* countPeak, int* listLle, int* countLle,
+                       unsigned long long vseed, unsigned int vthr, int* listVal, int* countVal,
+                       unsigned char* flag, hipStream_t st);
 hipError_t launch_scatter(int n, const int* list, const int* peak, const double* lle,
                           int* peakMap, double* lleMap, hipStream_t st);
 hipError_t launch_fill_pred(long long n, const unsigned char* cls, const float* lle,
                             int* peakMap, double* lleMap, hipStream_t st);
+hipError_t launch_regime_changed(int n, const int* list, const double* lle,
+                                 const unsigned char* pch, double chaosThr, int* out, int* count,
+                                 unsigned char* flag, hipStream_t st);
 hipError_t launch_score(int nFull, int nTot, const int* list, const int* rpeak, const double* rlle,
-                        int nPeak, const int* plist, const int* ppeak, const unsigned char* pcls,
+                        int nPeak, const int* plist, const int* ppeak, int nLle,
+                        const int* llist, const double* llle, const unsigned char* pcls,
                         const unsigned char* pch, const float* plle, double chaosThr,
                         double* acc, hipStream_t st);
 namespace host {
@@ -75,6 +79,10 @@ int surrogate_resolve_peaks(int nPts, double h, int nIC, const double* ic, const
                             const int* mut, const double* values, int nVal, double peakTMax,
                             double peakTransient, int wv, int maxPeaks, double dbEps,
                             const int* h_list, const int* d_list, int n, int* d_peak);
+int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const double* ranges,
+                          const int* mut, const double* values, int nVal, double lleTMax,
+                          double NT, double lleTransient, double lleEps, const int* d_list, int n,
+                          double* d_lle);
 }  // namespace host
 }  // namespace bifhip
 
@@ -545,32 +553,34 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     const auto t3 = std::chrono::steady_clock::now();
     // full re-solves first, then the validation sample (cells drawn on the
     // device among the unflagged ones, a seeded hash per cell), in one list
-    int *d_all, *d_vlist, *d_plist, *d_count;
+    int *d_all, *d_vlist, *d_plist, *d_llist, *d_count;
     RC_TRY(pool_get("sg_all", (size_t)nf, &d_all));
     RC_TRY(pool_get("sg_vlist", (size_t)(validationFraction > 0 ? nf : 1), &d_vlist));
     RC_TRY(pool_get("sg_plist", (size_t)(partial ? nf : 1), &d_plist));
-    RC_TRY(pool_get("sg_count", 3, &d_count));
-    HIP_TRY(launch_fill(3, d_count, 0, s));
+    RC_TRY(pool_get("sg_llist", (size_t)(partial ? nf : 1), &d_llist));
+    RC_TRY(pool_get("sg_count", 5, &d_count));
+    HIP_TRY(launch_fill(5, d_count, 0, s));
     const double vf = std::min(1.0, std::max(0.0, validationFraction));
     const unsigned int vthr =
         vf > 0 ? (unsigned int)std::min(4294967295.0, std::floor(vf * 4294967296.0)) : 0u;
     HIP_TRY(launch_flag(nFine, d_pcls, d_pch, d_conf, d_cconf, (float)confThreshold,
                         (float)g.chaosConfidence, partial, d_all, d_count, d_plist, d_count + 1,
-                        seed + 17, vthr, d_vlist, d_count + 2, d_flag, s));
+                        d_llist, d_count + 3, seed + 17, vthr, d_vlist, d_count + 2, d_flag, s));
     HIP_TRY(launch_fill_pred(nf, d_pcls, d_plle, d_peak, d_lle, s));
-    int counts[3] = {0, 0, 0};
-    HIP_TRY(hipMemcpyAsync(counts, d_count, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+    int counts[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(counts, d_count, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    const int nflag = counts[0], npeak = counts[1], nval = counts[2];
+    const int nflag = counts[0], npeak = counts[1], nval = counts[2], nlle = counts[3];
     const int ntot = nflag + nval;
     if (nval)
         HIP_TRY(hipMemcpyAsync(d_all + nflag, d_vlist, sizeof(int) * nval,
                                hipMemcpyDeviceToDevice, s));
     int *d_rpeak, *d_ppeak;
-    double* d_rlle;
+    double *d_rlle, *d_llle;
     RC_TRY(pool_get("sg_rpeak", (size_t)std::max(1, ntot), &d_rpeak));
     RC_TRY(pool_get("sg_rlle", (size_t)std::max(1, ntot), &d_rlle));
     RC_TRY(pool_get("sg_ppeak", (size_t)std::max(1, npeak), &d_ppeak));
+    RC_TRY(pool_get("sg_llle", (size_t)std::max(1, nlle), &d_llle));
     HIP_TRY(hipStreamSynchronize(s));
     st[BIFHIP_SSTAT_MS_FLAG] = ms_since(t3);
     const auto t4 = std::chrono::steady_clock::now();
@@ -582,17 +592,35 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
         RC_TRY(surrogate_resolve_peaks(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
                                        peakTransient, wv, maxPeaks, dbEps, nullptr, d_plist,
                                        npeak, d_ppeak));
+    int nchg = 0;
+    int *d_clist = nullptr, *d_cpeak2 = nullptr;
+    if (nlle > 0) {
+        RC_TRY(surrogate_resolve_lle(nFine, h, nIC, ic, ranges, mut, values, nVal, lleTMax, NT,
+                                     lleTransient, lleEps, d_llist, nlle, d_llle));
+        // a regime that changed under the re-solve: the peak count as well
+        RC_TRY(pool_get("sg_clist", (size_t)nlle, &d_clist));
+        HIP_TRY(launch_regime_changed(nlle, d_llist, d_llle, d_pch, chaosThreshold, d_clist,
+                                      d_count + 4, d_flag, s));
+        HIP_TRY(hipMemcpyAsync(&nchg, d_count + 4, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (nchg > 0) {
+            RC_TRY(pool_get("sg_cpeak2", (size_t)nchg, &d_cpeak2));
+            RC_TRY(surrogate_resolve_peaks(nFine, h, nIC, ic, ranges, mut, values, nVal,
+                                           peakTMax, peakTransient, wv, maxPeaks, dbEps, nullptr,
+                                           d_clist, nchg, d_cpeak2));
+        }
+    }
     HIP_TRY(hipStreamSynchronize(s));
     st[BIFHIP_SSTAT_MS_SOLVE] = ms_since(t4);
     const auto t5 = std::chrono::steady_clock::now();
     // score the surrogate on every re-solved cell (on the device) before
     // the re-solved values overwrite the predictions
     double* d_acc;
-    RC_TRY(pool_get("sg_acc", 15, &d_acc));
-    HIP_TRY(launch_fill_f64(15, d_acc, 0.0, s));
-    HIP_TRY(launch_score(nflag, ntot, d_all, d_rpeak, d_rlle, npeak, d_plist, d_ppeak, d_pcls,
-                         d_pch, d_plle, chaosThreshold, d_acc, s));
-    double acc[15];
+    RC_TRY(pool_get("sg_acc", 20, &d_acc));
+    HIP_TRY(launch_fill_f64(20, d_acc, 0.0, s));
+    HIP_TRY(launch_score(nflag, ntot, d_all, d_rpeak, d_rlle, npeak, d_plist, d_ppeak, nlle,
+                         d_llist, d_llle, d_pcls, d_pch, d_plle, chaosThreshold, d_acc, s));
+    double acc[20];
     HIP_TRY(hipMemcpyAsync(acc, d_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     auto score = [&](int gi, double* clsAcc, double* regAcc, double* mae) {
@@ -606,9 +634,13 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     score(1, &st[BIFHIP_SSTAT_VALID_CLASS_ACC], &st[BIFHIP_SSTAT_VALID_REGIME_ACC],
           &st[BIFHIP_SSTAT_VALID_LLE_MAE]);
     st[BIFHIP_SSTAT_PEAK_ONLY_CLASS_ACC] = acc[10] > 0 ? acc[11] / acc[10] : 1.0;
+    st[BIFHIP_SSTAT_LLE_ONLY_REGIME_ACC] = acc[15] > 0 ? acc[17] / acc[15] : 1.0;
     HIP_TRY(launch_scatter(ntot, d_all, d_rpeak, d_rlle, d_peak, d_lle, s));
     HIP_TRY(launch_scatter(npeak, d_plist, d_ppeak, nullptr, d_peak, d_lle, s));
-    // 1 re-solved in full, 2 peak count only, 3 validation sample, 0 predicted
+    HIP_TRY(launch_scatter(nlle, d_llist, nullptr, d_llle, nullptr, d_lle, s));
+    HIP_TRY(launch_scatter(nchg, d_clist, d_cpeak2, nullptr, d_peak, nullptr, s));
+    // 1 re-solved in full, 2 peak count only, 3 validation sample, 4 LLE
+    // only, 5 LLE then peak count (regime changed), 0 predicted
     if (d_resolved)
         HIP_TRY(hipMemcpyAsync(d_resolved, d_flag, (size_t)nf, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -618,6 +650,8 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     st[BIFHIP_SSTAT_VALIDATION] = nval;
     st[BIFHIP_SSTAT_RESOLVED_FRAC] = (double)nflag / (double)nf;
     st[BIFHIP_SSTAT_PEAK_ONLY] = npeak;
+    st[BIFHIP_SSTAT_LLE_ONLY] = nlle;
+    st[BIFHIP_SSTAT_LLE_ONLY_RECOUNTED] = nchg;
     st[BIFHIP_SSTAT_MS_TOTAL] = ms_since(t0);
     st[BIFHIP_SSTAT_LLE_MEAN] = mean;
     st[BIFHIP_SSTAT_LLE_SD] = sd;

@@ -616,12 +616,13 @@ __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
                        const float* __restrict__ confChaos, float confThr, float chaosConfThr,
                        bool partial, int* __restrict__ listFull, int* __restrict__ countFull,
                        int* __restrict__ listPeak, int* __restrict__ countPeak,
+                       int* __restrict__ listLle, int* __restrict__ countLle,
                        unsigned long long vseed, unsigned int vthr, int* __restrict__ listVal,
                        int* __restrict__ countVal, unsigned char* __restrict__ flag)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = idx < (long long)n * n;   // no early return: the appends are wave-wide
-    bool fc = false, fh = false, fv = false;
+    bool fc = false, fh = false, fv = false, fl = false;
     if (live) {
         const int i = (int)(idx % n), j = (int)(idx / n);
         const int c0 = cls[idx];
@@ -637,16 +638,41 @@ __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
             fc |= cls[o] != c0;
             fh |= chaos[o] != ch0;
         }
-        if (!confChaos || !partial) fh = fh || fc;
-        fv = !fh && !fc && vthr != 0 && validation_pick(idx, vseed, vthr);
-        flag[idx] = fh ? 1 : (fc ? 2 : (fv ? 3 : 0));
+        if (!confChaos || !partial) {
+            fh = fh || fc;
+        } else if (fh && !fc) {
+            fl = true;          // regime only: the LLE alone, class kept
+            fh = false;
+        }
+        fv = !fh && !fc && !fl && vthr != 0 && validation_pick(idx, vseed, vthr);
+        flag[idx] = fh ? 1 : (fc ? 2 : (fl ? 4 : (fv ? 3 : 0)));
     }
     wave_append(live && fh, (int)idx, listFull, countFull);
     wave_append(live && !fh && fc, (int)idx, listPeak, countPeak);
+    wave_append(live && fl, (int)idx, listLle, countLle);
     if (vthr != 0) wave_append(live && fv, (int)idx, listVal, countVal);
 }
 
-// Write re-solved cells into the output maps (LLE only when lleIn is given).
+// Cells re-solved for the regime only whose exact regime (lle > chaosThr)
+// differs from the predicted one: mostly periodic windows inside the chaotic
+// region, where the predicted class is suspect too; appended to out[] and
+// marked 5 in flag[].
+__global__ void k_regime_changed(int n, const int* __restrict__ list,
+                                 const double* __restrict__ lle,
+                                 const unsigned char* __restrict__ pch, double chaosThr,
+                                 int* __restrict__ out, int* __restrict__ count,
+                                 unsigned char* __restrict__ flag)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < n;   // no early return: the append is wave-wide
+    const int cell = live ? list[t] : 0;
+    const bool take = live && ((lle[t] > chaosThr) != (pch[cell] != 0));
+    if (take) flag[cell] = 5;
+    wave_append(take, cell, out, count);
+}
+
+// Write re-solved cells into the output maps (the peak count when peak is
+// given, the LLE when lleIn is given).
 __global__ void k_scatter(int n, const int* __restrict__ list, const int* __restrict__ peak,
                           const double* __restrict__ lleIn, int* __restrict__ peakMap,
                           double* __restrict__ lleMap)
@@ -654,7 +680,7 @@ __global__ void k_scatter(int n, const int* __restrict__ list, const int* __rest
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int c = list[t];
-    peakMap[c] = peak[t];
+    if (peak) peakMap[c] = peak[t];
     if (lleIn) lleMap[c] = lleIn[t];
 }
 
@@ -672,20 +698,24 @@ __global__ void k_fill_pred(long long n, const unsigned char* __restrict__ cls,
 // Scores of the surrogate on re-solved cells, accumulated per group:
 // 0 = flagged and re-solved in full (entries [0, nFull) of list),
 // 1 = validation sample (entries [nFull, nTot)), 2 = peak count only
-// (plist).  acc[g*5 + 0..4] = cells, class right, regime right, finite
-// LLE cells, sum |LLE error| (as doubles).
+// (plist), 3 = LLE only (llist).  acc[g*5 + 0..4] = cells, class right,
+// regime right, finite LLE cells, sum |LLE error| (as doubles); group 2
+// fills entries 0..1, group 3 scores the chaos regime alone in entry 2.
+constexpr int kScoreSlots = 20;
 __global__ __launch_bounds__(256) void k_score(int nFull, int nTot, const int* __restrict__ list,
                                                const int* __restrict__ rpeak,
                                                const double* __restrict__ rlle, int nPeak,
                                                const int* __restrict__ plist,
-                                               const int* __restrict__ ppeak,
+                                               const int* __restrict__ ppeak, int nLle,
+                                               const int* __restrict__ llist,
+                                               const double* __restrict__ llle,
                                                const unsigned char* __restrict__ pcls,
                                                const unsigned char* __restrict__ pch,
                                                const float* __restrict__ plle, double chaosThr,
                                                double* __restrict__ acc)
 {
-    __shared__ double s_acc[15];
-    if (threadIdx.x < 15) s_acc[threadIdx.x] = 0;
+    __shared__ double s_acc[kScoreSlots];
+    if (threadIdx.x < kScoreSlots) s_acc[threadIdx.x] = 0;
     __syncthreads();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nTot) {
@@ -710,8 +740,19 @@ __global__ __launch_bounds__(256) void k_score(int nFull, int nTot, const int* _
         atomicAdd(&s_acc[10], 1.0);
         if (tc == (int)pcls[cell]) atomicAdd(&s_acc[11], 1.0);
     }
+    if (t < nLle) {
+        const int cell = llist[t];
+        const double tl = llle[t];
+        atomicAdd(&s_acc[15], 1.0);
+        if ((tl > chaosThr) == (pch[cell] != 0)) atomicAdd(&s_acc[17], 1.0);
+        if (isfinite(tl)) {
+            atomicAdd(&s_acc[18], 1.0);
+            atomicAdd(&s_acc[19], fabs(tl - (double)plle[cell]));
+        }
+    }
     __syncthreads();
-    if (threadIdx.x < 15 && s_acc[threadIdx.x] != 0) atomicAdd(&acc[threadIdx.x], s_acc[threadIdx.x]);
+    if (threadIdx.x < kScoreSlots && s_acc[threadIdx.x] != 0)
+        atomicAdd(&acc[threadIdx.x], s_acc[threadIdx.x]);
 }
 
 // ------------------------------------------------------------ launchers
@@ -795,14 +836,15 @@ hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, f
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
                        float chaosConfThr, bool partial, int* listFull, int* countFull,
-                       int* listPeak, int* countPeak, unsigned long long vseed,
-                       unsigned int vthr, int* listVal, int* countVal, unsigned char* flag,
-                       hipStream_t st)
+                       int* listPeak, int* countPeak, int* listLle, int* countLle,
+                       unsigned long long vseed, unsigned int vthr, int* listVal, int* countVal,
+                       unsigned char* flag, hipStream_t st)
 {
     const long long cells = (long long)n * n;
     hipLaunchKernelGGL(k_flag, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, n, cls,
                        chaos, conf, confChaos, confThr, chaosConfThr, partial, listFull,
-                       countFull, listPeak, countPeak, vseed, vthr, listVal, countVal, flag);
+                       countFull, listPeak, countPeak, listLle, countLle, vseed, vthr, listVal,
+                       countVal, flag);
     return hipGetLastError();
 }
 
@@ -826,15 +868,27 @@ hipError_t launch_fill_pred(long long n, const unsigned char* cls, const float* 
 }  // namespace bifhip
 
 namespace bifhip {
+hipError_t launch_regime_changed(int n, const int* list, const double* lle,
+                                 const unsigned char* pch, double chaosThr, int* out, int* count,
+                                 unsigned char* flag, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_regime_changed, dim3((n + 255) / 256), dim3(256), 0, st, n, list, lle,
+                       pch, chaosThr, out, count, flag);
+    return hipGetLastError();
+}
+
 hipError_t launch_score(int nFull, int nTot, const int* list, const int* rpeak, const double* rlle,
-                        int nPeak, const int* plist, const int* ppeak, const unsigned char* pcls,
+                        int nPeak, const int* plist, const int* ppeak, int nLle,
+                        const int* llist, const double* llle, const unsigned char* pcls,
                         const unsigned char* pch, const float* plle, double chaosThr,
                         double* acc, hipStream_t st)
 {
-    const int n = max(nTot, nPeak);
+    const int n = max(max(nTot, nPeak), nLle);
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_score, dim3((n + 255) / 256), dim3(256), 0, st, nFull, nTot, list, rpeak,
-                       rlle, nPeak, plist, ppeak, pcls, pch, plle, chaosThr, acc);
+                       rlle, nPeak, plist, ppeak, nLle, llist, llle, pcls, pch, plle, chaosThr,
+                       acc);
     return hipGetLastError();
 }
 }  // namespace bifhip

@@ -425,17 +425,22 @@ int bifhip_set_chaos_confidence(double threshold);
  * or a 4-neighbour has another predicted class, and its regime when the
  * chaos confidence is below bifhip_set_chaos_confidence's threshold (0.98
  * by default) or a 4-neighbour has another predicted regime.  Cells that
- * need their regime are re-solved by the plain FP64 fused kernel
- * (bifhip_lle_peak_map_device semantics).  Cells that need only their class
+ * need their class and regime are re-solved by the plain FP64 fused kernel
+ * (bifhip_lle_peak_map_device semantics).  With partial re-solves on
+ * (bifhip_set_partial_resolve, default), cells that need only their class
  * get the plain FP64 DBSCAN peak count (bifurcation2DDBSCAN semantics) and
- * keep the predicted LLE when partial re-solves are on
- * (bifhip_set_partial_resolve, default), else they are re-solved in full.
+ * keep the predicted LLE, and cells that need only their regime get the
+ * plain FP64 LLE (LLE2D semantics) and keep the predicted class unless the
+ * re-solved regime differs from the predicted one, in which case they get
+ * the peak count too; with them off, every flagged cell is re-solved in
+ * full.
  * The re-solves score the surrogate; a validation sample -- each remaining
  * cell drawn with probability validationFraction by a hash of its index and
  * the seed -- is re-solved in full to score it there.  Outputs: d_peak[nFine^2] (re-solved cells: DBSCAN count;
  * others: predicted class + 1 in 1..16), d_lle[nFine^2],
  * d_resolved[nFine^2] (1 re-solved in full, 2 peak count only, 3 validation
- * sample re-solved in full, 0 predicted; may be NULL),
+ * sample re-solved in full, 4 LLE only, 5 LLE then peak count, 0
+ * predicted; may be NULL),
  * stats[BIFHIP_SSTAT_COUNT]. */
 #define BIFHIP_SSTAT_MS_TOTAL 0
 #define BIFHIP_SSTAT_MS_COARSE 1
@@ -460,7 +465,10 @@ int bifhip_set_chaos_confidence(double threshold);
 #define BIFHIP_SSTAT_MS_FLAG 20            /* parts of MS_RESOLVE: flagging and   */
 #define BIFHIP_SSTAT_MS_SOLVE 21           /* sampling, the FP64 re-solves, and   */
 #define BIFHIP_SSTAT_MS_SCORE 22           /* scoring + writing the results       */
-#define BIFHIP_SSTAT_COUNT 23
+#define BIFHIP_SSTAT_LLE_ONLY 23           /* cells re-solved for the regime only */
+#define BIFHIP_SSTAT_LLE_ONLY_REGIME_ACC 24 /* surrogate chaos regime right there */
+#define BIFHIP_SSTAT_LLE_ONLY_RECOUNTED 25 /* of those, regime changed: peak count */
+#define BIFHIP_SSTAT_COUNT 26
 
 int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int amountOfInitialConditions,
                                 const double* initialConditions, const double* ranges,

@@ -155,8 +155,16 @@ def test_surrogate_pipeline_small(bh, partial, cap, pert):
     assert (rs == 1).sum() == st["resolved"]
     assert (rs == 2).sum() == st["peak_only"]
     assert (rs == 3).sum() == st["validation"] > 0
+    assert ((rs == 4) | (rs == 5)).sum() == st["lle_only"]
+    assert (rs == 5).sum() == st["lle_only_recounted"]
     if not partial:
-        assert st["peak_only"] == 0
+        assert st["peak_only"] == 0 and st["lle_only"] == 0
+    # cells re-solved for the regime only carry the exact LLE, and the exact
+    # peak count where that regime differs from the predicted one (5)
+    lo = (rs == 4) | (rs == 5)
+    same_lo = (np.isnan(ll[lo]) & np.isnan(tll[lo])) | (ll[lo] == tll[lo])
+    assert same_lo.all()
+    np.testing.assert_array_equal(pk[rs == 5], tpk[rs == 5])
     # cells re-solved in full equal the plain FP64 map exactly; cells
     # re-solved for the class carry the exact peak count
     np.testing.assert_array_equal(pk[full], tpk[full])

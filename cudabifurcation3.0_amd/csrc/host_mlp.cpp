@@ -27,12 +27,12 @@
 
 namespace bifhip {
 typedef __bf16 bf16;
-hipError_t launch_gemm(int epi, int M, int N, int K, const bf16* A, long long sAm, long long sAk,
-                       const bf16* B, long long sBk, long long sBn, const float* bias, void* C,
-                       long long sCm, hipStream_t st);
 hipError_t launch_sum_splits(int n, int splits, const float* part, float* out, hipStream_t st);
-hipError_t launch_features(int rows, int n, const int* list, long long base, const float* Bff,
-                           bf16* X, hipStream_t st);
+hipError_t launch_infer_fused(long long rows, long long base, const int* list, int nLat,
+                              const float* Bff, const bf16* wbf, const float* p, float lleMean,
+                              float lleStd, float chaosThr, unsigned char* cls,
+                              unsigned char* chaos, float* conf, float* confChaos, float* lle,
+                              float* rawOut, hipStream_t st);
 int train_fused_blocks(int rows);
 hipError_t launch_train_fused(int rows, int nLat, const int* list, const int* tlist,
                               const float* Bff, const bf16* wbf, const bf16* wbfT, const float* p,
@@ -46,9 +46,6 @@ hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, 
 hipError_t launch_fill_f32(int n, float* p, float v, hipStream_t st);
 hipError_t launch_fill(long long n, int* v, int value, hipStream_t st);
 hipError_t launch_fill_f64(int n, double* p, double v, hipStream_t st);
-hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, float chaosThr,
-                             unsigned char* cls, unsigned char* chaos, float* conf,
-                             float* confChaos, float* lle, hipStream_t st);
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
                        float chaosConfThr, bool partial, int* listFull, int* countFull,
@@ -213,38 +210,6 @@ int net_init(Net& N, unsigned long long seed, float sigma)
     return BIFHIP_OK;
 }
 
-struct Acts {
-    bf16 *x0, *h1, *h2, *h3;
-    float* out;
-};
-
-int acts_alloc(Acts& A, int rows, const char* tag)
-{
-    std::string t(tag);
-    RC_TRY(pool_get((t + "x0").c_str(), (size_t)rows * kFeat, &A.x0));
-    RC_TRY(pool_get((t + "h1").c_str(), (size_t)rows * kHid, &A.h1));
-    RC_TRY(pool_get((t + "h2").c_str(), (size_t)rows * kHid, &A.h2));
-    RC_TRY(pool_get((t + "h3").c_str(), (size_t)rows * kHid, &A.h3));
-    RC_TRY(pool_get((t + "out").c_str(), (size_t)rows * kHead, &A.out));
-    return BIFHIP_OK;
-}
-
-// forward of `rows` lattice cells (list or base..base+rows) into A.out
-int forward(const Net& N, Acts& A, int rows, int nLat, const int* list, long long base,
-            hipStream_t s)
-{
-    HIP_TRY(launch_features(rows, nLat, list, base, N.bff, A.x0, s));
-    HIP_TRY(launch_gemm(0, rows, kHid, kFeat, A.x0, kFeat, 1, N.wbf + bW1, 1, kFeat, N.p + oB1,
-                        A.h1, kHid, s));
-    HIP_TRY(launch_gemm(0, rows, kHid, kHid, A.h1, kHid, 1, N.wbf + bW2, 1, kHid, N.p + oB2,
-                        A.h2, kHid, s));
-    HIP_TRY(launch_gemm(0, rows, kHid, kHid, A.h2, kHid, 1, N.wbf + bW3, 1, kHid, N.p + oB3,
-                        A.h3, kHid, s));
-    HIP_TRY(launch_gemm(1, rows, kHead, kHid, A.h3, kHid, 1, N.wbf + bWh, 1, kHid, N.p + oBh,
-                        A.out, kHead, s));
-    return BIFHIP_OK;
-}
-
 struct TrainCfg {
     int epochs = 20, batch = 1024;
     float lr = 1e-3f, wLle = 1.0f, chaosStd = 0.0f;
@@ -266,8 +231,8 @@ int train_step(const Net& N, const TrainCfg& T, int nLat, const int* list, const
     return BIFHIP_OK;
 }
 
-// Inference of the whole nLat x nLat lattice in chunks; confChaos as in
-// k_infer_post (null: conf = min of class and chaos confidence).
+// Inference of the whole nLat x nLat lattice (one fused launch); confChaos
+// as in k_infer_fused (null: conf = min of class and chaos confidence).
 int infer_impl(int nLat, double lleMean, double lleSd, double chaosThr, unsigned char* d_cls,
                unsigned char* d_chaos, float* d_conf, float* d_confChaos, float* d_lle)
 {
@@ -277,18 +242,10 @@ int infer_impl(int nLat, double lleMean, double lleSd, double chaosThr, unsigned
     Net N;
     RC_TRY(net_alloc(N));
     const long long n = (long long)nLat * nLat;
-    const int chunk = (int)std::min<long long>(n, 1 << 21);
-    Acts A;
-    RC_TRY(acts_alloc(A, chunk, "inf_"));
-    for (long long off = 0; off < n; off += chunk) {
-        const int rows = (int)std::min<long long>(chunk, n - off);
-        Timed t("mlp_infer");
-        RC_TRY(forward(N, A, rows, nLat, nullptr, off, g.stream));
-        HIP_TRY(launch_infer_post(rows, A.out, (float)lleMean, (float)lleSd, (float)chaosThr,
-                                  d_cls + off, d_chaos + off, d_conf + off,
-                                  d_confChaos ? d_confChaos + off : nullptr, d_lle + off,
-                                  g.stream));
-    }
+    Timed t("mlp_infer");
+    HIP_TRY(launch_infer_fused(n, 0, nullptr, nLat, N.bff, N.wbf, N.p, (float)lleMean,
+                               (float)lleSd, (float)chaosThr, d_cls, d_chaos, d_conf, d_confChaos,
+                               d_lle, nullptr, g.stream));
     return BIFHIP_OK;
 }
 
@@ -420,7 +377,7 @@ int bifhip_mlp_get_params(float* params, int nParams, float* fourier, int nFouri
 // gradients (same layout as the parameters) and the three losses back to
 // the host.  Used to check the MFMA kernels against a CPU reference: the
 // gradient and losses come from the fused training kernel (targets by row),
-// the head outputs from the inference forward pass.
+// the head outputs from the fused inference kernel.
 int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, const int* cells,
                         const int* cls, const float* lle, int rows, double wLle, double chaosStd,
                         float* grads, float* losses, float* outputs)
@@ -453,11 +410,12 @@ int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, con
                                N.gpart, kParams, N.loss, nullptr, s));
     HIP_TRY(launch_sum_splits(kParams, train_fused_blocks(rows), N.gpart, N.g, s));
     if (outputs) {
-        Acts A;
-        RC_TRY(acts_alloc(A, rows, "gr_"));
-        RC_TRY(forward(N, A, rows, nLat, dc.as<int>(), 0, s));
+        float* d_out;
+        RC_TRY(pool_get("gr_out", (size_t)rows * kHead, &d_out));
+        HIP_TRY(launch_infer_fused(rows, 0, dc.as<int>(), nLat, N.bff, N.wbf, N.p, 0.0f, 1.0f,
+                                   0.0f, nullptr, nullptr, nullptr, nullptr, nullptr, d_out, s));
         HIP_TRY(hipStreamSynchronize(s));
-        HIP_TRY(hipMemcpy(outputs, A.out, sizeof(float) * rows * kHead, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(outputs, d_out, sizeof(float) * rows * kHead, hipMemcpyDeviceToHost));
     }
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipMemcpy(grads, N.g, sizeof(float) * kParams, hipMemcpyDeviceToHost));

@@ -3,10 +3,12 @@
 // The surrogate maps a parameter point (a, c) to the LLE and the peak-count
 // class of the cell.  Network: 64 Fourier features -> 3 x 128 ReLU -> one
 // 32-wide head (column 0: standardised LLE, columns 1..16: class logits,
-// column 17: chaos logit, the rest zero).  Inference runs layer by layer
-// through k_gemm, a v_mfma_f32_16x16x32_bf16 kernel with strided operands;
-// a training step is one k_train_fused launch (forward, loss gradient,
-// backward, gradient partials per workgroup) and one k_adam launch.
+// column 17: chaos logit, the rest zero).  Every product is a
+// v_mfma_f32_16x16x32_bf16 tile with FP32 accumulation.  Inference is one
+// k_infer_fused launch (features, layers and head of 64 rows per workgroup
+// with the activations in LDS); a training step is one k_train_fused launch
+// (forward, loss gradient, backward, gradient partials per workgroup) and
+// one k_adam launch.
 // Weights are kept in FP32 (Adam state) with BF16 copies, and transposed
 // BF16 copies for the backward products, for the matrix cores.
 #include "bifhip_dev.h"
@@ -23,121 +25,12 @@ constexpr int kHead = 32;      // head width (1 LLE + 16 classes + chaos + paddi
 constexpr int kClasses = 16;
 constexpr int kChaosCol = 1 + kClasses;   // chaos logit column
 
-enum Epi : int {
-    kEpiReluBf16 = 0,   // out(bf16) = relu(acc + bias)
-    kEpiBiasF32 = 1,    // out(f32)  = acc + bias
-};
-
-// C[M x N] = sum_k A(m, k) B(k, n) with A(m,k) = A[m*sAm + k*sAk],
-// B(k,n) = B[k*sBk + n*sBn].  Block 256 = 4 waves, block tile 64 x 64,
-// each wave a 32 x 32 quadrant = 2 x 2 tiles of 16 x 16, K in steps of 32.
-// MFMA 16x16x32 operand layout (one lane = 8 consecutive k):
-//   A: row (lane & 15), k = 8 * (lane >> 4) + e
-//   B: col (lane & 15), k = 8 * (lane >> 4) + e
-//   D: col (lane & 15), row = 4 * (lane >> 4) + e   (e < 4)
-template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* __restrict__ A,
-                                              long long sAm, long long sAk,
-                                              const bf16* __restrict__ B, long long sBk,
-                                              long long sBn, const float* __restrict__ bias,
-                                              void* __restrict__ C, long long sCm)
-{
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int m0 = blockIdx.y * 64 + (wave >> 1) * 32;
-    const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
-    const int li = lane & 15;
-    const int kq = (lane >> 4) * 8;
-    f32x4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
-
-    const bool aVec = (sAk == 1) && (K % 8 == 0) && (sAm % 8 == 0);
-    const bool bVec = (sBk == 1) && (K % 8 == 0) && (sBn % 8 == 0);
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 af[2], bfr[2];
-        const int kb = k0 + kq;
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int r = m0 + 16 * t + li;
-            if (aVec && r < M && kb + 8 <= K) {
-                af[t] = *reinterpret_cast<const bf16x8*>(A + (long long)r * sAm + kb);
-            } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int k = kb + e;
-                    af[t][e] = (r < M && k < K) ? A[(long long)r * sAm + (long long)k * sAk]
-                                                : (bf16)0.0f;
-                }
-            }
-            const int c = n0 + 16 * t + li;
-            if (bVec && c < N && kb + 8 <= K) {
-                bfr[t] = *reinterpret_cast<const bf16x8*>(B + (long long)c * sBn + kb);
-            } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int k = kb + e;
-                    bfr[t][e] = (c < N && k < K) ? B[(long long)k * sBk + (long long)c * sBn]
-                                                 : (bf16)0.0f;
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int c = n0 + 16 * j + li;
-            if (c >= N) continue;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int r = m0 + 16 * i + (lane >> 4) * 4 + e;
-                if (r >= M) continue;
-                float v = acc[i][j][e];
-                v += bias ? bias[c] : 0.0f;
-                if (EPI == kEpiReluBf16)
-                    static_cast<bf16*>(C)[(long long)r * sCm + c] = (bf16)(v > 0 ? v : 0.0f);
-                else
-                    static_cast<float*>(C)[(long long)r * sCm + c] = v;
-            }
-        }
-    }
-}
-
-// Fourier features of lattice cells: cell = list ? list[i] : base + i on an
-// n x n lattice; p = (ix/(n-1), iy/(n-1)); x = [sin(2 pi B p), cos(2 pi B p)].
-__global__ void k_features(int rows, int n, const int* __restrict__ list, long long base,
-                           const float* __restrict__ Bff, bf16* __restrict__ X)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = t / (kFeat / 2);
-    const int f = t % (kFeat / 2);
-    if (r >= rows) return;
-    const long long cell = list ? (long long)list[r] : base + r;
-    const float inv = n > 1 ? 1.0f / (float)(n - 1) : 0.0f;
-    const float u = (float)(cell % n) * inv;
-    const float v = (float)(cell / n) * inv;
-    const float ph = 6.283185307179586f * (Bff[2 * f] * u + Bff[2 * f + 1] * v);
-    float s, c;
-    sincosf(ph, &s, &c);
-    X[(long long)r * kFeat + f] = (bf16)s;
-    X[(long long)r * kFeat + kFeat / 2 + f] = (bf16)c;
-}
-
-// Loss gradient of one batch: softmax cross-entropy over columns 1..16 and
-// w_lle * squared error on column 0 (skipped where the target is not
-// finite).  dOut = dLoss/dOut (mean over the batch) in BF16; loss summed
-// into loss[0] (CE) and loss[1] (MSE).
-// Column 17 is the chaos logit: binary cross-entropy against
-// (standardised LLE > chaosStd), a non-finite LLE counting as not chaotic;
-// its loss goes to loss[2].
+// Loss gradient of one row of a batch of 1/inv rows: softmax cross-entropy
+// over columns 1..16 and w_lle * squared error on column 0 (skipped where
+// the target is not finite); column 17 is the chaos logit, binary
+// cross-entropy against (standardised LLE > chaosStd), a non-finite LLE
+// counting as not chaotic.  d = dLoss/dOut (mean over the batch) in BF16;
+// the row's CE, squared error and BCE in ce, mse, bce.
 __device__ __forceinline__ void loss_row(const float* o, int y, float target, float wLle,
                                          float chaosStd, float inv, bf16 (&d)[kHead], float& ce,
                                          float& mse, float& bce)
@@ -226,7 +119,10 @@ __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, floa
 //   backward  dX^T = W^T dY^T    A = W^T (BF16 copy)     B = dY  row-major
 //   weights   dW   = dY^T X      A = dY^T transposed     B = X^T transposed
 // Each D tile (4 consecutive units of one row per lane) is stored to both
-// layouts.  The forward products run the same MFMA sequence as k_gemm
+// layouts.  MFMA 16x16x32 operand layout (one lane = 8 consecutive k):
+//   A: row (lane & 15), k = 8 * (lane >> 4) + e
+//   B: col (lane & 15), k = 8 * (lane >> 4) + e
+//   D: col (lane & 15), row = 4 * (lane >> 4) + e   (e < 4)  The forward products run the same MFMA sequence as k_gemm
 // (K ascending in steps of 32), so the activations equal the unfused path's.
 // Rows past `rows` enter with zero features and a zero loss gradient and
 // add nothing.
@@ -246,7 +142,8 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bf16x8 ld8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
-// D tile (units u0..u0+3 of row r, one lane) to both layouts
+// D tile (units u0..u0+3 of row r, one lane) to the row-major layout and,
+// when tr is given, the transposed one
 __device__ __forceinline__ void store_tile(bf16* rm, int ld, bf16* tr, int u0, int r,
                                            const float (&v)[4])
 {
@@ -254,8 +151,63 @@ __device__ __forceinline__ void store_tile(bf16* rm, int ld, bf16* tr, int u0, i
 #pragma unroll
     for (int e = 0; e < 4; ++e) q[e] = (bf16)v[e];
     *reinterpret_cast<bf16x4*>(rm + r * ld + u0) = q;
+    if (tr) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) tr[(u0 + e) * kTr + r] = q[e];
+        for (int e = 0; e < 4; ++e) tr[(u0 + e) * kTr + r] = q[e];
+    }
+}
+
+// Fourier features of kRows rows (k_features' arithmetic) into the
+// row-major layout and, when xt is given, the transposed one; cell of row r
+// = list ? list[row0 + r] : base + row0 + r; rows from nrows on are zero
+__device__ __forceinline__ void features_rows(const int* __restrict__ list, long long base,
+                                              long long row0, int nrows, int nLat,
+                                              const float* __restrict__ Bff, bf16* x, bf16* xt)
+{
+    const float inv = nLat > 1 ? 1.0f / (float)(nLat - 1) : 0.0f;
+    for (int t = threadIdx.x; t < kRows * (kFeat / 2); t += blockDim.x) {
+        const int r = t / (kFeat / 2), f = t % (kFeat / 2);
+        float sv = 0.0f, cv = 0.0f;
+        if (r < nrows) {
+            const long long cell = list ? (long long)list[row0 + r] : base + row0 + r;
+            const float u = (float)(cell % nLat) * inv;
+            const float v = (float)(cell / nLat) * inv;
+            const float ph = 6.283185307179586f * (Bff[2 * f] * u + Bff[2 * f + 1] * v);
+            sincosf(ph, &sv, &cv);
+        }
+        const bf16 bs = (bf16)sv, bc = (bf16)cv;
+        x[r * kLdX + f] = bs;
+        x[r * kLdX + kFeat / 2 + f] = bc;
+        if (xt) {
+            xt[f * kTr + r] = bs;
+            xt[(kFeat / 2 + f) * kTr + r] = bc;
+        }
+    }
+}
+
+// head outputs (FP32, + bias) of the kRows rows held row-major in h3: this
+// wave's row tile, both unit tiles
+__device__ __forceinline__ void head_rows(const bf16* __restrict__ Wh, const float* __restrict__ bh,
+                                          const bf16* h3, float* out, int ldOut, int wave, int li,
+                                          int kq, int rq)
+{
+    f32x4 acc[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+#pragma unroll
+    for (int k0 = 0; k0 < kHid; k0 += 32) {
+        const bf16x8 x = ld8(h3 + (wave * 16 + li) * kLdH + k0 + kq);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                ld8(Wh + (i * 16 + li) * kHid + k0 + kq), x, acc[i], 0, 0, 0);
+    }
+    const int r = wave * 16 + li;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int u = i * 16 + rq + e;
+            out[r * ldOut + u] = acc[i][e] + bh[u];
+        }
 }
 
 // hidden layer forward: out = relu(W in^T + b), W [kHid][K] (global BF16)
@@ -401,24 +353,8 @@ __global__ __launch_bounds__(256) void k_train_fused(
     float* gp = gpart + (long long)blockIdx.x * stride;
     if (step && blockIdx.x == 0 && tid == 0) *step += 1;
 
-    // Fourier features (k_features' arithmetic); rows past the batch are zero
-    const float inv = nLat > 1 ? 1.0f / (float)(nLat - 1) : 0.0f;
-    for (int t = tid; t < kRows * (kFeat / 2); t += 256) {
-        const int r = t / (kFeat / 2), f = t % (kFeat / 2);
-        float sv = 0.0f, cv = 0.0f;
-        if (r < nrows) {
-            const long long cell = list[row0 + r];
-            const float u = (float)(cell % nLat) * inv;
-            const float v = (float)(cell / nLat) * inv;
-            const float ph = 6.283185307179586f * (Bff[2 * f] * u + Bff[2 * f + 1] * v);
-            sincosf(ph, &sv, &cv);
-        }
-        const bf16 bs = (bf16)sv, bc = (bf16)cv;
-        sX[r * kLdX + f] = bs;
-        sX[r * kLdX + kFeat / 2 + f] = bc;
-        sXT[f * kTr + r] = bs;
-        sXT[(kFeat / 2 + f) * kTr + r] = bc;
-    }
+    // Fourier features; rows past the batch are zero
+    features_rows(list, 0, row0, nrows, nLat, Bff, sX, sXT);
     __syncthreads();
 
     // forward
@@ -428,26 +364,7 @@ __global__ __launch_bounds__(256) void k_train_fused(
     __syncthreads();
     fwd_hidden<kHid>(wbf + qW3, p + pB3, sH[1], kLdH, sH[2], sHT[2], wave, li, kq, rq);
     __syncthreads();
-    {
-        // head: this wave's row tile, both unit tiles, FP32 + bias
-        f32x4 acc[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
-#pragma unroll
-        for (int k0 = 0; k0 < kHid; k0 += 32) {
-            const bf16x8 x = ld8(sH[2] + (wave * 16 + li) * kLdH + k0 + kq);
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    ld8(wbf + qWh + (i * 16 + li) * kHid + k0 + kq), x, acc[i], 0, 0, 0);
-        }
-        const int r = wave * 16 + li;
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int u = i * 16 + rq + e;
-                sOut[r * (kHead + 1) + u] = acc[i][e] + p[pBh + u];
-            }
-    }
+    head_rows(wbf + qWh, p + pBh, sH[2], sOut, kHead + 1, wave, li, kq, rq);
     __syncthreads();
 
     // loss gradient, one row per lane of wave 0
@@ -522,34 +439,22 @@ __global__ __launch_bounds__(256) void k_train_fused(
     }
 }
 
-__global__ void k_fill_f64(int n, double* p, double v)
+// Inference post-processing of one row's head outputs: class = argmax of
+// the logits, chaos regime from the chaos logit, LLE = column 0
+// de-standardised and kept on the side of chaosThr the chaos logit predicts
+// (the regime of a predicted cell is the classifier's; the regression
+// supplies the value).  Class confidence = the class's softmax probability;
+// chaos confidence = the chaos classifier's probability of its answer.
+// With confChaos null, conf = min of the two; else conf = class confidence
+// and confChaos = chaos confidence.
+__device__ __forceinline__ void infer_post_row(const float* o, float lleMean, float lleStd,
+                                               float chaosThr, long long r,
+                                               unsigned char* __restrict__ cls,
+                                               unsigned char* __restrict__ chaos,
+                                               float* __restrict__ conf,
+                                               float* __restrict__ confChaos,
+                                               float* __restrict__ lle)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__global__ void k_fill_f32(int n, float* p, float v)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// Inference post-processing of head outputs: class = argmax of the logits,
-// chaos regime from the chaos logit, LLE = column 0 de-standardised and
-// kept on the side of chaosThr the chaos logit predicts (the regime of a
-// predicted cell is the classifier's; the regression supplies the value).
-// Class confidence = the class's softmax probability; chaos confidence =
-// the chaos classifier's probability of its answer.  With confChaos null,
-// conf = min of the two; else conf = class confidence and confChaos =
-// chaos confidence.
-__global__ void k_infer_post(int rows, const float* __restrict__ out, float lleMean,
-                             float lleStd, float chaosThr, unsigned char* __restrict__ cls,
-                             unsigned char* __restrict__ chaos, float* __restrict__ conf,
-                             float* __restrict__ confChaos, float* __restrict__ lle)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    const float* o = out + (long long)r * kHead;
     int best = 0;
     float mx = o[1];
     for (int k = 2; k <= kClasses; ++k)
@@ -573,6 +478,58 @@ __global__ void k_infer_post(int rows, const float* __restrict__ out, float lleM
         conf[r] = fminf(ck, cc);
     }
     lle[r] = l;
+}
+
+// Fused inference: a workgroup takes kRows rows (cell = list ? list[row] :
+// base + row) through the features, the three layers and the head with the
+// activations in LDS (the forward arithmetic of k_train_fused), then writes
+// the per-row predictions (cls != null) and/or the raw head outputs
+// (rawOut != null, [rows][kHead]).
+__global__ __launch_bounds__(256) void k_infer_fused(
+    long long rows, long long base, const int* __restrict__ list, int nLat,
+    const float* __restrict__ Bff, const bf16* __restrict__ wbf, const float* __restrict__ p,
+    float lleMean, float lleStd, float chaosThr, unsigned char* __restrict__ cls,
+    unsigned char* __restrict__ chaos, float* __restrict__ conf, float* __restrict__ confChaos,
+    float* __restrict__ lle, float* __restrict__ rawOut)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sX[kRows * kLdX];
+    __shared__ __attribute__((aligned(16))) bf16 sH[2][kRows * kLdH];
+    __shared__ float sOut[kRows * (kHead + 1)];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, kq = (lane >> 4) * 8, rq = (lane >> 4) * 4;
+    const long long row0 = (long long)blockIdx.x * kRows;
+    const int nrows = (int)min((long long)kRows, rows - row0);
+
+    features_rows(list, base, row0, nrows, nLat, Bff, sX, nullptr);
+    __syncthreads();
+    fwd_hidden<kFeat>(wbf + qW1, p + pB1, sX, kLdX, sH[0], nullptr, wave, li, kq, rq);
+    __syncthreads();
+    fwd_hidden<kHid>(wbf + qW2, p + pB2, sH[0], kLdH, sH[1], nullptr, wave, li, kq, rq);
+    __syncthreads();
+    fwd_hidden<kHid>(wbf + qW3, p + pB3, sH[1], kLdH, sH[0], nullptr, wave, li, kq, rq);
+    __syncthreads();
+    head_rows(wbf + qWh, p + pBh, sH[0], sOut, kHead + 1, wave, li, kq, rq);
+    __syncthreads();
+    if (rawOut)
+        for (int t = tid; t < nrows * kHead; t += 256)
+            rawOut[(row0 + t / kHead) * kHead + t % kHead] =
+                sOut[(t / kHead) * (kHead + 1) + t % kHead];
+    if (cls && tid < nrows)
+        infer_post_row(sOut + tid * (kHead + 1), lleMean, lleStd, chaosThr, row0 + tid, cls,
+                       chaos, conf, confChaos, lle);
+}
+
+__global__ void k_fill_f64(int n, double* p, double v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f32(int n, float* p, float v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // Wave-aggregated append of the lanes with `take` to list[]: one atomic per
@@ -757,29 +714,6 @@ __global__ __launch_bounds__(256) void k_score(int nFull, int nTot, const int* _
 
 // ------------------------------------------------------------ launchers
 
-hipError_t launch_gemm(int epi, int M, int N, int K, const bf16* A, long long sAm, long long sAk,
-                       const bf16* B, long long sBk, long long sBn, const float* bias, void* C,
-                       long long sCm, hipStream_t st)
-{
-    const dim3 grd((N + 63) / 64, (M + 63) / 64), blk(256);
-    if (epi == kEpiReluBf16)
-        hipLaunchKernelGGL((k_gemm<kEpiReluBf16>), grd, blk, 0, st, M, N, K, A, sAm, sAk, B, sBk,
-                           sBn, bias, C, sCm);
-    else
-        hipLaunchKernelGGL((k_gemm<kEpiBiasF32>), grd, blk, 0, st, M, N, K, A, sAm, sAk, B, sBk,
-                           sBn, bias, C, sCm);
-    return hipGetLastError();
-}
-
-hipError_t launch_features(int rows, int n, const int* list, long long base, const float* Bff,
-                           bf16* X, hipStream_t st)
-{
-    const long long th = (long long)rows * (kFeat / 2);
-    hipLaunchKernelGGL(k_features, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, st, rows, n,
-                       list, base, Bff, X);
-    return hipGetLastError();
-}
-
 hipError_t launch_sum_splits(int n, int splits, const float* part, float* out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sum_splits, dim3((n + 255) / 256), dim3(256), 0, st, n, splits, part,
@@ -811,6 +745,20 @@ hipError_t launch_adam(int n, float* p, float* m, float* v, const float* gpart, 
 
 int train_fused_blocks(int rows) { return (rows + kRows - 1) / kRows; }
 
+hipError_t launch_infer_fused(long long rows, long long base, const int* list, int nLat,
+                              const float* Bff, const bf16* wbf, const float* p, float lleMean,
+                              float lleStd, float chaosThr, unsigned char* cls,
+                              unsigned char* chaos, float* conf, float* confChaos, float* lle,
+                              float* rawOut, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    const long long blocks = (rows + kRows - 1) / kRows;
+    hipLaunchKernelGGL(k_infer_fused, dim3((unsigned)blocks), dim3(256), 0, st, rows, base, list,
+                       nLat, Bff, wbf, p, lleMean, lleStd, chaosThr, cls, chaos, conf, confChaos,
+                       lle, rawOut);
+    return hipGetLastError();
+}
+
 hipError_t launch_train_fused(int rows, int nLat, const int* list, const int* tlist,
                               const float* Bff, const bf16* wbf, const bf16* wbfT, const float* p,
                               const int* cls, const float* lle, float wLle, float chaosStd,
@@ -824,14 +772,7 @@ hipError_t launch_train_fused(int rows, int nLat, const int* list, const int* tl
     return hipGetLastError();
 }
 
-hipError_t launch_infer_post(int rows, const float* out, float mean, float sd, float chaosThr,
-                             unsigned char* cls, unsigned char* chaos, float* conf,
-                             float* confChaos, float* lle, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_infer_post, dim3((rows + 255) / 256), dim3(256), 0, st, rows, out, mean,
-                       sd, chaosThr, cls, chaos, conf, confChaos, lle);
-    return hipGetLastError();
-}
+
 
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,

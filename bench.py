@@ -202,10 +202,11 @@ def run_lle(bh, np, args, out):
     1e-8 along x."""
     n = args.n or 1024
     cells = n * n
+    lle = dict(LLE, NT=args.nt)
     d64 = bh.DeviceArray((cells,), np.float64)
     d32 = bh.DeviceArray((cells,), np.float64)
-    lle_args = (2, LLE["tMax"], LLE["NT"], n, H, LLE["eps"], IC, RANGES, [0, 2], 0.0,
-                LLE["transient"], VALS)
+    lle_args = (2, lle["tMax"], lle["NT"], n, H, lle["eps"], IC, RANGES, [0, 2], 0.0,
+                lle["transient"], VALS)
 
     def timed(dst, prec, steps, kernel):
         bh.set_precision(prec)
@@ -231,9 +232,9 @@ def run_lle(bh, np, args, out):
     with np.errstate(invalid="ignore"):
         sign = float((np.sign(a[fin]) == np.sign(b[fin])).mean())
         chaos = float(((a > CHAOS) == (b > CHAOS)).mean())
-    skip = int(LLE["transient"] / H)
-    nNT = int(LLE["NT"] / H)
-    nb = int(LLE["tMax"] / LLE["NT"])
+    skip = int(lle["transient"] / H)
+    nNT = int(lle["NT"] / H)
+    nb = int(lle["tMax"] / lle["NT"])
     flop32 = (LLE32_STEP_FLOP * (skip + nb * nNT) + LLE32_DEV_FLOP * nb * nNT) * cells
     ach = flop32 * n32 / (k32 / 1000) / 1e12
     out.update(metric="grid points/s, 2D LLE map, FP32 bulk vs FP64", value=cells / (ms32 / 1000),
@@ -241,7 +242,7 @@ def run_lle(bh, np, args, out):
                config={"workload": f"2D LLE map, Rossler {n}x{n} (a, c) lattice, FP32 bulk "
                                    f"integration with FP64 norm / log / renormalisation, "
                                    f"perturbation 1e-8 along x",
-                       "grid": n, "ranges": RANGES, "h": H, "lle": LLE})
+                       "grid": n, "ranges": RANGES, "h": H, "lle": lle})
     out["fp64_reference"] = {"ms": ms64, "value": cells / (ms64 / 1000),
                              "kernel_ms": k64 / max(n64, 1), "speedup_fp32_vs_fp64": ms64 / ms32}
     out["error_vs_fp64"] = {"cells_compared": int(fin.sum()),
@@ -286,6 +287,9 @@ def main():
                     choices=["surrogate", "plain", "lle", "tiles", "copy"])
     ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
     ap.add_argument("--coarse", type=int, default=256)
+    ap.add_argument("--nt", type=float, default=LLE["NT"],
+                    help="lle workload: renormalisation interval in time units (0.5 in the "
+                         "configuration; larger values isolate the renormalisations' cost)")
     ap.add_argument("--conf", type=float, default=0.95, help="class softmax confidence threshold")
     ap.add_argument("--chaos-conf", type=float, default=0.98,
                     help="chaos-logit confidence threshold")

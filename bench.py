@@ -18,6 +18,10 @@ Workloads (one step = one whole map, inputs resident, outputs in HBM):
   plain      the plain FP64 fused LLE + peak-count map on an n x n lattice.
   lle        the 1024^2 LLE map with FP32 bulk integration, timed against
              the plain FP64 LLE map, with its error and sign agreement.
+  dbscan     the 512^2 DBSCAN peak-count map (capped at 32), plain FP64,
+             with sampled rows checked bit for bit against the oracle.
+  bif1d      the 1000-point 1D bifurcation diagram through the reference's
+             driver, every cell checked against the oracle (CPU baseline).
   tiles      Lyapunov spectrum of 4096 uneven 16x16 refinement tiles (FP32
              variational state, FP64 Gram-Schmidt), autotuned, one
              persistent graph-replayed launch per step.
@@ -260,6 +264,93 @@ def run_lle(bh, np, args, out):
                                f"instructions"}
 
 
+BIF_FLOP_PER_CELL = STEP_FLOP * int((PEAK["transient"] + PEAK["tMax"]) / H)   # 1.8 MFLOP
+
+
+def run_dbscan(bh, np, args, out):
+    """2D bifurcation map: DBSCAN cluster count of the (amplitude, interval)
+    peak pairs of x per cell, plain FP64, Rossler n x n over the (a, c) box,
+    transient 1000, record 500, eps 0.01, counts capped at 32 (the
+    reference's maxAmountOfPeaks is a threshold, so it is set to the record
+    length and the cap is the separate clamp).  Sampled rows are compared
+    with the oracle bit for bit."""
+    import orclib as O
+    n = args.n or 512
+    cells = n * n
+    d = bh.DeviceArray((cells,), np.int32)
+    bh.set_cluster_clamp(32)
+    dargs = (d.ptr, PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(), PEAK["transient"],
+             VALS, 1, PEAK["eps"])
+    for _ in range(args.warmup):
+        bh.dbscan_map_device(*dargs)
+    bh.synchronize()
+    bh.set_timing(True)
+    bh.reset_kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bh.dbscan_map_device(*dargs)
+    bh.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    kms, nl = bh.kernel_time("bif_peaks")
+    dbms, _ = bh.kernel_time("dbscan")
+    got = d.to_host().reshape(n, n)
+    rows = [0, n // 3, n - 1]
+    exact = True
+    for r in rows:
+        want = O.dbscan_cells(PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(),
+                              PEAK["transient"], VALS, 1, PEAK["eps"], r * n, n)
+        exact &= bool(np.array_equal(got[r], np.minimum(want, 32)))
+    bh.set_cluster_clamp(0)
+    ach = BIF_FLOP_PER_CELL * cells * args.steps / (kms / 1000) / 1e12
+    out.update(metric="grid points/s, 2D DBSCAN peak-count map (plain FP64)",
+               value=cells / (ms / 1000), unit="grid points/s", ms_per_step=ms, dtype="f64",
+               config={"workload": f"2D bifurcation map, Rossler {n}x{n} (a, c) lattice, DBSCAN "
+                                   f"peak count of x capped at 32, plain FP64",
+                       "grid": n, "ranges": RANGES, "h": H, "peak": PEAK, "cap": 32})
+    out["oracle_rows_bit_exact"] = {"rows": rows, "equal": exact}
+    out["kernels_ms_per_step"] = {"bif_peaks": kms / args.steps, "dbscan": dbms / args.steps}
+    out["roofline"] = {"bound": "valu_fp64", "kernel": "k_bif_peaks", "achieved": ach,
+                       "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                       "frac": ach / (FP64_VALU_PEAK / 1e12), "traffic": None,
+                       "avg_launch_ms": kms / max(nl, 1), "launches": nl,
+                       "note": "12 FLOP per Rossler step, 150 000 steps per cell; no-contraction "
+                               "FP64 reaches at most 50% of the FMA-counted peak"}
+
+
+def run_bif1d(bh, np, args, out):
+    """1D bifurcation diagram: local maxima of x for c over [2, 7] in 1000
+    points (a = b = 0.2), transient 1000, record 500, plain FP64 through the
+    reference's driver with host buffers (the time includes the copy of the
+    peak rows back to the host).  Every cell is compared with the oracle,
+    which is also timed as the CPU baseline."""
+    import orclib as O
+    n = 1000
+    vals = [0.2, 0.2, 0.0]
+    bargs = (PEAK["tMax"], n, H, IC, [2.0, 7.0], [2], 0, 0.0, PEAK["transient"], vals, 1)
+    for _ in range(args.warmup):
+        bh.bifurcation1D_to_buffer(*bargs)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cnt, pk = bh.bifurcation1D_to_buffer(*bargs)
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    O.lib()
+    t1 = time.perf_counter()
+    wc, wp, _ = O.peaks_cells(1, PEAK["tMax"], n, H, IC, [2.0, 7.0], [2], 0, 0.0,
+                              PEAK["transient"], vals, 1, 0, n, pk.shape[1])
+    cpu_s = time.perf_counter() - t1
+    exact = bool(np.array_equal(cnt, np.maximum(wc, 0))) and all(
+        np.array_equal(pk[k, :cnt[k]], wp[k, :cnt[k]]) for k in range(n))
+    out.update(metric="parameter points/s, 1D bifurcation diagram (plain FP64)",
+               value=n / (ms / 1000), unit="parameter points/s", ms_per_step=ms, dtype="f64",
+               config={"workload": "1D bifurcation diagram, Rossler c in [2, 7] x 1000 points, "
+                                   "a = b = 0.2, transient 1000, record 500, host buffers",
+                       "h": H, "peak": PEAK})
+    out["oracle_bit_exact_all_cells"] = exact
+    out["cpu_baseline"] = {"value": n / cpu_s, "unit": "parameter points/s",
+                           "cores": O.lib().orc_threads(), "kind": "port",
+                           "sample": f"all {n} cells, {cpu_s:.2f} s"}
+
+
 def run_copy(bh, np, args, out):
     """Device-to-device copy of a known size: the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters into bytes."""
@@ -284,7 +375,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="surrogate",
-                    choices=["surrogate", "plain", "lle", "tiles", "copy"])
+                    choices=["surrogate", "plain", "lle", "tiles", "dbscan", "bif1d", "copy"])
     ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
     ap.add_argument("--coarse", type=int, default=256)
     ap.add_argument("--nt", type=float, default=LLE["NT"],
@@ -328,8 +419,10 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (a, c) parameter lattice of the Rossler system"}
 
-    if args.workload in ("tiles", "copy", "lle"):
-        {"tiles": run_tiles, "copy": run_copy, "lle": run_lle}[args.workload](bh, np, args, out)
+    extra = {"tiles": run_tiles, "copy": run_copy, "lle": run_lle, "dbscan": run_dbscan,
+             "bif1d": run_bif1d}
+    if args.workload in extra:
+        extra[args.workload](bh, np, args, out)
         print(json.dumps(out), flush=True)
         return
     if args.workload == "plain":

@@ -1047,23 +1047,31 @@ int bif1d_impl(double tMax, int nPts, double h, int nIC, const double* ic, const
     RC_TRY(peak_spec(S, 1, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime,
                      values, nVal, pre));
     progress("BIFURCATION1D", S.cells);
-    stride = std::max(1, S.N / 2 + 1);
+    // device rows hold every peak a record can have; only the used prefix
+    // of each row (the largest count) comes back to the host
+    const int rowCap = std::max(1, S.N / 2 + 1);
     DBuf c, a;
     HIP_TRY(c.alloc(sizeof(int) * (size_t)S.cells));
-    HIP_TRY(a.alloc(sizeof(double) * (size_t)S.cells * stride));
-    RC_TRY(integrate_peaks(S, 0, nullptr, (int)S.cells, stride, c.as<int>(), a.as<double>(),
-                           nullptr));
+    HIP_TRY(a.alloc(sizeof(double) * (size_t)S.cells * rowCap));
+    int* d_c = c.as<int>();
+    double* d_a = a.as<double>();
+    RC_TRY(integrate_peaks(S, 0, nullptr, (int)S.cells, rowCap, d_c, d_a, nullptr));
     counts.resize((size_t)S.cells);
-    peaks.resize((size_t)S.cells * stride);
-    HIP_TRY(hipMemcpyAsync(counts.data(), c.p, sizeof(int) * counts.size(), hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(counts.data(), d_c, sizeof(int) * counts.size(), hipMemcpyDeviceToHost,
                            g.stream));
-    HIP_TRY(hipMemcpyAsync(peaks.data(), a.p, sizeof(double) * peaks.size(),
-                           hipMemcpyDeviceToHost, g.stream));
-    RC_TRY(finish());
+    HIP_TRY(hipStreamSynchronize(g.stream));
     // peakFinderCUDA turns the divergence flag into 0 peaks
-    for (auto& v : counts)
+    int used = 1;
+    for (auto& v : counts) {
         if (v < 0) v = 0;
-    return BIFHIP_OK;
+        used = std::max(used, v);
+    }
+    stride = used;
+    peaks.resize((size_t)S.cells * stride);
+    HIP_TRY(hipMemcpy2DAsync(peaks.data(), sizeof(double) * stride, d_a, sizeof(double) * rowCap,
+                             sizeof(double) * stride, (size_t)S.cells, hipMemcpyDeviceToHost,
+                             g.stream));
+    return finish();
 }
 
 int cluster_impl(int kind, double tMax, int nPts, double h, int nIC, const double* ic,

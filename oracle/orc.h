@@ -12,20 +12,26 @@
  * so every double operation is one IEEE-754 binary64 rounding in the
  * reference's source order, with no FMA contraction.
  *
- * Pinning (what checks each function; see tests/test_oracle_*.py):
- *   orc_step                 closed form of the linear model; brute-force
- *                            hex fixtures of Rossler samples (tests/golden)
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *   orc_step                 exact rational evaluation of the sequential
+ *                            Euler step (a simultaneous-Euler step fails it)
  *   orc_loop                 record-before-step / divergence KATs
  *   orc_value_by_idx         closed form a_i = s + i*(f-s)/(n-1)
- *   orc_peak_finder          survey KAT {0,1,3,3,2,0,1,4,1,...}; brute-force
+ *   orc_peak_finder          reference KAT {0,1,3,3,2,0,1,4,1,...}; brute-force
  *                            textbook comparison on plateau-free series
  *   orc_distance             glibc hypotf == (float)sqrt(double) check
- *   orc_dbscan               survey KATs ({1,2}->1, {1..8}->7, ...); Rossler
+ *   orc_dbscan               reference KATs ({1,2}->1, {1..8}->7, ...); Rossler
  *                            cluster counts at c=2.5,3.5,4.0,5.7,9.0
  *   orc_kde                  single-cluster / separated-cluster closed cases
  *   orc_gram_schmidt         orthonormality, |det| = prod(norms), numpy QR
  *   orc_lle_cell             linear model: LLE = log|1+h*lambda|/h
  *   orc_ls_cell              linear model spectrum; volume-contraction sum
+ *   orc_ls_var_cell          agreement with the finite-difference spectrum
+ *   memristive orc_step      first-order convergence to its ODE (solve_ivp)
+ *                            for explicit shares 1, 1/2, 0; float state
+ *   orc_ostrovsky_cells      signs over the reference Rossler counts;
+ *                            peak-free cells follow the last sample
+ *   orc_denis                linear-model closed form per sample
  */
 #ifndef BIFHIP_ORACLE_H
 #define BIFHIP_ORACLE_H

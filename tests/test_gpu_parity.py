@@ -119,7 +119,7 @@ def test_peak_finder_reference_kat_gpu(bh, dev):
 
 def test_bifurcation1d_config1_bit_exact(orc, bh):
     """1D diagram, Rossler c in [2,7] x 1000, transient 1000, record 500."""
-    say("bifurcation1D config-1")
+    say("bifurcation1D c in [2, 7] x 1000")
     cnt, pk = bh.bifurcation1D_to_buffer(500.0, 1000, 0.01, IC, [2.0, 7.0], [2], 0, 0.0, 1000.0,
                                          [0.2, 0.2, 0.0], 1)
     cap = pk.shape[1]

@@ -384,6 +384,9 @@ def main():
     ap.add_argument("--conf", type=float, default=0.95, help="class softmax confidence threshold")
     ap.add_argument("--chaos-conf", type=float, default=0.98,
                     help="chaos-logit confidence threshold")
+    ap.add_argument("--lle32-margin", type=float, default=0.02,
+                    help="regime-only re-solves: FP32 bulk LLE at least this far from the "
+                         "chaos threshold, FP64 nearer (negative: all FP64)")
     ap.add_argument("--validation", type=float, default=0.002)
     ap.add_argument("--batch", type=int, default=2048,
                     help="training batch rows (20 epochs of the 256^2 coarse map)")
@@ -411,6 +414,7 @@ def main():
     bh.set_graph(not args.no_graph)
     bh.set_partial_resolve(not args.full_resolve)
     bh.set_chaos_confidence(args.chaos_conf)
+    bh.set_lle32_margin(args.lle32_margin)
     n = args.n or (4096 if args.workload == "surrogate" else 1024)
     cells = n * n
     d_peak = bh.DeviceArray((cells,), np.int32)
@@ -473,19 +477,35 @@ def main():
         # peak-count-only re-solves run k_bif_peaks and are not counted here
         fused_cells = sum(args.coarse ** 2 + st["resolved"] + st["validation"] for st in stats)
         rf = roofline(bh, fused_cells)
-        s_cls, s_ch = regimes(np, d_peak.to_host(), d_lle.to_host())
+        s_lle = d_lle.to_host()
+        s_cls, s_ch = regimes(np, d_peak.to_host(), s_lle)
         agree = float(((s_cls == t_cls) & (s_ch == t_ch)).mean())
         last = stats[-1]
+        # regime-only cells that kept the FP32 bulk LLE: its error against
+        # the FP64 map and the regime it gives there
+        res = d_res.to_host()
+        k32 = (res == 6) | (res == 7)
+        e32 = np.abs(s_lle[k32] - ref_lle[k32])
+        e32 = e32[np.isfinite(e32)]
+        fp32_lle = {"cells": int(k32.sum()),
+                    "lle_abs_err_mean": float(e32.mean()) if e32.size else 0.0,
+                    "lle_abs_err_max": float(e32.max()) if e32.size else 0.0,
+                    "regime_agreement": float((s_ch[k32] == t_ch[k32]).mean())
+                    if k32.any() else 1.0}
         out.update(value=cells / (ms / 1000), unit="grid points/s", ms_per_step=ms,
                    config={"workload": f"surrogate-refined 2D LLE + DBSCAN peak-count map, "
                                        f"Rossler {n}x{n} (coarse {args.coarse}x{args.coarse}, "
                                        f"20 epochs; class confidence < {args.conf} or class "
                                        f"boundary: peak count re-solved in FP64; chaos "
                                        f"confidence < {args.chaos_conf} or regime boundary: "
-                                       f"LLE re-solved in FP64, with the peak count when the "
-                                       f"class is flagged too)",
+                                       f"LLE re-solved, FP32 bulk where it lies >= "
+                                       f"{args.lle32_margin} from the threshold and FP64 "
+                                       f"nearer, with the FP64 peak count when the class is "
+                                       f"flagged too or the regime changed)",
                            "grid": n, "coarse": args.coarse, "ranges": RANGES, "h": H,
                            "peak": PEAK, "lle": LLE, "chaos_threshold": CHAOS})
+        if args.lle32_margin >= 0:
+            out["dtype"] = "f64+f32"
         out["surrogate"] = {
             "regime_agreement": agree,
             "reference_boundary_fraction": boundary_fraction(np, t_cls, t_ch, n),
@@ -497,6 +517,9 @@ def main():
             "peak_only_fraction": last["peak_only"] / cells,
             "lle_only_fraction": last["lle_only"] / cells,
             "lle_only_recounted_fraction": last["lle_only_recounted"] / cells,
+            "lle_only_fp32_fraction": last["lle_only_fp32"] / cells,
+            "lle32_margin": args.lle32_margin,
+            "fp32_lle_cells": fp32_lle,
             "surrogate_on_lle_only": {"regime_acc": last["lle_only_regime_acc"]},
             "surrogate_on_peak_only": {"class_acc": last["peak_only_class_acc"]},
             "partial_resolve": not args.full_resolve,

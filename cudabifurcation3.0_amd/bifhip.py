@@ -115,6 +115,7 @@ def _declare(L):
         "bifhip_set_graph": [_I],
         "bifhip_set_partial_resolve": [_I],
         "bifhip_set_chaos_confidence": [_D],
+        "bifhip_set_lle32_margin": [_D],
         "bifhip_lle_peak_map_device": [_I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I, _I, _D,
                                        _D, _D, _D, _D, _VP, _VP],
         "bifhip_surrogate_map_device": [_I, _I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I,
@@ -278,6 +279,7 @@ def reset_context():
     set_graph(True)
     set_partial_resolve(True)
     set_chaos_confidence(0.98)
+    set_lle32_margin(0.02)
 
 
 def device_count():
@@ -540,7 +542,8 @@ SSTAT = ["ms_total", "ms_coarse", "ms_train", "ms_infer", "ms_resolve", "resolve
          "resolved_frac", "validation", "flagged_class_acc", "flagged_regime_acc",
          "flagged_lle_mae", "valid_class_acc", "valid_regime_acc", "valid_lle_mae",
          "loss_ce", "loss_mse", "lle_mean", "lle_sd", "peak_only", "peak_only_class_acc", "ms_flag",
-         "ms_solve", "ms_score", "lle_only", "lle_only_regime_acc", "lle_only_recounted"]
+         "ms_solve", "ms_score", "lle_only", "lle_only_regime_acc", "lle_only_recounted",
+         "lle_only_fp32"]
 
 
 def _pf(a):
@@ -644,6 +647,13 @@ def set_partial_resolve(on):
 
 def set_chaos_confidence(threshold):
     _check(load().bifhip_set_chaos_confidence(float(threshold)))
+
+
+def set_lle32_margin(margin):
+    """Regime-only re-solves of the surrogate map: FP32 bulk LLE where it
+    lies at least `margin` from the chaos threshold, FP64 elsewhere; a
+    negative margin keeps them all in FP64."""
+    _check(load().bifhip_set_lle32_margin(float(margin)))
 
 
 # --------------------------------------------------------------- writers

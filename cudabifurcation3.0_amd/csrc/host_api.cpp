@@ -805,13 +805,15 @@ int surrogate_resolve_peaks(int nPts, double h, int nIC, const double* ic, const
     return peak_cells_device(S, maxPeaks, dbEps, h_list, d_list, 0, n, d_peak);
 }
 
-// Plain FP64 LLE (LLE2D semantics) of n listed cells (flattened indices
+// Plain LLE (LLE2D semantics) of n listed cells (flattened indices
 // d_list[0..n) on an nPts x nPts lattice) into d_lle[0..n): the surrogate's
-// re-solve of cells whose class is predicted with confidence.
+// re-solve of cells whose class is predicted with confidence.  precision
+// BIFHIP_PREC_FP64 is the reference's FP64 kernel, BIFHIP_PREC_FP32 the
+// FP32 bulk kernel (k_lle32).
 int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const double* ranges,
                           const int* mut, const double* values, int nVal, double lleTMax,
                           double NT, double lleTransient, double lleEps, const int* d_list, int n,
-                          double* d_lle)
+                          double* d_lle, int precision)
 {
     if (n <= 0) return BIFHIP_OK;
     if (g.model == BIFHIP_MODEL_MEMRISTIVE)
@@ -855,8 +857,8 @@ int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const d
             A.dirs = d_dirs;
         }
         {
-            Timed t("lle");
-            HIP_TRY(launch_lle(g.model, BIFHIP_PREC_FP64, A, false, d_lle + off, g.stream));
+            Timed t(precision == BIFHIP_PREC_FP32 ? "lle32" : "lle");
+            HIP_TRY(launch_lle(g.model, precision, A, false, d_lle + off, g.stream));
         }
         if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
     }

@@ -29,6 +29,7 @@ struct Ctx {
     int sweepIC = 0;           // swept entries index the initial conditions (IC drivers)
     int partialResolve = 1;    // surrogate: class-only cells re-solve the peak count alone
     double chaosConfidence = 0.98;  // surrogate: chaos confidence below this re-solves the LLE
+    double lle32Margin = 0.02;  // surrogate: FP32 regime re-solves this far from the threshold
     std::string lastError;
     hipStream_t stream = nullptr;
 };

@@ -59,6 +59,9 @@ hipError_t launch_fill_pred(long long n, const unsigned char* cls, const float* 
 hipError_t launch_regime_changed(int n, const int* list, const double* lle,
                                  const unsigned char* pch, double chaosThr, int* out, int* count,
                                  unsigned char* flag, hipStream_t st);
+hipError_t launch_lle32_split(int n, const int* list, const double* lle32, double chaosThr,
+                              double margin, int* cells, int* pos, int* count,
+                              unsigned char* flag, hipStream_t st);
 hipError_t launch_score(int nFull, int nTot, const int* list, const int* rpeak, const double* rlle,
                         int nPeak, const int* plist, const int* ppeak, int nLle,
                         const int* llist, const double* llle, const unsigned char* pcls,
@@ -79,7 +82,7 @@ int surrogate_resolve_peaks(int nPts, double h, int nIC, const double* ic, const
 int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const double* ranges,
                           const int* mut, const double* values, int nVal, double lleTMax,
                           double NT, double lleTransient, double lleEps, const int* d_list, int n,
-                          double* d_lle);
+                          double* d_lle, int precision);
 }  // namespace host
 }  // namespace bifhip
 
@@ -516,8 +519,8 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     RC_TRY(pool_get("sg_vlist", (size_t)(validationFraction > 0 ? nf : 1), &d_vlist));
     RC_TRY(pool_get("sg_plist", (size_t)(partial ? nf : 1), &d_plist));
     RC_TRY(pool_get("sg_llist", (size_t)(partial ? nf : 1), &d_llist));
-    RC_TRY(pool_get("sg_count", 5, &d_count));
-    HIP_TRY(launch_fill(5, d_count, 0, s));
+    RC_TRY(pool_get("sg_count", 6, &d_count));
+    HIP_TRY(launch_fill(6, d_count, 0, s));
     const double vf = std::min(1.0, std::max(0.0, validationFraction));
     const unsigned int vthr =
         vf > 0 ? (unsigned int)std::min(4294967295.0, std::floor(vf * 4294967296.0)) : 0u;
@@ -550,11 +553,37 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
         RC_TRY(surrogate_resolve_peaks(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
                                        peakTransient, wv, maxPeaks, dbEps, nullptr, d_plist,
                                        npeak, d_ppeak));
-    int nchg = 0;
+    int nchg = 0, nnear = nlle;
     int *d_clist = nullptr, *d_cpeak2 = nullptr;
-    if (nlle > 0) {
+    const bool lle32 = g.lle32Margin >= 0 && g.model != BIFHIP_MODEL_MEMRISTIVE;
+    if (nlle > 0 && lle32) {
+        // FP32 bulk LLE of every regime-only cell; the cells it leaves within
+        // the margin of the threshold are re-solved in FP64 and put back in
+        // their list positions
         RC_TRY(surrogate_resolve_lle(nFine, h, nIC, ic, ranges, mut, values, nVal, lleTMax, NT,
-                                     lleTransient, lleEps, d_llist, nlle, d_llle));
+                                     lleTransient, lleEps, d_llist, nlle, d_llle,
+                                     BIFHIP_PREC_FP32));
+        int *d_ncell, *d_npos;
+        double* d_nlle;
+        RC_TRY(pool_get("sg_ncell", (size_t)nlle, &d_ncell));
+        RC_TRY(pool_get("sg_npos", (size_t)nlle, &d_npos));
+        HIP_TRY(launch_lle32_split(nlle, d_llist, d_llle, chaosThreshold, g.lle32Margin, d_ncell,
+                                   d_npos, d_count + 5, d_flag, s));
+        HIP_TRY(hipMemcpyAsync(&nnear, d_count + 5, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (nnear > 0) {
+            RC_TRY(pool_get("sg_nlle", (size_t)nnear, &d_nlle));
+            RC_TRY(surrogate_resolve_lle(nFine, h, nIC, ic, ranges, mut, values, nVal, lleTMax,
+                                         NT, lleTransient, lleEps, d_ncell, nnear, d_nlle,
+                                         BIFHIP_PREC_FP64));
+            HIP_TRY(launch_scatter(nnear, d_npos, nullptr, d_nlle, nullptr, d_llle, s));
+        }
+    } else if (nlle > 0) {
+        RC_TRY(surrogate_resolve_lle(nFine, h, nIC, ic, ranges, mut, values, nVal, lleTMax, NT,
+                                     lleTransient, lleEps, d_llist, nlle, d_llle,
+                                     BIFHIP_PREC_FP64));
+    }
+    if (nlle > 0) {
         // a regime that changed under the re-solve: the peak count as well
         RC_TRY(pool_get("sg_clist", (size_t)nlle, &d_clist));
         HIP_TRY(launch_regime_changed(nlle, d_llist, d_llle, d_pch, chaosThreshold, d_clist,
@@ -598,7 +627,8 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     HIP_TRY(launch_scatter(nlle, d_llist, nullptr, d_llle, nullptr, d_lle, s));
     HIP_TRY(launch_scatter(nchg, d_clist, d_cpeak2, nullptr, d_peak, nullptr, s));
     // 1 re-solved in full, 2 peak count only, 3 validation sample, 4 LLE
-    // only, 5 LLE then peak count (regime changed), 0 predicted
+    // only, 5 LLE then peak count (regime changed), 6 and 7 as 4 and 5 with
+    // the FP32 bulk LLE, 0 predicted
     if (d_resolved)
         HIP_TRY(hipMemcpyAsync(d_resolved, d_flag, (size_t)nf, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -610,6 +640,7 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     st[BIFHIP_SSTAT_PEAK_ONLY] = npeak;
     st[BIFHIP_SSTAT_LLE_ONLY] = nlle;
     st[BIFHIP_SSTAT_LLE_ONLY_RECOUNTED] = nchg;
+    st[BIFHIP_SSTAT_LLE_ONLY_FP32] = nlle - nnear;
     st[BIFHIP_SSTAT_MS_TOTAL] = ms_since(t0);
     st[BIFHIP_SSTAT_LLE_MEAN] = mean;
     st[BIFHIP_SSTAT_LLE_SD] = sd;

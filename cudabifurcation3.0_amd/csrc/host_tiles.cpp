@@ -200,6 +200,13 @@ int bifhip_set_chaos_confidence(double threshold)
     return BIFHIP_OK;
 }
 
+int bifhip_set_lle32_margin(double margin)
+{
+    if (std::isnan(margin)) return fail(BIFHIP_E_ARG, "FP32 LLE margin is NaN");
+    g.lle32Margin = margin;
+    return BIFHIP_OK;
+}
+
 static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, const int* originJ,
                       const int* nBlocks, int nPts, const double* ranges, const int* mut,
                       const double* ic, int nIC, const double* values, int nVal, double h,

@@ -624,8 +624,36 @@ __global__ void k_regime_changed(int n, const int* __restrict__ list,
     const bool live = t < n;   // no early return: the append is wave-wide
     const int cell = live ? list[t] : 0;
     const bool take = live && ((lle[t] > chaosThr) != (pch[cell] != 0));
-    if (take) flag[cell] = 5;
+    if (take) flag[cell] = flag[cell] == 6 ? 7 : 5;
     wave_append(take, cell, out, count);
+}
+
+// FP32/FP64 split of the regime-only re-solves.  A listed cell whose FP32
+// bulk LLE lies at least `margin` from the chaos threshold keeps it (flag
+// 6); the others, NaN included, are appended for the FP64 re-solve: the
+// cell index to cells[], its position in the list to pos[] (same slot).
+__global__ void k_lle32_split(int n, const int* __restrict__ list,
+                              const double* __restrict__ lle32, double chaosThr, double margin,
+                              int* __restrict__ cells, int* __restrict__ pos,
+                              int* __restrict__ count, unsigned char* __restrict__ flag)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < n;   // no early return: the append is wave-wide
+    const int cell = live ? list[t] : 0;
+    const bool near = live && !(fabs(lle32[t] - chaosThr) >= margin);
+    if (live && !near) flag[cell] = 6;
+    const uint64_t m = __ballot(near);
+    if (!m) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __builtin_ctzll(m);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader);
+    if (near) {
+        const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+        cells[slot] = cell;
+        pos[slot] = t;
+    }
 }
 
 // Write re-solved cells into the output maps (the peak count when peak is
@@ -816,6 +844,16 @@ hipError_t launch_regime_changed(int n, const int* list, const double* lle,
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_regime_changed, dim3((n + 255) / 256), dim3(256), 0, st, n, list, lle,
                        pch, chaosThr, out, count, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_lle32_split(int n, const int* list, const double* lle32, double chaosThr,
+                              double margin, int* cells, int* pos, int* count,
+                              unsigned char* flag, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lle32_split, dim3((n + 255) / 256), dim3(256), 0, st, n, list, lle32,
+                       chaosThr, margin, cells, pos, count, flag);
     return hipGetLastError();
 }
 

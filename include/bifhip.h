@@ -411,6 +411,13 @@ int bifhip_set_partial_resolve(int on);
 /* Surrogate-refined map: cells whose chaos-logit confidence is below this
  * threshold (in [0.5, 1], default 0.98) have their LLE re-solved. */
 int bifhip_set_chaos_confidence(double threshold);
+/* Surrogate-refined map: cells re-solved for the regime only take their LLE
+ * from the FP32 bulk kernel (FP32 state and separation, FP64 norms and log
+ * sums, as bifhip_set_precision(BIFHIP_PREC_FP32) computes the LLE map);
+ * those whose FP32 LLE lies within `margin` of chaosThreshold, or is NaN,
+ * are re-solved again in FP64.  Default 0.02; a negative margin keeps
+ * every regime re-solve in FP64.  The memristive model always uses FP64. */
+int bifhip_set_lle32_margin(double margin);
 
 /* ------------------------------------------ surrogate-refined map */
 
@@ -430,17 +437,19 @@ int bifhip_set_chaos_confidence(double threshold);
  * (bifhip_set_partial_resolve, default), cells that need only their class
  * get the plain FP64 DBSCAN peak count (bifurcation2DDBSCAN semantics) and
  * keep the predicted LLE, and cells that need only their regime get the
- * plain FP64 LLE (LLE2D semantics) and keep the predicted class unless the
- * re-solved regime differs from the predicted one, in which case they get
- * the peak count too; with them off, every flagged cell is re-solved in
- * full.
+ * plain LLE (LLE2D semantics; FP32 bulk away from the threshold, FP64
+ * near it, bifhip_set_lle32_margin) and keep the predicted class unless
+ * the re-solved regime differs from the predicted one, in which case they
+ * get the FP64 peak count too; with them off, every flagged cell is
+ * re-solved in full.
  * The re-solves score the surrogate; a validation sample -- each remaining
  * cell drawn with probability validationFraction by a hash of its index and
  * the seed -- is re-solved in full to score it there.  Outputs: d_peak[nFine^2] (re-solved cells: DBSCAN count;
  * others: predicted class + 1 in 1..16), d_lle[nFine^2],
  * d_resolved[nFine^2] (1 re-solved in full, 2 peak count only, 3 validation
- * sample re-solved in full, 4 LLE only, 5 LLE then peak count, 0
- * predicted; may be NULL),
+ * sample re-solved in full, 4 LLE only, 5 LLE then peak count, 6 and 7 as
+ * 4 and 5 with the LLE from the FP32 bulk kernel, 0 predicted; may be
+ * NULL),
  * stats[BIFHIP_SSTAT_COUNT]. */
 #define BIFHIP_SSTAT_MS_TOTAL 0
 #define BIFHIP_SSTAT_MS_COARSE 1
@@ -468,7 +477,8 @@ int bifhip_set_chaos_confidence(double threshold);
 #define BIFHIP_SSTAT_LLE_ONLY 23           /* cells re-solved for the regime only */
 #define BIFHIP_SSTAT_LLE_ONLY_REGIME_ACC 24 /* surrogate chaos regime right there */
 #define BIFHIP_SSTAT_LLE_ONLY_RECOUNTED 25 /* of those, regime changed: peak count */
-#define BIFHIP_SSTAT_COUNT 26
+#define BIFHIP_SSTAT_LLE_ONLY_FP32 26      /* of those, LLE kept from FP32       */
+#define BIFHIP_SSTAT_COUNT 27
 
 int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int amountOfInitialConditions,
                                 const double* initialConditions, const double* ranges,

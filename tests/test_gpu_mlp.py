@@ -124,18 +124,23 @@ def test_training_learns_a_synthetic_map(bh):
 
 # cap: peak-row capacity (8 sends every re-solved cell through the overflow
 # recompute, which gathers cell indices from the device lists); pert: the
-# reference perturbation directions read the listed cells' indices back
-@pytest.mark.parametrize("partial,cap,pert", [(True, 0, 1), (False, 0, 1), (True, 8, 1),
-                                              (True, 0, 0)])
-def test_surrogate_pipeline_small(bh, partial, cap, pert):
+# reference perturbation directions read the listed cells' indices back;
+# margin: FP32 bulk LLE for regime-only cells this far from the threshold
+# (negative: all FP64)
+@pytest.mark.parametrize("partial,cap,pert,margin", [(True, 0, 1, 0.02), (False, 0, 1, 0.02),
+                                                     (True, 8, 1, 0.02), (True, 0, 0, 0.02),
+                                                     (True, 0, 1, -1.0)])
+def test_surrogate_pipeline_small(bh, partial, cap, pert, margin):
     nf, nc = 96, 24
     peak = dict(tmax=40.0, trans=20.0)
     lle = dict(tmax=50.0, NT=0.5, trans=10.0, eps=1e-8)
     bh.set_perturbation(pert)
     bh.set_partial_resolve(partial)
+    bh.set_lle32_margin(margin)
     if cap:
         bh.set_peak_capacity(cap)
-    say("surrogate pipeline", nf, nc, "partial", partial, "cap", cap, "pert", pert)
+    say("surrogate pipeline", nf, nc, "partial", partial, "cap", cap, "pert", pert, "margin",
+        margin)
     d_pk = bh.DeviceArray((nf * nf,), np.int32).zero_()
     d_ll = bh.DeviceArray((nf * nf,), np.float64).zero_()
     d_rs = bh.DeviceArray((nf * nf,), np.uint8).zero_()
@@ -149,22 +154,45 @@ def test_surrogate_pipeline_small(bh, partial, cap, pert):
     bh.lle_peak_map_device(t_pk.ptr, t_ll.ptr, nf, 0.01, IC, R2, [0, 2], VALS, peak["tmax"],
                            peak["trans"], 0, 100000, 0.01, lle["tmax"], lle["NT"], lle["trans"],
                            lle["eps"])
+    # the FP32 bulk LLE map of the lattice (the surrogate's regime re-solves
+    # run the same kernel on listed cells)
+    t_32 = bh.DeviceArray((nf * nf,), np.float64).zero_()
+    bh.set_precision(bh.PREC_FP32)
+    try:
+        bh.lle_map_device(t_32.ptr, 2, lle["tmax"], lle["NT"], nf, 0.01, lle["eps"], IC, R2,
+                          [0, 2], 0.0, lle["trans"], VALS)
+    finally:
+        bh.set_precision(bh.PREC_FP64)
     pk, ll, rs = d_pk.to_host(), d_ll.to_host(), d_rs.to_host()
-    tpk, tll = t_pk.to_host(), t_ll.to_host()
+    tpk, tll, t32 = t_pk.to_host(), t_ll.to_host(), t_32.to_host()
     full = (rs == 1) | (rs == 3)
     assert (rs == 1).sum() == st["resolved"]
     assert (rs == 2).sum() == st["peak_only"]
     assert (rs == 3).sum() == st["validation"] > 0
-    assert ((rs == 4) | (rs == 5)).sum() == st["lle_only"]
-    assert (rs == 5).sum() == st["lle_only_recounted"]
+    assert ((rs >= 4) & (rs <= 7)).sum() == st["lle_only"]
+    assert ((rs == 5) | (rs == 7)).sum() == st["lle_only_recounted"]
+    assert ((rs == 6) | (rs == 7)).sum() == st["lle_only_fp32"]
     if not partial:
         assert st["peak_only"] == 0 and st["lle_only"] == 0
-    # cells re-solved for the regime only carry the exact LLE, and the exact
-    # peak count where that regime differs from the predicted one (5)
+    if margin < 0:
+        assert st["lle_only_fp32"] == 0
+    elif partial:
+        assert st["lle_only_fp32"] > 0
+    # cells re-solved for the regime only carry the FP64 LLE (4, 5: FP32
+    # LLE within the margin of the threshold, or NaN) or the FP32 bulk LLE
+    # (6, 7: outside it), and the exact peak count where that regime differs
+    # from the predicted one (5, 7)
     lo = (rs == 4) | (rs == 5)
     same_lo = (np.isnan(ll[lo]) & np.isnan(tll[lo])) | (ll[lo] == tll[lo])
     assert same_lo.all()
-    np.testing.assert_array_equal(pk[rs == 5], tpk[rs == 5])
+    lo32 = (rs == 6) | (rs == 7)
+    np.testing.assert_array_equal(ll[lo32], t32[lo32])
+    with np.errstate(invalid="ignore"):
+        assert (np.abs(t32[lo32] - 0.01) >= margin).all()
+        if margin >= 0:
+            assert not (np.abs(t32[lo] - 0.01) >= margin).any()
+    rc = (rs == 5) | (rs == 7)
+    np.testing.assert_array_equal(pk[rc], tpk[rc])
     # cells re-solved in full equal the plain FP64 map exactly; cells
     # re-solved for the class carry the exact peak count
     np.testing.assert_array_equal(pk[full], tpk[full])
@@ -189,8 +217,10 @@ def test_surrogate_gain_over_plain_map(bh):
     the plain map's work (0.38 measured) at >= 98% per-cell regime agreement
     (98.5% measured), while the same pipeline with an untrained network
     (0 epochs) re-solves everything.  Work is counted in plain-map cells: a
-    full re-solve 1, a peak-count re-solve 0.6, an LLE-only re-solve 0.86
-    (their kernel times per cell relative to the fused kernel plus DBSCAN).
+    full re-solve 1, a peak-count re-solve 0.6 (also the recount of a
+    regime-only cell whose regime changed), an FP32 bulk LLE 0.36 (every
+    regime-only cell), an FP64 LLE 0.86 (those near the threshold): their
+    kernel times per cell relative to the fused kernel plus DBSCAN.
     At the configured scale (4096^2 from a 256^2 coarse map, bench.py) the
     agreement is >= 99%; a coarser fine lattice has more regime boundaries
     per cell."""
@@ -215,7 +245,9 @@ def test_surrogate_gain_over_plain_map(bh):
                                      epochs=epochs, batch=512, seed=5, validation=0.0)
         s_cls, s_ch = _regimes(d_pk.to_host(), d_ll.to_host())
         agree = float(((s_cls == t_cls) & (s_ch == t_ch)).mean())
-        work = (st["resolved"] + 0.6 * st["peak_only"] + 0.86 * st["lle_only"]) / (nf * nf)
+        lle64 = st["lle_only"] - st["lle_only_fp32"]
+        work = (st["resolved"] + 0.6 * (st["peak_only"] + st["lle_only_recounted"]) +
+                0.36 * st["lle_only"] + 0.86 * lle64) / (nf * nf)
         res[epochs] = (agree, work)
         say("epochs", epochs, "agreement", agree, "work vs plain map", work, st)
     assert res[20][0] >= 0.98 and res[0][0] >= 0.999

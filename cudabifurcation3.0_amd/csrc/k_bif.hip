@@ -11,9 +11,10 @@
 //  k_peak_finder    peakFinderCUDA (Library/cudaLibrary.cu:270-284) over
 //                   series already in HBM, same streaming state machine.
 //  k_dbscan_wave    dbscan (Library/cudaLibrary.cu:457-529), one wavefront
-//                   per cell: neighbour rows built with __ballot, the
-//                   reference's sequential label/stack machine replayed on
-//                   64-bit masks.  Exact for P <= 256 peaks.
+//                   per cell: the reference's sequential label/stack
+//                   machine replayed on 64-bit masks, each neighbour row
+//                   built with __ballot when the machine scans it.  Exact
+//                   for P <= 256 peaks.
 //  k_dbscan_block   the same machine, one workgroup per cell, label masks
 //                   in LDS, for the rare cells with 257..16384 peaks.
 //  k_dbscan_lane    the same algorithm one lane per cell on HBM scratch,
@@ -265,12 +266,14 @@ __device__ __forceinline__ bool near(double xa, double ya, double xb, double yb,
 {
     const float fx = (float)(xb - xa);
     const float fy = (float)(yb - ya);
-    const double sx = (double)fx * (double)fx;
+    // both squares are exact in double, so the fused form rounds the sum
+    // once, exactly as the separate product and sum do
     const double sy = (double)fy * (double)fy;
-    return sx + sy < T;
+    return fma((double)fx, (double)fx, sy) < T;
 }
 
 constexpr int kDbR = 4;                  // register sets: P <= 64*kDbR
+constexpr int kDbIsoSeeds = 2;           // isolated seeds before the isolated-peak pass
 constexpr int kDbMaxP = kWave * kDbR;
 
 // Bits of 64-bit word w (global indices 64w .. 64w+63) with index >= i,
@@ -284,12 +287,18 @@ __device__ __forceinline__ uint64_t word_lt(int w, int q) { return ~word_ge(w, q
 
 // DBSCAN of one cell by one wavefront, specialised on the number NS of
 // 64-peak words (P <= 64*NS) so the scalar mask logic has no idle words.
-// Peak k lives in lane k%64 of register set k/64.  Neighbour row i (bits
-// over k, one 64-bit word per k-set) is one ballot per word; word w of
-// row i is kept in lane i%64 of the register pair R[i/64][w], so the
-// label machine reads a row with readlanes (no LDS round trip).  The
-// anchor peak of row i is read with scalar loads (ra, rb and i are
-// wave-uniform).
+// Peak k lives in lane k%64 of register set k/64.  The label machine is
+// wave-uniform (scalar registers).  The scalar unit, shared by the CU's
+// SIMDs, bounded the version that built every neighbour row up front, so:
+//  - a neighbour row is built, one ballot per word, only when the machine
+//    scans it: for a cluster seed or a popped index; nothing is scanned
+//    once every absorbable index is labelled.  A period-1 orbit scans one
+//    row, a period-2 orbit about a quarter of them.
+//  - after kDbIsoSeeds seeds with empty rows (a chaotic cell, whose peaks
+//    are mostly isolated) the isolated peaks are found at once by the
+//    vector lanes -- each lane counts the anchors j < P-1 near its peak,
+//    every anchor read from the lane that holds it -- and from then on
+//    runs of isolated seeds are labelled without scans.
 template <int NS>
 __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
                                                 const double* __restrict__ rb, int P, double T,
@@ -304,41 +313,15 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
         b[r] = k < P ? rb[k] : 0.0;
         valid[r] = __builtin_amdgcn_ballot_w64(k < P - 1);
     }
-    uint64_t R[NS][NS];
-    uint64_t Z[NS];                   // isolated peaks: empty neighbour row
-#pragma unroll
-    for (int ri = 0; ri < NS; ++ri) {
-        Z[ri] = 0;
-#pragma unroll
-        for (int w = 0; w < NS; ++w) R[ri][w] = 0;
-    }
-#pragma unroll
-    for (int ri = 0; ri < NS; ++ri) {
-        const int iend = min(kWave, P - ri * kWave);
-        for (int li = 0; li < iend; ++li) {
-            const int i = ri * kWave + li;
-            const double ai = ra[i];
-            const double bi = rb[i];
-            const bool mine = lane == li;             // the lane that keeps row i
-            uint64_t any = 0;
-#pragma unroll
-            for (int w = 0; w < NS; ++w) {
-                uint64_t m = __builtin_amdgcn_ballot_w64(near(ai, bi, a[w], b[w], T)) & valid[w];
-                if (w == ri) m &= ~(1ull << li);      // k != i
-                R[ri][w] = mine ? m : R[ri][w];
-                any |= m;
-            }
-            if (any == 0) Z[ri] |= 1ull << li;
-        }
-    }
-
-    // label / stack machine, wave-uniform (scalar registers)
+    uint64_t Z[NS];                   // isolated peaks, once found (else 0)
     uint64_t U[NS], S[NS];
 #pragma unroll
     for (int w = 0; w < NS; ++w) {
         U[w] = word_lt(w, P);
         S[w] = 0;
+        Z[w] = 0;
     }
+    int isoSeeds = 0;                  // seeds with empty rows before Z is known
     // The reference visits i = 0, 1, ... and after each stack pop jumps to
     // i = slot (or 0); an empty-stack step at a labelled index changes
     // nothing.  Steps that provably change nothing but i and nn are taken in
@@ -347,15 +330,17 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
     //    can absorb.  At rem == 0 pops and scans change no label, P-1 is
     //    never absorbed (scans test k < P-1) and the walk still reaches it
     //    (a popped slot is at most P-2), so it opens one more cluster
-    //    exactly when unlabelled.
+    //    exactly when unlabelled.  A scan at rem == 0 is skipped.
     //  - A pop of an unset slot sets i = 0 and scans row 0.  Row 0 was
-    //    scanned first (the walk starts at 0), so it has no unlabelled
-    //    neighbour left: the run of unset slots above the highest set slot
-    //    below nn is popped at once, leaving i = 1.
+    //    scanned first or is empty (the walk starts at 0; any later scan
+    //    needs rem > 0 after it), so it has no unlabelled neighbour left:
+    //    the run of unset slots above the highest set slot below nn is
+    //    popped at once, leaving i = 1.
     //  - With an empty stack the walk visits the unlabelled indices >= i in
     //    order.  An isolated index (empty row) only opens a cluster of its
-    //    own, so every isolated unlabelled index before the next
-    //    non-isolated one is labelled at once.
+    //    own, so once Z is known every isolated unlabelled index before the
+    //    next non-isolated one is labelled at once (with Z = 0 the next
+    //    unlabelled index is the seed).
     int nn = 0, cluster = 0, i = 0, rem = P - 1;
     bool lastFree = true;
     while (i < P) {
@@ -363,6 +348,7 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
             if (lastFree) ++cluster;
             break;
         }
+        bool seed = false;
         if (nn >= 1) {
             int hs = -1;                       // highest set slot below nn
 #pragma unroll
@@ -409,6 +395,7 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
                 if (rem == 0) continue;
             }
             if (q >= P) break;
+            seed = true;
             i = q;
             ++cluster;
 #pragma unroll
@@ -417,33 +404,57 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
             if (i < P - 1) --rem;
             else lastFree = false;
         }
-        {
-            const int ri = i >> 6, li = i & 63;
-            uint64_t row[NS];
-#pragma unroll
-            for (int r = 0; r < NS; ++r)
-                if (r == ri) {
-#pragma unroll
-                    for (int w = 0; w < NS; ++w) row[w] = readlane_u64(R[r][w], li);
-                }
+        if (rem != 0) {
+            // scan row i: its unlabelled neighbours k < P-1 (i itself is
+            // labelled, so k != i holds through U); the anchor peak is read
+            // with scalar loads
+            const double ai = ra[i];
+            const double bi = rb[i];
             int c = 0;
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
-                const uint64_t m = row[w] & U[w];
+                const uint64_t m =
+                    __builtin_amdgcn_ballot_w64(near(ai, bi, a[w], b[w], T)) & valid[w] & U[w];
                 U[w] &= ~m;
                 S[w] |= m;
                 c += __popcll(m);
             }
             nn += c;
             rem -= c;
+            if (seed && c == 0 && isoSeeds >= 0 && ++isoSeeds == kDbIsoSeeds) {
+                // Row k is empty when no valid j != k is near k (near is
+                // symmetric).  Lane k counts the valid anchors j near it,
+                // its own test included when k is valid (a point is near
+                // itself exactly when T > 0).
+                int cnt[NS];
+#pragma unroll
+                for (int w = 0; w < NS; ++w) cnt[w] = 0;
+#pragma unroll
+                for (int r = 0; r < NS; ++r) {
+                    const int jend = min(kWave, P - 1 - r * kWave);
+                    for (int lj = 0; lj < jend; ++lj) {
+                        const double aj = readlane_d(a[r], lj);
+                        const double bj = readlane_d(b[r], lj);
+#pragma unroll
+                        for (int w = 0; w < NS; ++w) cnt[w] += near(aj, bj, a[w], b[w], T) ? 1 : 0;
+                    }
+                }
+#pragma unroll
+                for (int w = 0; w < NS; ++w) {
+                    const int k = w * kWave + lane;
+                    const int self = (k < P - 1 && T > 0.0) ? 1 : 0;
+                    Z[w] = __builtin_amdgcn_ballot_w64(k < P && cnt[w] == self);
+                }
+                isoSeeds = -1;             // known
+            }
         }
         ++i;
     }
     return cluster - 1;
 }
 
-// One wavefront per cell (dbscan_wave_cell holds the peaks and neighbour
-// rows in registers).  The reference's machine:
+// One wavefront per cell (dbscan_wave_cell holds the peaks in registers
+// and builds the neighbour rows it scans).  The reference's machine:
 //   U = unlabelled k (label == 0), S = stack slots holding their own index
 //   (slot k is written with value k; popping slot NumNeibor-1 yields that
 //   slot's index if set, else 0; the popped value becomes the loop

@@ -309,12 +309,57 @@ def run_dbscan(bh, np, args, out):
                        "grid": n, "ranges": RANGES, "h": H, "peak": PEAK, "cap": 32})
     out["oracle_rows_bit_exact"] = {"rows": rows, "equal": exact}
     out["kernels_ms_per_step"] = {"bif_peaks": kms / args.steps, "dbscan": dbms / args.steps}
+    if args.fp32:
+        out["fp32"] = run_dbscan_fp32(bh, np, args, d, dargs, got, ms, rows)
     out["roofline"] = {"bound": "valu_fp64", "kernel": "k_bif_peaks", "achieved": ach,
                        "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s",
                        "frac": ach / (FP64_VALU_PEAK / 1e12), "traffic": None,
                        "avg_launch_ms": kms / max(nl, 1), "launches": nl,
                        "note": "12 FLOP per Rossler step, 150 000 steps per cell; no-contraction "
                                "FP64 reaches at most 50% of the FMA-counted peak"}
+
+
+def run_dbscan_fp32(bh, np, args, d, dargs, got64, ms64, rows):
+    """The same map with FP32 trajectories (bifhip_set_precision FP32):
+    time, bit-exactness of sampled rows against the oracle's FP32
+    restatement, and peak-count agreement with the FP64 map."""
+    import orclib as O
+    n = int(round(got64.size ** 0.5))
+    cells = n * n
+    bh.set_precision(bh.PREC_FP32)
+    bh.set_cluster_clamp(32)
+    try:
+        for _ in range(args.warmup):
+            bh.dbscan_map_device(*dargs)
+        bh.synchronize()
+        bh.reset_kernel_times()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bh.dbscan_map_device(*dargs)
+        bh.synchronize()
+        ms = (time.perf_counter() - t0) * 1000 / args.steps
+        kms, nl = bh.kernel_time("bif_peaks")
+        dbms, _ = bh.kernel_time("dbscan")
+        got = d.to_host().reshape(n, n)
+    finally:
+        bh.set_precision(bh.PREC_FP64)
+        bh.set_cluster_clamp(0)
+    exact = True
+    for r in rows:
+        want = O.dbscan_cells(PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(),
+                              PEAK["transient"], VALS, 1, PEAK["eps"], r * n, n, fp32=True)
+        exact &= bool(np.array_equal(got[r], np.minimum(want, 32)))
+    c64, c32 = np.clip(got64, 1, 16), np.clip(got, 1, 16)
+    # 11 FLOP per FP32 step (4 fused multiply-adds, 3 adds / multiplies)
+    ach = 11 * 150000 * cells * args.steps / (kms / 1000) / 1e12
+    return {"ms_per_step": ms, "value": cells / (ms / 1000), "speedup_vs_fp64": ms64 / ms,
+            "oracle_rows_bit_exact": {"rows": rows, "equal": exact, "oracle": "FP32 restatement"},
+            "count_agreement_vs_fp64": float((got == got64).mean()),
+            "class_agreement_vs_fp64": float((c32 == c64).mean()),
+            "kernels_ms_per_step": {"bif_peaks32": kms / args.steps, "dbscan": dbms / args.steps},
+            "roofline": {"bound": "valu_fp32", "kernel": "k_bif_peaks32", "achieved": ach,
+                         "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                         "frac": ach / (FP32_VALU_PEAK / 1e12), "launches": nl}}
 
 
 def run_bif1d(bh, np, args, out):
@@ -393,6 +438,9 @@ def main():
     ap.add_argument("--cpu-cells", type=int, default=65536,
                     help="cells of the CPU baseline sample (whole lattice rows, ~10 s on 16 cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp32", action="store_true",
+                    help="dbscan workload: also time the FP32-trajectory map and report its "
+                         "agreement with the FP64 map")
     ap.add_argument("--full-resolve", action="store_true",
                     help="re-solve every flagged cell in full (no peak-count-only re-solves)")
     ap.add_argument("--no-graph", action="store_true",

@@ -25,8 +25,8 @@
 
 // kernel launchers (k_bif.hip, k_lyap.hip) -------------------------------
 namespace bifhip {
-hipError_t launch_bif(int model, const BifArgs& A, int wv, bool check, int* counts, double* amps,
-                      double* ivls, hipStream_t st);
+hipError_t launch_bif(int model, int precision, const BifArgs& A, int wv, bool check, int* counts,
+                      double* amps, double* ivls, hipStream_t st);
 hipError_t launch_series(int model, const BifArgs& A, int wv, double* data, int* flags,
                          hipStream_t st);
 hipError_t launch_peak_finder(const double* data, int sizeOfBlock, int nBlocks, int* counts,
@@ -251,6 +251,7 @@ struct PeakMapSpec {
     long long cells;
     double h, maxValue;
     int skip, N, pre, wv;
+    int precision;    // BIFHIP_PREC_FP64, or FP32 for the peak maps that honour it
 };
 
 int peak_spec(PeakMapSpec& S, int dimension, double tMax, int nPts, double h, int nIC,
@@ -272,6 +273,18 @@ int peak_spec(PeakMapSpec& S, int dimension, double tMax, int nPts, double h, in
     S.N = N;
     S.pre = pre < 1 ? 1 : pre;
     S.wv = wv;
+    S.precision = BIFHIP_PREC_FP64;
+    return BIFHIP_OK;
+}
+
+// The 1D diagram and the 2D cluster maps follow bifhip_set_precision; the
+// surrogate's re-solves always integrate in FP64.
+int peak_precision(PeakMapSpec& S)
+{
+    if (g.precision != BIFHIP_PREC_FP32) return BIFHIP_OK;
+    if (g.model == BIFHIP_MODEL_MEMRISTIVE)
+        return fail(BIFHIP_E_ARG, "FP32 peak maps support the Rossler and linear models");
+    S.precision = BIFHIP_PREC_FP32;
     return BIFHIP_OK;
 }
 
@@ -302,7 +315,7 @@ int integrate_peaks(const PeakMapSpec& S, int base, const int* d_list, int n, in
         const int m = (int)std::min<long long>(chunk, n - off);
         BifArgs A = bif_args(S, base + (int)off, d_list ? d_list + off : nullptr, m, cap);
         Timed t("bif_peaks");
-        HIP_TRY(launch_bif(g.model, A, S.wv, S.maxValue != 0, d_counts + off,
+        HIP_TRY(launch_bif(g.model, S.precision, A, S.wv, S.maxValue != 0, d_counts + off,
                            d_amps ? d_amps + off * cap : nullptr,
                            d_ivls ? d_ivls + off * cap : nullptr, g.stream));
     }
@@ -1048,6 +1061,7 @@ int bif1d_impl(double tMax, int nPts, double h, int nIC, const double* ic, const
     PeakMapSpec S;
     RC_TRY(peak_spec(S, 1, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime,
                      values, nVal, pre));
+    RC_TRY(peak_precision(S));
     progress("BIFURCATION1D", S.cells);
     // device rows hold every peak a record can have; only the used prefix
     // of each row (the largest count) comes back to the host
@@ -1085,6 +1099,7 @@ int cluster_impl(int kind, double tMax, int nPts, double h, int nIC, const doubl
     PeakMapSpec S;
     RC_TRY(peak_spec(S, 2, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime,
                      values, nVal, pre));
+    RC_TRY(peak_precision(S));
     progress("BIFURCATION2D", S.cells);
     ClusterParams C{kind, maxPeaks, eps, kdeSampling, I1, I2, H};
     RC_TRY(cluster_map_device(S, C, d_out));

@@ -112,6 +112,87 @@ __global__ __launch_bounds__(256) void k_bif_peaks(const BifArgs A, int* __restr
     counts[t] = bad ? -1 : ps.count();
 }
 
+template <int WV>
+__device__ __forceinline__ float pick32(float x0, float x1, float x2)
+{
+    return WV == 0 ? x0 : (WV == 1 ? x1 : x2);
+}
+
+// FP32 form of k_bif_peaks (bifhip_set_precision(BIFHIP_PREC_FP32)): the
+// state in FP32, stepped with the model's fused FP32 map (M::step32, as the
+// FP32 LLE kernel), each sample widened exactly to double for the same peak
+// stream, intervals and divergence test.  Every update of the FP32 map
+// contains the component itself, so NaN stays absorbing (the trajectory
+// form of the stream applies).  No reference counterpart: the counts are
+// judged by their agreement with the FP64 map.
+template <class M, int WV, bool CHECK>
+__global__ __launch_bounds__(256) void k_bif_peaks32(const BifArgs A, int* __restrict__ counts,
+                                                     double* __restrict__ amps,
+                                                     double* __restrict__ ivls)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nCells) return;
+    const int idx = A.cellList ? A.cellList[t] : A.cellBase + t;
+
+    double v[kMaxVal];
+    double s0, s1, s2;
+    cell_setup(A.g, idx, v, s0, s1, s2);
+    const typename M::P32 q = M::params32(v, A.h);
+    float x0 = (float)s0, x1 = (float)s1, x2 = (float)s2;
+
+    int s = 0;
+    for (; s + 4 <= A.skip; s += 4) {
+        M::step32(x0, x1, x2, q);
+        M::step32(x0, x1, x2, q);
+        M::step32(x0, x1, x2, q);
+        M::step32(x0, x1, x2, q);
+    }
+    for (; s < A.skip; ++s) M::step32(x0, x1, x2, q);
+
+    PeakSink sink;
+    sink.amps = amps ? amps + (long long)t * A.rowStride : nullptr;
+    sink.ivls = ivls ? ivls + (long long)t * A.rowStride : nullptr;
+    sink.cap = A.cap;
+    sink.hpf = A.hpf;
+
+    TrajPeakStream ps;
+    ps.init();
+    bool bad = false;
+    const int N = A.N;
+    const int pre = A.pre;
+    int i = 0;
+    for (; i < N && i < 2; ++i) {
+        ps.prime((double)pick32<WV>(x0, x1, x2), i);
+        for (int j = 0; j < pre - 1; ++j) M::step32(x0, x1, x2, q);
+        M::step32(x0, x1, x2, q);
+        if (CHECK && diverged((double)pick32<WV>(x0, x1, x2), A.maxValue)) { bad = true; break; }
+    }
+    if (!CHECK && pre == 1) {
+        constexpr int K = kPeakBatch;
+        for (; i + K <= N; i += K) {
+            double va[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                va[k] = (double)pick32<WV>(x0, x1, x2);
+                M::step32(x0, x1, x2, q);
+            }
+            ps.push_batch(va, i, sink);
+        }
+        for (; i < N; ++i) {
+            ps.push((double)pick32<WV>(x0, x1, x2), i, sink);
+            M::step32(x0, x1, x2, q);
+        }
+    } else {
+        for (; i < N && !bad; ++i) {
+            ps.push((double)pick32<WV>(x0, x1, x2), i, sink);
+            for (int j = 0; j < pre - 1; ++j) M::step32(x0, x1, x2, q);
+            M::step32(x0, x1, x2, q);
+            if (CHECK && diverged((double)pick32<WV>(x0, x1, x2), A.maxValue)) bad = true;
+        }
+    }
+    counts[t] = bad ? -1 : ps.count();
+}
+
 // below this many cells a launch keeps one cell per lane (occupancy first)
 constexpr int kPairCells = 1 << 20;
 
@@ -912,11 +993,21 @@ __global__ void k_fill_i32(long long n, int* __restrict__ v, int value)
 // --------------------------------------------------------- launchers
 
 template <class M>
-static hipError_t launch_bif_m(const BifArgs& A, int wv, bool check, int* counts, double* amps,
-                               double* ivls, hipStream_t st)
+static hipError_t launch_bif_m(const BifArgs& A, int wv, bool check, bool fp32, int* counts,
+                               double* amps, double* ivls, hipStream_t st)
 {
     const dim3 blk(256), grd((A.nCells + 255) / 256);
     if (A.nCells <= 0) return hipSuccess;
+    if (fp32) {
+#define BIF32_CASE(W, C) hipLaunchKernelGGL((k_bif_peaks32<M, W, C>), grd, blk, 0, st, A, counts, amps, ivls)
+        if (check) {
+            if (wv == 0) BIF32_CASE(0, true); else if (wv == 1) BIF32_CASE(1, true); else BIF32_CASE(2, true);
+        } else {
+            if (wv == 0) BIF32_CASE(0, false); else if (wv == 1) BIF32_CASE(1, false); else BIF32_CASE(2, false);
+        }
+#undef BIF32_CASE
+        return hipGetLastError();
+    }
     if (!check && A.nCells >= kPairCells) {
         // enough cells to fill the device at two per lane
         const dim3 grd2((A.nCells / 2 + 1 + 255) / 256);
@@ -940,13 +1031,17 @@ hipError_t launch_gen_bif(int model, const BifArgs& A, int wv, bool check, int* 
 hipError_t launch_gen_series(int model, const BifArgs& A, int wv, double* data, int* flags,
                              hipStream_t st);
 
-hipError_t launch_bif(int model, const BifArgs& A, int wv, bool check, int* counts, double* amps,
-                      double* ivls, hipStream_t st)
+// precision 1: the FP32 peak kernel (Rossler and linear models only)
+hipError_t launch_bif(int model, int precision, const BifArgs& A, int wv, bool check, int* counts,
+                      double* amps, double* ivls, hipStream_t st)
 {
-    if (model == kMemristive)
+    const bool fp32 = precision == 1;
+    if (model == kMemristive) {
+        if (fp32) return hipErrorInvalidValue;
         return launch_gen_bif(model, A, wv, check, counts, amps, ivls, nullptr, st);
-    if (model == kLinear) return launch_bif_m<Linear>(A, wv, check, counts, amps, ivls, st);
-    return launch_bif_m<Rossler>(A, wv, check, counts, amps, ivls, st);
+    }
+    if (model == kLinear) return launch_bif_m<Linear>(A, wv, check, fp32, counts, amps, ivls, st);
+    return launch_bif_m<Rossler>(A, wv, check, fp32, counts, amps, ivls, st);
 }
 
 hipError_t launch_series(int model, const BifArgs& A, int wv, double* data, int* flags,

@@ -53,7 +53,13 @@ const char* bifhip_get_output_path(void);
 /* Model plug-in.  Replaces editing calculateDiscreteModel
  * (Library/cudaLibrary.cu:2-44).  Default BIFHIP_MODEL_ROSSLER. */
 int bifhip_set_model(int model);
-/* Integration precision of the LLE drivers.  Default BIFHIP_PREC_FP64. */
+/* Integration precision.  Default BIFHIP_PREC_FP64 (the reference's).
+ * BIFHIP_PREC_FP32 applies to the LLE drivers (FP32 state and separation,
+ * FP64 norms and log sums), to the 1D diagram and to the 2D DBSCAN / KDE
+ * maps (FP32 state with the fused FP32 step, samples widened to double for
+ * the same peak finder and clustering), for the Rossler and linear models.
+ * No reference counterpart: FP32 results are judged by their agreement
+ * with the FP64 ones.  The surrogate's re-solves ignore it. */
 int bifhip_set_precision(int precision);
 /* LLE/LS initial perturbation directions.  Default BIFHIP_PERT_REFERENCE
  * (Library/cudaLibrary.cu:720-732, :1025-1040). */

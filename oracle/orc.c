@@ -627,6 +627,137 @@ void orc_dbscan_cells(int model, double tMax, int nPts, double h, int nIC,
     }
 }
 
+/* FP32 form of the peak maps (the library's bifhip_set_precision(FP32) for
+ * the 1D diagram and the 2D cluster maps; no reference counterpart).  The
+ * state is float, stepped with the fused FP32 map below (fmaf is correctly
+ * rounded in glibc, as v_fma_f32 is), each sample widened to double for the
+ * reference's peak finder and DBSCAN.  Rossler:
+ *   x0 <- fma(-h, x1 + x2, x0)
+ *   x1 <- fma(x1, (float)(1 + h*a), h*x0)
+ *   x2 <- fma(x2, fma(h, x0 - c, 1), (float)(h*b))
+ * linear: x_i <- x_i * (float)(1 + h*a_i). */
+static void step32_params(int model, const double *v, double h, float *q)
+{
+    if (model == ORC_MODEL_ROSSLER) {
+        q[0] = (float)h;
+        q[1] = (float)(1.0 + h * v[0]);
+        q[2] = (float)v[2];
+        q[3] = (float)(h * v[1]);
+    } else {
+        for (int i = 0; i < 3; ++i)
+            q[i] = (float)(1.0 + h * v[i]);
+        q[3] = 0;
+    }
+}
+
+static inline void step32(int model, float *x, const float *q)
+{
+    if (model == ORC_MODEL_ROSSLER) {
+        x[0] = fmaf(-q[0], x[1] + x[2], x[0]);
+        x[1] = fmaf(x[1], q[1], q[0] * x[0]);
+        x[2] = fmaf(x[2], fmaf(q[0], x[0] - q[2], 1.0f), q[3]);
+    } else {
+        x[0] *= q[0];
+        x[1] *= q[1];
+        x[2] *= q[2];
+    }
+}
+
+int orc_bif_cell32(int model, int nPts, long cellIdx, int dimension,
+                   const double *ranges, const int *mutIdx, const double *ic,
+                   int nIC, const double *values, int nVal, double h, int skip,
+                   int N, int preScaller, int writableVar, double maxValue,
+                   double *series)
+{
+    double x[ORC_MAXD], v[ORC_MAXV];
+    float q[4], y[3];
+    if (model == ORC_MODEL_MEMRISTIVE || nIC != 3)
+        return -1;
+    setup_cell(model, nPts, cellIdx, dimension, ranges, mutIdx, ic, nIC,
+               values, nVal, x, v);
+    step32_params(model, v, h, q);
+    for (int i = 0; i < 3; ++i)
+        y[i] = (float)x[i];
+    for (int i = 0; i < skip; ++i)
+        step32(model, y, q);
+    for (int i = 0; i < N; ++i) {
+        series[i] = (double)y[writableVar];
+        for (int j = 0; j < preScaller; ++j)
+            step32(model, y, q);
+        if (maxValue != 0 && fabsf(y[writableVar]) > maxValue)
+            return 0;
+    }
+    return 1;
+}
+
+void orc_peaks_cells32(int model, int dimension, double tMax, int nPts,
+                       double h, int nIC, const double *ic,
+                       const double *ranges, const int *mutIdx,
+                       int writableVar, double maxValue, double transientTime,
+                       const double *values, int nVal, int preScaller,
+                       long c0, long nc, int *counts, double *peaks,
+                       double *intervals, int cap)
+{
+    int N = (int)(tMax / h / preScaller);
+    int skip = (int)(transientTime / h);
+#pragma omp parallel
+    {
+        double *s = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+        double *iv = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+#pragma omp for schedule(dynamic, 64)
+        for (long c = 0; c < nc; ++c) {
+            int ok = orc_bif_cell32(model, nPts, c0 + c, dimension, ranges,
+                                    mutIdx, ic, nIC, values, nVal, h, skip,
+                                    N, preScaller, writableVar, maxValue, s);
+            int P = 0;
+            if (ok == 1)
+                P = orc_peak_finder(s, 0, N, s, iv, 0.01);
+            counts[c] = ok == 1 ? P : -1;
+            int m = P < cap ? P : cap;
+            if (peaks)
+                for (int k = 0; k < m; ++k)
+                    peaks[c * (long)cap + k] = s[k];
+            if (intervals)
+                for (int k = 0; k < m; ++k)
+                    intervals[c * (long)cap + k] = iv[k];
+        }
+        free(s);
+        free(iv);
+    }
+}
+
+void orc_dbscan_cells32(int model, double tMax, int nPts, double h, int nIC,
+                        const double *ic, const double *ranges,
+                        const int *mutIdx, int writableVar, double maxValue,
+                        int maxAmountOfPeaks, double transientTime,
+                        const double *values, int nVal, int preScaller,
+                        double eps, long c0, long nc, int *out)
+{
+    int N = (int)(tMax / h / preScaller);
+    int skip = (int)(transientTime / h);
+#pragma omp parallel
+    {
+        double *s = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+        double *iv = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+        double *hp = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+#pragma omp for schedule(dynamic, 64)
+        for (long c = 0; c < nc; ++c) {
+            int ok = orc_bif_cell32(model, nPts, c0 + c, 2, ranges, mutIdx,
+                                    ic, nIC, values, nVal, h, skip, N,
+                                    preScaller, writableVar, maxValue, s);
+            if (ok != 1) {
+                out[c] = 0;
+                continue;
+            }
+            int P = orc_peak_finder(s, 0, N, s, iv, 0.01);
+            out[c] = orc_dbscan(s, iv, hp, 0, P, N, maxAmountOfPeaks, eps);
+        }
+        free(s);
+        free(iv);
+        free(hp);
+    }
+}
+
 /* calculationOfPeriodicityByOstrovsky, Library/hostLibrary.cu:748-925:
  * initial-condition sweep (calculateDiscreteModelICCUDA), peaks with five
  * flags (peakFinderCUDAForCalculationOfPeriodicityByOstrovsky,

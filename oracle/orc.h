@@ -32,6 +32,11 @@
  *   orc_ostrovsky_cells      signs over the reference Rossler counts;
  *                            peak-free cells follow the last sample
  *   orc_denis                linear-model closed form per sample
+ *   orc_bif_cell32 (FP32)    each step against the FP64 step from the same
+ *                            float state (relative error at float rounding;
+ *                            a dropped term or sign fails); linear model
+ *                            x_n = x_0 * (float)(1+h*a)^n rounded per step;
+ *                            period-1 Rossler peak counts equal the FP64 ones
  */
 #ifndef BIFHIP_ORACLE_H
 #define BIFHIP_ORACLE_H
@@ -133,6 +138,29 @@ void orc_dbscan_cells(int model, double tMax, int nPts, double h, int nIC,
                       int maxAmountOfPeaks, double transientTime,
                       const double *values, int nVal, int preScaller,
                       double eps, long c0, long nc, int *out);
+
+/* FP32 form of the peak maps (the library's FP32 precision; no reference
+ * counterpart): float state with the fused FP32 step, samples widened to
+ * double.  orc_bif_cell32 returns 1, 0 (diverged) or -1 (model or nIC not
+ * supported: Rossler and linear, nIC = 3). */
+int  orc_bif_cell32(int model, int nPts, long cellIdx, int dimension,
+                    const double *ranges, const int *mutIdx, const double *ic,
+                    int nIC, const double *values, int nVal, double h, int skip,
+                    int N, int preScaller, int writableVar, double maxValue,
+                    double *series);
+void orc_peaks_cells32(int model, int dimension, double tMax, int nPts,
+                       double h, int nIC, const double *ic,
+                       const double *ranges, const int *mutIdx,
+                       int writableVar, double maxValue, double transientTime,
+                       const double *values, int nVal, int preScaller,
+                       long c0, long nc, int *counts, double *peaks,
+                       double *intervals, int cap);
+void orc_dbscan_cells32(int model, double tMax, int nPts, double h, int nIC,
+                        const double *ic, const double *ranges,
+                        const int *mutIdx, int writableVar, double maxValue,
+                        int maxAmountOfPeaks, double transientTime,
+                        const double *values, int nVal, int preScaller,
+                        double eps, long c0, long nc, int *out);
 
 /* bifurcation2DKDE (hostLibrary.cu:397-567) over cells [c0, c0+nc). */
 /* calculationOfPeriodicityByOstrovsky cells (initial-condition sweep) */

@@ -66,6 +66,8 @@ def _declare(L):
     L.orc_bif_cell.argtypes = [_I, _I, _L, _I, _PD, _PI, _PD, _I, _PD, _I,
                                _D, _I, _I, _I, _I, _D, _PD]
     L.orc_bif_cell.restype = _I
+    L.orc_bif_cell32.argtypes = L.orc_bif_cell.argtypes
+    L.orc_bif_cell32.restype = _I
     L.orc_lle_cell.argtypes = [_I, _I, _L, _L, _I, _PD, _PI, _PD, _I, _PD, _I,
                                _D, _D, _D, _D, _I, _D, _I]
     L.orc_lle_cell.restype = _D
@@ -78,6 +80,8 @@ def _declare(L):
                                     _D, _PD, _I, _I, _PI, _PD, _I]
     L.orc_dbscan_cells.argtypes = [_I, _D, _I, _D, _I, _PD, _PD, _PI, _I, _D,
                                    _I, _D, _PD, _I, _I, _D, _L, _L, _PI]
+    L.orc_peaks_cells32.argtypes = L.orc_peaks_cells.argtypes
+    L.orc_dbscan_cells32.argtypes = L.orc_dbscan_cells.argtypes
     L.orc_kde_cells.argtypes = [_I, _D, _I, _D, _I, _PD, _PD, _PI, _I, _D, _I,
                                 _D, _PD, _I, _I, _I, _D, _D, _D, _L, _L, _PI]
     L.orc_lle_cells.argtypes = [_I, _I, _D, _D, _I, _D, _D, _PD, _I, _PD, _PI,
@@ -162,36 +166,36 @@ def gram_schmidt(a):
 
 
 def bif_cell(cell, nPts, ranges, mut, ic, values, h, skip, N, pre=1, wv=0,
-             max_value=0.0, dimension=None, model=MODEL_ROSSLER):
+             max_value=0.0, dimension=None, model=MODEL_ROSSLER, fp32=False):
+    """fp32: the FP32 form (float state, fused FP32 step)."""
     ranges = f64(ranges); mut = i32(mut); ic = f64(ic); values = f64(values)
     dimension = dimension if dimension is not None else len(mut)
     s = np.zeros(N)
-    ok = lib().orc_bif_cell(model, nPts, cell, dimension, pd(ranges), pi(mut),
-                            pd(ic), len(ic), pd(values), len(values), h, skip,
-                            N, pre, wv, max_value, pd(s))
+    fn = lib().orc_bif_cell32 if fp32 else lib().orc_bif_cell
+    ok = fn(model, nPts, cell, dimension, pd(ranges), pi(mut), pd(ic), len(ic), pd(values),
+            len(values), h, skip, N, pre, wv, max_value, pd(s))
     return ok, s
 
 
 def peaks_cells(dimension, tMax, nPts, h, ic, ranges, mut, wv, max_value,
-                transient, values, pre, c0, nc, cap, model=MODEL_ROSSLER):
+                transient, values, pre, c0, nc, cap, model=MODEL_ROSSLER, fp32=False):
     ic = f64(ic); ranges = f64(ranges); mut = i32(mut); values = f64(values)
     counts = np.zeros(nc, dtype=np.int32)
     peaks = np.zeros(nc * cap)
     ivl = np.zeros(nc * cap)
-    lib().orc_peaks_cells(model, dimension, tMax, nPts, h, len(ic), pd(ic),
-                          pd(ranges), pi(mut), wv, max_value, transient,
-                          pd(values), len(values), pre, c0, nc, pi(counts),
-                          pd(peaks), pd(ivl), cap)
+    fn = lib().orc_peaks_cells32 if fp32 else lib().orc_peaks_cells
+    fn(model, dimension, tMax, nPts, h, len(ic), pd(ic), pd(ranges), pi(mut), wv, max_value,
+       transient, pd(values), len(values), pre, c0, nc, pi(counts), pd(peaks), pd(ivl), cap)
     return counts, peaks.reshape(nc, cap), ivl.reshape(nc, cap)
 
 
 def dbscan_cells(tMax, nPts, h, ic, ranges, mut, wv, max_value, max_peaks,
-                 transient, values, pre, eps, c0, nc, model=MODEL_ROSSLER):
+                 transient, values, pre, eps, c0, nc, model=MODEL_ROSSLER, fp32=False):
     ic = f64(ic); ranges = f64(ranges); mut = i32(mut); values = f64(values)
     out = np.zeros(nc, dtype=np.int32)
-    lib().orc_dbscan_cells(model, tMax, nPts, h, len(ic), pd(ic), pd(ranges),
-                           pi(mut), wv, max_value, max_peaks, transient,
-                           pd(values), len(values), pre, eps, c0, nc, pi(out))
+    fn = lib().orc_dbscan_cells32 if fp32 else lib().orc_dbscan_cells
+    fn(model, tMax, nPts, h, len(ic), pd(ic), pd(ranges), pi(mut), wv, max_value, max_peaks,
+       transient, pd(values), len(values), pre, eps, c0, nc, pi(out))
     return out
 
 

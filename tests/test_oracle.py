@@ -125,6 +125,69 @@ def test_rossler_step_is_sequential_euler(orc):
     assert abs(F(got[1]) - s1) > F(1000) * F(math.ulp(float(s1)))
 
 
+def test_fp32_step_is_sequential_euler(orc):
+    """The FP32 form (float state, fused FP32 step) against the same exact
+    rational sequential step from the same float state: within a few float
+    ulps, while the simultaneous variant (x1 from the old x0) or a flipped
+    sign misses by far more."""
+    from fractions import Fraction as F
+    x = [0.75, -1.25, 0.375]                 # exact in float
+    a, b, c, h = 0.2, 0.2, 5.7, 0.01
+    X = [F(v) for v in x]
+    A, B, Cc, H = F(a), F(b), F(c), F(h)
+    n0 = X[0] + H * (-X[1] - X[2])
+    n1 = X[1] + H * (n0 + A * X[1])
+    n2 = X[2] + H * (B + X[2] * (n0 - Cc))
+    got = []
+    for wv in range(3):
+        ok, s = orc.bif_cell(0, 2, [c, c], [2], x, [a, b, c], h, 0, 2, wv=wv, fp32=True)
+        assert ok == 1 and s[0] == x[wv]
+        got.append(s[1])
+    for g, w in zip(got, (n0, n1, n2)):
+        ulp = F(float(np.spacing(np.float32(abs(float(w))))))
+        assert abs(F(g) - w) <= 4 * ulp, (g, float(w))
+    s1 = X[1] + H * (X[0] + A * X[1])        # simultaneous variant
+    assert abs(F(got[1]) - s1) > 100 * F(float(np.spacing(np.float32(abs(float(s1))))))
+
+
+def test_fp32_linear_model_power_law(orc):
+    """Linear model in FP32: x_n = x_0 (1+h*a)^n up to one float rounding of
+    the factor and one per step; divergence sets the -1 count as in FP64."""
+    h, a, n = 0.01, 0.3, 1000
+    ok, s = orc.bif_cell(0, 2, [a, a], [0], [1.0, 1.0, 1.0], [a, a, a], h, 0, n + 1,
+                         model=orc.MODEL_LINEAR, fp32=True)
+    assert ok == 1
+    k = np.arange(n + 1)
+    want = np.exp(k * math.log1p(h * a))
+    rel = np.abs(s - want) / want
+    assert rel.max() < 3 * n * 2.0 ** -24, rel.max()
+    assert rel[-1] > 1e-9                    # the state is float, not double
+    for fp32 in (False, True):
+        cnt, _, _ = orc.peaks_cells(1, 50, 2, h, [1.0, 1.0, 1.0], [50.0, 50.0], [0], 0, 1000.0,
+                                    0, [50.0, 50.0, 50.0], 1, 0, 1, 16, model=orc.MODEL_LINEAR,
+                                    fp32=fp32)
+        assert cnt[0] == -1
+
+
+def test_fp32_period1_rossler_matches_fp64(orc):
+    """c = 2.5 (period 1): the FP32 trajectory settles on the same limit
+    cycle -- the same DBSCAN count as the reference's (1) and the same
+    peak amplitudes to float accuracy."""
+    c = 2.5
+    rng2 = [0.2, 0.2, c, c]
+    args = (500, 2, 0.01, IC, rng2, [0, 2], 0, 0.0, 100000, 1000, [0.2, 0.2, c], 1, 0.01, 0, 1)
+    assert orc.dbscan_cells(*args)[0] == 1
+    assert orc.dbscan_cells(*args, fp32=True)[0] == 1
+    c64, p64, _ = orc.peaks_cells(1, 500, 2, 0.01, IC, [c, c], [2], 0, 0.0, 1000, [0.2, 0.2, c],
+                                  1, 0, 1, 256)
+    c32, p32, _ = orc.peaks_cells(1, 500, 2, 0.01, IC, [c, c], [2], 0, 0.0, 1000, [0.2, 0.2, c],
+                                  1, 0, 1, 256, fp32=True)
+    assert abs(int(c32[0]) - int(c64[0])) <= 1
+    m = min(c32[0], c64[0])
+    assert np.abs(p32[0, :m].mean() - p64[0, :m].mean()) < 1e-4
+    assert not np.array_equal(p32[0, :m], p64[0, :m])
+
+
 # ------------------------------------------------------------ peak finder
 
 def test_peak_finder_plateau_free_is_textbook(orc):

@@ -322,6 +322,84 @@ int integrate_peaks(const PeakMapSpec& S, int base, const int* d_list, int n, in
     return BIFHIP_OK;
 }
 
+// ------------------------------------------ chunked integrate / cluster
+//
+// Peak maps run in passes of at most `pass` cells whose peak rows fit in
+// ~8 GiB.  Two row sets alternate: while pass k+1 integrates on the library
+// stream, pass k is clustered on a side stream.  The integrators are bound
+// by the vector ALUs and the clustering by the scalar unit, so the two
+// share the CUs well.  Events order the work: clustering k waits for
+// integration k, integration k+2 waits for clustering k (the same rows),
+// and the library stream waits for the last clustering before the caller
+// sees the results.
+struct PeakRows {
+    int* counts;
+    double* amps;
+    double* ivls;
+};
+
+namespace {
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t integrated[2] = {nullptr, nullptr};
+    hipEvent_t clustered[2] = {nullptr, nullptr};
+};
+SideStream g_side;
+}  // namespace
+
+static int side_stream()
+{
+    if (g_side.s) return BIFHIP_OK;
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipEventCreateWithFlags(&g_side.integrated[b], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&g_side.clustered[b], hipEventDisableTiming));
+    }
+    // non-blocking: the library stream is the null stream
+    HIP_TRY(hipStreamCreateWithFlags(&g_side.s, hipStreamNonBlocking));
+    return BIFHIP_OK;
+}
+
+// Two row sets of `pass` cells x `cap` slots (pool keys <tag>_<what><b>);
+// no interval rows when ivls is false.
+static int peak_rows(const char* tag, long long pass, int cap, bool ivls, PeakRows rows[2])
+{
+    char key[64];
+    for (int b = 0; b < 2; ++b) {
+        std::snprintf(key, sizeof(key), "%s_counts%d", tag, b);
+        RC_TRY(pool_get(key, (size_t)pass, &rows[b].counts));
+        std::snprintf(key, sizeof(key), "%s_amps%d", tag, b);
+        RC_TRY(pool_get(key, (size_t)(pass * cap), &rows[b].amps));
+        rows[b].ivls = nullptr;
+        if (ivls) {
+            std::snprintf(key, sizeof(key), "%s_ivls%d", tag, b);
+            RC_TRY(pool_get(key, (size_t)(pass * cap), &rows[b].ivls));
+        }
+    }
+    return BIFHIP_OK;
+}
+
+// integrate(off, m, rows) enqueues on g.stream; cluster(off, m, rows, st)
+// enqueues on st.
+template <class Integrate, class Cluster>
+static int chunk_pipeline(long long n, long long pass, const PeakRows rows[2],
+                          Integrate integrate, Cluster cluster)
+{
+    RC_TRY(side_stream());
+    int k = 0;
+    for (long long off = 0; off < n; off += pass, ++k) {
+        const int m = (int)std::min(pass, n - off);
+        const int b = k & 1;
+        if (k >= 2) HIP_TRY(hipStreamWaitEvent(g.stream, g_side.clustered[b], 0));
+        RC_TRY(integrate(off, m, rows[b]));
+        HIP_TRY(hipEventRecord(g_side.integrated[b], g.stream));
+        HIP_TRY(hipStreamWaitEvent(g_side.s, g_side.integrated[b], 0));
+        RC_TRY(cluster(off, m, rows[b], g_side.s));
+        HIP_TRY(hipEventRecord(g_side.clustered[b], g_side.s));
+    }
+    if (k > 0) HIP_TRY(hipStreamWaitEvent(g.stream, g_side.clustered[(k - 1) & 1], 0));
+    return BIFHIP_OK;
+}
+
 // Whole 2D/1D cluster map (DBSCAN when kind==0, KDE when kind==1) into d_out.
 struct ClusterParams {
     int kind;
@@ -387,31 +465,32 @@ int cluster_map_device(const PeakMapSpec& S, const ClusterParams& C, int* d_out)
     pass = std::min(pass, cells);
     if (g.maxCellsPerLaunch > 0) pass = std::min(pass, g.maxCellsPerLaunch);
 
-    int *d_counts, *d_ovf, *d_novf;
-    double *d_amps, *d_ivls;
-    RC_TRY(pool_get("counts", (size_t)pass, &d_counts));
-    RC_TRY(pool_get("amps", (size_t)(pass * cap), &d_amps));
-    RC_TRY(pool_get("ivls", (size_t)(C.kind == 0 ? pass * cap : 1), &d_ivls));
+    int *d_ovf, *d_novf;
+    PeakRows rows[2];
+    RC_TRY(peak_rows("m", pass, cap, C.kind == 0, rows));
     RC_TRY(pool_get("ovf", (size_t)cells, &d_ovf));
     RC_TRY(pool_get("novf", 1, &d_novf));
     HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
     const double T = dbscan_threshold(C.eps);
 
-    for (long long off = 0; off < cells; off += pass) {
-        const int n = (int)std::min(pass, cells - off);
-        RC_TRY(integrate_peaks(S, (int)off, nullptr, n, cap, d_counts, d_amps,
-                               C.kind == 0 ? d_ivls : nullptr));
-        if (C.kind == 0) {
-            Timed t("dbscan");
-            HIP_TRY(launch_dbscan_wave(n, d_counts, d_amps, d_ivls, cap, cap, C.maxPeaks, T,
-                                       d_out + off, d_ovf, d_novf, (int)off, g.stream));
-        } else {
-            Timed t("kde");
-            HIP_TRY(launch_kde_wave(n, d_counts, d_amps, cap, cap, C.maxPeaks, C.kdeSampling,
-                                    C.I1, C.I2, C.H, d_out + off, nullptr, d_ovf, d_novf,
-                                    (int)off, g.stream));
-        }
-    }
+    RC_TRY(chunk_pipeline(
+        cells, pass, rows,
+        [&](long long off, int n, const PeakRows& r) {
+            return integrate_peaks(S, (int)off, nullptr, n, cap, r.counts, r.amps, r.ivls);
+        },
+        [&](long long off, int n, const PeakRows& r, hipStream_t st) {
+            if (C.kind == 0) {
+                Timed t("dbscan", st);
+                HIP_TRY(launch_dbscan_wave(n, r.counts, r.amps, r.ivls, cap, cap, C.maxPeaks, T,
+                                           d_out + off, d_ovf, d_novf, (int)off, st));
+            } else {
+                Timed t("kde", st);
+                HIP_TRY(launch_kde_wave(n, r.counts, r.amps, cap, cap, C.maxPeaks,
+                                        C.kdeSampling, C.I1, C.I2, C.H, d_out + off, nullptr,
+                                        d_ovf, d_novf, (int)off, st));
+            }
+            return BIFHIP_OK;
+        }));
     int novf = 0;
     HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
@@ -703,11 +782,10 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
     pass = std::min<long long>(pass, std::max(1, n));
     if (g.maxCellsPerLaunch > 0) pass = std::min(pass, g.maxCellsPerLaunch);
 
-    int *d_counts, *d_ovf, *d_novf;
-    double *d_amps, *d_ivls, *d_dirs = nullptr;
-    RC_TRY(pool_get("f_counts", (size_t)pass, &d_counts));
-    RC_TRY(pool_get("f_amps", (size_t)(pass * cap), &d_amps));
-    RC_TRY(pool_get("f_ivls", (size_t)(pass * cap), &d_ivls));
+    int *d_ovf, *d_novf;
+    double* d_dirs = nullptr;
+    PeakRows rows[2];
+    RC_TRY(peak_rows("f", pass, cap, true, rows));
     RC_TRY(pool_get("f_ovf", (size_t)std::max(1, n), &d_ovf));
     RC_TRY(pool_get("f_novf", 1, &d_novf));
     HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
@@ -717,8 +795,7 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
     std::vector<double> hdirs;
     if (refDirs) RC_TRY(pool_get("f_dirs", (size_t)pass * 3, &d_dirs));
 
-    for (long long off = 0; off < n; off += pass) {
-        const int m = (int)std::min<long long>(pass, n - off);
+    auto integrate = [&](long long off, int m, const PeakRows& r) -> int {
         FusedArgs F;
         std::memset(&F, 0, sizeof(F));
         F.pk = bif_args(S, base + (int)off, d_list ? d_list + off : nullptr, m, cap);
@@ -747,16 +824,19 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
         }
         {
             Timed t("lle_peaks");
-            HIP_TRY(launch_lle_peaks(g.model, S.wv, F, d_lle + off, d_counts, d_amps, d_ivls,
+            HIP_TRY(launch_lle_peaks(g.model, S.wv, F, d_lle + off, r.counts, r.amps, r.ivls,
                                      g.stream));
         }
-        {
-            Timed t("dbscan");
-            HIP_TRY(launch_dbscan_wave(m, d_counts, d_amps, d_ivls, cap, cap, maxPeaks, T,
-                                       d_peak + off, d_ovf, d_novf, (int)off, g.stream));
-        }
-        if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));
-    }
+        if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
+        return BIFHIP_OK;
+    };
+    auto cluster = [&](long long off, int m, const PeakRows& r, hipStream_t st) -> int {
+        Timed t("dbscan", st);
+        HIP_TRY(launch_dbscan_wave(m, r.counts, r.amps, r.ivls, cap, cap, maxPeaks, T,
+                                   d_peak + off, d_ovf, d_novf, (int)off, st));
+        return BIFHIP_OK;
+    };
+    RC_TRY(chunk_pipeline(n, pass, rows, integrate, cluster));
     int novf = 0;
     HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
@@ -779,23 +859,25 @@ int peak_cells_device(const PeakMapSpec& S, int maxPeaks, double epsDb, const in
     long long pass = std::max<long long>(1, (8LL << 30) / rowBytes);
     pass = std::min<long long>(pass, n);
     if (g.maxCellsPerLaunch > 0) pass = std::min(pass, g.maxCellsPerLaunch);
-    int *d_counts, *d_ovf, *d_novf;
-    double *d_amps, *d_ivls;
-    RC_TRY(pool_get("f_counts", (size_t)pass, &d_counts));
-    RC_TRY(pool_get("f_amps", (size_t)(pass * cap), &d_amps));
-    RC_TRY(pool_get("f_ivls", (size_t)(pass * cap), &d_ivls));
+    int *d_ovf, *d_novf;
+    PeakRows rows[2];
+    RC_TRY(peak_rows("f", pass, cap, true, rows));
     RC_TRY(pool_get("f_ovf", (size_t)n, &d_ovf));
     RC_TRY(pool_get("f_novf", 1, &d_novf));
     HIP_TRY(launch_fill(1, d_novf, 0, g.stream));
     const double T = dbscan_threshold(epsDb);
-    for (long long off = 0; off < n; off += pass) {
-        const int m = (int)std::min<long long>(pass, n - off);
-        RC_TRY(integrate_peaks(S, base + (int)off, d_list ? d_list + off : nullptr, m, cap,
-                               d_counts, d_amps, d_ivls));
-        Timed t("dbscan");
-        HIP_TRY(launch_dbscan_wave(m, d_counts, d_amps, d_ivls, cap, cap, maxPeaks, T,
-                                   d_peak + off, d_ovf, d_novf, (int)off, g.stream));
-    }
+    RC_TRY(chunk_pipeline(
+        n, pass, rows,
+        [&](long long off, int m, const PeakRows& r) {
+            return integrate_peaks(S, base + (int)off, d_list ? d_list + off : nullptr, m, cap,
+                                   r.counts, r.amps, r.ivls);
+        },
+        [&](long long off, int m, const PeakRows& r, hipStream_t st) {
+            Timed t("dbscan", st);
+            HIP_TRY(launch_dbscan_wave(m, r.counts, r.amps, r.ivls, cap, cap, maxPeaks, T,
+                                       d_peak + off, d_ovf, d_novf, (int)off, st));
+            return BIFHIP_OK;
+        }));
     int novf = 0;
     HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
@@ -1258,6 +1340,7 @@ int bifhip_kernel_time(const char* kernel, double* total_ms, long long* launches
 
 int bifhip_release(void)
 {
+    if (g_side.s) HIP_TRY(hipStreamSynchronize(g_side.s));  // clustering still reading rows
     g_pool.release();
     return BIFHIP_OK;
 }

@@ -6,44 +6,71 @@
 // across the 256 CUs without a host round trip.  Per lane: the state and
 // three tangent vectors in FP32 registers, advanced with the variational
 // equation of the discrete step; every nNT steps the vectors are promoted to
-// FP64, orthonormalised with the reference's classical Gram-Schmidt
-// (Library/cudaLibrary.cu:913-957) and log(norm) accumulates in FP64.
+// FP64 and orthonormalised with classical Gram-Schmidt (the reference's
+// scheme, Library/cudaLibrary.cu:913-957, with normalised projections), and
+// the norms accumulate in FP64 as a product, logged once at the end.
 #include "bifhip_dev.h"
 
 namespace bifhip {
 
-__device__ __forceinline__ void gs3_f(float (&v)[9], double (&acc)[3])
-{
-    double a[9], b[9];
+// Running product of squared norms in FP64 as mantissa x 2^exponent: the
+// spectrum needs sum(log norm) only at the end, so one log per exponent per
+// cell replaces one per renormalisation (a product of 4 000 norms rounds
+// 4 000 times, an error of about 1e-13 in the log sum).
+struct NormLog {
+    double m[3];
+    int e[3];
+    __device__ __forceinline__ void init()
+    {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) a[i] = v[i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) b[i * 3 + j] = a[i * 3 + j];
-#pragma unroll
-        for (int j = 0; j < i; ++j) {
-            double num = 0, dd = 0;
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                num += a[i * 3 + m] * b[j * 3 + m];
-                dd += b[j * 3 + m] * b[j * 3 + m];
-            }
-            const double f = dd == 0 ? 0 : num / dd;
-#pragma unroll
-            for (int m = 0; m < 3; ++m) b[i * 3 + m] -= f * b[j * 3 + m];
+        for (int i = 0; i < 3; ++i) {
+            m[i] = 1.0;
+            e[i] = 0;
         }
     }
+    __device__ __forceinline__ void add(int i, double d2)
+    {
+        int ea, eb;
+        const double a = frexp(d2, &ea);
+        m[i] = frexp(m[i] * a, &eb);
+        e[i] += ea + eb;
+    }
+    // sum of log(norm) = log(product of squared norms) / 2
+    __device__ __forceinline__ double log_sum(int i) const
+    {
+        return 0.5 * (log(m[i]) + (double)e[i] * 0.69314718055994530942);
+    }
+};
+
+// Gram-Schmidt of the three FP32 tangent vectors in FP64: the classical
+// scheme (each vector minus its projections on the previous ones, as
+// Library/cudaLibrary.cu:913-957), written with the previous vectors already
+// normalised, so a projection is a dot product and no division; each norm
+// from a reciprocal square root.  Zero vectors stay zero (the reference's
+// guards), their log norm is -inf.
+__device__ __forceinline__ void gs3_f(float (&v)[9], NormLog& acc)
+{
+    double u[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) u[i] = v[i];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        double d = 0;
+        double a[3] = {u[i * 3], u[i * 3 + 1], u[i * 3 + 2]};
+        double b[3] = {a[0], a[1], a[2]};
 #pragma unroll
-        for (int j = 0; j < 3; ++j) d += b[i * 3 + j] * b[i * 3 + j];
-        d = sqrt(d);
-        const double inv = d == 0 ? 0 : 1 / d;
+        for (int j = 0; j < i; ++j) {
+            const double p = fma(a[2], u[j * 3 + 2], fma(a[1], u[j * 3 + 1], a[0] * u[j * 3]));
 #pragma unroll
-        for (int j = 0; j < 3; ++j) v[i * 3 + j] = (float)(b[i * 3 + j] * inv);
-        acc[i] += log(d);
+            for (int m = 0; m < 3; ++m) b[m] = fma(-p, u[j * 3 + m], b[m]);
+        }
+        const double d2 = fma(b[2], b[2], fma(b[1], b[1], b[0] * b[0]));
+        const double r = d2 > 0.0 ? rsqrt(d2) : 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            u[i * 3 + m] = b[m] * r;
+            v[i * 3 + m] = (float)u[i * 3 + m];
+        }
+        acc.add(i, d2);
     }
 }
 
@@ -65,7 +92,7 @@ __global__ void k_ls_tiles(const TileArgs A)
 
         typename M::P32 q[CPL];
         float x0[CPL], x1[CPL], x2[CPL], v[CPL][9];
-        double acc[CPL][3];
+        NormLog acc[CPL];
         bool live[CPL];
         int cellOf[CPL];
 #pragma unroll
@@ -84,13 +111,16 @@ __global__ void k_ls_tiles(const TileArgs A)
             x2[c] = (float)s2;
 #pragma unroll
             for (int k = 0; k < 9; ++k) v[c][k] = (k % 4 == 0) ? 1.0f : 0.0f;
-            acc[c][0] = acc[c][1] = acc[c][2] = 0;
+            acc[c].init();
         }
         for (int s = 0; s < A.skip; ++s)
 #pragma unroll
             for (int c = 0; c < CPL; ++c) M::step32(x0[c], x1[c], x2[c], q[c]);
 
         for (int b = 0; b < nb; ++b) {
+            // unrolled so the loop-carried tangent components keep their
+            // registers (one step per iteration rotates them with moves)
+#pragma unroll 2
             for (int s = 0; s < A.nNT; ++s) {
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
@@ -109,9 +139,9 @@ __global__ void k_ls_tiles(const TileArgs A)
         for (int c = 0; c < CPL; ++c) {
             if (!live[c]) continue;
             double* o = A.out + ((long long)tile * tileCells + cellOf[c]) * 3;
-            o[0] = acc[c][0] / T;
-            o[1] = acc[c][1] / T;
-            o[2] = acc[c][2] / T;
+            o[0] = acc[c].log_sum(0) / T;
+            o[1] = acc[c].log_sum(1) / T;
+            o[2] = acc[c].log_sum(2) / T;
         }
     }
 }

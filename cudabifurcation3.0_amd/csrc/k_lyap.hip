@@ -7,7 +7,8 @@
 //  k_lle32   FP32 bulk: x in FP32 with fused multiply-adds, the separation
 //            carried as a scaled deviation u (y = x + eps*u, exact
 //            difference of the two maps, so eps = 1e-8 survives FP32);
-//            norm, log accumulation and renormalisation in FP64.
+//            norm, renormalisation (rsqrt) and the log-norm sum in FP64,
+//            the sum kept as a product of squared norms and logged once.
 //  k_ls64    finite-difference Lyapunov spectrum (LSKernelCUDA,
 //            Library/cudaLibrary.cu:961-1100) with classical Gram-Schmidt
 //            (Library/cudaLibrary.cu:913-957), FP64, reference order.
@@ -116,7 +117,8 @@ __global__ __launch_bounds__(256) void k_lle32(const LyapArgs A, double* __restr
         M::step32(x0, x1, x2, q);
         if (CHECK && fabsf(x0) > mv) break;
     }
-    double result = 0;
+    LogNormSum result;
+    result.init();
     bool live = true;
     for (int b = 0; b < A.nBlocks; ++b) {
         if (live) {
@@ -129,15 +131,14 @@ __global__ __launch_bounds__(256) void k_lle32(const LyapArgs A, double* __restr
             }
         }
         const double d0 = u0, d1 = u1, d2 = u2;
-        double n = d0 * d0 + d1 * d1 + d2 * d2;
-        n = sqrt(n);
-        result += log(n);
-        const double inv = n != 0 ? 1 / n : 1.0;
+        const double n2 = fma(d2, d2, fma(d1, d1, d0 * d0));
+        result.add(n2);
+        const double inv = n2 > 0.0 ? rsqrt(n2) : 1.0;
         u0 = (float)(d0 * inv);
         u1 = (float)(d1 * inv);
         u2 = (float)(d2 * inv);
     }
-    out[t] = result / A.tMax;
+    out[t] = result.value() / A.tMax;
 }
 
 // Classical Gram-Schmidt of three row vectors a -> b with norms

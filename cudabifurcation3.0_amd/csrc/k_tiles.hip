@@ -13,42 +13,13 @@
 
 namespace bifhip {
 
-// Running product of squared norms in FP64 as mantissa x 2^exponent: the
-// spectrum needs sum(log norm) only at the end, so one log per exponent per
-// cell replaces one per renormalisation (a product of 4 000 norms rounds
-// 4 000 times, an error of about 1e-13 in the log sum).
-struct NormLog {
-    double m[3];
-    int e[3];
-    __device__ __forceinline__ void init()
-    {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            m[i] = 1.0;
-            e[i] = 0;
-        }
-    }
-    __device__ __forceinline__ void add(int i, double d2)
-    {
-        int ea, eb;
-        const double a = frexp(d2, &ea);
-        m[i] = frexp(m[i] * a, &eb);
-        e[i] += ea + eb;
-    }
-    // sum of log(norm) = log(product of squared norms) / 2
-    __device__ __forceinline__ double log_sum(int i) const
-    {
-        return 0.5 * (log(m[i]) + (double)e[i] * 0.69314718055994530942);
-    }
-};
-
 // Gram-Schmidt of the three FP32 tangent vectors in FP64: the classical
 // scheme (each vector minus its projections on the previous ones, as
 // Library/cudaLibrary.cu:913-957), written with the previous vectors already
 // normalised, so a projection is a dot product and no division; each norm
 // from a reciprocal square root.  Zero vectors stay zero (the reference's
 // guards), their log norm is -inf.
-__device__ __forceinline__ void gs3_f(float (&v)[9], NormLog& acc)
+__device__ __forceinline__ void gs3_f(float (&v)[9], LogNormSum (&acc)[3])
 {
     double u[9];
 #pragma unroll
@@ -70,7 +41,7 @@ __device__ __forceinline__ void gs3_f(float (&v)[9], NormLog& acc)
             u[i * 3 + m] = b[m] * r;
             v[i * 3 + m] = (float)u[i * 3 + m];
         }
-        acc.add(i, d2);
+        acc[i].add(d2);
     }
 }
 
@@ -92,7 +63,7 @@ __global__ void k_ls_tiles(const TileArgs A)
 
         typename M::P32 q[CPL];
         float x0[CPL], x1[CPL], x2[CPL], v[CPL][9];
-        NormLog acc[CPL];
+        LogNormSum acc[CPL][3];
         bool live[CPL];
         int cellOf[CPL];
 #pragma unroll
@@ -111,7 +82,9 @@ __global__ void k_ls_tiles(const TileArgs A)
             x2[c] = (float)s2;
 #pragma unroll
             for (int k = 0; k < 9; ++k) v[c][k] = (k % 4 == 0) ? 1.0f : 0.0f;
-            acc[c].init();
+            acc[c][0].init();
+            acc[c][1].init();
+            acc[c][2].init();
         }
         for (int s = 0; s < A.skip; ++s)
 #pragma unroll
@@ -139,9 +112,9 @@ __global__ void k_ls_tiles(const TileArgs A)
         for (int c = 0; c < CPL; ++c) {
             if (!live[c]) continue;
             double* o = A.out + ((long long)tile * tileCells + cellOf[c]) * 3;
-            o[0] = acc[c].log_sum(0) / T;
-            o[1] = acc[c].log_sum(1) / T;
-            o[2] = acc[c].log_sum(2) / T;
+            o[0] = acc[c][0].value() / T;
+            o[1] = acc[c][1].value() / T;
+            o[2] = acc[c][2].value() / T;
         }
     }
 }

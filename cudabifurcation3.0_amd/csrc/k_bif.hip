@@ -120,8 +120,9 @@ __device__ __forceinline__ float pick32(float x0, float x1, float x2)
 
 // FP32 form of k_bif_peaks (bifhip_set_precision(BIFHIP_PREC_FP32)): the
 // state in FP32, stepped with the model's fused FP32 map (M::step32, as the
-// FP32 LLE kernel), each sample widened exactly to double for the same peak
-// stream, intervals and divergence test.  Every update of the FP32 map
+// FP32 LLE kernel); the peak stream compares the float samples (the same
+// outcomes as on the samples widened to double) and widens the peaks it
+// writes.  Every update of the FP32 map
 // contains the component itself, so NaN stays absorbing (the trajectory
 // form of the stream applies).  No reference counterpart: the counts are
 // judged by their agreement with the FP64 map.
@@ -155,14 +156,14 @@ __global__ __launch_bounds__(256) void k_bif_peaks32(const BifArgs A, int* __res
     sink.cap = A.cap;
     sink.hpf = A.hpf;
 
-    TrajPeakStream ps;
+    TrajPeakStream32 ps;
     ps.init();
     bool bad = false;
     const int N = A.N;
     const int pre = A.pre;
     int i = 0;
     for (; i < N && i < 2; ++i) {
-        ps.prime((double)pick32<WV>(x0, x1, x2), i);
+        ps.prime(pick32<WV>(x0, x1, x2), i);
         for (int j = 0; j < pre - 1; ++j) M::step32(x0, x1, x2, q);
         M::step32(x0, x1, x2, q);
         if (CHECK && diverged((double)pick32<WV>(x0, x1, x2), A.maxValue)) { bad = true; break; }
@@ -170,21 +171,21 @@ __global__ __launch_bounds__(256) void k_bif_peaks32(const BifArgs A, int* __res
     if (!CHECK && pre == 1) {
         constexpr int K = kPeakBatch;
         for (; i + K <= N; i += K) {
-            double va[K];
+            float va[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                va[k] = (double)pick32<WV>(x0, x1, x2);
+                va[k] = pick32<WV>(x0, x1, x2);
                 M::step32(x0, x1, x2, q);
             }
             ps.push_batch(va, i, sink);
         }
         for (; i < N; ++i) {
-            ps.push((double)pick32<WV>(x0, x1, x2), i, sink);
+            ps.push(pick32<WV>(x0, x1, x2), i, sink);
             M::step32(x0, x1, x2, q);
         }
     } else {
         for (; i < N && !bad; ++i) {
-            ps.push((double)pick32<WV>(x0, x1, x2), i, sink);
+            ps.push(pick32<WV>(x0, x1, x2), i, sink);
             for (int j = 0; j < pre - 1; ++j) M::step32(x0, x1, x2, q);
             M::step32(x0, x1, x2, q);
             if (CHECK && diverged((double)pick32<WV>(x0, x1, x2), A.maxValue)) bad = true;

@@ -36,6 +36,11 @@ namespace bifhip {
 //     pending plateau / skip.
 // Without TRAJ (series from memory, general models) the rising test is an
 // explicit compare.
+//
+// T: the sample type.  FP32 trajectories stream float samples: widening a
+// float to double is exact and keeps the order (NaN included), so every
+// comparison gives what it gives on the widened samples, and only the peaks
+// written to the rows are widened.
 
 struct PeakSink {
     double* amps;     // row base (nullptr: count only)
@@ -44,9 +49,9 @@ struct PeakSink {
     double hpf;       // peakFinder's h (0.01 by default)
 };
 
-template <bool TRAJ>
+template <bool TRAJ, class T = double>
 struct PeakStreamT {
-    double p1, p2;    // d[t-1], d[t-2] (p2 unused with TRAJ)
+    T p1, p2;         // d[t-1], d[t-2] (p2 unused with TRAJ)
     int found, lastj;
     // wavefront-uniform lane masks
     uint64_t plat;    // lanes on a plateau
@@ -62,19 +67,19 @@ struct PeakStreamT {
     }
 
     // samples 0 and 1 fill the window
-    __device__ __forceinline__ void prime(double v, int i)
+    __device__ __forceinline__ void prime(T v, int i)
     {
         if (TRAJ && i == 1) geprev = __builtin_amdgcn_ballot_w64(p1 >= v);
         p2 = p1;
         p1 = v;
     }
 
-    __device__ __forceinline__ void emit(double amp, int j, const PeakSink& s)
+    __device__ __forceinline__ void emit(T amp, int j, const PeakSink& s)
     {
         if (found >= 1) {
             const int slot = found - 1;
             if (slot < s.cap) {
-                if (s.amps) s.amps[slot] = amp;
+                if (s.amps) s.amps[slot] = (double)amp;
                 if (s.ivls) s.ivls[slot] = (double)(j - lastj) * s.hpf;
             }
         }
@@ -84,7 +89,7 @@ struct PeakStreamT {
 
     // One sample of the machine: d[t-1] = prev, d[t] = v, rising candidates
     // (before the skip / plateau masks) in c.
-    __device__ __forceinline__ void machine(double prev, double v, uint64_t c, int t,
+    __device__ __forceinline__ void machine(T prev, T v, uint64_t c, int t,
                                             const PeakSink& s)
     {
         const uint64_t cand = c & ~skip & ~plat;
@@ -102,7 +107,7 @@ struct PeakStreamT {
     }
 
     // sample v has index t >= 2
-    __device__ __forceinline__ void push(double v, int t, const PeakSink& s)
+    __device__ __forceinline__ void push(T v, int t, const PeakSink& s)
     {
         const uint64_t ge = __builtin_amdgcn_ballot_w64(p1 >= v);
         const uint64_t rise = TRAJ ? ~geprev : __builtin_amdgcn_ballot_w64(p1 > p2);
@@ -118,7 +123,7 @@ struct PeakStreamT {
     // K consecutive samples v[0..K-1] with indices t..t+K-1 (t >= 2), all
     // lanes of the wavefront active (TRAJ only).
     template <int K>
-    __device__ __forceinline__ void push_batch(const double (&v)[K], int t, const PeakSink& s)
+    __device__ __forceinline__ void push_batch(const T (&v)[K], int t, const PeakSink& s)
     {
         static_assert(TRAJ, "batched samples need the trajectory form");
         uint64_t ge[K], c[K];
@@ -146,5 +151,6 @@ struct PeakStreamT {
 
 using PeakStream = PeakStreamT<false>;
 using TrajPeakStream = PeakStreamT<true>;
+using TrajPeakStream32 = PeakStreamT<true, float>;
 
 }  // namespace bifhip

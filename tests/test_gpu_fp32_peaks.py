@@ -5,7 +5,8 @@ Standards:
   * against the oracle's FP32 restatement (float state, the same fused FP32
     step, samples widened to double, the reference's peak finder and
     DBSCAN): bit-exact -- every FP32 operation is a correctly rounded add,
-    multiply or fma on both sides;
+    multiply or fma on both sides; KDE over those peaks: count agreement
+    >= 99.5%, as for the FP64 KDE map (glibc expf);
   * against the FP64 map: agreement is measured and printed, and checked
     only as a sanity band (the FP32 path has no reference counterpart).
 """
@@ -99,6 +100,25 @@ def test_fp32_dbscan_map_agreement_with_fp64(orc, bh):
     agree = float((np.clip(m32, 1, 16) == np.clip(m64, 1, 16)).mean())
     say("peak-count class agreement FP32 vs FP64", agree)
     assert agree > 0.5
+
+
+def test_fp32_kde_map_agreement(orc, bh):
+    """KDE over FP32 peaks: the oracle's FP32 peak rows clustered by the
+    oracle's kde.  KDE sums glibc expf, which is not correctly rounded, so
+    the standard is the FP64 KDE map's: count agreement >= 99.5%."""
+    n, cap = 24, 1024
+    say("fp32 kde map", n)
+    _fp32(bh)
+    m = bh.kde_map(200.0, n, 0.01, IC, R2, [0, 2], 0, 0.0, 1000, 200.0, VALS, 1, 10, -15.0,
+                   15.0, 0.05)
+    cnt, pk, _ = orc.peaks_cells(2, 200.0, n, 0.01, IC, R2, [0, 2], 0, 0.0, 200.0, VALS, 1, 0,
+                                 n * n, cap, fp32=True)
+    assert cnt.max() < cap
+    want = np.array([orc.kde(pk[k, :max(cnt[k], 0)], 1000, 10, -15.0, 15.0, 0.05)
+                     for k in range(n * n)])
+    agree = float((m == want).mean())
+    say("fp32 kde count agreement with the oracle", agree)
+    assert agree >= 0.995
 
 
 def test_fp32_peak_map_refuses_the_memristive_model(bh):

@@ -379,10 +379,11 @@ static int peak_rows(const char* tag, long long pass, int cap, bool ivls, PeakRo
 }
 
 // integrate(off, m, rows) enqueues on g.stream; cluster(off, m, rows, st)
-// enqueues on st.
+// enqueues on st.  join false leaves the last clustering running on the side
+// stream (the caller synchronises it before reading the results).
 template <class Integrate, class Cluster>
 static int chunk_pipeline(long long n, long long pass, const PeakRows rows[2],
-                          Integrate integrate, Cluster cluster)
+                          Integrate integrate, Cluster cluster, bool join = true)
 {
     RC_TRY(side_stream());
     int k = 0;
@@ -396,7 +397,38 @@ static int chunk_pipeline(long long n, long long pass, const PeakRows rows[2],
         RC_TRY(cluster(off, m, rows[b], g_side.s));
         HIP_TRY(hipEventRecord(g_side.clustered[b], g_side.s));
     }
-    if (k > 0) HIP_TRY(hipStreamWaitEvent(g.stream, g_side.clustered[(k - 1) & 1], 0));
+    if (join && k > 0) HIP_TRY(hipStreamWaitEvent(g.stream, g_side.clustered[(k - 1) & 1], 0));
+    return BIFHIP_OK;
+}
+
+// A peak-count job split in two so independent work can be enqueued while
+// its last clustering pass still runs on the side stream: begin enqueues the
+// passes; finish waits for the side stream, reads the overflow count and
+// recomputes the overflowing cells.  One job at a time (the rows and
+// overflow buffers are shared).
+struct PeakCellsJob {
+    PeakMapSpec S;
+    int maxPeaks;
+    double T;
+    const int* h_list;
+    const int* d_list;
+    int base, n;
+    int* d_peak;
+    int* d_ovf;
+    int* d_novf;
+    bool pending = false;
+};
+
+static PeakCellsJob g_peakJob;   // the surrogate's pending peak-count job
+
+// Entry points that use the shared rows first drop a job left pending by a
+// call that failed between its two halves: its clustering is waited for and
+// its results are not processed.
+static int abandon_pending_peaks()
+{
+    if (!g_peakJob.pending) return BIFHIP_OK;
+    g_peakJob.pending = false;
+    HIP_TRY(hipStreamSynchronize(g_side.s));
     return BIFHIP_OK;
 }
 
@@ -455,6 +487,7 @@ int dbscan_full_rows(int n, const int* d_list, int nRows, const int* d_counts,
 
 int cluster_map_device(const PeakMapSpec& S, const ClusterParams& C, int* d_out)
 {
+    RC_TRY(abandon_pending_peaks());
     const long long cells = S.cells;
     const int capMax = std::max(2, std::min(g.peakCap, S.N / 2 + 1));
     int cap = capMax;
@@ -768,6 +801,7 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
     if (g.model == BIFHIP_MODEL_MEMRISTIVE)
         return fail(BIFHIP_E_ARG, "the fused map supports the Rossler and linear models");
     if (S.pre != 1) return fail(BIFHIP_E_ARG, "the fused map needs preScaller 1");
+    RC_TRY(abandon_pending_peaks());
     if (S.maxValue != 0 || L.maxValue != 0)
         return fail(BIFHIP_E_ARG, "the fused map runs without divergence checks (maxValue 0)");
     const long long span = (long long)L.skip + (long long)L.nBlocks * L.nNT;
@@ -845,12 +879,10 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
     return BIFHIP_OK;
 }
 
-// DBSCAN peak count (bifurcation2DDBSCAN semantics) of listed cells:
-// cells = h_list/d_list (flattened lattice indices) or base..base+n;
-// d_peak[t] for entry t.
-int peak_cells_device(const PeakMapSpec& S, int maxPeaks, double epsDb, const int* h_list,
-                      const int* d_list, int base, int n, int* d_peak)
+static int peak_cells_begin(const PeakMapSpec& S, int maxPeaks, double epsDb, const int* h_list,
+                            const int* d_list, int base, int n, int* d_peak, PeakCellsJob& J)
 {
+    J.pending = false;
     if (n <= 0) return BIFHIP_OK;
     const int capMax = std::max(2, std::min(g.peakCap, S.N / 2 + 1));
     int cap = capMax;
@@ -877,13 +909,46 @@ int peak_cells_device(const PeakMapSpec& S, int maxPeaks, double epsDb, const in
             HIP_TRY(launch_dbscan_wave(m, r.counts, r.amps, r.ivls, cap, cap, maxPeaks, T,
                                        d_peak + off, d_ovf, d_novf, (int)off, st));
             return BIFHIP_OK;
-        }));
-    int novf = 0;
-    HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    RC_TRY(recompute_overflow(S, maxPeaks, T, h_list, d_list, base, novf, d_ovf, d_peak));
-    if (g.clusterClamp > 0) HIP_TRY(launch_clamp(n, d_peak, g.clusterClamp, g.stream));
+        },
+        false));
+    J.S = S;
+    J.maxPeaks = maxPeaks;
+    J.T = T;
+    J.h_list = h_list;
+    J.d_list = d_list;
+    J.base = base;
+    J.n = n;
+    J.d_peak = d_peak;
+    J.d_ovf = d_ovf;
+    J.d_novf = d_novf;
+    J.pending = true;
     return BIFHIP_OK;
+}
+
+static int peak_cells_finish(PeakCellsJob& J)
+{
+    if (!J.pending) return BIFHIP_OK;
+    J.pending = false;
+    HIP_TRY(hipStreamSynchronize(g_side.s));   // the last clustering pass
+    int novf = 0;
+    HIP_TRY(hipMemcpyAsync(&novf, J.d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    RC_TRY(recompute_overflow(J.S, J.maxPeaks, J.T, J.h_list, J.d_list, J.base, novf, J.d_ovf,
+                              J.d_peak));
+    if (g.clusterClamp > 0) HIP_TRY(launch_clamp(J.n, J.d_peak, g.clusterClamp, g.stream));
+    return BIFHIP_OK;
+}
+
+// DBSCAN peak count (bifurcation2DDBSCAN semantics) of listed cells:
+// cells = h_list/d_list (flattened lattice indices) or base..base+n;
+// d_peak[t] for entry t.
+int peak_cells_device(const PeakMapSpec& S, int maxPeaks, double epsDb, const int* h_list,
+                      const int* d_list, int base, int n, int* d_peak)
+{
+    RC_TRY(abandon_pending_peaks());
+    PeakCellsJob J;
+    RC_TRY(peak_cells_begin(S, maxPeaks, epsDb, h_list, d_list, base, n, d_peak, J));
+    return peak_cells_finish(J);
 }
 
 // Plain FP64 DBSCAN peak count of listed cells (surrogate pipeline, cells
@@ -899,6 +964,25 @@ int surrogate_resolve_peaks(int nPts, double h, int nIC, const double* ic, const
                      values, nVal, 1));
     return peak_cells_device(S, maxPeaks, dbEps, h_list, d_list, 0, n, d_peak);
 }
+
+// The same, in two halves (one pending job at a time): the begin half
+// enqueues the passes and returns with the last clustering pass still on the
+// side stream, so independent work enqueued next overlaps it; the finish
+// half completes the job before its results are read.
+int surrogate_resolve_peaks_begin(int nPts, double h, int nIC, const double* ic,
+                                  const double* ranges, const int* mut, const double* values,
+                                  int nVal, double peakTMax, double peakTransient, int wv,
+                                  int maxPeaks, double dbEps, const int* d_list, int n,
+                                  int* d_peak)
+{
+    RC_TRY(abandon_pending_peaks());
+    PeakMapSpec S;
+    RC_TRY(peak_spec(S, 2, peakTMax, nPts, h, nIC, ic, ranges, mut, wv, 0.0, peakTransient,
+                     values, nVal, 1));
+    return peak_cells_begin(S, maxPeaks, dbEps, nullptr, d_list, 0, n, d_peak, g_peakJob);
+}
+
+int surrogate_resolve_peaks_finish() { return peak_cells_finish(g_peakJob); }
 
 // Plain LLE (LLE2D semantics) of n listed cells (flattened indices
 // d_list[0..n) on an nPts x nPts lattice) into d_lle[0..n): the surrogate's
@@ -1340,6 +1424,7 @@ int bifhip_kernel_time(const char* kernel, double* total_ms, long long* launches
 
 int bifhip_release(void)
 {
+    RC_TRY(abandon_pending_peaks());
     if (g_side.s) HIP_TRY(hipStreamSynchronize(g_side.s));  // clustering still reading rows
     g_pool.release();
     return BIFHIP_OK;

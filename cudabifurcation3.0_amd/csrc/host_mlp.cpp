@@ -79,6 +79,12 @@ int surrogate_resolve_peaks(int nPts, double h, int nIC, const double* ic, const
                             const int* mut, const double* values, int nVal, double peakTMax,
                             double peakTransient, int wv, int maxPeaks, double dbEps,
                             const int* h_list, const int* d_list, int n, int* d_peak);
+int surrogate_resolve_peaks_begin(int nPts, double h, int nIC, const double* ic,
+                                  const double* ranges, const int* mut, const double* values,
+                                  int nVal, double peakTMax, double peakTransient, int wv,
+                                  int maxPeaks, double dbEps, const int* d_list, int n,
+                                  int* d_peak);
+int surrogate_resolve_peaks_finish();
 int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const double* ranges,
                           const int* mut, const double* values, int nVal, double lleTMax,
                           double NT, double lleTransient, double lleEps, const int* d_list, int n,
@@ -549,10 +555,12 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
         RC_TRY(surrogate_resolve(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
                                  peakTransient, wv, maxPeaks, dbEps, lleTMax, NT, lleTransient,
                                  lleEps, nullptr, d_all, ntot, d_rlle, d_rpeak));
+    // the class-only peak counts: their last clustering pass overlaps the
+    // regime-only LLE integration enqueued next
     if (npeak > 0)
-        RC_TRY(surrogate_resolve_peaks(nFine, h, nIC, ic, ranges, mut, values, nVal, peakTMax,
-                                       peakTransient, wv, maxPeaks, dbEps, nullptr, d_plist,
-                                       npeak, d_ppeak));
+        RC_TRY(surrogate_resolve_peaks_begin(nFine, h, nIC, ic, ranges, mut, values, nVal,
+                                             peakTMax, peakTransient, wv, maxPeaks, dbEps,
+                                             d_plist, npeak, d_ppeak));
     int nchg = 0, nnear = nlle;
     int *d_clist = nullptr, *d_cpeak2 = nullptr;
     const bool lle32 = g.lle32Margin >= 0 && g.model != BIFHIP_MODEL_MEMRISTIVE;
@@ -583,6 +591,7 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
                                      lleTransient, lleEps, d_llist, nlle, d_llle,
                                      BIFHIP_PREC_FP64));
     }
+    RC_TRY(surrogate_resolve_peaks_finish());
     if (nlle > 0) {
         // a regime that changed under the re-solve: the peak count as well
         RC_TRY(pool_get("sg_clist", (size_t)nlle, &d_clist));

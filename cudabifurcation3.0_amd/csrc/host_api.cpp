@@ -380,21 +380,26 @@ static int peak_rows(const char* tag, long long pass, int cap, bool ivls, PeakRo
 
 // integrate(off, m, rows) enqueues on g.stream; cluster(off, m, rows, st)
 // enqueues on st.  join false leaves the last clustering running on the side
-// stream (the caller synchronises it before reading the results).
+// stream (the caller synchronises it before reading the results).  tail > 0
+// makes the last pass `tail` cells (when more than that remain), so that
+// only a small clustering pass is left with no integration beside it.
 template <class Integrate, class Cluster>
 static int chunk_pipeline(long long n, long long pass, const PeakRows rows[2],
-                          Integrate integrate, Cluster cluster, bool join = true)
+                          Integrate integrate, Cluster cluster, bool join = true,
+                          long long tail = 0)
 {
     RC_TRY(side_stream());
     int k = 0;
-    for (long long off = 0; off < n; off += pass, ++k) {
-        const int m = (int)std::min(pass, n - off);
+    for (long long off = 0, m = 0; off < n; off += m, ++k) {
+        const long long r = n - off;
+        m = std::min(pass, r);
+        if (tail > 0 && r > tail && r <= pass) m = r - tail;
         const int b = k & 1;
         if (k >= 2) HIP_TRY(hipStreamWaitEvent(g.stream, g_side.clustered[b], 0));
-        RC_TRY(integrate(off, m, rows[b]));
+        RC_TRY(integrate(off, (int)m, rows[b]));
         HIP_TRY(hipEventRecord(g_side.integrated[b], g.stream));
         HIP_TRY(hipStreamWaitEvent(g_side.s, g_side.integrated[b], 0));
-        RC_TRY(cluster(off, m, rows[b], g_side.s));
+        RC_TRY(cluster(off, (int)m, rows[b], g_side.s));
         HIP_TRY(hipEventRecord(g_side.clustered[b], g_side.s));
     }
     if (join && k > 0) HIP_TRY(hipStreamWaitEvent(g.stream, g_side.clustered[(k - 1) & 1], 0));
@@ -870,7 +875,12 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
                                    d_peak + off, d_ovf, d_novf, (int)off, st));
         return BIFHIP_OK;
     };
-    RC_TRY(chunk_pipeline(n, pass, rows, integrate, cluster));
+    // with several passes the last is an eighth of a pass: the clustering of
+    // the pass before runs beside its integration (a single pass is not
+    // split: its short tail would integrate at low occupancy for as long as
+    // the clustering it hides)
+    const long long tail = n > pass ? std::max<long long>(1 << 16, pass / 8) : 0;
+    RC_TRY(chunk_pipeline(n, pass, rows, integrate, cluster, true, tail));
     int novf = 0;
     HIP_TRY(hipMemcpyAsync(&novf, d_novf, sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));

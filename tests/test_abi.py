@@ -111,6 +111,12 @@ def test_context_knob_validation():
         B.set_precision(5)
     with pytest.raises(B.BifhipError):
         B.set_peak_capacity(1)
+    with pytest.raises(B.BifhipError):
+        B.set_chaos_confidence(0.4)
+    with pytest.raises(B.BifhipError):
+        B.set_lle32_margin(float("nan"))
+    B.set_lle32_margin(-1.0)          # negative: every regime re-solve in FP64
+    B.set_lle32_margin(0.05)
     B.reset_context()
     B.set_output_path("/tmp/x.csv")
     assert B.load().bifhip_get_output_path() == b"/tmp/x.csv"

@@ -127,11 +127,11 @@ def cpu_baseline(n, rows):
 
 # HBM bytes per cell of k_lle_peaks at the benchmark's settings, from the
 # FETCH_SIZE / WRITE_SIZE counter runs of the plain 1024^2 map scaled by the
-# same counters on a 1 GiB copy (FETCH x 2.0, WRITE x 1.0): 396 KiB fetched
-# and 5 386 628 KiB written per 1 048 576-cell launch.  Algorithmic bytes are
-# about 1 372 per cell (85 peaks x 16 B, plus 12 B of results); the rest is
-# the 32-byte write granule charged for each isolated 8-byte peak store.
-LLE_PEAKS_HBM_BYTES_PER_CELL = (396 + 5386628) * 1024 / 1048576
+# same counters on a 1 GiB copy (FETCH x 2.0, WRITE x 1.0): 446 KiB fetched
+# and 1 495 424 KiB written per 1 048 576-cell launch.  Algorithmic bytes are
+# about 1 372 per cell (85 peaks x 16 B, plus 12 B of results): the peaks
+# reach the rows four at a time from an LDS stage, so the writes are 1.06x.
+LLE_PEAKS_HBM_BYTES_PER_CELL = (446 + 1495424) * 1024 / 1048576
 
 
 def roofline(bh, cells_total, kernel="lle_peaks"):

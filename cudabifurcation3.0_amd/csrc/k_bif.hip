@@ -33,6 +33,11 @@ namespace bifhip {
 #define BIFHIP_PEAK_BATCH 4
 #endif
 constexpr int kPeakBatch = BIFHIP_PEAK_BATCH;
+// peaks staged in LDS per lane before they go to the row: 4 in the fused
+// kernel (one cell per lane), 2 per cell in the two-cell peak kernel (16 KiB
+// per workgroup either way)
+constexpr int kStage = 4;
+constexpr int kStage2 = 2;
 
 template <class M, int WV>
 __device__ __forceinline__ double pick(double x0, double x1, double x2)
@@ -68,6 +73,10 @@ __global__ __launch_bounds__(256) void k_bif_peaks(const BifArgs A, int* __restr
     }
     for (; s < A.skip; ++s) M::step(x0, x1, x2, p, h);
 
+    // direct stores: this one-chain kernel serves small maps, the 1D diagram
+    // and the overflow recompute, where it is latency-bound and an LDS stage
+    // cost 3% (512^2 map); the two-cell and fused kernels stage their peaks
+    // (their rows are the bulk of the peak-row traffic)
     PeakSink sink;
     sink.amps = amps ? amps + (long long)t * A.rowStride : nullptr;
     sink.ivls = ivls ? ivls + (long long)t * A.rowStride : nullptr;
@@ -109,6 +118,7 @@ __global__ __launch_bounds__(256) void k_bif_peaks(const BifArgs A, int* __restr
             if (CHECK && diverged(pick<M, WV>(x0, x1, x2), A.maxValue)) bad = true;
         }
     }
+    ps.finish(sink);
     counts[t] = bad ? -1 : ps.count();
 }
 
@@ -150,6 +160,7 @@ __global__ __launch_bounds__(256) void k_bif_peaks32(const BifArgs A, int* __res
     }
     for (; s < A.skip; ++s) M::step32(x0, x1, x2, q);
 
+    // direct stores, as k_bif_peaks (one chain per lane: an LDS stage cost 2%)
     PeakSink sink;
     sink.amps = amps ? amps + (long long)t * A.rowStride : nullptr;
     sink.ivls = ivls ? ivls + (long long)t * A.rowStride : nullptr;
@@ -191,6 +202,7 @@ __global__ __launch_bounds__(256) void k_bif_peaks32(const BifArgs A, int* __res
             if (CHECK && diverged((double)pick32<WV>(x0, x1, x2), A.maxValue)) bad = true;
         }
     }
+    ps.finish(sink);
     counts[t] = bad ? -1 : ps.count();
 }
 
@@ -231,6 +243,9 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
     }
     for (; s < A.skip; ++s) M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
 
+    // two cells per lane: 2 peaks per cell staged (16 KiB per workgroup keeps
+    // the occupancy that the registers allow)
+    __shared__ double s_stage[256 * 2 * 2 * kStage2];
     PeakSink sa, sb;
     sa.amps = amps ? amps + (long long)c0 * A.rowStride : nullptr;
     sa.ivls = ivls ? ivls + (long long)c0 * A.rowStride : nullptr;
@@ -238,6 +253,9 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
     sb.ivls = ivls && two ? ivls + (long long)c1 * A.rowStride : nullptr;
     sa.cap = sb.cap = A.cap;
     sa.hpf = sb.hpf = A.hpf;
+    sa.stage = (amps || ivls) ? s_stage + threadIdx.x * 4 * kStage2 : nullptr;
+    sb.stage = sa.stage ? sa.stage + 2 * kStage2 : nullptr;
+    sa.stageN = sb.stageN = kStage2;
     TrajPeakStream pa, pb;
     pa.init();
     pb.init();
@@ -276,6 +294,8 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
             M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
         }
     }
+    pa.finish(sa);
+    pb.finish(sb);
     counts[c0] = pa.count();
     if (two) counts[c1] = pb.count();
 }
@@ -327,6 +347,8 @@ __global__ __launch_bounds__(256) void k_peak_finder(const double* data, int siz
     sink.ivls = outIntervals ? outIntervals + (long long)c * sizeOfBlock : nullptr;
     sink.cap = sizeOfBlock;
     sink.hpf = hpf;
+    sink.stage = nullptr;   // in place over the series: direct stores
+    sink.stageN = 1;
     PeakStream ps;
     ps.init();
     if (sizeOfBlock >= 1) ps.p1 = d[0];
@@ -869,11 +891,14 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
     const double h = A.h, eps = F.eps;
 
 
+    __shared__ double s_stage[256 * 2 * kStage];   // 16 KiB: 4 peaks per lane
     PeakSink sink;
     sink.amps = amps + (long long)t * A.rowStride;
     sink.ivls = ivls + (long long)t * A.rowStride;
     sink.cap = A.cap;
     sink.hpf = A.hpf;
+    sink.stage = s_stage + threadIdx.x * 2 * kStage;
+    sink.stageN = kStage;
     TrajPeakStream ps;
     ps.init();
     const int w0 = A.skip, w1 = A.skip + A.N;   // window of recorded states [w0, w1)
@@ -955,6 +980,7 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
         y2 = x2 - ((x2 - y2) * s);
     }
     lle[t] = result / F.tMax;
+    ps.finish(sink);
     counts[t] = ps.count();
 }
 

@@ -374,10 +374,13 @@ def run_bif1d(bh, np, args, out):
     bargs = (PEAK["tMax"], n, H, IC, [2.0, 7.0], [2], 0, 0.0, PEAK["transient"], vals, 1)
     for _ in range(args.warmup):
         bh.bifurcation1D_to_buffer(*bargs)
+    bh.set_timing(True)
+    bh.reset_kernel_times()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         cnt, pk = bh.bifurcation1D_to_buffer(*bargs)
     ms = (time.perf_counter() - t0) * 1000 / args.steps
+    kms, _ = bh.kernel_time("bif_peaks")
     O.lib()
     t1 = time.perf_counter()
     wc, wp, _ = O.peaks_cells(1, PEAK["tMax"], n, H, IC, [2.0, 7.0], [2], 0, 0.0,
@@ -391,6 +394,7 @@ def run_bif1d(bh, np, args, out):
                                    "a = b = 0.2, transient 1000, record 500, host buffers",
                        "h": H, "peak": PEAK})
     out["oracle_bit_exact_all_cells"] = exact
+    out["kernels_ms_per_step"] = {"bif_peaks": kms / args.steps}
     out["cpu_baseline"] = {"value": n / cpu_s, "unit": "parameter points/s",
                            "cores": O.lib().orc_threads(), "kind": "port",
                            "sample": f"all {n} cells, {cpu_s:.2f} s"}

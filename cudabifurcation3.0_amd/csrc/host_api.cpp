@@ -29,6 +29,8 @@ hipError_t launch_bif(int model, int precision, const BifArgs& A, int wv, bool c
                       double* amps, double* ivls, hipStream_t st);
 hipError_t launch_series(int model, const BifArgs& A, int wv, double* data, int* flags,
                          hipStream_t st);
+hipError_t launch_pack_rows(long long nCells, int rowCap, int stride, const double* src,
+                            double* dst, hipStream_t st);
 hipError_t launch_peak_finder(const double* data, int sizeOfBlock, int nBlocks, int* counts,
                               double* outPeaks, double* outIntervals, double hpf, hipStream_t st);
 hipError_t launch_dbscan_wave(int nCells, const int* counts, const double* amps,
@@ -1239,30 +1241,40 @@ int bif1d_impl(double tMax, int nPts, double h, int nIC, const double* ic, const
                      values, nVal, pre));
     RC_TRY(peak_precision(S));
     progress("BIFURCATION1D", S.cells);
-    // device rows hold every peak a record can have; only the used prefix
-    // of each row (the largest count) comes back to the host
-    const int rowCap = std::max(1, S.N / 2 + 1);
+    // rows of the library's peak capacity first (as the maps); a cell with
+    // more peaks than that sends the call round again with rows that hold
+    // every peak a record can have.  Only the used prefix of each row (the
+    // largest count) comes back to the host, packed on the device (a
+    // strided 2D copy to pageable memory took most of the call).
+    const int fullCap = std::max(1, S.N / 2 + 1);
+    int rowCap = std::min(fullCap, std::max(2, g.peakCap));
     DBuf c, a;
     HIP_TRY(c.alloc(sizeof(int) * (size_t)S.cells));
-    HIP_TRY(a.alloc(sizeof(double) * (size_t)S.cells * rowCap));
-    int* d_c = c.as<int>();
-    double* d_a = a.as<double>();
-    RC_TRY(integrate_peaks(S, 0, nullptr, (int)S.cells, rowCap, d_c, d_a, nullptr));
     counts.resize((size_t)S.cells);
-    HIP_TRY(hipMemcpyAsync(counts.data(), d_c, sizeof(int) * counts.size(), hipMemcpyDeviceToHost,
-                           g.stream));
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    // peakFinderCUDA turns the divergence flag into 0 peaks
     int used = 1;
-    for (auto& v : counts) {
-        if (v < 0) v = 0;
-        used = std::max(used, v);
+    for (;;) {
+        HIP_TRY(a.alloc(sizeof(double) * (size_t)S.cells * rowCap));
+        RC_TRY(integrate_peaks(S, 0, nullptr, (int)S.cells, rowCap, c.as<int>(), a.as<double>(),
+                               nullptr));
+        HIP_TRY(hipMemcpyAsync(counts.data(), c.p, sizeof(int) * counts.size(),
+                               hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        used = 1;
+        for (int v : counts) used = std::max(used, v);
+        if (used <= rowCap || rowCap == fullCap) break;
+        rowCap = fullCap;
     }
+    // peakFinderCUDA turns the divergence flag into 0 peaks
+    for (auto& v : counts)
+        if (v < 0) v = 0;
     stride = used;
     peaks.resize((size_t)S.cells * stride);
-    HIP_TRY(hipMemcpy2DAsync(peaks.data(), sizeof(double) * stride, d_a, sizeof(double) * rowCap,
-                             sizeof(double) * stride, (size_t)S.cells, hipMemcpyDeviceToHost,
+    DBuf packed;
+    HIP_TRY(packed.alloc(sizeof(double) * peaks.size()));
+    HIP_TRY(launch_pack_rows(S.cells, rowCap, stride, a.as<double>(), packed.as<double>(),
                              g.stream));
+    HIP_TRY(hipMemcpyAsync(peaks.data(), packed.p, sizeof(double) * peaks.size(),
+                           hipMemcpyDeviceToHost, g.stream));
     return finish();
 }
 

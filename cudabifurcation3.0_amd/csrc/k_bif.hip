@@ -1009,6 +1009,18 @@ __global__ void k_clamp_i32(int n, int* __restrict__ v, int clamp)
     if (i < n && v[i] > clamp) v[i] = clamp;
 }
 
+// The used prefix of each peak row, packed for one contiguous copy:
+// dst[c*stride + j] = src[c*rowCap + j] for j < stride.
+__global__ void k_pack_rows(long long nCells, int rowCap, int stride,
+                            const double* __restrict__ src, double* __restrict__ dst)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nCells * stride) return;
+    const long long c = i / stride;
+    const int j = (int)(i - c * stride);
+    dst[i] = src[c * rowCap + j];
+}
+
 // Fill kernel (used instead of hipMemset so graph-captured work stays
 // kernel-only).
 __global__ void k_fill_i32(long long n, int* __restrict__ v, int value)
@@ -1139,6 +1151,16 @@ hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, lo
     hipLaunchKernelGGL(k_kde_wave, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
                        nCells, counts, amps, rowStride, cap, maxPeaks, kdeSampling, I1, I2, H,
                        out, outIndex, overflow, nOverflow, listBase);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_rows(long long nCells, int rowCap, int stride, const double* src,
+                            double* dst, hipStream_t st)
+{
+    const long long n = nCells * stride;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nCells,
+                       rowCap, stride, src, dst);
     return hipGetLastError();
 }
 

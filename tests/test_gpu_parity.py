@@ -131,6 +131,27 @@ def test_bifurcation1d_config1_bit_exact(orc, bh):
     assert 80 <= cnt.min() and cnt.max() <= 90
 
 
+def test_bifurcation1d_full_rows_round(orc, bh):
+    """Rows of 8 slots cannot hold the ~85 peaks a cell has: the driver goes
+    round again with rows that hold every peak a record can have, and the
+    diagram is the same, bit for bit."""
+    say("bifurcation1D, 8-slot rows then full rows")
+    bh.set_peak_capacity(8)
+    cnt, pk = bh.bifurcation1D_to_buffer(500.0, 200, 0.01, IC, [2.0, 7.0], [2], 0, 0.0, 1000.0,
+                                         [0.2, 0.2, 0.0], 1)
+    wc, wp, _ = orc.peaks_cells(1, 500.0, 200, 0.01, IC, [2.0, 7.0], [2], 0, 0.0, 1000.0,
+                                [0.2, 0.2, 0.0], 1, 0, 200, pk.shape[1])
+    assert cnt.max() > 8
+    np.testing.assert_array_equal(cnt, np.maximum(wc, 0))
+    for k in range(200):
+        np.testing.assert_array_equal(pk[k, :cnt[k]], wp[k, :cnt[k]], err_msg=str(k))
+    # a caller's row stride shorter than a point's peaks is an error, not a cut
+    with pytest.raises(bh.BifhipError) as e:
+        bh.bifurcation1D_to_buffer(500.0, 20, 0.01, IC, [2.0, 7.0], [2], 0, 0.0, 1000.0,
+                                   [0.2, 0.2, 0.0], 1, stride=16)
+    assert e.value.code == bh.E_LIMIT
+
+
 def test_bifurcation1d_file_matches_reference_format(orc, bh, tmp_path):
     say("bifurcation1D file")
     out = tmp_path / "mat.csv"

@@ -431,7 +431,7 @@ def main():
                     help="lle workload: renormalisation interval in time units (0.5 in the "
                          "configuration; larger values isolate the renormalisations' cost)")
     ap.add_argument("--conf", type=float, default=0.95, help="class softmax confidence threshold")
-    ap.add_argument("--chaos-conf", type=float, default=0.98,
+    ap.add_argument("--chaos-conf", type=float, default=0.975,
                     help="chaos-logit confidence threshold")
     ap.add_argument("--lle32-margin", type=float, default=0.02,
                     help="regime-only re-solves: FP32 bulk LLE at least this far from the "

@@ -278,7 +278,7 @@ def reset_context():
     set_sweep(SWEEP_PARAMETERS)
     set_graph(True)
     set_partial_resolve(True)
-    set_chaos_confidence(0.98)
+    set_chaos_confidence(0.975)
     set_lle32_margin(0.02)
 
 

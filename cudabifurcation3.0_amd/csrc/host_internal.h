@@ -28,7 +28,7 @@ struct Ctx {
     int useGraph = 1;          // replay tile launches and training epochs from HIP graphs
     int sweepIC = 0;           // swept entries index the initial conditions (IC drivers)
     int partialResolve = 1;    // surrogate: class-only cells re-solve the peak count alone
-    double chaosConfidence = 0.98;  // surrogate: chaos confidence below this re-solves the LLE
+    double chaosConfidence = 0.975;  // surrogate: chaos confidence below this re-solves the LLE
     double lle32Margin = 0.02;  // surrogate: FP32 regime re-solves this far from the threshold
     std::string lastError;
     hipStream_t stream = nullptr;

@@ -415,7 +415,7 @@ int bifhip_set_graph(int on);
  * count alone (default 1); 0 re-solves every flagged cell in full. */
 int bifhip_set_partial_resolve(int on);
 /* Surrogate-refined map: cells whose chaos-logit confidence is below this
- * threshold (in [0.5, 1], default 0.98) have their LLE re-solved. */
+ * threshold (in [0.5, 1], default 0.975) have their LLE re-solved. */
 int bifhip_set_chaos_confidence(double threshold);
 /* Surrogate-refined map: cells re-solved for the regime only take their LLE
  * from the FP32 bulk kernel (FP32 state and separation, FP64 norms and log
@@ -436,7 +436,7 @@ int bifhip_set_lle32_margin(double margin);
  * head, kept on the predicted side of chaosThreshold.  A cell needs its
  * class re-solved when the class softmax confidence is below confThreshold
  * or a 4-neighbour has another predicted class, and its regime when the
- * chaos confidence is below bifhip_set_chaos_confidence's threshold (0.98
+ * chaos confidence is below bifhip_set_chaos_confidence's threshold (0.975
  * by default) or a 4-neighbour has another predicted regime.  Cells that
  * need their class and regime are re-solved by the plain FP64 fused kernel
  * (bifhip_lle_peak_map_device semantics).  With partial re-solves on

@@ -89,6 +89,7 @@ def _declare(L):
         "bifhip_LLE_device": [_I] + drv_ly + [_VP],
         "bifhip_peak_finder_device": [_VP, _I, _I, _VP, _VP, _VP, _D],
         "bifhip_dbscan_device": [_VP, _VP, _I, _I, _VP, _I, _D, _VP],
+        "bifhip_dbscan_device_noalloc": [_VP, _VP, _I, _I, _VP, _I, _D, _VP],
         "bifhip_kde_device": [_VP, _I, _I, _VP, _I, _I, _D, _D, _D, _VP],
         "bifhip_gram_schmidt_device": [_VP, _VP, _VP, _I, _I],
         "bifhip_series_device": [_I, _D, _I, _D, _I, _PD, _PD, _PI, _I, _D, _D, _PD, _I,
@@ -513,6 +514,19 @@ def peak_finder_device(d_data, sizeOfBlock, nBlocks, d_counts, d_peaks, d_times,
 def dbscan_device(d_peaks, d_ivls, sizeOfBlock, nBlocks, d_counts, maxPeaks, eps, d_out):
     _check(load().bifhip_dbscan_device(d_peaks, d_ivls, sizeOfBlock, nBlocks, d_counts, maxPeaks,
                                        eps, d_out))
+
+
+def dbscan_device_noalloc(d_peaks, d_ivls, sizeOfBlock, nBlocks, d_counts, maxPeaks, eps,
+                          d_out):
+    """dbscan_device with the library's fixed per-GPU scratch array: no
+    device allocation; at most NOALLOC_MAX_BLOCKS blocks and rows of at
+    most NOALLOC_MAX_ROW peaks per call."""
+    _check(load().bifhip_dbscan_device_noalloc(d_peaks, d_ivls, sizeOfBlock, nBlocks, d_counts,
+                                               maxPeaks, eps, d_out))
+
+
+NOALLOC_MAX_BLOCKS = 1 << 20
+NOALLOC_MAX_ROW = 1 << 20
 
 
 def kde_device(d_peaks, sizeOfBlock, nBlocks, d_counts, maxPeaks, sampling, i1, i2, smooth,

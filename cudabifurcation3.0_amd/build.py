@@ -32,7 +32,7 @@ COMMON = [
     "-Wno-unused-command-line-argument", "-I", INC, "-I", CSRC,
 ]
 
-SOURCES = ["k_bif.hip", "k_lyap.hip", "k_generic.hip", "k_tiles.hip", "k_mlp.hip", "host_api.cpp",
+SOURCES = ["k_bif.hip", "k_lyap.hip", "k_generic.hip", "k_tiles.hip", "k_mlp.hip", "noalloc.hip", "host_api.cpp",
            "host_tiles.cpp", "host_mlp.cpp"]
 
 

@@ -348,6 +348,23 @@ int bifhip_dbscan_device(const double* d_peaks, const double* d_intervals, int s
                          int amountOfBlocks, const int* d_counts, int maxAmountOfPeaks,
                          double eps, int* d_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): bifhip_dbscan_device with the same arguments and
+ * results, whose scratch (overflow lists, counters, lane label rows) is a
+ * fixed device array of BIFHIP_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; it calls no allocating routine.  Limits: amountOfBlocks <=
+ * BIFHIP_NOALLOC_MAX_BLOCKS per call, and sizeOfBlock <=
+ * BIFHIP_NOALLOC_MAX_ROW; beyond them it returns BIFHIP_E_LIMIT.  Calls
+ * share the array of their GPU: calls that overlap on one GPU race. */
+#define BIFHIP_NOALLOC_MAX_BLOCKS (1 << 20)
+#define BIFHIP_NOALLOC_MAX_ROW (1 << 20)
+#define BIFHIP_NOALLOC_SCRATCH_BYTES \
+    (4LL * (64 + 2LL * BIFHIP_NOALLOC_MAX_BLOCKS + 2LL * BIFHIP_NOALLOC_MAX_ROW))
+int bifhip_dbscan_device_noalloc(const double* d_peaks, const double* d_intervals,
+                                 int sizeOfBlock, int amountOfBlocks, const int* d_counts,
+                                 int maxAmountOfPeaks, double eps, int* d_out);
+
 /* kdeCUDA (Library/cudaLibrary.cu:426-441). */
 int bifhip_kde_device(const double* d_peaks, int sizeOfBlock, int amountOfBlocks,
                       const int* d_counts, int maxAmountOfPeaks, int kdeSampling,

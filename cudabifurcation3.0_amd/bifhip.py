@@ -127,6 +127,9 @@ def _declare(L):
                                     C.POINTER(C.c_float)],
         "bifhip_mlp_infer_device": [_I, _D, _D, _D, _VP, _VP, _VP, _VP, _VP],
         "bifhip_mlp_get_params": [C.POINTER(C.c_float), _I, C.POINTER(C.c_float), _I],
+        "bifhip_mlp_adam": [_I, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                            C.POINTER(C.c_float), C.POINTER(C.c_float), _I, _I,
+                            C.POINTER(C.c_ubyte), _D, _D, _D, _D, C.POINTER(C.c_ushort)],
         "bifhip_mlp_gradient": [_I, C.POINTER(C.c_float), C.POINTER(C.c_float), _PI, _PI,
                                 C.POINTER(C.c_float), _I, _D, _D, C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), C.POINTER(C.c_float)],
@@ -610,6 +613,23 @@ def mlp_get_params():
     f = np.zeros(64, dtype=np.float32)
     _check(load().bifhip_mlp_get_params(_pf(p), len(p), _pf(f), len(f)))
     return p, f
+
+
+def mlp_adam(params, m, v, grads, keep, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """k_adam applied len(grads) times (steps t = 1..) to FP32 params/m/v;
+    grads[k] is (splits, n) partials.  Returns (params, m, v, bf16 bits)."""
+    p = np.ascontiguousarray(params, dtype=np.float32).copy()
+    m = np.ascontiguousarray(m, dtype=np.float32).copy()
+    v = np.ascontiguousarray(v, dtype=np.float32).copy()
+    g = np.ascontiguousarray(grads, dtype=np.float32)
+    steps, splits, n = g.shape
+    assert n == len(p) == len(m) == len(v)
+    k = np.ascontiguousarray(keep, dtype=np.uint8)
+    w = np.zeros(n, dtype=np.uint16)
+    _check(load().bifhip_mlp_adam(n, _pf(p), _pf(m), _pf(v), _pf(g), splits, steps,
+                                  k.ctypes.data_as(C.POINTER(C.c_ubyte)), lr, b1, b2, eps,
+                                  w.ctypes.data_as(C.POINTER(C.c_ushort))))
+    return p, m, v, w
 
 
 def mlp_gradient(n_lat, params, fourier, cells, cls, lle, w_lle=1.0, chaos_std=0.0):

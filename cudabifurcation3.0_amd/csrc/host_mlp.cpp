@@ -432,6 +432,53 @@ int bifhip_mlp_gradient(int nLat, const float* params, const float* fourier, con
     return BIFHIP_OK;
 }
 
+// k_adam on caller-given state: `steps` successive Adam updates of n FP32
+// parameters (host arrays params, m, v updated in place), the gradient of
+// step k given as `splits` partials grads[(k * splits + z) * n + i] that the
+// kernel adds in order z = 0..splits-1; keep[i] == 0 holds parameter i at
+// zero.  The BF16 copy of every parameter after the last step goes to wbf
+// (bit patterns) when it is not NULL.  For checks against a CPU Adam.
+int bifhip_mlp_adam(int n, float* params, float* m, float* v, const float* grads, int splits,
+                    int steps, const unsigned char* keep, double lr, double b1, double b2,
+                    double eps, unsigned short* wbf)
+{
+    RC_TRY(ensure_device());
+    if (n < 1 || splits < 1 || steps < 0 || !params || !m || !v || !keep || (steps && !grads))
+        return fail(BIFHIP_E_ARG, "bad Adam arguments");
+    DBuf dp, dm, dv, dg, dk, di, dw, ds;
+    HIP_TRY(dp.alloc(sizeof(float) * n));
+    HIP_TRY(dm.alloc(sizeof(float) * n));
+    HIP_TRY(dv.alloc(sizeof(float) * n));
+    HIP_TRY(dg.alloc(sizeof(float) * (size_t)n * splits));
+    HIP_TRY(dk.alloc((size_t)n));
+    HIP_TRY(di.alloc(sizeof(int) * n));
+    HIP_TRY(dw.alloc(sizeof(bf16) * n));
+    HIP_TRY(ds.alloc(sizeof(int)));
+    std::vector<int> ident(n);
+    std::iota(ident.begin(), ident.end(), 0);
+    HIP_TRY(hipMemcpy(dp.p, params, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dm.p, m, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dv.p, v, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dk.p, keep, (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(di.p, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    for (int k = 0; k < steps; ++k) {
+        const int t = k + 1;
+        HIP_TRY(hipMemcpy(dg.p, grads + (size_t)k * splits * n, sizeof(float) * (size_t)n * splits,
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ds.p, &t, sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(launch_adam(n, dp.as<float>(), dm.as<float>(), dv.as<float>(), dg.as<float>(),
+                            splits, n, dk.as<unsigned char>(), di.as<int>(), dw.as<bf16>(),
+                            nullptr, nullptr, (float)lr, (float)b1, (float)b2, (float)eps,
+                            ds.as<int>(), g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+    }
+    HIP_TRY(hipMemcpy(params, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(m, dm.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(v, dv.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (wbf && steps > 0) HIP_TRY(hipMemcpy(wbf, dw.p, sizeof(bf16) * n, hipMemcpyDeviceToHost));
+    return BIFHIP_OK;
+}
+
 int bifhip_mlp_param_count(void) { return kParams; }
 
 int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const double* ic,

@@ -531,6 +531,17 @@ int bifhip_mlp_infer_device(int nLat, double lleMean, double lleSd, double chaos
                             unsigned char* d_cls, unsigned char* d_chaos, float* d_conf,
                             float* d_confChaos, float* d_lle);
 int bifhip_mlp_get_params(float* params, int nParams, float* fourier, int nFourier);
+
+/* No counterpart in the reference: the optimiser kernel of the surrogate's
+ * training on caller-given state, for checks against a CPU Adam.  `steps`
+ * updates (step t = 1..steps) of n FP32 parameters, params/m/v host arrays
+ * updated in place; the gradient of step k is `splits` partials
+ * grads[(k*splits + z)*n + i], added in order z; keep[i] == 0 holds
+ * parameter i at zero; wbf (may be NULL) receives the BF16 bit patterns of
+ * the parameters after the last step. */
+int bifhip_mlp_adam(int n, float* params, float* m, float* v, const float* grads, int splits,
+                    int steps, const unsigned char* keep, double lr, double b1, double b2,
+                    double eps, unsigned short* wbf);
 /* One forward + backward pass (host arrays): cells are flattened indices on
  * an nLat x nLat lattice, cls/lle their targets; grads in parameter
  * layout W1[128x64] b1 W2[128x128] b2 W3[128x128] b3 Wh[32x128] bh[32];

@@ -105,3 +105,23 @@ def loss_only(flat, fourier, cells, n_lat, cls, lle, w_lle=1.0, chaos_std=0.0):
     losses, _, _ = forward_backward(flat, fourier, cells, n_lat, cls, lle, w_lle, bf=False,
                                     chaos_std=chaos_std)
     return losses[0] + w_lle * losses[1] + losses[2]
+
+
+def adam_step(p, m, v, grad, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, keep=None):
+    """One Adam step (Kingma & Ba, 2015, Algorithm 1) in float64 at step t
+    (1-based): m <- b1 m + (1-b1) g; v <- b2 v + (1-b2) g^2;
+    p <- p - lr * (m / (1-b1^t)) / (sqrt(v / (1-b2^t)) + eps).  Entries with
+    keep == 0 are held at zero with m and v unchanged (the padded head rows
+    of the network).  Returns new (p, m, v)."""
+    p = np.asarray(p, dtype=np.float64).copy()
+    m = np.asarray(m, dtype=np.float64).copy()
+    v = np.asarray(v, dtype=np.float64).copy()
+    g = np.asarray(grad, dtype=np.float64)
+    live = np.ones(p.shape, dtype=bool) if keep is None else np.asarray(keep, dtype=bool)
+    m[live] = b1 * m[live] + (1 - b1) * g[live]
+    v[live] = b2 * v[live] + (1 - b2) * g[live] ** 2
+    mh = m / (1 - b1 ** t)
+    vh = v / (1 - b2 ** t)
+    p[live] = p[live] - lr * mh[live] / (np.sqrt(vh[live]) + eps)
+    p[~live] = 0.0
+    return p, m, v

@@ -215,6 +215,29 @@ int orc_kde(const double *data, long start, int amountOfPoints,
     return result;
 }
 
+/* The KDE values kde() compares, Library/cudaLibrary.cu:368-386: sum w of
+ * the k1-1 sample points (w < k1-1, k1 = kdeSampling*P, step (I2-I1)/(k1-1),
+ * point w*step + I1) of sum_m expf(-(((p_m - point)/H)^2)/2), in the same
+ * order and precision as orc_kde.  Writes min(k1-1, cap) values to out and
+ * returns k1-1 (0 when the loop would not run). */
+int orc_kde_sums(const double *data, int amountOfPoints, int kdeSampling, double I1,
+                 double I2, double smoothH, double *out, int cap)
+{
+    double k1 = kdeSampling * amountOfPoints;
+    double k2 = (I2 - I1) / (k1 - 1);
+    int n = 0;
+    for (int w = 0; w < k1 - 1; ++w, ++n) {
+        double delt = w * k2 + I1, cur = 0;
+        for (int m = 0; m < amountOfPoints; ++m) {
+            double t = (data[m] - delt) / smoothH;
+            cur += expf((float)(-((t * t) / 2)));
+        }
+        if (n < cap)
+            out[n] = cur;
+    }
+    return n;
+}
+
 /* distance, Library/cudaLibrary.cu:445-453: float hypot of the double
  * differences, 0 for identical points. */
 double orc_distance(double x1, double y1, double x2, double y2)

@@ -22,10 +22,17 @@
  *   orc_distance             glibc hypotf == (float)sqrt(double) check
  *   orc_dbscan               reference KATs ({1,2}->1, {1..8}->7, ...); Rossler
  *                            cluster counts at c=2.5,3.5,4.0,5.7,9.0
- *   orc_kde                  single-cluster / separated-cluster closed cases
+ *   orc_kde                  the values it compares (orc_kde_sums) against the
+ *                            KDE definition; the count against a run-length
+ *                            statement of the rule on exact-plateau / exact-
+ *                            zero fuzz; hand cases for each rule; planted
+ *                            mistakes caught (tests/test_oracle_pins.py,
+ *                            tests/test_oracle_mutants.py)
  *   orc_gram_schmidt         orthonormality, |det| = prod(norms), numpy QR
  *   orc_lle_cell             linear model: LLE = log|1+h*lambda|/h
- *   orc_ls_cell              linear model spectrum; volume-contraction sum
+ *   orc_ls_cell              linear model spectrum; volume-contraction sum;
+ *                            reference directions by one renormalisation of
+ *                            the diagonal linear model against numpy QR
  *   orc_ls_var_cell          agreement with the finite-difference spectrum
  *   memristive orc_step      first-order convergence to its ODE (solve_ivp)
  *                            for explicit shares 1, 1/2, 0; float state
@@ -70,6 +77,9 @@ int  orc_peak_finder(double *data, long start, int amountOfPoints,
 int  orc_kde(const double *data, long start, int amountOfPoints,
              int maxAmountOfPeaks, int kdeSampling, double I1, double I2,
              double smoothH);
+
+int  orc_kde_sums(const double *data, int amountOfPoints, int kdeSampling, double I1,
+                  double I2, double smoothH, double *out, int cap);
 
 double orc_distance(double x1, double y1, double x2, double y2);
 

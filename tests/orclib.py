@@ -41,6 +41,29 @@ def lib():
     return _lib
 
 
+class use_library:
+    """Context manager: route every call of this module through another
+    build of the oracle (a copy with a planted mistake, for checking that
+    the pinning tests catch it)."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        lib()
+        self.saved = _lib
+        other = C.CDLL(self.path)
+        _declare(other)
+        _lib = other
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
 def _declare(L):
     L.orc_set_sweep_ic.argtypes = [_I]
     L.orc_set_sweep_ic.restype = None
@@ -58,6 +81,8 @@ def _declare(L):
     L.orc_peak_finder.restype = _I
     L.orc_kde.argtypes = [_PD, _L, _I, _I, _I, _D, _D, _D]
     L.orc_kde.restype = _I
+    L.orc_kde_sums.argtypes = [_PD, _I, _I, _D, _D, _D, _PD, _I]
+    L.orc_kde_sums.restype = _I
     L.orc_distance.argtypes = [_D, _D, _D, _D]
     L.orc_distance.restype = _D
     L.orc_dbscan.argtypes = [_PD, _PD, _PD, _L, _I, _I, _I, _D]
@@ -150,6 +175,16 @@ def kde(amps, max_peaks, sampling, i1, i2, smooth):
     a = f64(amps)
     a2 = np.zeros(max(len(a), 1)); a2[:len(a)] = a
     return lib().orc_kde(pd(a2), 0, len(a), max_peaks, sampling, i1, i2, smooth)
+
+
+def kde_sums(amps, sampling, i1, i2, smooth):
+    """The k1-1 KDE values orc_kde compares (k1 = sampling * P)."""
+    a = f64(amps)
+    a2 = np.zeros(max(len(a), 1)); a2[:len(a)] = a
+    cap = max(int(sampling) * len(a), 1)
+    out = np.zeros(cap)
+    n = lib().orc_kde_sums(pd(a2), len(a), sampling, i1, i2, smooth, pd(out), cap)
+    return out[:n].copy()
 
 
 def distance(x1, y1, x2, y2):

@@ -182,3 +182,41 @@ def test_tile_generator_is_seeded_and_in_bounds():
     assert h.max() > 3 * h.min()
     c = B.tiles_generate(64, 16, 16, 4096, 5, 1, 200.0, 2000.0, 0.5)
     assert not np.array_equal(c[0], oi[:64])
+
+
+def _header_prototypes():
+    """name -> list of parameter declarations, from include/bifhip.h."""
+    import re
+    txt = open(B.HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", " ", txt, flags=re.S)
+    protos = {}
+    for m in re.finditer(r"\b(bifhip_[A-Za-z0-9_]+)\s*\(([^()]*)\)\s*;", txt):
+        params = [p.strip() for p in m.group(2).split(",")]
+        protos[m.group(1)] = [] if params == ["void"] or params == [""] else params
+    return protos
+
+
+def test_binding_argtypes_match_header_prototypes():
+    """Every function the Python binding declares takes as many arguments as
+    its prototype, each of the same kind (pointer, floating, integer)."""
+    import ctypes as C
+    lib = B.load()
+    protos = _header_prototypes()
+    checked = 0
+    for name, params in protos.items():
+        fn = getattr(lib, name)
+        if fn.argtypes is None:
+            continue
+        assert len(fn.argtypes) == len(params), (name, len(fn.argtypes), params)
+        for t, p in zip(fn.argtypes, params):
+            if "*" in p:
+                ok = t in (C.c_void_p, C.c_char_p) or hasattr(t, "_type_") and \
+                    issubclass(t, C._Pointer)
+            elif p.split()[0] == "double":
+                ok = t is C.c_double
+            else:
+                ok = t in (C.c_int, C.c_long, C.c_longlong, C.c_size_t, C.c_ulonglong,
+                           C.c_uint, C.c_ulong)
+            assert ok, (name, p, t)
+        checked += 1
+    assert checked >= 40, checked

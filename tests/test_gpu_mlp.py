@@ -86,6 +86,44 @@ def _synthetic_labels(n):
     return cls, lle
 
 
+def test_adam_kernel_matches_reference_adam(bh):
+    """k_adam (through bifhip_mlp_adam) against the float64 Adam of
+    oracle/mlp_ref.py with the kernel's float32 constants: six steps, the
+    gradient as three partials, magnitudes from 1e-9 (where eps dominates)
+    to 10, a quarter of the entries held at zero.  Tolerance: float32
+    rounding of the update (relative 2e-6 of p plus 2e-9 absolute; the first
+    moment within 1e-6 of the gradients' scale, the second within a
+    relative 1e-5); the BF16
+    copy must be the round-to-nearest-even of the FP32 parameter, bit for
+    bit."""
+    rng = np.random.default_rng(12)
+    n, steps, splits = 5000, 6, 3
+    f = lambda x: float(np.float32(x))
+    lr, b1, b2, eps = f(1e-3), f(0.9), f(0.999), f(1e-8)
+    p0 = rng.normal(size=n).astype(np.float32)
+    mag = 10.0 ** rng.uniform(-9, 1, size=n)
+    grads = (rng.normal(size=(steps, splits, n)) * mag).astype(np.float32)
+    keep = (np.arange(n) % 4 != 1).astype(np.uint8)
+    say("adam", n, steps, splits)
+    p, m, v, wbf = bh.mlp_adam(p0, np.zeros(n), np.zeros(n), grads, keep, lr, b1, b2, eps)
+    pr, mr, vr = p0.astype(np.float64), np.zeros(n), np.zeros(n)
+    for k in range(steps):
+        gk = grads[k].astype(np.float32).sum(axis=0, dtype=np.float32)
+        pr, mr, vr = M.adam_step(pr, mr, vr, gk, k + 1, lr, b1, b2, eps, keep=keep)
+    np.testing.assert_allclose(p, pr, rtol=2e-6, atol=2e-9)
+    # m sums gradients of either sign: its float32 rounding is relative to
+    # the gradients' scale, not to m itself
+    gscale = np.abs(grads).sum(axis=1).max(axis=0)
+    assert np.all(np.abs(m - mr) <= 1e-5 * np.abs(mr) + 1e-6 * gscale)
+    np.testing.assert_allclose(v, vr, rtol=1e-5, atol=1e-36)
+    assert not p[keep == 0].any()
+    got_bf = (wbf.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    np.testing.assert_array_equal(got_bf, M.bf16(p))
+    # the update moved every live parameter by up to lr per step
+    live = keep == 1
+    assert np.all(np.abs(p[live] - p0[live]) <= steps * lr * 1.01)
+
+
 def test_training_graph_equals_eager(bh):
     n = 32
     cls, lle = _synthetic_labels(n)

@@ -312,6 +312,23 @@ def test_dbscan_map_config2_rows_bit_exact(orc, bh, max_peaks):
         assert len(np.unique(m)) > 10
 
 
+def test_dbscan_map_config2_cluster_clamp(orc, bh):
+    """Config 2's "capped at 32": bifhip_set_cluster_clamp(32) on the 512x512
+    map with every peak admitted gives min(oracle count, 32) on whole rows,
+    and the clamp binds somewhere (chaotic cells have 70-85 clusters)."""
+    say("dbscan map 512^2, cluster clamp 32")
+    bh.set_cluster_clamp(32)
+    m = bh.dbscan_map(500.0, 512, 0.01, IC, R2, [0, 2], 0, 0.0, 50000, 1000.0, VALS, 1, 0.01)
+    m = m.reshape(512, 512)
+    bound = 0
+    for row in (5, 260, 480):
+        want = orc.dbscan_cells(500.0, 512, 0.01, IC, R2, [0, 2], 0, 0.0, 50000, 1000.0,
+                                VALS, 1, 0.01, row * 512, 512)
+        np.testing.assert_array_equal(m[row], np.minimum(want, 32), err_msg=f"row {row}")
+        bound += int((want > 32).sum())
+    assert bound > 0 and m.max() == 32
+
+
 def test_dbscan_map_two_cells_per_lane(orc, bh):
     """A launch of at least 2^20 cells integrates two cells per lane; a
     1024 x 1024 map (short window) against the oracle on whole rows,
@@ -508,17 +525,56 @@ def test_driver_files_match_reference_format(orc, bh, tmp_path):
         np.testing.assert_allclose(body, [float(f"{v:g}") for v in ls[:, k]], rtol=1e-5,
                                    atol=1e-12)
 
+    # LLE1D (Library/hostLibrary.cu:1366-1373): "c, lle" per line at the
+    # stream's default 6 significant digits; the directions use the
+    # cell's global index (one chunk)
+    def same6(b_, w_):
+        return b_ == w_ or (b_ in ("nan", "-nan") and w_ == "nan") or \
+            abs(float(b_) - float(w_)) <= 1e-12 * max(1.0, abs(float(w_)))
+
+    def c_of(k):
+        return orc.value_by_idx(k, n, 2.0, 7.0, 0)
+
     bh.LLE1D(40.0, 0.5, n, 0.01, 1e-8, IC, 3, [2.0, 7.0], [2], 0, 0.0, 20.0, VALS, 3)
-    lines = out.read_text().splitlines()
-    assert len(lines) == n and lines[0].startswith("2, ")
+    lle1 = orc.lle_cells(1, 40.0, 0.5, n, 0.01, 1e-8, IC, [2.0, 7.0], [2], 0.0, 20.0, VALS,
+                         orc.PERT_REFERENCE, 0, 0, n)
+    lines = out.read_text().split("\n")
+    assert lines[-1] == "" and len(lines) == n + 1
+    for k in range(n):
+        c_txt, v_txt = lines[k].split(", ")
+        assert c_txt == f"{c_of(k):g}", (k, c_txt)
+        assert same6(v_txt, f"{lle1[k]:g}"), (k, v_txt, lle1[k])
 
+    # LS1D (Library/hostLibrary.cu:1898-1908): "c, l1, l2, l3" per line
     bh.LS1D(30.0, 0.5, n, 0.01, 1e-6, IC, 3, [2.0, 7.0], [2], 0, 0.0, 10.0, VALS, 3)
-    lines = out.read_text().splitlines()
-    assert len(lines) == n and len(lines[0].split(", ")) == 4
+    ls1 = orc.ls_cells(1, 30.0, 0.5, n, 0.01, 1e-6, IC, [2.0, 7.0], [2], 0.0, 10.0, VALS,
+                       orc.PERT_REFERENCE, 0, 0, n)
+    lines = out.read_text().split("\n")
+    assert lines[-1] == "" and len(lines) == n + 1
+    for k in range(n):
+        f = lines[k].split(", ")
+        assert len(f) == 4 and f[0] == f"{c_of(k):g}", (k, lines[k])
+        for j in range(3):
+            assert same6(f[1 + j], f"{ls1[k, j]:g}"), (k, j, f[1 + j], ls1[k, j])
 
-    bh.bifurcation2DKDE(80.0, n, 0.01, 3, IC, R2, [0, 2], 0, 0.0, 1000, 50.0, VALS, 3, 1, 10,
-                        -15.0, 15.0, 0.05)
-    assert out.read_text().split("\n")[:2] == ["0.05 0.45", "2 10"]
+    # KDE map (Library/hostLibrary.cu:474, 537-545): the header at 12
+    # significant digits (a range that needs them), the counts with the 2D
+    # separator logic; the counts are the driver's own (its buffer twin) and
+    # agree with the oracle (glibc expf is not correctly rounded)
+    rk = [0.0512345678901, 0.45, 2.0, 10.0]
+    args = (80.0, n, 0.01, 3, IC, rk, [0, 2], 0, 0.0, 1000, 50.0, VALS, 3, 1, 10, -15.0, 15.0,
+            0.05)
+    bh.bifurcation2DKDE(*args)
+    text = out.read_text()
+    assert text.split("\n")[:2] == ["0.0512345678901 0.45", "2 10"]
+    kde = bh.kde_map(80.0, n, 0.01, IC, rk, [0, 2], 0, 0.0, 1000, 50.0, VALS, 1, 10, -15.0,
+                     15.0, 0.05)
+    assert text == _ref_map_text(n, rk, kde, 12)
+    want = orc.kde_cells(80.0, n, 0.01, IC, rk, [0, 2], 0, 0.0, 1000, 50.0, VALS, 1, 10, -15.0,
+                         15.0, 0.05, 0, n * n)
+    agree = float((kde == want).mean())
+    say("KDE file counts vs oracle agreement", agree)
+    assert agree >= 0.99
 
 
 def test_denis_series_and_file(orc, bh, tmp_path):

@@ -64,3 +64,33 @@ def test_bf16_rounding():
     assert r[0] == 1.0 and r[1] == 1.0          # tie rounds to even
     assert r[2] == 1.0078125                     # above the tie rounds up
     assert abs(r[3] + 3.0e-3) < 2e-5 and r[5] == 0.0
+
+
+def test_adam_reference_closed_forms():
+    """With a constant gradient g the bias-corrected moments are exactly g
+    and g^2 at every step, so each step moves p by lr*g/(|g|+eps); with a
+    gradient that changes, m_t = (1-b1) sum_k b1^(t-k) g_k and v_t likewise.
+    Held entries stay zero."""
+    rng = np.random.default_rng(3)
+    n, lr, b1, b2, eps = 64, 1e-3, 0.9, 0.999, 1e-8
+    g = rng.normal(size=n) * np.logspace(-12, 2, n)
+    p0 = rng.normal(size=n)
+    p, m, v = p0.copy(), np.zeros(n), np.zeros(n)
+    for t in range(1, 8):
+        p, m, v = M.adam_step(p, m, v, g, t, lr, b1, b2, eps)
+        np.testing.assert_allclose(p, p0 - t * lr * g / (np.abs(g) + eps), rtol=1e-12,
+                                   atol=1e-15)
+    gs = rng.normal(size=(6, n))
+    p, m, v = p0.copy(), np.zeros(n), np.zeros(n)
+    for t in range(1, 7):
+        p_prev = p
+        p, m, v = M.adam_step(p, m, v, gs[t - 1], t, lr, b1, b2, eps)
+        mt = (1 - b1) * sum(b1 ** (t - k) * gs[k - 1] for k in range(1, t + 1))
+        vt = (1 - b2) * sum(b2 ** (t - k) * gs[k - 1] ** 2 for k in range(1, t + 1))
+        np.testing.assert_allclose(m, mt, rtol=1e-12)
+        np.testing.assert_allclose(v, vt, rtol=1e-12)
+        step = lr * (mt / (1 - b1 ** t)) / (np.sqrt(vt / (1 - b2 ** t)) + eps)
+        np.testing.assert_allclose(p, p_prev - step, rtol=1e-12)
+    keep = np.arange(n) % 5 != 0
+    p, m, v = M.adam_step(p0, np.zeros(n), np.zeros(n), g, 1, keep=keep)
+    assert not p[~keep].any() and not m[~keep].any() and not v[~keep].any()

@@ -21,6 +21,7 @@ MODEL_LINEAR = 1
 MODEL_MEMRISTIVE = 2
 PREC_FP64 = 0
 PREC_FP32 = 1
+READING_CLASS_AND_CHAOS, READING_CHAOS, READING_CLASS = 0, 1, 2
 PERT_REFERENCE = 0
 PERT_AXIS_X = 1
 
@@ -117,6 +118,7 @@ def _declare(L):
         "bifhip_set_partial_resolve": [_I],
         "bifhip_set_chaos_confidence": [_D],
         "bifhip_set_lle32_margin": [_D],
+        "bifhip_set_regime_reading": [_I],
         "bifhip_lle_peak_map_device": [_I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I, _I, _D,
                                        _D, _D, _D, _D, _VP, _VP],
         "bifhip_surrogate_map_device": [_I, _I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I,
@@ -284,6 +286,7 @@ def reset_context():
     set_partial_resolve(True)
     set_chaos_confidence(0.975)
     set_lle32_margin(0.02)
+    set_regime_reading(READING_CLASS_AND_CHAOS)
 
 
 def device_count():
@@ -695,6 +698,12 @@ def set_lle32_margin(margin):
     lies at least `margin` from the chaos threshold, FP64 elsewhere; a
     negative margin keeps them all in FP64."""
     _check(load().bifhip_set_lle32_margin(float(margin)))
+
+
+def set_regime_reading(reading):
+    """What a cell's regime is for the surrogate's re-solve decision:
+    READING_CLASS_AND_CHAOS (default), READING_CHAOS or READING_CLASS."""
+    _check(load().bifhip_set_regime_reading(int(reading)))
 
 
 # --------------------------------------------------------------- writers

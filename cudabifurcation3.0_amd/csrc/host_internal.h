@@ -30,6 +30,7 @@ struct Ctx {
     int partialResolve = 1;    // surrogate: class-only cells re-solve the peak count alone
     double chaosConfidence = 0.975;  // surrogate: chaos confidence below this re-solves the LLE
     double lle32Margin = 0.02;  // surrogate: FP32 regime re-solves this far from the threshold
+    int reading = BIFHIP_READING_CLASS_AND_CHAOS;  // surrogate: what a cell's regime is
     std::string lastError;
     hipStream_t stream = nullptr;
 };

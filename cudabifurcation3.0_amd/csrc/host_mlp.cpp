@@ -48,8 +48,8 @@ hipError_t launch_fill(long long n, int* v, int value, hipStream_t st);
 hipError_t launch_fill_f64(int n, double* p, double v, hipStream_t st);
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
-                       float chaosConfThr, bool partial, int* listFull, int* countFull,
-                       int* listPeak, int* countPeak, int* listLle, int* countLle,
+                       float chaosConfThr, bool partial, int reading, int* listFull,
+                       int* countFull, int* listPeak, int* countPeak, int* listLle, int* countLle,
                        unsigned long long vseed, unsigned int vthr, int* listVal, int* countVal,
                        unsigned char* flag, hipStream_t st);
 hipError_t launch_scatter(int n, const int* list, const int* peak, const double* lle,
@@ -578,8 +578,9 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     const unsigned int vthr =
         vf > 0 ? (unsigned int)std::min(4294967295.0, std::floor(vf * 4294967296.0)) : 0u;
     HIP_TRY(launch_flag(nFine, d_pcls, d_pch, d_conf, d_cconf, (float)confThreshold,
-                        (float)g.chaosConfidence, partial, d_all, d_count, d_plist, d_count + 1,
-                        d_llist, d_count + 3, seed + 17, vthr, d_vlist, d_count + 2, d_flag, s));
+                        (float)g.chaosConfidence, partial, g.reading, d_all, d_count, d_plist,
+                        d_count + 1, d_llist, d_count + 3, seed + 17, vthr, d_vlist, d_count + 2,
+                        d_flag, s));
     HIP_TRY(launch_fill_pred(nf, d_pcls, d_plle, d_peak, d_lle, s));
     int counts[4] = {0, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(counts, d_count, 4 * sizeof(int), hipMemcpyDeviceToHost, s));

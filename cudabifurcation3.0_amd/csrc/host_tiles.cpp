@@ -207,6 +207,15 @@ int bifhip_set_lle32_margin(double margin)
     return BIFHIP_OK;
 }
 
+int bifhip_set_regime_reading(int reading)
+{
+    if (reading != BIFHIP_READING_CLASS_AND_CHAOS && reading != BIFHIP_READING_CHAOS &&
+        reading != BIFHIP_READING_CLASS)
+        return fail(BIFHIP_E_ARG, "unknown regime reading %d", reading);
+    g.reading = reading;
+    return BIFHIP_OK;
+}
+
 static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, const int* originJ,
                       const int* nBlocks, int nPts, const double* ranges, const int* mut,
                       const double* ic, int nIC, const double* values, int nVal, double h,

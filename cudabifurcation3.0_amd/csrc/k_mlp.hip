@@ -571,7 +571,8 @@ __device__ __forceinline__ bool validation_pick(long long idx, unsigned long lon
 __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
                        const unsigned char* __restrict__ chaos, const float* __restrict__ conf,
                        const float* __restrict__ confChaos, float confThr, float chaosConfThr,
-                       bool partial, int* __restrict__ listFull, int* __restrict__ countFull,
+                       bool partial, int reading, int* __restrict__ listFull,
+                       int* __restrict__ countFull,
                        int* __restrict__ listPeak, int* __restrict__ countPeak,
                        int* __restrict__ listLle, int* __restrict__ countLle,
                        unsigned long long vseed, unsigned int vthr, int* __restrict__ listVal,
@@ -595,6 +596,9 @@ __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
             fc |= cls[o] != c0;
             fh |= chaos[o] != ch0;
         }
+        // the regime reading decides what a cell can be unsure of
+        if (reading == 1) fc = false;        // chaos bit only
+        else if (reading == 2) fh = false;   // peak-count class only
         if (!confChaos || !partial) {
             fh = fh || fc;
         } else if (fh && !fc) {
@@ -804,14 +808,14 @@ hipError_t launch_train_fused(int rows, int nLat, const int* list, const int* tl
 
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
-                       float chaosConfThr, bool partial, int* listFull, int* countFull,
+                       float chaosConfThr, bool partial, int reading, int* listFull, int* countFull,
                        int* listPeak, int* countPeak, int* listLle, int* countLle,
                        unsigned long long vseed, unsigned int vthr, int* listVal, int* countVal,
                        unsigned char* flag, hipStream_t st)
 {
     const long long cells = (long long)n * n;
     hipLaunchKernelGGL(k_flag, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, n, cls,
-                       chaos, conf, confChaos, confThr, chaosConfThr, partial, listFull,
+                       chaos, conf, confChaos, confThr, chaosConfThr, partial, reading, listFull,
                        countFull, listPeak, countPeak, listLle, countLle, vseed, vthr, listVal,
                        countVal, flag);
     return hipGetLastError();

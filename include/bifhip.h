@@ -441,6 +441,18 @@ int bifhip_set_chaos_confidence(double threshold);
  * are re-solved again in FP64.  Default 0.02; a negative margin keeps
  * every regime re-solve in FP64.  The memristive model always uses FP64. */
 int bifhip_set_lle32_margin(double margin);
+/* Surrogate-refined map: what a cell's regime is when the map decides
+ * which cells to re-solve.  No reference counterpart.
+ *   BIFHIP_READING_CLASS_AND_CHAOS (default): the peak-count class
+ *     clamp(count, 1, 16) and the chaos bit (LLE > chaosThreshold); cells
+ *     unsure of either are re-solved for it;
+ *   BIFHIP_READING_CHAOS: the chaos bit only; peak counts stay predicted
+ *     except where a re-solved LLE changes the predicted regime;
+ *   BIFHIP_READING_CLASS: the class only; LLE values stay predicted. */
+#define BIFHIP_READING_CLASS_AND_CHAOS 0
+#define BIFHIP_READING_CHAOS 1
+#define BIFHIP_READING_CLASS 2
+int bifhip_set_regime_reading(int reading);
 
 /* ------------------------------------------ surrogate-refined map */
 

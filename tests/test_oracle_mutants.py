@@ -71,7 +71,7 @@ def _build(src_text, out_dir, k):
     so = os.path.join(out_dir, f"liborc_m{k}.so")
     with open(c, "w") as f:
         f.write(src_text)
-    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-fopenmp",
+    subprocess.run(["gcc", "-O3", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-fopenmp",
                     "-std=gnu11", "-w", "-I", os.path.join(ROOT, "oracle"), "-shared", "-o", so,
                     c, "-lm"], check=True)
     return so

@@ -169,11 +169,29 @@ def roofline(bh, cells_total, kernel="lle_peaks"):
                     "most 50% of the FMA-counted peak"}
 
 
-def run_tiles(bh, np, args, out):
-    """Lyapunov spectrum of uneven refinement tiles: 4096 tiles of 16x16
-    cells on the 4096^2 lattice, seeded lengths U[200, 2000] time units,
-    FP32 variational state, FP64 Gram-Schmidt every 0.5; one persistent
-    graph-replayed launch per step, configuration from the autotuner."""
+def tiles_fp32_vs_fp64(bh, np, f32, f64):
+    """Per-cell error of the FP32 tile spectrum against the FP64 tile
+    kernel (FP64 state and tangent vectors, the reference's Gram-Schmidt)."""
+    a, b = f32.reshape(-1, 3), f64.reshape(-1, 3)
+    fa, fb = np.isfinite(a).all(axis=1), np.isfinite(b).all(axis=1)
+    fin = fa & fb
+    e = np.abs(a[fin] - b[fin])
+    ssum = np.abs(a[fin].sum(axis=1) - b[fin].sum(axis=1)) / \
+        np.maximum(np.abs(b[fin].sum(axis=1)), 1e-12)
+    return {"cells_compared": int(fin.sum()), "finite_in_one_only": float((fa != fb).mean()),
+            "regime_agreement": float(((a[fin, 0] > CHAOS) == (b[fin, 0] > CHAOS)).mean()),
+            "abs_err_median": np.median(e, axis=0).tolist(),
+            "abs_err_p90": np.quantile(e, 0.9, axis=0).tolist(),
+            "abs_err_p99": np.quantile(e, 0.99, axis=0).tolist(),
+            "spectrum_sum_rel_err_median": float(np.median(ssum)),
+            "spectrum_sum_rel_err_p99": float(np.quantile(ssum, 0.99))}
+
+
+def run_config4(bh, np, steps=1, warmup=0):
+    """Config 4: 4096 tiles of 16x16 cells on the 4096^2 lattice, seeded
+    lengths U[200, 2000] time units, FP32 variational state, FP64
+    Gram-Schmidt every 0.5; autotuned, one persistent graph-replayed launch
+    per map; then the same tiles with the FP64 tile kernel for the error."""
     nt, w, n_pts = 4096, 16, 4096
     tiles = bh.tiles_generate(nt, w, w, n_pts, 0, 1, 200.0, 2000.0, LLE["NT"])
     nsteps = int(np.asarray(tiles[2], dtype=np.int64).sum()) * int(LLE["NT"] / H) * w * w
@@ -181,33 +199,62 @@ def run_tiles(bh, np, args, out):
     bh.set_tile_config(0)
     args_ = (d.ptr, tiles, w, w, n_pts, RANGES, [0, 2], IC, VALS, H, LLE["NT"], 0.0)
     bh.ls_tiles_device(*args_, autotune=True)
-    for _ in range(args.warmup):
+    tune = bh.tile_autotune_log().tolist()
+    for _ in range(warmup):
         bh.ls_tiles_device(*args_)
     bh.synchronize()
     bh.set_timing(True)
     bh.reset_kernel_times()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         bh.ls_tiles_device(*args_)
     bh.synchronize()
-    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    ms = (time.perf_counter() - t0) * 1000 / steps
     kms, nl = bh.kernel_time("ls_tiles")
-    flop = TILE_STEP_FLOP * nsteps * args.steps
-    ach = flop / (kms / 1000) / 1e12
+    f32 = d.to_host()
     block, cpl, _ = bh.get_tile_config()
+    lanes = bh.get_tile_lanes()
+    bh.set_tile_precision(bh.PREC_FP64)
+    try:
+        bh.set_tile_config(128, 1, 0)
+        bh.reset_kernel_times()
+        t1 = time.perf_counter()
+        bh.ls_tiles_device(*args_)
+        bh.synchronize()
+        ms64 = (time.perf_counter() - t1) * 1000
+        f64 = d.to_host()
+    finally:
+        bh.set_tile_precision(bh.PREC_FP32)
+        bh.set_tile_config(block, cpl, 0)
+        bh.set_tile_lanes(lanes)
+    flop = TILE_STEP_FLOP * nsteps * steps
+    ach = flop / (kms / 1000) / 1e12
     cells = nt * w * w
+    return {"ms_per_map": ms, "value": cells / (ms / 1000), "integration_steps_per_map": nsteps,
+            "autotuner": {"block": block, "cells_per_lane": cpl, "lanes_per_cell": lanes,
+                          "candidates_block_lanes_cpl_ms": tune},
+            "fp64_tiles_ms": ms64, "fp32_vs_fp64": tiles_fp32_vs_fp64(bh, np, f32, f64),
+            "roofline": {"bound": "valu_fp32", "kernel": "k_ls_tiles", "achieved": ach,
+                         "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                         "frac": ach / (FP32_VALU_PEAK / 1e12), "traffic": None,
+                         "avg_launch_ms": kms / max(nl, 1), "launches": nl,
+                         "note": "48 FLOP per step (state + 3 tangent vectors, FMA = 2); built "
+                                 "without packed FP32 instructions"}}
+
+
+def run_tiles(bh, np, args, out):
+    """Lyapunov spectrum of uneven refinement tiles (config 4), with the
+    FP32 spectrum's error against the FP64 tile kernel."""
+    r = run_config4(bh, np, args.steps, args.warmup)
     out.update(metric="grid points/s, uneven-tile Lyapunov spectrum (FP32 state, FP64 GS)",
-               value=cells / (ms / 1000), unit="grid points/s", ms_per_step=ms, dtype="f32",
-               config={"workload": f"{nt} tiles of {w}x{w} cells on the Rossler {n_pts}x{n_pts} "
-                                   f"(a, c) lattice, lengths U[200, 2000] t.u., NT {LLE['NT']}",
-                       "integration_steps_per_map": nsteps, "tile_block": block,
-                       "cells_per_lane": cpl})
-    out["roofline"] = {"bound": "valu_fp32", "kernel": "k_ls_tiles", "achieved": ach,
-                       "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s",
-                       "frac": ach / (FP32_VALU_PEAK / 1e12), "traffic": None,
-                       "avg_launch_ms": kms / max(nl, 1), "launches": nl,
-                       "note": "48 FLOP per step (state + 3 tangent vectors, FMA = 2); built "
-                               "without packed FP32 instructions"}
+               value=r["value"], unit="grid points/s", ms_per_step=r["ms_per_map"], dtype="f32",
+               config={"workload": "4096 tiles of 16x16 cells on the Rossler 4096x4096 (a, c) "
+                                   f"lattice, lengths U[200, 2000] t.u., NT {LLE['NT']}",
+                       "integration_steps_per_map": r["integration_steps_per_map"]})
+    out["autotuner"] = r["autotuner"]
+    out["fp64_tiles_ms"] = r["fp64_tiles_ms"]
+    out["fp32_vs_fp64"] = r["fp32_vs_fp64"]
+    out["roofline"] = r["roofline"]
 
 
 LLE32_STEP_FLOP = 11         # FP32 fused-multiply-add form of the Rossler step (FMA = 2)
@@ -278,6 +325,41 @@ def run_lle(bh, np, args, out):
                        "note": f"{LLE32_STEP_FLOP} FLOP per state step, {LLE32_DEV_FLOP} per "
                                f"deviation step (FMA = 2); built without packed FP32 "
                                f"instructions"}
+
+
+def run_config3(bh, np):
+    """Config 3 in brief: the 1024^2 LLE map in FP64 and with FP32 bulk
+    integration (transient 500, measure 1000, NT 0.5, perturbation 1e-8
+    along x), one map each, and the FP32 map's agreement with the FP64 one."""
+    n = 1024
+    d64 = bh.DeviceArray((n * n,), np.float64)
+    d32 = bh.DeviceArray((n * n,), np.float64)
+    la = (2, LLE["tMax"], LLE["NT"], n, H, LLE["eps"], IC, RANGES, [0, 2], 0.0,
+          LLE["transient"], VALS)
+    res = {}
+    try:
+        for name, d, prec in (("fp64", d64, bh.PREC_FP64), ("fp32_bulk", d32, bh.PREC_FP32)):
+            bh.set_precision(prec)
+            bh.lle_map_device(d.ptr, *la)     # warm-up
+            bh.synchronize()
+            t0 = time.perf_counter()
+            bh.lle_map_device(d.ptr, *la)
+            bh.synchronize()
+            res[name + "_ms"] = (time.perf_counter() - t0) * 1000
+    finally:
+        bh.set_precision(bh.PREC_FP64)
+    a, b = d64.to_host(), d32.to_host()
+    ok = np.isfinite(a) & np.isfinite(b)
+    err = np.abs(a[ok] - b[ok])
+    with np.errstate(invalid="ignore"):
+        res.update(regime_agreement=float(((a > CHAOS) == (b > CHAOS)).mean()),
+                   sign_agreement=float((np.sign(a[ok]) == np.sign(b[ok])).mean()),
+                   nan_in_one_only=float((np.isnan(a) != np.isnan(b)).mean()),
+                   lle_abs_err_mean=float(err.mean()),
+                   lle_abs_err_p99=float(np.quantile(err, 0.99)),
+                   lle_abs_err_max=float(err.max()))
+    res["speedup_fp32_vs_fp64"] = res["fp64_ms"] / res["fp32_bulk_ms"]
+    return res
 
 
 BIF_FLOP_PER_CELL = STEP_FLOP * int((PEAK["transient"] + PEAK["tMax"]) / H)   # 1.8 MFLOP
@@ -800,6 +882,8 @@ def main():
                     help="cells of the CPU baseline sample (whole lattice rows, ~6 s per run on "
                          "16 cores, median of 3 runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="surrogate workload: skip the brief config-3 / config-4 summaries")
     ap.add_argument("--fp32", action="store_true",
                     help="dbscan workload: also time the FP32-trajectory map and report its "
                          "agreement with the FP64 map")
@@ -859,6 +943,15 @@ def main():
                                       for k in ("lle_peaks", "dbscan")}
     else:
         run_surrogate(bh, np, args, out, n, d_peak, d_lle)
+        if not args.no_configs:
+            bh.set_timing(False)
+            c4 = run_config4(bh, np)
+            out["configs"] = {
+                "lle_fp32_bulk_1024": run_config3(bh, np),
+                "ls_uneven_tiles_4096x16x16": {k: c4[k] for k in (
+                    "ms_per_map", "autotuner", "fp64_tiles_ms", "fp32_vs_fp64")},
+                "note": "configs 3 and 4 in brief, one map each (their own workloads: "
+                        "--workload lle / tiles)"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, max(1, args.cpu_cells // n))
     print(json.dumps(out), flush=True)

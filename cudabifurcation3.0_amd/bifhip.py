@@ -114,6 +114,10 @@ def _declare(L):
                                      _I, _D, _D, _D, _VP],
         "bifhip_set_tile_config": [_I, _I, _I],
         "bifhip_get_tile_config": [_PI, _PI, _PI],
+        "bifhip_set_tile_lanes": [_I],
+        "bifhip_get_tile_lanes": [_PI],
+        "bifhip_tile_autotune_log": [_PD, _I],
+        "bifhip_set_tile_precision": [_I],
         "bifhip_set_graph": [_I],
         "bifhip_set_partial_resolve": [_I],
         "bifhip_set_chaos_confidence": [_D],
@@ -287,6 +291,8 @@ def reset_context():
     set_chaos_confidence(0.975)
     set_lle32_margin(0.02)
     set_regime_reading(READING_CLASS_AND_CHAOS)
+    set_tile_precision(PREC_FP32)
+    set_tile_lanes(1)
 
 
 def device_count():
@@ -679,6 +685,31 @@ def get_tile_config():
     a, b, c = C.c_int(), C.c_int(), C.c_int()
     _check(load().bifhip_get_tile_config(C.byref(a), C.byref(b), C.byref(c)))
     return a.value, b.value, c.value
+
+
+def set_tile_lanes(lanes):
+    """Lanes per cell of the FP32 tile kernel: 1 or 3."""
+    _check(load().bifhip_set_tile_lanes(int(lanes)))
+
+
+def get_tile_lanes():
+    a = C.c_int()
+    _check(load().bifhip_get_tile_lanes(C.byref(a)))
+    return a.value
+
+
+def tile_autotune_log():
+    """Candidates of the last autotuner run: rows (block, lanes, cells per
+    lane, ms on the sample)."""
+    n = load().bifhip_tile_autotune_log(None, 0)
+    out = np.zeros(max(n, 1) * 4)
+    load().bifhip_tile_autotune_log(_pd(out), n)
+    return out[:4 * n].reshape(n, 4)
+
+
+def set_tile_precision(p):
+    """PREC_FP32 (default) or PREC_FP64 state for the tile path."""
+    _check(load().bifhip_set_tile_precision(int(p)))
 
 
 def set_graph(on):

@@ -78,6 +78,11 @@ struct LyapArgs {
 };
 
 // Arguments of the persistent tile kernel (k_tiles.hip).
+// tile kernel variants
+constexpr int kTileFp32 = 0;     // FP32 state, one lane per cell
+constexpr int kTileFp32x3 = 1;   // FP32 state, three lanes per cell (one tangent vector each)
+constexpr int kTileFp64 = 2;     // FP64 state and tangent vectors, one lane per cell
+
 struct TileArgs {
     Grid g;                 // lattice the tile cells live on (nPts x nPts)
     int nTiles;
@@ -201,6 +206,17 @@ struct Rossler {
         v1 = fmaf(v1, p.ha1p, p.h * v0);
         v2 = fmaf(v2, fmaf(p.h, x0n - p.c, 1.0f), (p.h * x2o) * v0);
     }
+    // Tangent map of `step` in FP64, one rounding per operation in this
+    // order (no contraction): given the post-step x0 (x0n) and the pre-step
+    // x2 (x2o),
+    //   v0' = v0 - h(v1+v2); v1' = (1+ha)v1 + h v0'; v2' = (1+h(x0n-c))v2 + h x2o v0'
+    __device__ __forceinline__ static void tan64(double& v0, double& v1, double& v2, double x0n,
+                                                 double x2o, const P64& p, double h)
+    {
+        v0 = v0 - h * (v1 + v2);
+        v1 = (1 + h * p.a) * v1 + h * v0;
+        v2 = (1 + h * (x0n - p.c)) * v2 + h * x2o * v0;
+    }
 };
 
 // Linear diagonal test model x_i <- x_i + h*(a_i*x_i); its Lyapunov
@@ -248,6 +264,11 @@ struct Linear {
                                                  const P32& p)
     {
         v0 *= p.a; v1 *= p.b; v2 *= p.c;
+    }
+    __device__ __forceinline__ static void tan64(double& v0, double& v1, double& v2, double,
+                                                 double, const P64& p, double h)
+    {
+        step(v0, v1, v2, p, h);
     }
 };
 
@@ -374,6 +395,39 @@ struct LogNormSum {
         return 0.5 * (log(m) + (double)e * 0.69314718055994530942);
     }
 };
+
+// Classical Gram-Schmidt of three row vectors a -> b with norms
+// (projectionOperator/gramSchmidtProcess, Library/cudaLibrary.cu:913-957).
+__device__ __forceinline__ void gs3(const double (&a)[9], double (&b)[9], double (&den)[3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b[i * 3 + j] = a[i * 3 + j];
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            double num = 0, dd = 0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                num += a[i * 3 + m] * b[j * 3 + m];
+                dd += b[j * 3 + m] * b[j * 3 + m];
+            }
+            const double f = dd == 0 ? 0 : num / dd;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) b[i * 3 + m] -= f * b[j * 3 + m];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double d = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) d += b[i * 3 + j] * b[i * 3 + j];
+        d = sqrt(d);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b[i * 3 + j] = d == 0 ? 0 : b[i * 3 + j] / d;
+        den[i] = d;
+    }
+}
 
 // ------------------------------------------------------- wave utilities
 

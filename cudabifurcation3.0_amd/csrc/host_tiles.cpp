@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -14,9 +15,9 @@
 #include "host_internal.h"
 
 namespace bifhip {
-hipError_t launch_ls_tiles(int model, int cpl, int block, int grid, const TileArgs& A,
-                           hipStream_t st);
-const void* tile_kernel_ptr(int model, int cpl);
+hipError_t launch_ls_tiles(int model, int variant, int cpl, int block, int grid,
+                           const TileArgs& A, hipStream_t st);
+const void* tile_kernel_ptr(int model, int variant, int cpl);
 }  // namespace bifhip
 
 using namespace bifhip;
@@ -26,8 +27,9 @@ namespace {
 
 struct TileConfig {
     int block = 0;        // 0: not chosen yet (autotune on first use)
-    int cpl = 1;          // cells per lane
+    int cpl = 1;          // cells per lane (one lane per cell)
     int blocksPerCU = 0;  // 0: occupancy maximum
+    int lanes = 1;        // lanes per cell: 1, or 3 (one tangent vector per lane)
 };
 TileConfig g_cfg;
 hipStream_t g_tstream = nullptr;
@@ -46,6 +48,8 @@ struct GraphCache {
     }
 };
 GraphCache g_graph;
+// the last autotuner run: (block, lanes per cell, cells per lane, ms) per candidate
+std::vector<std::array<double, 4>> tuneLog;
 
 double u01(std::mt19937_64& r) { return (double)(r() >> 11) * (1.0 / 9007199254740992.0); }
 
@@ -72,26 +76,33 @@ int occupancy(K kernel, int block, int* blocks)
     return BIFHIP_OK;
 }
 
+// the kernel variant a configuration runs at the context's tile precision
+int tile_variant(int lanes)
+{
+    if (g.tilePrecision == BIFHIP_PREC_FP64) return kTileFp64;
+    return lanes == 3 ? kTileFp32x3 : kTileFp32;
+}
+
 // persistent grid = resident capacity of the device for this configuration
-int grid_for(int block, int cpl, int bpcu, int* grid)
+int grid_for(int variant, int block, int cpl, int bpcu, int* grid)
 {
     int cus = 0;
     RC_TRY(num_cus(&cus));
     int per = bpcu;
-    if (per <= 0) RC_TRY(occupancy(tile_kernel_ptr(g.model, cpl), block, &per));
+    if (per <= 0) RC_TRY(occupancy(tile_kernel_ptr(g.model, variant, cpl), block, &per));
     *grid = std::max(1, cus * per);
     return BIFHIP_OK;
 }
 
 struct TileJob {
     TileArgs A;
-    int cpl, block, grid;
+    int variant, cpl, block, grid;
 };
 
 int run_job(const TileJob& J, hipStream_t s, bool graph)
 {
     if (!graph) {
-        HIP_TRY(launch_ls_tiles(g.model, J.cpl, J.block, J.grid, J.A, s));
+        HIP_TRY(launch_ls_tiles(g.model, J.variant, J.cpl, J.block, J.grid, J.A, s));
         return BIFHIP_OK;
     }
     std::vector<unsigned char> key(sizeof(TileJob) + sizeof(int));
@@ -100,7 +111,7 @@ int run_job(const TileJob& J, hipStream_t s, bool graph)
     if (!g_graph.exec || g_graph.key != key) {
         g_graph.reset();
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipError_t le = launch_ls_tiles(g.model, J.cpl, J.block, J.grid, J.A, s);
+        hipError_t le = launch_ls_tiles(g.model, J.variant, J.cpl, J.block, J.grid, J.A, s);
         hipGraph_t gr = nullptr;
         hipError_t ce = hipStreamEndCapture(s, &gr);
         HIP_TRY(le);
@@ -167,6 +178,41 @@ int bifhip_set_tile_config(int blockSize, int cellsPerLane, int blocksPerCU)
     g_cfg.block = blockSize;
     g_cfg.cpl = cellsPerLane;
     g_cfg.blocksPerCU = blocksPerCU;
+    if (cellsPerLane != 1) g_cfg.lanes = 1;
+    g_graph.reset();
+    return BIFHIP_OK;
+}
+
+int bifhip_set_tile_lanes(int lanesPerCell)
+{
+    if (lanesPerCell != 1 && lanesPerCell != 3)
+        return fail(BIFHIP_E_ARG, "lanes per cell must be 1 or 3");
+    g_cfg.lanes = lanesPerCell;
+    if (lanesPerCell == 3) g_cfg.cpl = 1;
+    g_graph.reset();
+    return BIFHIP_OK;
+}
+
+int bifhip_get_tile_lanes(int* lanesPerCell)
+{
+    if (lanesPerCell) *lanesPerCell = g_cfg.lanes;
+    return BIFHIP_OK;
+}
+
+int bifhip_tile_autotune_log(double* out, int maxEntries)
+{
+    const int n = (int)tuneLog.size();
+    if (out)
+        for (int i = 0; i < std::min(n, maxEntries); ++i)
+            for (int k = 0; k < 4; ++k) out[4 * i + k] = tuneLog[i][k];
+    return n;
+}
+
+int bifhip_set_tile_precision(int precision)
+{
+    if (precision != BIFHIP_PREC_FP32 && precision != BIFHIP_PREC_FP64)
+        return fail(BIFHIP_E_ARG, "unknown tile precision %d", precision);
+    g.tilePrecision = precision;
     g_graph.reset();
     return BIFHIP_OK;
 }
@@ -269,19 +315,22 @@ static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, cons
     A.out = d_out;
     A.counter = d_counter;
 
-    auto job_for = [&](int block, int cpl, int bpcu, TileJob& J) -> int {
+    auto job_for = [&](int block, int cpl, int lanes, int bpcu, TileJob& J) -> int {
         J.A = A;
-        J.cpl = cpl;
+        J.variant = tile_variant(lanes);
+        J.cpl = J.variant == kTileFp32 ? cpl : 1;
         J.block = block;
-        J.A.itemCells = block * cpl;
+        // three lanes per cell: 21 cells per wavefront
+        J.A.itemCells = J.variant == kTileFp32x3 ? (block / 64) * 21 : block * J.cpl;
         J.A.itemsPerTile = (tileW * tileH + J.A.itemCells - 1) / J.A.itemCells;
-        return grid_for(block, cpl, bpcu, &J.grid);
+        return grid_for(J.variant, block, J.cpl, bpcu, &J.grid);
     };
 
     if (tune || g_cfg.block == 0) {
-        // Autotuner: time each (block, cells-per-lane) on a strided sample of
-        // the queue (every 8th tile, lengths cut to a tenth) and keep the
-        // fastest.  The sample keeps the length mix of the real job.
+        // Autotuner: time each (block, lanes per cell, cells per lane) on a
+        // strided sample of the queue (every 8th tile, lengths cut to a
+        // tenth) and keep the fastest.  The sample keeps the length mix of
+        // the real job.  The FP64 variant has one lane and one cell per lane.
         const int ns = std::max(1, nTiles / 8);
         std::vector<int> so(ns), sb(ns);
         for (int k = 0; k < ns; ++k) {
@@ -299,15 +348,20 @@ static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, cons
         std::memcpy(st.data() + 2 * nTiles, originJ, sizeof(int) * nTiles);
         HIP_TRY(hipMemcpy(d_s, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
         const int blocks[3] = {64, 128, 256};
+        // (lanes per cell, cells per lane)
+        const int shapes[3][2] = {{1, 1}, {1, 2}, {3, 1}};
+        const int nShapes = g.tilePrecision == BIFHIP_PREC_FP64 ? 1 : 3;
         float best = 1e30f;
-        TileConfig pick{256, 1, 0};
+        TileConfig pick{256, 1, 0, 1};
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
+        tuneLog.clear();
         for (int bi = 0; bi < 3; ++bi) {
-            for (int cpl = 1; cpl <= 2; ++cpl) {
+            for (int si = 0; si < nShapes; ++si) {
+                const int lanes = shapes[si][0], cpl = shapes[si][1];
                 TileJob J;
-                RC_TRY(job_for(blocks[bi], cpl, 0, J));
+                RC_TRY(job_for(blocks[bi], cpl, lanes, 0, J));
                 J.A.nTiles = ns;
                 J.A.order = d_s;
                 J.A.originI = d_s + nTiles;
@@ -321,11 +375,13 @@ static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, cons
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0;
                 HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                if (g.verbose) printf("tile autotune: block %d cpl %d grid %d: %.3f ms\n",
-                                      blocks[bi], cpl, J.grid, ms);
+                if (g.verbose)
+                    printf("tile autotune: block %d lanes %d cpl %d grid %d: %.3f ms\n",
+                           blocks[bi], lanes, cpl, J.grid, ms);
+                tuneLog.push_back({(double)blocks[bi], (double)lanes, (double)cpl, ms});
                 if (ms < best) {
                     best = ms;
-                    pick = {blocks[bi], cpl, 0};
+                    pick = {blocks[bi], cpl, 0, lanes};
                 }
             }
         }
@@ -337,7 +393,7 @@ static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, cons
 
     TileJob J;
     std::memset(&J, 0, sizeof(J));
-    RC_TRY(job_for(g_cfg.block, g_cfg.cpl, g_cfg.blocksPerCU, J));
+    RC_TRY(job_for(g_cfg.block, g_cfg.cpl, g_cfg.lanes, g_cfg.blocksPerCU, J));
     {
         Timed t("ls_tiles", s);
         RC_TRY(run_job(J, s, g.useGraph != 0));

@@ -141,39 +141,6 @@ __global__ __launch_bounds__(256) void k_lle32(const LyapArgs A, double* __restr
     out[t] = result.value() / A.tMax;
 }
 
-// Classical Gram-Schmidt of three row vectors a -> b with norms
-// (projectionOperator/gramSchmidtProcess, Library/cudaLibrary.cu:913-957).
-__device__ __forceinline__ void gs3(const double (&a)[9], double (&b)[9], double (&den)[3])
-{
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) b[i * 3 + j] = a[i * 3 + j];
-#pragma unroll
-        for (int j = 0; j < i; ++j) {
-            double num = 0, dd = 0;
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                num += a[i * 3 + m] * b[j * 3 + m];
-                dd += b[j * 3 + m] * b[j * 3 + m];
-            }
-            const double f = dd == 0 ? 0 : num / dd;
-#pragma unroll
-            for (int m = 0; m < 3; ++m) b[i * 3 + m] -= f * b[j * 3 + m];
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        double d = 0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) d += b[i * 3 + j] * b[i * 3 + j];
-        d = sqrt(d);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) b[i * 3 + j] = d == 0 ? 0 : b[i * 3 + j] / d;
-        den[i] = d;
-    }
-}
-
 template <class M, bool CHECK>
 __global__ __launch_bounds__(256) void k_ls64(const LyapArgs A, double* __restrict__ out)
 {

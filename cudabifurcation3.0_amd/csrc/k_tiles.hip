@@ -119,28 +119,166 @@ __global__ void k_ls_tiles(const TileArgs A)
     }
 }
 
+// Three lanes per cell: lane 3k+j of a wavefront carries tangent vector j
+// of the wave's cell k (21 cells per wavefront, lane 63 idle), and each of
+// the three lanes steps the cell's state itself (the same operations, so
+// the same values).  At each renormalisation the three lanes exchange their
+// vectors by cross-lane reads and all three run the one-lane kernel's
+// Gram-Schmidt on the same nine values, so every result is bit-identical to
+// k_ls_tiles; the lane holding vector j writes exponent j.  A work item is
+// 21 cells per wavefront of the block.
+template <class M>
+__global__ void k_ls_tiles3(const TileArgs A)
+{
+    __shared__ int s_item;
+    const int totalItems = A.nTiles * A.itemsPerTile;
+    const int tileCells = A.tileW * A.tileH;
+    const int lane = (int)threadIdx.x & 63;
+    const int j = lane % 3;
+    const int base = lane - j;
+    const int slot = ((int)threadIdx.x >> 6) * 21 + lane / 3;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(A.counter, 1);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();   // every lane has read s_item before lane 0 rewrites it
+        if (item >= totalItems) return;
+        const int tile = A.order[item / A.itemsPerTile];
+        const int part = item % A.itemsPerTile;
+        const int nb = A.nBlocks[tile];
+        const int cell = part * A.itemCells + slot;
+        const bool live = lane < 63 && cell < tileCells;
+        const int ci = live ? cell % A.tileW : 0;
+        const int cj = live ? cell / A.tileW : 0;
+        const int idx = (A.originJ[tile] + cj) * A.g.nPts + (A.originI[tile] + ci);
+        double vals[kMaxVal], s0, s1, s2;
+        cell_setup(A.g, idx, vals, s0, s1, s2);
+        const typename M::P32 q = M::params32(vals, A.h);
+        float x0 = (float)s0, x1 = (float)s1, x2 = (float)s2;
+        float w0 = j == 0 ? 1.0f : 0.0f, w1 = j == 1 ? 1.0f : 0.0f, w2 = j == 2 ? 1.0f : 0.0f;
+        LogNormSum acc[3];
+        acc[0].init();
+        acc[1].init();
+        acc[2].init();
+        for (int s = 0; s < A.skip; ++s) M::step32(x0, x1, x2, q);
+        for (int b = 0; b < nb; ++b) {
+#pragma unroll 2
+            for (int s = 0; s < A.nNT; ++s) {
+                const float x2o = x2;
+                M::step32(x0, x1, x2, q);
+                M::tan32(w0, w1, w2, x0, x2o, q);
+            }
+            float v[9];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                v[3 * k] = __shfl(w0, base + k, 64);
+                v[3 * k + 1] = __shfl(w1, base + k, 64);
+                v[3 * k + 2] = __shfl(w2, base + k, 64);
+            }
+            gs3_f(v, acc);
+            w0 = j == 0 ? v[0] : (j == 1 ? v[3] : v[6]);
+            w1 = j == 0 ? v[1] : (j == 1 ? v[4] : v[7]);
+            w2 = j == 0 ? v[2] : (j == 1 ? v[5] : v[8]);
+        }
+        if (live) {
+            const double T = (double)nb * A.nNT * A.h;
+            const double r = j == 0 ? acc[0].value() : (j == 1 ? acc[1].value() : acc[2].value());
+            A.out[((long long)tile * tileCells + cell) * 3 + j] = r / T;
+        }
+    }
+}
+
+// FP64 state and tangent vectors, one lane per cell: the tangent map of the
+// FP64 step (M::tan64), the reference's Gram-Schmidt (gs3, with divisions
+// and square roots, Library/cudaLibrary.cu:913-957) every nNT steps and a
+// sum of log norms -- the operations of the FP64 variational oracle in its
+// order.  The reference against which the FP32 tiles' error is measured.
+template <class M>
+__global__ void k_ls_tiles64(const TileArgs A)
+{
+    __shared__ int s_item;
+    const int totalItems = A.nTiles * A.itemsPerTile;
+    const int tileCells = A.tileW * A.tileH;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(A.counter, 1);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();   // every lane has read s_item before lane 0 rewrites it
+        if (item >= totalItems) return;
+        const int tile = A.order[item / A.itemsPerTile];
+        const int part = item % A.itemsPerTile;
+        const int nb = A.nBlocks[tile];
+        const int cell = part * A.itemCells + (int)threadIdx.x;
+        const bool live = cell < tileCells;
+        const int ci = live ? cell % A.tileW : 0;
+        const int cj = live ? cell / A.tileW : 0;
+        const int idx = (A.originJ[tile] + cj) * A.g.nPts + (A.originI[tile] + ci);
+        double vals[kMaxVal], x0, x1, x2;
+        cell_setup(A.g, idx, vals, x0, x1, x2);
+        const typename M::P64 p = M::params(vals);
+        const double h = A.h;
+        double v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        double res[3] = {0, 0, 0};
+        for (int s = 0; s < A.skip; ++s) M::step(x0, x1, x2, p, h);
+        for (int b = 0; b < nb; ++b) {
+            for (int s = 0; s < A.nNT; ++s) {
+                const double x2o = x2;
+                M::step(x0, x1, x2, p, h);
+                M::tan64(v[0], v[1], v[2], x0, x2o, p, h);
+                M::tan64(v[3], v[4], v[5], x0, x2o, p, h);
+                M::tan64(v[6], v[7], v[8], x0, x2o, p, h);
+            }
+            double w[9], den[3];
+            gs3(v, w, den);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) res[k] += log(den[k]);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) v[k] = w[k];
+        }
+        if (live) {
+            const double T = (double)nb * A.nNT * h;
+            double* o = A.out + ((long long)tile * tileCells + cell) * 3;
+            o[0] = res[0] / T;
+            o[1] = res[1] / T;
+            o[2] = res[2] / T;
+        }
+    }
+}
+
 __global__ void k_zero_counter(int* c) { *c = 0; }
 
-hipError_t launch_ls_tiles(int model, int cpl, int block, int grid, const TileArgs& A,
-                           hipStream_t st)
+// variant: kTileFp32 (one lane per cell, cpl cells per lane), kTileFp32x3
+// (three lanes per cell), kTileFp64 (FP64 state, one lane per cell)
+hipError_t launch_ls_tiles(int model, int variant, int cpl, int block, int grid,
+                           const TileArgs& A, hipStream_t st)
 {
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, A.counter);
-#define TILE_CASE(MOD, C) \
-    hipLaunchKernelGGL((k_ls_tiles<MOD, C>), dim3(grid), dim3(block), 0, st, A)
+#define TILE_LAUNCH(K) hipLaunchKernelGGL(K, dim3(grid), dim3(block), 0, st, A)
     if (model == kLinear) {
-        if (cpl == 2) TILE_CASE(Linear, 2); else TILE_CASE(Linear, 1);
+        if (variant == kTileFp64) TILE_LAUNCH((k_ls_tiles64<Linear>));
+        else if (variant == kTileFp32x3) TILE_LAUNCH((k_ls_tiles3<Linear>));
+        else if (cpl == 2) TILE_LAUNCH((k_ls_tiles<Linear, 2>));
+        else TILE_LAUNCH((k_ls_tiles<Linear, 1>));
     } else {
-        if (cpl == 2) TILE_CASE(Rossler, 2); else TILE_CASE(Rossler, 1);
+        if (variant == kTileFp64) TILE_LAUNCH((k_ls_tiles64<Rossler>));
+        else if (variant == kTileFp32x3) TILE_LAUNCH((k_ls_tiles3<Rossler>));
+        else if (cpl == 2) TILE_LAUNCH((k_ls_tiles<Rossler, 2>));
+        else TILE_LAUNCH((k_ls_tiles<Rossler, 1>));
     }
-#undef TILE_CASE
+#undef TILE_LAUNCH
     return hipGetLastError();
 }
 
 // kernel handle for the occupancy query of the persistent grid
-const void* tile_kernel_ptr(int model, int cpl)
+const void* tile_kernel_ptr(int model, int variant, int cpl)
 {
-    if (model == kLinear)
+    if (model == kLinear) {
+        if (variant == kTileFp64) return (const void*)&k_ls_tiles64<Linear>;
+        if (variant == kTileFp32x3) return (const void*)&k_ls_tiles3<Linear>;
         return cpl == 2 ? (const void*)&k_ls_tiles<Linear, 2> : (const void*)&k_ls_tiles<Linear, 1>;
+    }
+    if (variant == kTileFp64) return (const void*)&k_ls_tiles64<Rossler>;
+    if (variant == kTileFp32x3) return (const void*)&k_ls_tiles3<Rossler>;
     return cpl == 2 ? (const void*)&k_ls_tiles<Rossler, 2> : (const void*)&k_ls_tiles<Rossler, 1>;
 }
 

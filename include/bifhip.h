@@ -424,6 +424,21 @@ int bifhip_ls_tiles_autotune(int nTiles, int tileW, int tileH, const int* origin
  * cells per lane 1 or 2; blocks per CU 0 = occupancy maximum. */
 int bifhip_set_tile_config(int blockSize, int cellsPerLane, int blocksPerCU);
 int bifhip_get_tile_config(int* blockSize, int* cellsPerLane, int* blocksPerCU);
+/* Lanes per cell of the FP32 tile kernel: 1 (a lane carries the state and
+ * the three tangent vectors) or 3 (a lane per tangent vector, each stepping
+ * the state itself; vectors exchanged across lanes for Gram-Schmidt).
+ * Results are bit-identical either way; the autotuner times both. */
+int bifhip_set_tile_lanes(int lanesPerCell);
+int bifhip_get_tile_lanes(int* lanesPerCell);
+/* The last autotuner run: up to maxEntries rows of (block size, lanes per
+ * cell, cells per lane, ms on the sample) to out[4*i..]; returns the number
+ * of candidates timed. */
+int bifhip_tile_autotune_log(double* out, int maxEntries);
+/* State precision of the tile path: BIFHIP_PREC_FP32 (default, FP32 state
+ * and tangent vectors, FP64 Gram-Schmidt) or BIFHIP_PREC_FP64 (FP64 state
+ * and tangent vectors, the reference's Gram-Schmidt and log sum: the
+ * reference against which the FP32 path's error is stated). */
+int bifhip_set_tile_precision(int precision);
 /* Replay the tile launches and the surrogate's training epochs from captured
  * HIP graphs (default 1); 0 launches the same kernels directly. */
 int bifhip_set_graph(int on);

@@ -484,6 +484,44 @@ def test_lle_fp32_bulk_vs_fp64(bh):
     assert np.median(err) < 5e-3
 
 
+def test_lle_config3_fp32_bulk_vs_fp64(orc, bh):
+    """The configured LLE workload: 1024^2 lattice over the (a, c) box,
+    transient 500, measure 1000 time units, renormalisation every 0.5,
+    perturbation 1e-8 along x.  The FP64 map's rows against the oracle on 4
+    sampled rows (|error| <= 1e-12, NaN cells as a class); the FP32 bulk map
+    against the FP64 map on every cell.  Bounds: cells NaN in one precision
+    only <= 0.2%, regime (LLE > 0.01, NaN not chaotic) agreement >= 98.5%,
+    sign agreement >= 98% and mean |error| <= 6e-3 over cells finite in
+    both."""
+    n = 1024
+    bh.set_perturbation(bh.PERT_AXIS_X)
+    args = (2, 1000.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 500.0, VALS)
+    say("config-3 LLE map fp64", n)
+    f64 = bh.lle_map(*args).reshape(n, n)
+    rows = (0, 300, 611, 1023)
+    for r in rows:
+        want = orc.lle_cells(2, 1000.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 500.0, VALS,
+                             orc.PERT_AXIS_X, 0, r * n, n)
+        _close_with_nan_class(f64[r], want, 1e-12, f"config-3 fp64 row {r}")
+    bh.set_precision(bh.PREC_FP32)
+    say("config-3 LLE map fp32 bulk", n)
+    f32 = bh.lle_map(*args).reshape(n, n)
+    a, b = f64.ravel(), f32.ravel()
+    # escaping orbits (NaN) near the escape boundary can escape in one
+    # precision and not the other; a NaN cell counts as not chaotic
+    nan_diff = float((np.isnan(a) != np.isnan(b)).mean())
+    ok = np.isfinite(a) & np.isfinite(b)
+    err = np.abs(a[ok] - b[ok])
+    with np.errstate(invalid="ignore"):
+        regime = float(((a > 0.01) == (b > 0.01)).mean())
+    sign = float((np.sign(a[ok]) == np.sign(b[ok])).mean())
+    say("config-3 fp32 vs fp64: cells", a.size, "NaN in one precision only", nan_diff,
+        "regime agreement", regime, "sign agreement (finite)", sign, "mean abs err",
+        float(err.mean()), "p99", float(np.quantile(err, 0.99)), "max", float(err.max()))
+    assert nan_diff <= 2e-3
+    assert regime >= 0.985 and sign >= 0.98 and err.mean() <= 6e-3
+
+
 # ------------------------------------------------------------- drivers
 
 def test_driver_files_match_reference_format(orc, bh, tmp_path):

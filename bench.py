@@ -698,17 +698,19 @@ def run_surrogate(bh, np, args, out, n, d_peak, d_lle):
     bufs = [(bh.DeviceArray((cells,), np.int32), bh.DeviceArray((cells,), np.float64),
              bh.DeviceArray((cells,), np.uint8)) for _ in range(max(K, args.variant_steps, 1))]
 
-    def run(k, seed0, warm):
+    def run(k, seed0, warm, validation=None, batch=None):
+        validation = args.validation if validation is None else validation
+        batch = args.batch if batch is None else batch
         for w in range(warm):
             surrogate_map(bh, n, args.coarse, *bufs[0], seed0 + 100 + w, args.conf,
-                          args.validation, args.batch)
+                          validation, batch)
         bh.synchronize()
         bh.reset_kernel_times()
         st = []
         t0 = time.perf_counter()
         for s in range(k):
             st.append(surrogate_map(bh, n, args.coarse, *bufs[s], seed0 + s, args.conf,
-                                    args.validation, args.batch))
+                                    validation, batch))
         bh.synchronize()
         return (time.perf_counter() - t0) * 1000 / max(k, 1), st
 
@@ -754,16 +756,22 @@ def run_surrogate(bh, np, args, out, n, d_peak, d_lle):
                 "note": "every regime re-solve in FP64 (lle32 margin < 0)"}
         finally:
             bh.set_lle32_margin(args.lle32_margin)
-        for name, reading, conf in (("chaos_reading", bh.READING_CHAOS, args.chaos_reading_conf),
-                                    ("class_reading", bh.READING_CLASS, args.chaos_conf)):
+        for name, reading, conf, margin, val, batch in (
+                ("chaos_reading", bh.READING_CHAOS, args.chaos_reading_conf,
+                 args.chaos_reading_margin, args.chaos_reading_validation,
+                 args.chaos_reading_batch),
+                ("class_reading", bh.READING_CLASS, args.chaos_conf, args.lle32_margin,
+                 args.validation, args.batch)):
             try:
                 bh.set_regime_reading(reading)
                 bh.set_chaos_confidence(conf)
-                vms, vst = run(kv, args.seed_base, 1)
+                bh.set_lle32_margin(margin)
+                vms, vst = run(kv, args.seed_base, 1, val, batch)
                 sc = scored(kv)
                 variants[name] = {
                     "ms_per_step": vms, "speedup_vs_plain_fp64": plain_ms / vms, "steps": kv,
-                    "chaos_confidence": conf,
+                    "chaos_confidence": conf, "quad_confidence": args.quad_conf,
+                    "lle32_margin": margin, "validation": val, "batch": batch,
                     "agreement": {k: summarise(np, sc, k)
                                   for k in ("regime", "chaos", "class", "count_exact")},
                     "lle_abs_err_mean": float(np.mean([a["lle_abs_err_mean"] for a in sc])),
@@ -774,8 +782,13 @@ def run_surrogate(bh, np, args, out, n, d_peak, d_lle):
             finally:
                 bh.set_regime_reading(bh.READING_CLASS_AND_CHAOS)
                 bh.set_chaos_confidence(args.chaos_conf)
+                bh.set_lle32_margin(args.lle32_margin)
         variants["chaos_reading"]["reading"] = ("regime = chaos bit (LLE > 0.01) only; peak "
                                                 "counts predicted, their agreement reported")
+        variants["chaos_reading"]["gate_passed"] = \
+            variants["chaos_reading"]["agreement"]["chaos"]["mean"] >= 0.99
+        variants["class_reading"]["gate_passed"] = \
+            variants["class_reading"]["agreement"]["class"]["mean"] >= 0.99
         variants["class_reading"]["reading"] = ("regime = peak-count class only; LLE values "
                                                 "predicted, their error reported")
 
@@ -785,7 +798,9 @@ def run_surrogate(bh, np, args, out, n, d_peak, d_lle):
                                    f"Rossler {n}x{n} (coarse {args.coarse}x{args.coarse}, "
                                    f"20 epochs; class confidence < {args.conf} or class "
                                    f"boundary: peak count re-solved in FP64; chaos "
-                                   f"confidence < {args.chaos_conf} or regime boundary: "
+                                   f"confidence < {args.chaos_conf}, regime boundary, or a "
+                                   f"coarse quad with mixed or mispredicted exact chaos "
+                                   f"bits (chaos confidence < {args.quad_conf}): "
                                    f"LLE re-solved, FP32 bulk where it lies >= "
                                    f"{args.lle32_margin} from the threshold and FP64 "
                                    f"nearer, with the FP64 peak count when the class is "
@@ -861,8 +876,12 @@ def main():
                     help="lle workload: renormalisation interval in time units (0.5 in the "
                          "configuration; larger values isolate the renormalisations' cost)")
     ap.add_argument("--conf", type=float, default=0.95, help="class softmax confidence threshold")
-    ap.add_argument("--chaos-conf", type=float, default=0.975,
-                    help="chaos-logit confidence threshold")
+    ap.add_argument("--chaos-conf", type=float, default=0.95,
+                    help="chaos-logit confidence threshold (chosen on held-out seeds 1000-1002)")
+    ap.add_argument("--quad-conf", type=float, default=2.0,
+                    help="coarse-quad chaos rule: cells in coarse quads with mixed exact chaos "
+                         "bits, or whose corners the network gets wrong, re-solve their LLE "
+                         "below this chaos confidence (0 off, above 1 always)")
     ap.add_argument("--lle32-margin", type=float, default=0.02,
                     help="regime-only re-solves: FP32 bulk LLE at least this far from the "
                          "chaos threshold, FP64 nearer (negative: all FP64)")
@@ -874,8 +893,15 @@ def main():
     ap.add_argument("--variant-steps", type=int, default=3,
                     help="timed steps of each variant (all-FP64 re-solves, chaos-only and "
                          "class-only readings); 0 skips them")
-    ap.add_argument("--chaos-reading-conf", type=float, default=0.9,
-                    help="chaos-confidence threshold of the chaos-only reading variant")
+    ap.add_argument("--chaos-reading-conf", type=float, default=0.6,
+                    help="chaos-only reading variant: chaos-confidence threshold")
+    ap.add_argument("--chaos-reading-margin", type=float, default=0.01,
+                    help="chaos-only reading variant: FP32 LLE margin")
+    ap.add_argument("--chaos-reading-batch", type=int, default=4096,
+                    help="chaos-only reading variant: training batch rows")
+    ap.add_argument("--chaos-reading-validation", type=float, default=0.0,
+                    help="chaos-only reading variant: validation fraction (its agreement is "
+                         "measured here against the plain map)")
     ap.add_argument("--batch", type=int, default=2048,
                     help="training batch rows (20 epochs of the 256^2 coarse map)")
     ap.add_argument("--cpu-cells", type=int, default=32768,
@@ -908,6 +934,7 @@ def main():
     bh.set_graph(not args.no_graph)
     bh.set_partial_resolve(not args.full_resolve)
     bh.set_chaos_confidence(args.chaos_conf)
+    bh.set_quad_confidence(args.quad_conf)
     bh.set_lle32_margin(args.lle32_margin)
     n = args.n or (4096 if args.workload == "surrogate" else 1024)
     cells = n * n

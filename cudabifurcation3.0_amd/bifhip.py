@@ -123,6 +123,7 @@ def _declare(L):
         "bifhip_set_chaos_confidence": [_D],
         "bifhip_set_lle32_margin": [_D],
         "bifhip_set_regime_reading": [_I],
+        "bifhip_set_quad_confidence": [_D],
         "bifhip_lle_peak_map_device": [_I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I, _I, _D,
                                        _D, _D, _D, _D, _VP, _VP],
         "bifhip_surrogate_map_device": [_I, _I, _D, _I, _PD, _PD, _PI, _PD, _I, _D, _D, _I,
@@ -291,6 +292,7 @@ def reset_context():
     set_chaos_confidence(0.975)
     set_lle32_margin(0.02)
     set_regime_reading(READING_CLASS_AND_CHAOS)
+    set_quad_confidence(0.0)
     set_tile_precision(PREC_FP32)
     set_tile_lanes(1)
 
@@ -729,6 +731,13 @@ def set_lle32_margin(margin):
     lies at least `margin` from the chaos threshold, FP64 elsewhere; a
     negative margin keeps them all in FP64."""
     _check(load().bifhip_set_lle32_margin(float(margin)))
+
+
+def set_quad_confidence(threshold):
+    """Coarse-quad chaos rule of the surrogate: 0 off (default); cells in a
+    coarse quad with mixed exact chaos bits re-solve their LLE when their
+    chaos confidence is below the threshold."""
+    _check(load().bifhip_set_quad_confidence(float(threshold)))
 
 
 def set_regime_reading(reading):

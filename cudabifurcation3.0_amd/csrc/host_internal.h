@@ -31,6 +31,7 @@ struct Ctx {
     double chaosConfidence = 0.975;  // surrogate: chaos confidence below this re-solves the LLE
     double lle32Margin = 0.02;  // surrogate: FP32 regime re-solves this far from the threshold
     int reading = BIFHIP_READING_CLASS_AND_CHAOS;  // surrogate: what a cell's regime is
+    double quadConfidence = 0.0;  // surrogate: coarse-quad chaos rule (0: off)
     int tilePrecision = BIFHIP_PREC_FP32;  // uneven tiles: FP32 state (FP64: the reference)
     std::string lastError;
     hipStream_t stream = nullptr;

@@ -48,8 +48,9 @@ hipError_t launch_fill(long long n, int* v, int value, hipStream_t st);
 hipError_t launch_fill_f64(int n, double* p, double v, hipStream_t st);
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
-                       float chaosConfThr, bool partial, int reading, int* listFull,
-                       int* countFull, int* listPeak, int* countPeak, int* listLle, int* countLle,
+                       float chaosConfThr, bool partial, int reading,
+                       const unsigned char* coarseChaos, int nCoarse, float quadConf,
+                       int* listFull, int* countFull, int* listPeak, int* countPeak, int* listLle, int* countLle,
                        unsigned long long vseed, unsigned int vthr, int* listVal, int* countVal,
                        unsigned char* flag, hipStream_t st);
 hipError_t launch_scatter(int n, const int* list, const int* peak, const double* lle,
@@ -526,10 +527,15 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
                            : 1.0;
     std::vector<int> cls(nc);
     std::vector<float> lstd(nc);
+    std::vector<unsigned char> cchaos(nc);
     for (int i = 0; i < nc; ++i) {
         cls[i] = std::min(kClasses, std::max(1, cpeak[i])) - 1;
         lstd[i] = std::isfinite(clle[i]) ? (float)((clle[i] - mean) / sd) : NAN;
+        cchaos[i] = clle[i] > chaosThreshold ? 1 : 0;     // NaN: not chaotic
     }
+    unsigned char* d_cchaos;
+    RC_TRY(pool_get("sg_cchaos", nc, &d_cchaos));
+    HIP_TRY(hipMemcpy(d_cchaos, cchaos.data(), (size_t)nc, hipMemcpyHostToDevice));
     int* d_cls;
     float* d_lstd;
     RC_TRY(pool_get("sg_cls", nc, &d_cls));
@@ -578,7 +584,9 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
     const unsigned int vthr =
         vf > 0 ? (unsigned int)std::min(4294967295.0, std::floor(vf * 4294967296.0)) : 0u;
     HIP_TRY(launch_flag(nFine, d_pcls, d_pch, d_conf, d_cconf, (float)confThreshold,
-                        (float)g.chaosConfidence, partial, g.reading, d_all, d_count, d_plist,
+                        (float)g.chaosConfidence, partial, g.reading,
+                        g.quadConfidence > 0 ? d_cchaos : nullptr, nCoarse,
+                        (float)g.quadConfidence, d_all, d_count, d_plist,
                         d_count + 1, d_llist, d_count + 3, seed + 17, vthr, d_vlist, d_count + 2,
                         d_flag, s));
     HIP_TRY(launch_fill_pred(nf, d_pcls, d_plle, d_peak, d_lle, s));
@@ -640,8 +648,9 @@ int bifhip_surrogate_map_device(int nFine, int nCoarse, double h, int nIC, const
                                      BIFHIP_PREC_FP64));
     }
     RC_TRY(surrogate_resolve_peaks_finish());
-    if (nlle > 0) {
+    if (nlle > 0 && g.reading != BIFHIP_READING_CHAOS) {
         // a regime that changed under the re-solve: the peak count as well
+        // (not under the chaos-only reading, whose peak counts stay predicted)
         RC_TRY(pool_get("sg_clist", (size_t)nlle, &d_clist));
         HIP_TRY(launch_regime_changed(nlle, d_llist, d_llle, d_pch, chaosThreshold, d_clist,
                                       d_count + 4, d_flag, s));

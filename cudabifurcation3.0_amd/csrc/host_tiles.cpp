@@ -262,6 +262,14 @@ int bifhip_set_regime_reading(int reading)
     return BIFHIP_OK;
 }
 
+int bifhip_set_quad_confidence(double threshold)
+{
+    if (!(threshold >= 0.0 && threshold <= 2.0))
+        return fail(BIFHIP_E_ARG, "quad confidence threshold %g outside [0, 2]", threshold);
+    g.quadConfidence = threshold;
+    return BIFHIP_OK;
+}
+
 static int tiles_impl(int nTiles, int tileW, int tileH, const int* originI, const int* originJ,
                       const int* nBlocks, int nPts, const double* ranges, const int* mut,
                       const double* ic, int nIC, const double* values, int nVal, double h,

@@ -571,7 +571,8 @@ __device__ __forceinline__ bool validation_pick(long long idx, unsigned long lon
 __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
                        const unsigned char* __restrict__ chaos, const float* __restrict__ conf,
                        const float* __restrict__ confChaos, float confThr, float chaosConfThr,
-                       bool partial, int reading, int* __restrict__ listFull,
+                       bool partial, int reading, const unsigned char* __restrict__ coarseChaos,
+                       int nCoarse, float quadConf, int* __restrict__ listFull,
                        int* __restrict__ countFull,
                        int* __restrict__ listPeak, int* __restrict__ countPeak,
                        int* __restrict__ listLle, int* __restrict__ countLle,
@@ -595,6 +596,29 @@ __global__ void k_flag(int n, const unsigned char* __restrict__ cls,
             const long long o = (long long)jj * n + ii;
             fc |= cls[o] != c0;
             fh |= chaos[o] != ch0;
+        }
+        // the exact coarse map: a cell inside a coarse quad whose four
+        // corners disagree on the chaos bit sits where a regime boundary
+        // crosses the coarse lattice, and one whose corners the network
+        // itself gets wrong lies where the network is unreliable; unless the
+        // chaos head is sure of the cell (confidence >= quadConf), its LLE
+        // is re-solved.  Coarse corner (ci, cj) is fine cell
+        // (rint(ci (n-1)/(nc-1)), rint(cj (n-1)/(nc-1))).
+        if (coarseChaos && confChaos[idx] < quadConf) {
+            const int qi = min((int)((long long)i * (nCoarse - 1) / (n - 1)), nCoarse - 2);
+            const int qj = min((int)((long long)j * (nCoarse - 1) / (n - 1)), nCoarse - 2);
+            int s4 = 0;
+            bool wrong = false;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int ci = qi + (c & 1), cj = qj + (c >> 1);
+                const int b = coarseChaos[(long long)cj * nCoarse + ci];
+                const long long fi = (2LL * ci * (n - 1) + (nCoarse - 1)) / (2LL * (nCoarse - 1));
+                const long long fj = (2LL * cj * (n - 1) + (nCoarse - 1)) / (2LL * (nCoarse - 1));
+                s4 += b;
+                wrong |= (chaos[fj * n + fi] != 0) != (b != 0);
+            }
+            fh |= (s4 > 0 && s4 < 4) || wrong;
         }
         // the regime reading decides what a cell can be unsure of
         if (reading == 1) fc = false;        // chaos bit only
@@ -808,14 +832,17 @@ hipError_t launch_train_fused(int rows, int nLat, const int* list, const int* tl
 
 hipError_t launch_flag(int n, const unsigned char* cls, const unsigned char* chaos,
                        const float* conf, const float* confChaos, float confThr,
-                       float chaosConfThr, bool partial, int reading, int* listFull, int* countFull,
+                       float chaosConfThr, bool partial, int reading,
+                       const unsigned char* coarseChaos, int nCoarse, float quadConf,
+                       int* listFull, int* countFull,
                        int* listPeak, int* countPeak, int* listLle, int* countLle,
                        unsigned long long vseed, unsigned int vthr, int* listVal, int* countVal,
                        unsigned char* flag, hipStream_t st)
 {
     const long long cells = (long long)n * n;
     hipLaunchKernelGGL(k_flag, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, n, cls,
-                       chaos, conf, confChaos, confThr, chaosConfThr, partial, reading, listFull,
+                       chaos, conf, confChaos, confThr, chaosConfThr, partial, reading, coarseChaos,
+                       nCoarse, quadConf, listFull,
                        countFull, listPeak, countPeak, listLle, countLle, vseed, vthr, listVal,
                        countVal, flag);
     return hipGetLastError();

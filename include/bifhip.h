@@ -464,6 +464,13 @@ int bifhip_set_lle32_margin(double margin);
  *   BIFHIP_READING_CHAOS: the chaos bit only; peak counts stay predicted
  *     except where a re-solved LLE changes the predicted regime;
  *   BIFHIP_READING_CLASS: the class only; LLE values stay predicted. */
+/* Surrogate-refined map: a fine cell inside a quad of the coarse lattice
+ * whose four corners (exact, from the coarse map) disagree on the chaos
+ * bit, or whose corners the network predicts wrongly, has its LLE re-solved
+ * unless its chaos confidence is >= threshold (in [0, 2]; 0 turns the rule
+ * off, above 1 applies it to every such cell).  Default 0.  No reference
+ * counterpart. */
+int bifhip_set_quad_confidence(double threshold);
 #define BIFHIP_READING_CLASS_AND_CHAOS 0
 #define BIFHIP_READING_CHAOS 1
 #define BIFHIP_READING_CLASS 2

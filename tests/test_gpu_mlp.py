@@ -165,20 +165,26 @@ def test_training_learns_a_synthetic_map(bh):
 # reference perturbation directions read the listed cells' indices back;
 # margin: FP32 bulk LLE for regime-only cells this far from the threshold
 # (negative: all FP64)
-@pytest.mark.parametrize("partial,cap,pert,margin", [(True, 0, 1, 0.02), (False, 0, 1, 0.02),
-                                                     (True, 8, 1, 0.02), (True, 0, 0, 0.02),
-                                                     (True, 0, 1, -1.0)])
-def test_surrogate_pipeline_small(bh, partial, cap, pert, margin):
+# reading: what a cell's regime is (0 class and chaos bit, 1 chaos bit, 2
+# class); quad: the coarse-quad chaos rule's confidence threshold (0 off)
+@pytest.mark.parametrize("partial,cap,pert,margin,reading,quad",
+                         [(True, 0, 1, 0.02, 0, 0.0), (False, 0, 1, 0.02, 0, 0.0),
+                          (True, 8, 1, 0.02, 0, 0.0), (True, 0, 0, 0.02, 0, 0.0),
+                          (True, 0, 1, -1.0, 0, 0.0), (True, 0, 1, 0.02, 0, 2.0),
+                          (True, 0, 1, 0.01, 1, 2.0), (True, 0, 1, 0.02, 2, 0.0)])
+def test_surrogate_pipeline_small(bh, partial, cap, pert, margin, reading, quad):
     nf, nc = 96, 24
     peak = dict(tmax=40.0, trans=20.0)
     lle = dict(tmax=50.0, NT=0.5, trans=10.0, eps=1e-8)
     bh.set_perturbation(pert)
     bh.set_partial_resolve(partial)
     bh.set_lle32_margin(margin)
+    bh.set_regime_reading(reading)
+    bh.set_quad_confidence(quad)
     if cap:
         bh.set_peak_capacity(cap)
     say("surrogate pipeline", nf, nc, "partial", partial, "cap", cap, "pert", pert, "margin",
-        margin)
+        margin, "reading", reading, "quad", quad)
     d_pk = bh.DeviceArray((nf * nf,), np.int32).zero_()
     d_ll = bh.DeviceArray((nf * nf,), np.float64).zero_()
     d_rs = bh.DeviceArray((nf * nf,), np.uint8).zero_()
@@ -214,7 +220,7 @@ def test_surrogate_pipeline_small(bh, partial, cap, pert, margin):
         assert st["peak_only"] == 0 and st["lle_only"] == 0
     if margin < 0:
         assert st["lle_only_fp32"] == 0
-    elif partial:
+    elif partial and reading != 2:
         assert st["lle_only_fp32"] > 0
     # cells re-solved for the regime only carry the FP64 LLE (4, 5: FP32
     # LLE within the margin of the threshold, or NaN) or the FP32 bulk LLE
@@ -239,9 +245,41 @@ def test_surrogate_pipeline_small(bh, partial, cap, pert, margin):
     np.testing.assert_array_equal(pk[rs == 2], tpk[rs == 2])
     cls_t = np.clip(tpk, 1, 16)
     cls_s = np.clip(pk, 1, 16)
-    reg = (cls_t == cls_s) & ((tll > 0.01) == (ll > 0.01))
-    say("regime agreement", reg.mean(), "full", full.mean(), "peak only", (rs == 2).mean())
-    assert reg.mean() > 0.9
+    with np.errstate(invalid="ignore"):
+        ch_ok = (tll > 0.01) == (ll > 0.01)
+    reg = (cls_t == cls_s) & ch_ok
+    say("regime agreement", reg.mean(), "chaos", ch_ok.mean(), "class",
+        (cls_t == cls_s).mean(), "full", full.mean(), "peak only", (rs == 2).mean())
+    if reading == 1:
+        # chaos bit only: no class re-solves and no recounts
+        assert st["resolved"] == 0 and st["peak_only"] == 0
+        assert st["lle_only_recounted"] == 0
+        assert ch_ok.mean() > 0.9
+    elif reading == 2:
+        # class only: no LLE re-solves
+        assert st["resolved"] == 0 and st["lle_only"] == 0
+        assert (cls_t == cls_s).mean() > 0.9
+    else:
+        assert reg.mean() > 0.9
+    if quad > 1.0:
+        # every cell of a coarse quad whose exact corners disagree on the
+        # chaos bit has its LLE re-solved (the network-independent half of
+        # the rule); the coarse map is the plain map of the coarse lattice
+        c_pk = bh.DeviceArray((nc * nc,), np.int32).zero_()
+        c_ll = bh.DeviceArray((nc * nc,), np.float64).zero_()
+        bh.lle_peak_map_device(c_pk.ptr, c_ll.ptr, nc, 0.01, IC, R2, [0, 2], VALS, peak["tmax"],
+                               peak["trans"], 0, 100000, 0.01, lle["tmax"], lle["NT"],
+                               lle["trans"], lle["eps"])
+        with np.errstate(invalid="ignore"):
+            q = (c_ll.to_host() > 0.01).reshape(nc, nc).astype(int)
+        s4 = q[:-1, :-1] + q[1:, :-1] + q[:-1, 1:] + q[1:, 1:]
+        mixed = (s4 > 0) & (s4 < 4)
+        k = np.minimum(np.arange(nf) * (nc - 1) // (nf - 1), nc - 2)
+        mq = mixed[np.ix_(k, k)].ravel()
+        say("cells in mixed coarse quads", int(mq.sum()))
+        assert mq.any()
+        lle_resolved = (rs == 1) | ((rs >= 4) & (rs <= 7))
+        assert lle_resolved[mq].all()
 
 
 def _regimes(peak, lle):

@@ -370,9 +370,8 @@ def run_dbscan(bh, np, args, out):
     peak pairs of x per cell, plain FP64, Rossler n x n over the (a, c) box,
     transient 1000, record 500, eps 0.01, counts capped at 32 (the
     reference's maxAmountOfPeaks is a threshold, so it is set to the record
-    length and the cap is the separate clamp).  Sampled rows are compared
-    with the oracle bit for bit."""
-    import orclib as O
+    length and the cap is the separate clamp).  The CPU baseline leg times
+    the oracle on three rows and compares them with the map bit for bit."""
     n = args.n or 512
     cells = n * n
     d = bh.DeviceArray((cells,), np.int32)
@@ -389,15 +388,10 @@ def run_dbscan(bh, np, args, out):
         bh.dbscan_map_device(*dargs)
     bh.synchronize()
     ms = (time.perf_counter() - t0) * 1000 / args.steps
-    kms, nl = bh.kernel_time("bif_peaks")
+    fam = "bif_peaks2" if bh.kernel_time("bif_peaks2")[1] else "bif_peaks"
+    kms, nl = bh.kernel_time(fam)
     dbms, _ = bh.kernel_time("dbscan")
     got = d.to_host().reshape(n, n)
-    rows = [0, n // 3, n - 1]
-    exact = True
-    for r in rows:
-        want = O.dbscan_cells(PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(),
-                              PEAK["transient"], VALS, 1, PEAK["eps"], r * n, n)
-        exact &= bool(np.array_equal(got[r], np.minimum(want, 32)))
     bh.set_cluster_clamp(0)
     ach = BIF_FLOP_PER_CELL * cells * args.steps / (kms / 1000) / 1e12
     out.update(metric="grid points/s, 2D DBSCAN peak-count map (plain FP64)",
@@ -405,11 +399,12 @@ def run_dbscan(bh, np, args, out):
                config={"workload": f"2D bifurcation map, Rossler {n}x{n} (a, c) lattice, DBSCAN "
                                    f"peak count of x capped at 32, plain FP64",
                        "grid": n, "ranges": RANGES, "h": H, "peak": PEAK, "cap": 32})
-    out["oracle_rows_bit_exact"] = {"rows": rows, "equal": exact}
-    out["kernels_ms_per_step"] = {"bif_peaks": kms / args.steps, "dbscan": dbms / args.steps}
+    out["kernels_ms_per_step"] = {fam: kms / args.steps, "dbscan": dbms / args.steps}
     if args.fp32:
-        out["fp32"] = run_dbscan_fp32(bh, np, args, d, dargs, got, ms, rows)
-    out["roofline"] = {"bound": "valu_fp64", "kernel": "k_bif_peaks", "achieved": ach,
+        out["fp32"] = run_dbscan_fp32(bh, np, args, d, dargs, got, ms)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = dbscan_cpu_baseline(np, n, got)
+    out["roofline"] = {"bound": "valu_fp64", "kernel": "k_" + fam, "achieved": ach,
                        "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s",
                        "frac": ach / (FP64_VALU_PEAK / 1e12), "traffic": None,
                        "avg_launch_ms": kms / max(nl, 1), "launches": nl,
@@ -417,11 +412,29 @@ def run_dbscan(bh, np, args, out):
                                "FP64 reaches at most 50% of the FMA-counted peak"}
 
 
-def run_dbscan_fp32(bh, np, args, d, dargs, got64, ms64, rows):
-    """The same map with FP32 trajectories (bifhip_set_precision FP32):
-    time, bit-exactness of sampled rows against the oracle's FP32
-    restatement, and peak-count agreement with the FP64 map."""
+def dbscan_cpu_baseline(np, n, got):
+    """CPU baseline of the DBSCAN map: the oracle (OpenMP over cells) on
+    three whole rows, timed, and its counts compared with the GPU map's."""
     import orclib as O
+    O.lib()
+    rows = [0, n // 3, n - 1]
+    exact = True
+    t0 = time.perf_counter()
+    wants = [O.dbscan_cells(PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(),
+                            PEAK["transient"], VALS, 1, PEAK["eps"], r * n, n) for r in rows]
+    dt = time.perf_counter() - t0
+    for r, want in zip(rows, wants):
+        exact &= bool(np.array_equal(got[r], np.minimum(want, 32)))
+    return {"value": len(rows) * n / dt, "unit": "grid points/s", "cores": O.lib().orc_threads(),
+            "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"rows {rows} of the {n}x{n} map, {dt:.2f} s",
+            "rows_bit_exact_vs_gpu": exact}
+
+
+def run_dbscan_fp32(bh, np, args, d, dargs, got64, ms64):
+    """The same map with FP32 trajectories (bifhip_set_precision FP32):
+    time and peak-count agreement with the FP64 map (bit-exactness against
+    the oracle's FP32 restatement is tests/test_gpu_fp32_peaks.py's)."""
     n = int(round(got64.size ** 0.5))
     cells = n * n
     bh.set_precision(bh.PREC_FP32)
@@ -442,16 +455,10 @@ def run_dbscan_fp32(bh, np, args, d, dargs, got64, ms64, rows):
     finally:
         bh.set_precision(bh.PREC_FP64)
         bh.set_cluster_clamp(0)
-    exact = True
-    for r in rows:
-        want = O.dbscan_cells(PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(),
-                              PEAK["transient"], VALS, 1, PEAK["eps"], r * n, n, fp32=True)
-        exact &= bool(np.array_equal(got[r], np.minimum(want, 32)))
     c64, c32 = np.clip(got64, 1, 16), np.clip(got, 1, 16)
     # 11 FLOP per FP32 step (4 fused multiply-adds, 3 adds / multiplies)
     ach = 11 * 150000 * cells * args.steps / (kms / 1000) / 1e12
     return {"ms_per_step": ms, "value": cells / (ms / 1000), "speedup_vs_fp64": ms64 / ms,
-            "oracle_rows_bit_exact": {"rows": rows, "equal": exact, "oracle": "FP32 restatement"},
             "count_agreement_vs_fp64": float((got == got64).mean()),
             "class_agreement_vs_fp64": float((c32 == c64).mean()),
             "kernels_ms_per_step": {"bif_peaks32": kms / args.steps, "dbscan": dbms / args.steps},
@@ -464,9 +471,8 @@ def run_bif1d(bh, np, args, out):
     """1D bifurcation diagram: local maxima of x for c over [2, 7] in 1000
     points (a = b = 0.2), transient 1000, record 500, plain FP64 through the
     reference's driver with host buffers (the time includes the copy of the
-    peak rows back to the host).  Every cell is compared with the oracle,
-    which is also timed as the CPU baseline."""
-    import orclib as O
+    peak rows back to the host).  The CPU baseline leg times the oracle on
+    every cell and compares its peaks with the GPU's."""
     n = 1000
     vals = [0.2, 0.2, 0.0]
     bargs = (PEAK["tMax"], n, H, IC, [2.0, 7.0], [2], 0, 0.0, PEAK["transient"], vals, 1)
@@ -479,23 +485,27 @@ def run_bif1d(bh, np, args, out):
         cnt, pk = bh.bifurcation1D_to_buffer(*bargs)
     ms = (time.perf_counter() - t0) * 1000 / args.steps
     kms, _ = bh.kernel_time("bif_peaks")
-    O.lib()
-    t1 = time.perf_counter()
-    wc, wp, _ = O.peaks_cells(1, PEAK["tMax"], n, H, IC, [2.0, 7.0], [2], 0, 0.0,
-                              PEAK["transient"], vals, 1, 0, n, pk.shape[1])
-    cpu_s = time.perf_counter() - t1
-    exact = bool(np.array_equal(cnt, np.maximum(wc, 0))) and all(
-        np.array_equal(pk[k, :cnt[k]], wp[k, :cnt[k]]) for k in range(n))
     out.update(metric="parameter points/s, 1D bifurcation diagram (plain FP64)",
                value=n / (ms / 1000), unit="parameter points/s", ms_per_step=ms, dtype="f64",
                config={"workload": "1D bifurcation diagram, Rossler c in [2, 7] x 1000 points, "
                                    "a = b = 0.2, transient 1000, record 500, host buffers",
                        "h": H, "peak": PEAK})
-    out["oracle_bit_exact_all_cells"] = exact
     out["kernels_ms_per_step"] = {"bif_peaks": kms / args.steps}
-    out["cpu_baseline"] = {"value": n / cpu_s, "unit": "parameter points/s",
-                           "cores": O.lib().orc_threads(), "kind": "port",
-                           "sample": f"all {n} cells, {cpu_s:.2f} s"}
+    if not args.no_cpu_baseline:
+        # CPU baseline leg: the oracle on every cell, timed; its peaks are
+        # compared with the GPU diagram's
+        import orclib as O
+        O.lib()
+        t1 = time.perf_counter()
+        wc, wp, _ = O.peaks_cells(1, PEAK["tMax"], n, H, IC, [2.0, 7.0], [2], 0, 0.0,
+                                  PEAK["transient"], vals, 1, 0, n, pk.shape[1])
+        cpu_s = time.perf_counter() - t1
+        exact = bool(np.array_equal(cnt, np.maximum(wc, 0))) and all(
+            np.array_equal(pk[k, :cnt[k]], wp[k, :cnt[k]]) for k in range(n))
+        out["cpu_baseline"] = {"value": n / cpu_s, "unit": "parameter points/s",
+                               "cores": O.lib().orc_threads(), "cpu_model": cpu_model(),
+                               "kind": "port", "sample": f"all {n} cells, {cpu_s:.2f} s",
+                               "all_cells_bit_exact_vs_gpu": exact}
 
 
 def run_copy(bh, np, args, out):
@@ -521,13 +531,15 @@ HBM_PEAK = 8.0e12            # MI355X HBM3E
 MLP_TRAIN_FLOP_PER_ROW = 3 * 2 * 45312   # forward, input- and weight-gradient products
 MLP_INFER_FLOP_PER_CELL = 86272
 # DBSCAN reads each clustered cell's peak row once: about 85 (amplitude,
-# interval) pairs per cell on this lattice (Rossler, 500-unit window), 16 B
-# each, plus the count in and the result out
+# interval) pairs per cell on this lattice (Rossler, 500-unit window; 85.3
+# from the fused kernel's row writes), 16 B each, plus the count in and the
+# result out
 DBSCAN_BYTES_PER_CELL = 85 * 16 + 8
-# HBM bytes written per algorithmic byte, from the FETCH_SIZE / WRITE_SIZE
-# counter runs (profiles/: plain_fp64_lle_peak_map_1024.md and
-# peak_count_rows_1024.md)
-TRAFFIC_RATIO = {"k_lle_peaks": 1.06, "k_bif_peaks2": 2.0, "k_dbscan_wave": 1.05}
+# HBM bytes per algorithmic byte, from the FETCH_SIZE / WRITE_SIZE counter
+# runs scaled by the same counters on a 1 GiB copy (FETCH x 2.0, WRITE x
+# 1.0): k_lle_peaks and k_bif_peaks2 writes, k_dbscan_wave reads
+# (profiles/plain_fp64_lle_peak_map_1024.md, dbscan_peak_rows_1024.md)
+TRAFFIC_RATIO = {"k_lle_peaks": 1.06, "k_bif_peaks2": 1.03, "k_dbscan_wave": 1.05}
 
 
 def lle32_flop_per_cell():
@@ -613,38 +625,35 @@ def noise_ceiling(np, ref, n, plain_ms, fixed_ms, cost):
 
 def kernel_rooflines(bh, args, stats, n, step_ms):
     """Every kernel family of the timed surrogate steps: time per step, share
-    of the step, algorithmic work / HIP-event time against its peak.  DBSCAN
-    runs on a side stream beside the integrators, so its share overlaps."""
+    of the step, algorithmic work (the cells each family's launches covered,
+    counted by the library, times the work per cell) / HIP-event time
+    against its peak.  DBSCAN runs on a side stream beside the integrators,
+    so its share overlaps theirs."""
     K = len(stats)
-
-    def cnt(key):
-        return sum(st[key] for st in stats)
-
-    fused = args.coarse ** 2 * K + cnt("resolved") + cnt("validation")
-    peak_cells = cnt("peak_only") + cnt("lle_only_recounted")
-    lle_only = cnt("lle_only")
-    lle32 = lle_only if args.lle32_margin >= 0 else 0
-    lle64 = lle_only - cnt("lle_only_fp32")
     steps_per_epoch = args.coarse ** 2 // min(args.batch, args.coarse ** 2)
     train_rows = steps_per_epoch * min(args.batch, args.coarse ** 2) * 20 * K
     rows = [
-        ("k_lle_peaks", "lle_peaks", "valu_fp64", lle_flop_per_cell() * fused, FP64_VALU_PEAK),
-        ("k_bif_peaks2", "bif_peaks", "valu_fp64", BIF_FLOP_PER_CELL * peak_cells,
-         FP64_VALU_PEAK),
-        ("k_lle32", "lle32", "valu_fp32", lle32_flop_per_cell() * lle32, FP32_VALU_PEAK),
-        ("k_lle64", "lle", "valu_fp64", lle_flop_per_cell() * lle64, FP64_VALU_PEAK),
-        ("k_dbscan_wave", "dbscan", "hbm", DBSCAN_BYTES_PER_CELL * (fused + peak_cells),
-         HBM_PEAK),
-        ("k_train_fused", "mlp_train", "mfma_bf16", MLP_TRAIN_FLOP_PER_ROW * train_rows,
-         MFMA_BF16_PEAK),
-        ("k_infer_fused", "mlp_infer", "mfma_bf16", MLP_INFER_FLOP_PER_CELL * n * n * K,
-         MFMA_BF16_PEAK),
+        ("k_lle_peaks", "lle_peaks", "valu_fp64", lle_flop_per_cell(), FP64_VALU_PEAK),
+        ("k_bif_peaks2", "bif_peaks2", "valu_fp64", BIF_FLOP_PER_CELL, FP64_VALU_PEAK),
+        ("k_bif_peaks", "bif_peaks", "valu_fp64", BIF_FLOP_PER_CELL, FP64_VALU_PEAK),
+        ("k_lle32", "lle32", "valu_fp32", lle32_flop_per_cell(), FP32_VALU_PEAK),
+        ("k_lle64", "lle", "valu_fp64", lle_flop_per_cell(), FP64_VALU_PEAK),
+        ("k_dbscan_wave", "dbscan", "hbm", DBSCAN_BYTES_PER_CELL, HBM_PEAK),
+        ("k_train_fused", "mlp_train", "mfma_bf16", None, MFMA_BF16_PEAK),
+        ("k_infer_fused", "mlp_infer", "mfma_bf16", None, MFMA_BF16_PEAK),
     ]
     out = {}
-    for name, timer, bound, work, peak in rows:
+    for name, timer, bound, per_cell, peak in rows:
         ms, launches = bh.kernel_time(timer)
         if launches == 0:
             continue
+        cells = bh.kernel_cells(timer)
+        if name == "k_train_fused":
+            work = MLP_TRAIN_FLOP_PER_ROW * train_rows
+        elif name == "k_infer_fused":
+            work = MLP_INFER_FLOP_PER_CELL * n * n * K
+        else:
+            work = per_cell * cells
         ach = work / (ms / 1000) if ms > 0 else 0.0
         hbm = bound == "hbm"
         row = {"bound": bound, "ms_per_step": ms / K, "share_of_step": ms / K / step_ms,
@@ -652,21 +661,20 @@ def kernel_rooflines(bh, args, stats, n, step_ms):
                "achieved": ach / (1e9 if hbm else 1e12),
                "peak": peak / (1e9 if hbm else 1e12),
                "unit": "GB/s" if hbm else "TFLOP/s", "frac": ach / peak,
-               "work_per_launch": work / launches,
+               "work_per_launch": work / launches, "cells": cells,
                "traffic_ratio": TRAFFIC_RATIO.get(name)}
         # HBM bytes written per launch by the peak kernels: counter ratio x
         # algorithmic peak-row bytes
         if name == "k_lle_peaks":
-            row["traffic"] = LLE_PEAKS_HBM_BYTES_PER_CELL * fused / launches
+            row["traffic"] = LLE_PEAKS_HBM_BYTES_PER_CELL * cells / launches
         elif name == "k_bif_peaks2":
-            row["traffic"] = TRAFFIC_RATIO[name] * (DBSCAN_BYTES_PER_CELL - 8) * \
-                peak_cells / launches
+            row["traffic"] = TRAFFIC_RATIO[name] * (DBSCAN_BYTES_PER_CELL - 4) * cells / launches
         out[name] = row
     if "k_dbscan_wave" in out:
-        out["k_dbscan_wave"]["note"] = "side stream, overlapped with the integrators"
-    if "k_bif_peaks2" in out:
-        out["k_bif_peaks2"]["note"] = ("class-only and recount re-solves; short lists take the "
-                                       "one-chain k_bif_peaks under the same timer")
+        out["k_dbscan_wave"]["note"] = ("side stream, overlapped with the integrators; the "
+                                        "family includes the rare workgroup-kernel cells")
+    if "k_bif_peaks" in out:
+        out["k_bif_peaks"]["note"] = "one chain per lane: recounts and overflow recomputes"
     return out
 
 
@@ -735,10 +743,10 @@ def run_surrogate(bh, np, args, out, n, d_peak, d_lle):
                 "regime_agreement": float((s_ch[k32] == t_ch[k32]).mean()) if k32.any() else 1.0}
     fixed_ms = float(np.mean([st["ms_coarse"] + st["ms_train"] + st["ms_infer"] + st["ms_flag"]
                               for st in stats]))
-    peak_ms, _ = bh.kernel_time("bif_peaks")
-    peak_cells = sum(st["peak_only"] + st["lle_only_recounted"] for st in stats)
+    peak_ms = bh.kernel_time("bif_peaks2")[0] + bh.kernel_time("bif_peaks")[0]
+    peak_cells = bh.kernel_cells("bif_peaks2") + bh.kernel_cells("bif_peaks")
     lle32_ms, _ = bh.kernel_time("lle32")
-    lle32_cells = sum(st["lle_only"] for st in stats) if args.lle32_margin >= 0 else 0
+    lle32_cells = bh.kernel_cells("lle32")
     cost = {"class": peak_ms / max(peak_cells, 1)}
     cost["chaos"] = lle32_ms / lle32_cells if lle32_cells and lle32_ms > 0 else cost["class"]
 

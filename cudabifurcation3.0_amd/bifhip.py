@@ -66,6 +66,7 @@ def _declare(L):
         "bifhip_set_cluster_clamp": [_I], "bifhip_set_verbose": [_I],
         "bifhip_set_timing": [_I],
         "bifhip_kernel_time": [C.c_char_p, _PD, C.POINTER(_LL)],
+        "bifhip_kernel_cells": [C.c_char_p, _PD],
         "bifhip_release": [], "bifhip_device_count": [],
         "bifhip_bifurcation1D": drv_bif1,
         "bifhip_bifurcation2DKDE": drv_kde,
@@ -266,6 +267,13 @@ def kernel_time(name):
     n = C.c_longlong(0)
     _check(load().bifhip_kernel_time(name.encode(), C.byref(ms), C.byref(n)))
     return ms.value, n.value
+
+
+def kernel_cells(name):
+    """Cells covered by the timed launches of a kernel family."""
+    c = C.c_double(0)
+    _check(load().bifhip_kernel_cells(name.encode(), C.byref(c)))
+    return c.value
 
 
 def reset_kernel_times():

@@ -45,6 +45,7 @@ hipError_t launch_dbscan_block(int nList, const int* list, const int* counts, co
                                const double* ivls, long long rowStride, double T, int* out,
                                const int* outIndex, hipStream_t st);
 int dbscan_block_max_peaks();
+const char* bif_kernel_family(int model, int precision, long long nCells, bool check);
 hipError_t launch_gen_bif(int model, const BifArgs& A, int wv, bool check, int* counts,
                           double* amps, double* ivls, double* last, hipStream_t st);
 hipError_t launch_osc_flags(int n, const int* counts, const double* amps, long long rowStride,
@@ -79,11 +80,13 @@ Ctx g;
 struct TimerSlot {
     double ms = 0;
     long long launches = 0;
+    double cells = 0;
 };
 std::map<std::string, TimerSlot> g_times;
 struct PendingEv {
     std::string name;
     hipEvent_t a, b;
+    double cells;
 };
 std::vector<PendingEv> g_pending;
 
@@ -98,9 +101,9 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-void add_pending_timer(const char* name, hipEvent_t a, hipEvent_t b)
+void add_pending_timer(const char* name, hipEvent_t a, hipEvent_t b, double cells)
 {
-    g_pending.push_back({name, a, b});
+    g_pending.push_back({name, a, b, cells});
 }
 
 void resolve_timers()
@@ -111,6 +114,7 @@ void resolve_timers()
             auto& s = g_times[p.name];
             s.ms += ms;
             s.launches += 1;
+            s.cells += p.cells;
         }
         hipEventDestroy(p.a);
         hipEventDestroy(p.b);
@@ -316,7 +320,7 @@ int integrate_peaks(const PeakMapSpec& S, int base, const int* d_list, int n, in
     for (long long off = 0; off < n; off += chunk) {
         const int m = (int)std::min<long long>(chunk, n - off);
         BifArgs A = bif_args(S, base + (int)off, d_list ? d_list + off : nullptr, m, cap);
-        Timed t("bif_peaks");
+        Timed t(bif_kernel_family(g.model, S.precision, m, S.maxValue != 0), nullptr, m);
         HIP_TRY(launch_bif(g.model, S.precision, A, S.wv, S.maxValue != 0, d_counts + off,
                            d_amps ? d_amps + off * cap : nullptr,
                            d_ivls ? d_ivls + off * cap : nullptr, g.stream));
@@ -471,7 +475,7 @@ int dbscan_full_rows(int n, const int* d_list, int nRows, const int* d_counts,
         }
     }
     {
-        Timed tm("dbscan");
+        Timed tm("dbscan", nullptr, n);
         HIP_TRY(launch_dbscan_block(n, d_list, d_counts, d_amps, d_ivls, rowStride, T, d_out,
                                     d_outIndex, g.stream));
     }
@@ -483,7 +487,7 @@ int dbscan_full_rows(int n, const int* d_list, int nRows, const int* d_counts,
         HIP_TRY(hipMemcpy(drow.p, bigRow.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(dout.p, bigOut.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
         HIP_TRY(scr.alloc(sizeof(int) * (size_t)nb * 2 * rowStride));
-        Timed tm("dbscan");
+        Timed tm("dbscan", nullptr, nb);
         HIP_TRY(launch_dbscan_lane(nb, drow.as<int>(), d_counts, d_amps, d_ivls, rowStride, T,
                                    scr.as<int>(), 2LL * rowStride, d_out, dout.as<int>(),
                                    g.stream));
@@ -520,7 +524,7 @@ int cluster_map_device(const PeakMapSpec& S, const ClusterParams& C, int* d_out)
         },
         [&](long long off, int n, const PeakRows& r, hipStream_t st) {
             if (C.kind == 0) {
-                Timed t("dbscan", st);
+                Timed t("dbscan", st, n);
                 HIP_TRY(launch_dbscan_wave(n, r.counts, r.amps, r.ivls, cap, cap, C.maxPeaks, T,
                                            d_out + off, d_ovf, d_novf, (int)off, st));
             } else {
@@ -757,7 +761,7 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
             Timed t("ls");
             HIP_TRY(launch_ls(g.model, A, S.maxValue != 0, d_out + off * D, g.stream));
         } else {
-            Timed t(g.precision == BIFHIP_PREC_FP32 ? "lle32" : "lle");
+            Timed t(g.precision == BIFHIP_PREC_FP32 ? "lle32" : "lle", nullptr, A.nCells);
             HIP_TRY(launch_lle(g.model, g.precision, A, S.maxValue != 0, d_out + off, g.stream));
         }
         if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
@@ -864,7 +868,7 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
             F.dirs = d_dirs;
         }
         {
-            Timed t("lle_peaks");
+            Timed t("lle_peaks", nullptr, m);
             HIP_TRY(launch_lle_peaks(g.model, S.wv, F, d_lle + off, r.counts, r.amps, r.ivls,
                                      g.stream));
         }
@@ -872,7 +876,7 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
         return BIFHIP_OK;
     };
     auto cluster = [&](long long off, int m, const PeakRows& r, hipStream_t st) -> int {
-        Timed t("dbscan", st);
+        Timed t("dbscan", st, m);
         HIP_TRY(launch_dbscan_wave(m, r.counts, r.amps, r.ivls, cap, cap, maxPeaks, T,
                                    d_peak + off, d_ovf, d_novf, (int)off, st));
         return BIFHIP_OK;
@@ -917,7 +921,7 @@ static int peak_cells_begin(const PeakMapSpec& S, int maxPeaks, double epsDb, co
                                    r.counts, r.amps, r.ivls);
         },
         [&](long long off, int m, const PeakRows& r, hipStream_t st) {
-            Timed t("dbscan", st);
+            Timed t("dbscan", st, m);
             HIP_TRY(launch_dbscan_wave(m, r.counts, r.amps, r.ivls, cap, cap, maxPeaks, T,
                                        d_peak + off, d_ovf, d_novf, (int)off, st));
             return BIFHIP_OK;
@@ -1048,7 +1052,7 @@ int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const d
             A.dirs = d_dirs;
         }
         {
-            Timed t(precision == BIFHIP_PREC_FP32 ? "lle32" : "lle");
+            Timed t(precision == BIFHIP_PREC_FP32 ? "lle32" : "lle", nullptr, m);
             HIP_TRY(launch_lle(g.model, precision, A, false, d_lle + off, g.stream));
         }
         if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
@@ -1441,6 +1445,14 @@ int bifhip_kernel_time(const char* kernel, double* total_ms, long long* launches
     auto it = g_times.find(kernel);
     if (total_ms) *total_ms = it == g_times.end() ? 0.0 : it->second.ms;
     if (launches) *launches = it == g_times.end() ? 0 : it->second.launches;
+    return BIFHIP_OK;
+}
+
+int bifhip_kernel_cells(const char* kernel, double* cells)
+{
+    if (!kernel || !cells) return fail(BIFHIP_E_ARG, "null argument");
+    auto it = g_times.find(kernel);
+    *cells = it == g_times.end() ? 0.0 : it->second.cells;
     return BIFHIP_OK;
 }
 

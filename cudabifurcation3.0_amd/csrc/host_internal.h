@@ -55,15 +55,19 @@ int fail(int code, const char* fmt, ...);
         if (rc_ != BIFHIP_OK) return rc_;  \
     } while (0)
 
-// HIP events around a launch family when timing is on (bifhip_kernel_time).
-void add_pending_timer(const char* name, hipEvent_t a, hipEvent_t b);
+// HIP events around a launch family when timing is on (bifhip_kernel_time);
+// `cells` is the work the launch covers, summed per family
+// (bifhip_kernel_cells).
+void add_pending_timer(const char* name, hipEvent_t a, hipEvent_t b, double cells);
 void resolve_timers();
 
 struct Timed {
     const char* name;
     hipStream_t st;
+    double cells;
     hipEvent_t a = nullptr, b = nullptr;
-    explicit Timed(const char* n, hipStream_t s = nullptr) : name(n), st(s ? s : g.stream)
+    explicit Timed(const char* n, hipStream_t s = nullptr, double c = 0)
+        : name(n), st(s ? s : g.stream), cells(c)
     {
         if (!g.timing) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
@@ -76,7 +80,7 @@ struct Timed {
     {
         if (!a) return;
         (void)hipEventRecord(b, st);
-        add_pending_timer(name, a, b);
+        add_pending_timer(name, a, b, cells);
     }
 };
 

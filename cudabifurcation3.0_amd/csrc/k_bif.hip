@@ -33,11 +33,13 @@ namespace bifhip {
 #define BIFHIP_PEAK_BATCH 4
 #endif
 constexpr int kPeakBatch = BIFHIP_PEAK_BATCH;
-// peaks staged in LDS per lane before they go to the row: 4 in the fused
-// kernel (one cell per lane), 2 per cell in the two-cell peak kernel (16 KiB
-// per workgroup either way)
+// peaks staged in LDS per lane before they go to the row, 12 bytes each
+// (amplitude and sample gap): 4 in the fused kernel (one cell per lane, 12
+// KiB per workgroup), 4 per cell in the two-cell peak kernel (24 KiB: six
+// workgroups of 256 lanes per CU, near the seven its registers allow; with
+// 2 per cell its rows took 2.0x their bytes in HBM writes, with 4 1.08x)
 constexpr int kStage = 4;
-constexpr int kStage2 = 2;
+constexpr int kStage2 = 4;
 
 template <class M, int WV>
 __device__ __forceinline__ double pick(double x0, double x1, double x2)
@@ -243,9 +245,9 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
     }
     for (; s < A.skip; ++s) M::step2(x0, x1, x2, p, y0, y1, y2, q, h);
 
-    // two cells per lane: 2 peaks per cell staged (16 KiB per workgroup keeps
-    // the occupancy that the registers allow)
-    __shared__ double s_stage[256 * 2 * 2 * kStage2];
+    // two cells per lane, kStage2 peaks staged per cell
+    __shared__ double s_stage[256 * 2 * kStage2];
+    __shared__ int s_gap[256 * 2 * kStage2];
     PeakSink sa, sb;
     sa.amps = amps ? amps + (long long)c0 * A.rowStride : nullptr;
     sa.ivls = ivls ? ivls + (long long)c0 * A.rowStride : nullptr;
@@ -253,8 +255,10 @@ __global__ __launch_bounds__(256) void k_bif_peaks2(const BifArgs A, int* __rest
     sb.ivls = ivls && two ? ivls + (long long)c1 * A.rowStride : nullptr;
     sa.cap = sb.cap = A.cap;
     sa.hpf = sb.hpf = A.hpf;
-    sa.stage = (amps || ivls) ? s_stage + threadIdx.x * 4 * kStage2 : nullptr;
-    sb.stage = sa.stage ? sa.stage + 2 * kStage2 : nullptr;
+    sa.stage = (amps || ivls) ? s_stage + threadIdx.x * 2 * kStage2 : nullptr;
+    sb.stage = sa.stage ? sa.stage + kStage2 : nullptr;
+    sa.stageGap = s_gap + threadIdx.x * 2 * kStage2;
+    sb.stageGap = sa.stageGap + kStage2;
     sa.stageN = sb.stageN = kStage2;
     TrajPeakStream pa, pb;
     pa.init();
@@ -891,13 +895,15 @@ __global__ __launch_bounds__(256) void k_lle_peaks(const FusedArgs F, double* __
     const double h = A.h, eps = F.eps;
 
 
-    __shared__ double s_stage[256 * 2 * kStage];   // 16 KiB: 4 peaks per lane
+    __shared__ double s_stage[256 * kStage];   // 4 peaks per lane
+    __shared__ int s_gap[256 * kStage];
     PeakSink sink;
     sink.amps = amps + (long long)t * A.rowStride;
     sink.ivls = ivls + (long long)t * A.rowStride;
     sink.cap = A.cap;
     sink.hpf = A.hpf;
-    sink.stage = s_stage + threadIdx.x * 2 * kStage;
+    sink.stage = s_stage + threadIdx.x * kStage;
+    sink.stageGap = s_gap + threadIdx.x * kStage;
     sink.stageN = kStage;
     TrajPeakStream ps;
     ps.init();
@@ -1063,6 +1069,14 @@ static hipError_t launch_bif_m(const BifArgs& A, int wv, bool check, bool fp32, 
     }
 #undef BIF_CASE
     return hipGetLastError();
+}
+
+// Timer family of the kernel launch_bif runs for these arguments:
+// "bif_peaks2" for the two-cell FP64 kernel, else "bif_peaks".
+const char* bif_kernel_family(int model, int precision, long long nCells, bool check)
+{
+    const bool two = model != kMemristive && precision != 1 && !check && nCells >= kPairCells;
+    return two ? "bif_peaks2" : "bif_peaks";
 }
 
 hipError_t launch_gen_bif(int model, const BifArgs& A, int wv, bool check, int* counts,

@@ -42,37 +42,40 @@ namespace bifhip {
 // comparison gives what it gives on the widened samples, and only the peaks
 // written to the rows are widened.
 
-// Where a lane's peaks go.  With a stage (a per-lane LDS buffer of
-// 2*stageN doubles: stageN amplitudes, then stageN intervals, stageN a power
-// of two), peaks collect there and reach the row stageN at a time, as
-// back-to-back stores into one stretch of the row: a lane's peaks arrive
-// hundreds of steps apart, and single 8-byte stores into rows 2 KiB apart
-// leave partly written sectors to be evicted from L2 (about 3.8x the bytes).
+// Where a lane's peaks go.  With a stage (per-lane LDS buffers of stageN
+// amplitudes and stageN sample gaps, stageN a power of two), peaks collect
+// there and reach the row stageN at a time, as back-to-back stores into one
+// stretch of the row: a lane's peaks arrive hundreds of steps apart, and
+// single 8-byte stores into rows 2 KiB apart leave partly written sectors to
+// be evicted from L2 (about 3.8x the bytes).  An interval is staged as its
+// integer sample gap and written as (double)gap * hpf, the operation
+// peakFinder's interval is, so the stage holds 12 bytes per peak.
 struct PeakSink {
     double* amps;     // row base (nullptr: count only)
     double* ivls;     // row base (nullptr: no intervals)
     int cap;          // slots in the row
     double hpf;       // peakFinder's h (0.01 by default)
-    double* stage = nullptr;   // LDS stage of this lane (nullptr: direct stores)
+    double* stage = nullptr;   // LDS amplitudes of this lane (nullptr: direct stores)
+    int* stageGap = nullptr;   // LDS sample gaps of this lane
     int stageN = 1;
 
-    __device__ __forceinline__ void put(int slot, double amp, double ivl) const
+    __device__ __forceinline__ void put(int slot, double amp, int gap) const
     {
         if (!stage) {
             if (amps) amps[slot] = amp;
-            if (ivls) ivls[slot] = ivl;
+            if (ivls) ivls[slot] = (double)gap * hpf;
             return;
         }
         const int k = slot & (stageN - 1);
         stage[k] = amp;
-        stage[stageN + k] = ivl;
+        stageGap[k] = gap;
         if (k == stageN - 1) flush(slot - k, stageN);
     }
     __device__ __forceinline__ void flush(int base, int n) const
     {
         for (int k = 0; k < n; ++k) {
             if (amps) amps[base + k] = stage[k];
-            if (ivls) ivls[base + k] = stage[stageN + k];
+            if (ivls) ivls[base + k] = (double)stageGap[k] * hpf;
         }
     }
 };
@@ -106,7 +109,7 @@ struct PeakStreamT {
     {
         if (found >= 1) {
             const int slot = found - 1;
-            if (slot < s.cap) s.put(slot, (double)amp, (double)(j - lastj) * s.hpf);
+            if (slot < s.cap) s.put(slot, (double)amp, j - lastj);
         }
         lastj = j;
         ++found;

@@ -84,6 +84,10 @@ int bifhip_set_timing(int on);
  * ("bif_peaks", "dbscan", "kde", "lle", "ls", ...) since the last reset;
  * kernel == NULL resets all counters. */
 int bifhip_kernel_time(const char* kernel, double* total_ms, long long* launches);
+/* Cells the timed launches of a kernel family covered since the last reset
+ * (families that count them: "bif_peaks", "bif_peaks2" -- the two-cell FP64
+ * peak kernel --, "lle_peaks", "lle", "lle32", "dbscan"). */
+int bifhip_kernel_cells(const char* kernel, double* cells);
 /* Release cached device workspaces. */
 int bifhip_release(void);
 const char* bifhip_last_error(void);

@@ -342,6 +342,23 @@ def test_dbscan_map_two_cells_per_lane(orc, bh):
         np.testing.assert_array_equal(m[row], want, err_msg=f"row {row}")
 
 
+def test_kernel_timers_count_cells(bh):
+    """bifhip_kernel_time / bifhip_kernel_cells: a 1024^2 map's peak pass is
+    one two-cell launch ("bif_peaks2") of 2^20 cells, a 64^2 map's one
+    single-chain launch ("bif_peaks") of 4096 cells; every cell is
+    clustered once ("dbscan")."""
+    bh.set_timing(True)
+    for n, fam in ((1024, "bif_peaks2"), (64, "bif_peaks")):
+        bh.reset_kernel_times()
+        bh.dbscan_map(10.0, n, 0.01, IC, R2, [0, 2], 0, 0.0, 100000, 5.0, VALS, 1, 0.01)
+        ms, launches = bh.kernel_time(fam)
+        say("timer", fam, n, ms, launches, bh.kernel_cells(fam))
+        assert launches == 1 and ms > 0 and bh.kernel_cells(fam) == n * n
+        assert bh.kernel_cells("dbscan") == n * n
+        other = "bif_peaks" if fam == "bif_peaks2" else "bif_peaks2"
+        assert bh.kernel_time(other)[1] == 0
+
+
 def test_dbscan_overflow_recompute(orc, bh):
     """Rows of 8 slots force every cell through the exact recompute path."""
     say("dbscan overflow recompute")

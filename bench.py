@@ -752,13 +752,22 @@ def run_surrogate(bh, np, args, out, n, d_peak, d_lle):
 
     variants = {}
     kv = max(1, min(args.variant_steps, len(bufs)))
+    def breakdown(k, vst):
+        fams = ("lle_peaks", "bif_peaks2", "bif_peaks", "lle32", "lle", "dbscan", "mlp_train",
+                "mlp_infer")
+        kern = {f: {"ms_per_step": bh.kernel_time(f)[0] / k, "cells_per_step": bh.kernel_cells(f) / k}
+                for f in fams if bh.kernel_time(f)[1]}
+        phases = {p_: float(np.mean([st[p_] for st in vst]))
+                  for p_ in ("ms_coarse", "ms_train", "ms_infer", "ms_flag", "ms_solve")}
+        return {"kernels": kern, "phase_ms": phases}
+
     if args.variant_steps > 0:
-        bh.set_timing(False)
         try:
             bh.set_lle32_margin(-1.0)
             vms, vst = run(kv, args.seed_base, 1)
             variants["all_fp64_resolves"] = {
                 "ms_per_step": vms, "speedup_vs_plain_fp64": plain_ms / vms, "steps": kv,
+                **breakdown(kv, vst),
                 "agreement": {k: summarise(np, scored(kv), k) for k in ("regime", "chaos",
                                                                          "class")},
                 "note": "every regime re-solve in FP64 (lle32 margin < 0)"}
@@ -778,6 +787,7 @@ def run_surrogate(bh, np, args, out, n, d_peak, d_lle):
                 sc = scored(kv)
                 variants[name] = {
                     "ms_per_step": vms, "speedup_vs_plain_fp64": plain_ms / vms, "steps": kv,
+                    **breakdown(kv, vst),
                     "chaos_confidence": conf, "quad_confidence": args.quad_conf,
                     "lle32_margin": margin, "validation": val, "batch": batch,
                     "agreement": {k: summarise(np, sc, k)

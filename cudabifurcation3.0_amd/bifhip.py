@@ -446,19 +446,21 @@ def LLE2DIC(*a):
 def bifurcation1D_to_buffer(tMax, nPts, h, initialConditions, ranges, indicesOfMutVars,
                             writableVar, maxValue, transientTime, values, preScaller, stride=None):
     """Counts and peaks per point; peaks[k, :counts[k]].  Without a stride,
-    rows of 1024 slots are tried first and rows that hold every peak a
-    record can have (N/2 + 1) only when a point has more peaks."""
+    rows of 1024 slots are tried first; when a point has more peaks the
+    library refuses them with every count filled and keeps the diagram, and
+    the retry with the stride max(counts) copies it (no second
+    integration)."""
     ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
-    N = int(tMax / h / preScaller)
-    full = max(1, N // 2 + 1)
-    for st in ([stride] if stride else sorted({min(full, 1024), full})):
+    st = stride or 1024
+    for attempt in range(2):
         counts = np.zeros(nPts, dtype=np.int32)
         peaks = np.zeros(nPts * st)
         rc = load().bifhip_bifurcation1D_to_buffer(tMax, nPts, h, len(ic), _pd(ic), _pd(r),
                                                    _pi(m), writableVar, maxValue, transientTime,
                                                    _pd(v), len(v), preScaller, _pi(counts),
                                                    _pd(peaks), st)
-        if rc == E_LIMIT and st < full and not stride:
+        if rc == E_LIMIT and not stride and attempt == 0:
+            st = int(counts.max())
             continue
         _check(rc)
         return counts, peaks.reshape(nPts, st)

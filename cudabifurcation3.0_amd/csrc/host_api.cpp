@@ -1824,21 +1824,61 @@ int bifhip_bifurcation1D_to_buffer(double tMax, int nPts, double h, int nIC, con
                                    int* counts, double* peaks, int peakStride)
 {
     if (!counts) return fail(BIFHIP_E_ARG, "null counts");
+    // A call refused for its short stride keeps its diagram; the caller's
+    // retry with the same arguments and the stride that max(counts) gives
+    // copies it instead of integrating again.
+    static struct {
+        std::vector<unsigned char> key;
+        std::vector<int> c;
+        std::vector<double> p;
+        int stride = 0;
+    } kept;
+    std::vector<unsigned char> key;
+    auto put = [&](const void* q, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(q);
+        key.insert(key.end(), b, b + n);
+    };
+    const int dimension = 1;
+    put(&tMax, sizeof tMax); put(&nPts, sizeof nPts); put(&h, sizeof h); put(&nIC, sizeof nIC);
+    put(&wv, sizeof wv); put(&maxValue, sizeof maxValue); put(&transientTime, sizeof transientTime);
+    put(&nVal, sizeof nVal); put(&pre, sizeof pre); put(&g.model, sizeof g.model);
+    put(&g.precision, sizeof g.precision); put(&g.sweepIC, sizeof g.sweepIC);
+    put(&g.peakCap, sizeof g.peakCap);
+    if (ic && nIC > 0) put(ic, sizeof(double) * (size_t)nIC);
+    if (ranges) put(ranges, sizeof(double) * 2 * dimension);
+    if (mut) put(mut, sizeof(int) * dimension);
+    if (values && nVal > 0) put(values, sizeof(double) * (size_t)nVal);
     std::vector<int> c;
     std::vector<double> p;
     int stride = 0;
-    RC_TRY(bif1d_impl(tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime, values,
-                      nVal, pre, c, p, stride));
+    if (!kept.key.empty() && kept.key == key) {
+        c.swap(kept.c);
+        p.swap(kept.p);
+        stride = kept.stride;
+    } else {
+        RC_TRY(bif1d_impl(tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, transientTime,
+                          values, nVal, pre, c, p, stride));
+    }
+    kept.key.clear();
+    kept.c.clear();
+    kept.p.clear();
+    int most = 0;
     for (int k = 0; k < nPts; ++k) {
         counts[k] = c[k];
-        if (peaks) {
-            if (c[k] > peakStride)
-                return fail(BIFHIP_E_LIMIT, "cell %d has %d peaks > peakStride %d", k, c[k],
-                            peakStride);
+        most = std::max(most, c[k]);
+    }
+    if (peaks && most > peakStride) {
+        kept.key.swap(key);
+        kept.c.swap(c);
+        kept.p.swap(p);
+        kept.stride = stride;
+        return fail(BIFHIP_E_LIMIT, "a cell has %d peaks > peakStride %d (counts are filled)",
+                    most, peakStride);
+    }
+    if (peaks)
+        for (int k = 0; k < nPts; ++k)
             for (int j = 0; j < c[k]; ++j)
                 peaks[(long long)k * peakStride + j] = p[(long long)k * stride + j];
-        }
-    }
     return BIFHIP_OK;
 }
 

@@ -152,6 +152,27 @@ def test_bifurcation1d_full_rows_round(orc, bh):
     assert e.value.code == bh.E_LIMIT
 
 
+def test_bifurcation1d_long_record_integrates_once(bh):
+    """A record with more than 1024 peaks in a point: the binding's first
+    call (1024-slot rows) is refused with every count filled, and its retry
+    with the stride max(counts) copies the kept diagram: as many peak-kernel
+    launches as one call with that stride, and the same result."""
+    args = (6000.0, 8, 0.01, IC, [2.0, 7.0], [2], 0, 0.0, 100.0, [0.2, 0.2, 0.0], 1)
+    say("bifurcation1D long record")
+    bh.set_timing(True)
+    bh.reset_kernel_times()
+    cnt, pk = bh.bifurcation1D_to_buffer(*args)
+    first = bh.kernel_time("bif_peaks")[1]
+    assert cnt.max() > 1024 and pk.shape[1] == cnt.max()
+    bh.reset_kernel_times()
+    cnt2, pk2 = bh.bifurcation1D_to_buffer(*args, stride=int(cnt.max()))
+    once = bh.kernel_time("bif_peaks")[1]
+    say("peak-kernel launches", first, "vs", once)
+    assert first == once > 0
+    np.testing.assert_array_equal(cnt, cnt2)
+    np.testing.assert_array_equal(pk, pk2)
+
+
 def test_bifurcation1d_file_matches_reference_format(orc, bh, tmp_path):
     say("bifurcation1D file")
     out = tmp_path / "mat.csv"

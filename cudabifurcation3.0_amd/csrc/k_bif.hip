@@ -384,181 +384,231 @@ constexpr int kDbR = 4;                  // register sets: P <= 64*kDbR
 constexpr int kDbIsoSeeds = 2;           // isolated seeds before the isolated-peak pass
 constexpr int kDbMaxP = kWave * kDbR;
 
-// Bits of 64-bit word w (global indices 64w .. 64w+63) with index >= i,
-// and with index < q.
-__device__ __forceinline__ uint64_t word_ge(int w, int i)
+// Returned by dbscan_wave_cell if its machine has not finished after 4P+64
+// steps.  It finishes in at most 3P+4 (each step seeds, pops or labels), so
+// the bound only guarantees that the loop ends; no count can equal it.
+constexpr int kDbUnfinished = -0x7fffffff - 1;
+
+// Isolated peaks of one cell: bit k%64 of Z[k/64] is set when peak k < P has
+// no neighbour among the valid peaks j < P-1, j != k.  Lanes hold the peaks
+// (a, b); anchor j is read from the lane that holds it.  An anchor is tested
+// exactly only when another lane's amplitude lies within B of its own:
+// |dx| >= B = sqrt(T)(1 + 2^-20) makes (float)dx squared, and so the
+// distance sum, exceed T, so such a pair is never near.  With T <= 0 no pair
+// is near and every peak is isolated.
+template <int NS>
+__device__ __forceinline__ void isolated_peaks(int P, double T, const double (&a)[NS],
+                                               const double (&b)[NS], const int (&kw)[NS],
+                                               uint64_t (&Z)[NS])
 {
-    const int d = i - w * kWave;
-    return d <= 0 ? ~0ull : (d >= kWave ? 0ull : (~0ull << d));
+    const int last = P - 1;
+    int cnt[NS];                       // near anchors j != k of the lane's peak
+#pragma unroll
+    for (int w = 0; w < NS; ++w) cnt[w] = 0;
+    if (T > 0.0) {
+        const double B = sqrt(T) * (1.0 + 0x1p-20);
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int jend = min(kWave, last - r * kWave);
+            uint64_t self = 1;                     // the anchor's own lane in word r
+            for (int lj = 0; lj < jend; ++lj, self <<= 1) {
+                const double aj = readlane_d(a[r], lj);
+                uint64_t other = 0;
+#pragma unroll
+                for (int w = 0; w < NS; ++w) {
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(!(fabs(a[w] - aj) >= B));
+                    other |= w == r ? (m & ~self) : m;
+                }
+                if (other != 0) {
+                    const double bj = readlane_d(b[r], lj);
+                    const int j = r * kWave + lj;
+#pragma unroll
+                    for (int w = 0; w < NS; ++w)
+                        cnt[w] += (near(aj, bj, a[w], b[w], T) && kw[w] != j) ? 1 : 0;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < NS; ++w) Z[w] = __builtin_amdgcn_ballot_w64(kw[w] < P && cnt[w] == 0);
 }
-__device__ __forceinline__ uint64_t word_lt(int w, int q) { return ~word_ge(w, q); }
+
+// Peak k's value v held in lane k%64 of register set k/64 (k wave-uniform).
+template <int NS>
+__device__ __forceinline__ double peak_value(const double (&v)[NS], int k)
+{
+    double x = readlane_d(v[0], k & (kWave - 1));
+#pragma unroll
+    for (int w = 1; w < NS; ++w)
+        if ((k >> 6) == w) x = readlane_d(v[w], k & (kWave - 1));
+    return x;
+}
 
 // DBSCAN of one cell by one wavefront, specialised on the number NS of
-// 64-peak words (P <= 64*NS) so the scalar mask logic has no idle words.
-// Peak k lives in lane k%64 of register set k/64.  The label machine is
-// wave-uniform (scalar registers).  The scalar unit, shared by the CU's
-// SIMDs, bounded the version that built every neighbour row up front, so:
+// 64-peak words (P <= 64*NS).  Peak k lives in lane k%64 of register set
+// k/64.  The label machine is wave-uniform, one 64-bit mask per word:
+//   U  unlabelled peaks k < P-1, the only ones a scan can absorb (whether
+//      P-1 is still unlabelled is the flag lastFree, whether it is isolated
+//      the flag zlast)
+//   S  stack slots that hold their own index
+//   Z  isolated peaks, once they are known (else 0)
+// Every mask that depends on an index (the indices below nn, those at or
+// after i, a range, all but one index) is the ballot of one vector compare
+// of the lane's peak index, so the scalar unit, which the CU's waves share,
+// only combines masks and finds bits.
 //  - a neighbour row is built, one ballot per word, only when the machine
 //    scans it: for a cluster seed or a popped index; nothing is scanned
 //    once every absorbable index is labelled.  A period-1 orbit scans one
 //    row, a period-2 orbit about a quarter of them.
 //  - after kDbIsoSeeds seeds with empty rows (a chaotic cell, whose peaks
-//    are mostly isolated) the isolated peaks are found at once by the
-//    vector lanes -- each lane counts the anchors j < P-1 near its peak,
-//    every anchor read from the lane that holds it -- and from then on
-//    runs of isolated seeds are labelled without scans.
+//    are mostly isolated) the isolated peaks are found at once
+//    (isolated_peaks), and from then on runs of isolated seeds are labelled
+//    without scans.
 template <int NS>
 __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
                                                 const double* __restrict__ rb, int P, double T,
                                                 int lane)
 {
+    // lanes past the last peak get an infinite amplitude: never near and
+    // never a candidate of the isolated-peak test
     double a[NS], b[NS];
-    uint64_t valid[NS];               // k < P-1: the reference never absorbs the last peak
-#pragma unroll
-    for (int r = 0; r < NS; ++r) {
-        const int k = r * kWave + lane;
-        a[r] = k < P ? ra[k] : 0.0;
-        b[r] = k < P ? rb[k] : 0.0;
-        valid[r] = __builtin_amdgcn_ballot_w64(k < P - 1);
-    }
-    uint64_t Z[NS];                   // isolated peaks, once found (else 0)
-    uint64_t U[NS], S[NS];
+    int kw[NS];                       // the lane's peak index in word w
 #pragma unroll
     for (int w = 0; w < NS; ++w) {
-        U[w] = word_lt(w, P);
+        kw[w] = w * kWave + lane;
+        a[w] = kw[w] < P ? ra[kw[w]] : __builtin_inf();
+        b[w] = kw[w] < P ? rb[kw[w]] : 0.0;
+    }
+    const int last = P - 1;
+    uint64_t U[NS], S[NS], Z[NS];
+#pragma unroll
+    for (int w = 0; w < NS; ++w) {
+        U[w] = __builtin_amdgcn_ballot_w64(kw[w] < last);
         S[w] = 0;
         Z[w] = 0;
     }
-    int isoSeeds = 0;                  // seeds with empty rows before Z is known
-    // The reference visits i = 0, 1, ... and after each stack pop jumps to
-    // i = slot (or 0); an empty-stack step at a labelled index changes
-    // nothing.  Steps that provably change nothing but i and nn are taken in
-    // bulk, with the state the reference reaches after them:
+    // The reference's loop (i = 0 .. P-1, a stack pop jumping to i = slot or
+    // 0, then i + 1) runs as two phases.  Steps that provably change nothing
+    // but i and nn are taken in bulk, with the state the reference reaches
+    // after them:
     //  - rem counts the unlabelled indices below P-1, the only ones a scan
     //    can absorb.  At rem == 0 pops and scans change no label, P-1 is
     //    never absorbed (scans test k < P-1) and the walk still reaches it
     //    (a popped slot is at most P-2), so it opens one more cluster
-    //    exactly when unlabelled.  A scan at rem == 0 is skipped.
-    //  - A pop of an unset slot sets i = 0 and scans row 0.  Row 0 was
-    //    scanned first or is empty (the walk starts at 0; any later scan
-    //    needs rem > 0 after it), so it has no unlabelled neighbour left:
-    //    the run of unset slots above the highest set slot below nn is
-    //    popped at once, leaving i = 1.
-    //  - With an empty stack the walk visits the unlabelled indices >= i in
+    //    exactly when unlabelled: the machine stops there.
+    //  - walk (empty stack): the unlabelled indices >= i are visited in
     //    order.  An isolated index (empty row) only opens a cluster of its
     //    own, so once Z is known every isolated unlabelled index before the
     //    next non-isolated one is labelled at once (with Z = 0 the next
-    //    unlabelled index is the seed).
-    int nn = 0, cluster = 0, i = 0, rem = P - 1;
-    bool lastFree = true;
-    while (i < P) {
-        if (rem == 0) {
-            if (lastFree) ++cluster;
-            break;
+    //    unlabelled index is the seed).  A seed at P-1 ends the reference's
+    //    loop after its scan.
+    //  - pops: slot nn-1 is popped.  A pop of an unset slot sets i = 0 and
+    //    scans row 0, which was scanned first (the walk starts at 0), so it
+    //    has no unlabelled neighbour left: the run of unset slots above the
+    //    highest set slot hs below nn is popped at once, leaving i = 1, and
+    //    hs is popped next (or, with no set slot, the walk resumes at 1).
+    bool lastFree = true, zlast = false;
+    int isoSeeds = 0;                  // seeds with empty rows before Z is known
+    int nn = 0, cluster = 0, i = 0, rem = last;
+    int guard = 4 * P + 64;
+    for (;;) {
+        // walk: nn == 0, rem > 0, i < P
+        if (--guard < 0) return kDbUnfinished;
+        unsigned qu = ~0u;                 // next non-isolated unlabelled >= i
+#pragma unroll
+        for (int w = 0; w < NS; ++w) {
+            const uint64_t m = (U[w] & ~Z[w]) & __builtin_amdgcn_ballot_w64(kw[w] >= i);
+            const unsigned f = m != 0 ? (unsigned)__builtin_ctzll(m) : ~0u;
+            qu = min(qu, f | (unsigned)(w * kWave));
         }
-        bool seed = false;
-        if (nn >= 1) {
-            int hs = -1;                       // highest set slot below nn
-#pragma unroll
-            for (int w = NS - 1; w >= 0; --w) {
-                const uint64_t m = S[w] & word_lt(w, nn);
-                if (hs < 0 && m != 0) hs = w * kWave + 63 - __builtin_clzll(m);
-            }
-            if (hs != nn - 1) {
-                nn = hs + 1;
-                i = 1;
-                continue;
-            }
-#pragma unroll
-            for (int w = 0; w < NS; ++w)
-                if (w == (hs >> 6)) S[w] &= ~(1ull << (hs & 63));
-            i = hs;
-            nn = hs;
-        } else {
-            int q = P;                         // next non-isolated unlabelled >= i
-#pragma unroll
-            for (int w = 0; w < NS; ++w) {
-                const uint64_t m = U[w] & ~Z[w] & word_ge(w, i);
-                if (q == P && m != 0) q = w * kWave + __builtin_ctzll(m);
-            }
+        const int q = qu != ~0u ? (int)qu : (lastFree && !zlast ? last : P);
+        if (isoSeeds < 0) {
+            // the isolated unlabelled indices in [i, q) each open a cluster
+            // of their own; P-1 is among them when q == P
             int c = 0;
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
-                const uint64_t m = U[w] & Z[w] & word_ge(w, i) & word_lt(w, q);
+                const uint64_t m = U[w] & Z[w] &
+                                   __builtin_amdgcn_ballot_w64((unsigned)(kw[w] - i) <
+                                                               (unsigned)(q - i));
                 U[w] &= ~m;
                 c += __popcll(m);
             }
-            if (c != 0) {
-                cluster += c;
-                const int wl = (P - 1) >> 6;
-                bool lastNow = false;
-#pragma unroll
-                for (int w = 0; w < NS; ++w)
-                    if (w == wl) lastNow = ((U[w] >> ((P - 1) & 63)) & 1ull) != 0;
-                if (lastFree && !lastNow) {
-                    lastFree = false;
-                    --c;
-                }
-                rem -= c;
-                if (rem == 0) continue;
+            cluster += c;
+            rem -= c;
+            if (q == P && lastFree && zlast) {
+                ++cluster;
+                lastFree = false;
             }
-            if (q >= P) break;
-            seed = true;
-            i = q;
-            ++cluster;
-#pragma unroll
-            for (int w = 0; w < NS; ++w)
-                if (w == (i >> 6)) U[w] &= ~(1ull << (i & 63));
-            if (i < P - 1) --rem;
-            else lastFree = false;
+            if (rem == 0) break;
         }
-        if (rem != 0) {
-            // scan row i: its unlabelled neighbours k < P-1 (i itself is
-            // labelled, so k != i holds through U); the anchor peak is read
-            // with scalar loads
-            const double ai = ra[i];
-            const double bi = rb[i];
-            int c = 0;
+        if (q >= P) return cluster - 1;
+        ++cluster;
+        if (q < last) {
+#pragma unroll
+            for (int w = 0; w < NS; ++w) U[w] &= __builtin_amdgcn_ballot_w64(kw[w] != q);
+            if (--rem == 0) break;
+        } else {
+            lastFree = false;
+        }
+        // scan the seed's row: its unlabelled neighbours k < P-1 (q itself
+        // is labelled, so k != q holds through U)
+        {
+            const double ai = peak_value<NS>(a, q);
+            const double bi = peak_value<NS>(b, q);
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
-                const uint64_t m =
-                    __builtin_amdgcn_ballot_w64(near(ai, bi, a[w], b[w], T)) & valid[w] & U[w];
+                const uint64_t m = __builtin_amdgcn_ballot_w64(near(ai, bi, a[w], b[w], T)) & U[w];
                 U[w] &= ~m;
                 S[w] |= m;
-                c += __popcll(m);
-            }
-            nn += c;
-            rem -= c;
-            if (seed && c == 0 && isoSeeds >= 0 && ++isoSeeds == kDbIsoSeeds) {
-                // Row k is empty when no valid j != k is near k (near is
-                // symmetric).  Lane k counts the valid anchors j near it,
-                // its own test included when k is valid (a point is near
-                // itself exactly when T > 0).
-                int cnt[NS];
-#pragma unroll
-                for (int w = 0; w < NS; ++w) cnt[w] = 0;
-#pragma unroll
-                for (int r = 0; r < NS; ++r) {
-                    const int jend = min(kWave, P - 1 - r * kWave);
-                    for (int lj = 0; lj < jend; ++lj) {
-                        const double aj = readlane_d(a[r], lj);
-                        const double bj = readlane_d(b[r], lj);
-#pragma unroll
-                        for (int w = 0; w < NS; ++w) cnt[w] += near(aj, bj, a[w], b[w], T) ? 1 : 0;
-                    }
-                }
-#pragma unroll
-                for (int w = 0; w < NS; ++w) {
-                    const int k = w * kWave + lane;
-                    const int self = (k < P - 1 && T > 0.0) ? 1 : 0;
-                    Z[w] = __builtin_amdgcn_ballot_w64(k < P && cnt[w] == self);
-                }
-                isoSeeds = -1;             // known
+                nn += __popcll(m);
             }
         }
-        ++i;
+        rem -= nn;
+        if (rem == 0) break;
+        if (q == last) return cluster - 1;     // i = P ends the reference's loop
+        if (nn == 0 && isoSeeds >= 0 && ++isoSeeds == kDbIsoSeeds) {
+            isolated_peaks<NS>(P, T, a, b, kw, Z);
+#pragma unroll
+            for (int w = 0; w < NS; ++w)
+                if (w == (last >> 6)) zlast = ((Z[w] >> (last & 63)) & 1ull) != 0;
+            isoSeeds = -1;                 // known
+        }
+        i = q + 1;
+        // pops
+        while (nn >= 1) {
+            if (--guard < 0) return kDbUnfinished;
+            int hs = -1;                   // highest set slot below nn
+#pragma unroll
+            for (int w = 0; w < NS; ++w) {
+                const uint64_t m = S[w] & __builtin_amdgcn_ballot_w64(kw[w] < nn);
+                if (m != 0) hs = w * kWave + 63 - __builtin_clzll(m);
+            }
+            if (hs < 0) {
+                nn = 0;
+                i = 1;
+                break;
+            }
+            const double ai = peak_value<NS>(a, hs);
+            const double bi = peak_value<NS>(b, hs);
+            int c = 0;
+#pragma unroll
+            for (int w = 0; w < NS; ++w) {
+                const uint64_t m = __builtin_amdgcn_ballot_w64(near(ai, bi, a[w], b[w], T)) & U[w];
+                U[w] = (U[w] & ~m);
+                S[w] = (S[w] | m) & __builtin_amdgcn_ballot_w64(kw[w] != hs);
+                c += __popcll(m);
+            }
+            nn = hs + c;
+            rem -= c;
+            if (rem == 0) break;
+            i = hs + 1;
+        }
+        if (rem == 0) break;
     }
-    return cluster - 1;
+    // rem == 0: no scan can label anything; P-1 opens a last cluster when
+    // still unlabelled
+    return cluster - 1 + (lastFree ? 1 : 0);
 }
 
 // One wavefront per cell (dbscan_wave_cell holds the peaks in registers

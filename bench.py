@@ -672,7 +672,19 @@ def kernel_rooflines(bh, args, stats, n, step_ms):
         out[name] = row
     if "k_dbscan_wave" in out:
         out["k_dbscan_wave"]["note"] = ("side stream, overlapped with the integrators; the "
-                                        "family includes the rare workgroup-kernel cells")
+                                        "family includes the rare workgroup-kernel cells; bound "
+                                        "by the label machine's dependent compare and mask "
+                                        "steps, not by HBM")
+    if "k_train_fused" in out:
+        blocks = -(-min(args.batch, args.coarse ** 2) // 64)
+        out["k_train_fused"]["note"] = (
+            f"BF16 MFMA, latency-bound: each step is {blocks} workgroups of 64 rows, one "
+            f"dependent layer after another; training is "
+            f"{100 * out['k_train_fused']['share_of_step']:.1f}% of the step")
+    if "k_infer_fused" in out:
+        out["k_infer_fused"]["note"] = (
+            f"BF16 MFMA over every fine cell; "
+            f"{100 * out['k_infer_fused']['share_of_step']:.1f}% of the step")
     if "k_bif_peaks" in out:
         out["k_bif_peaks"]["note"] = "one chain per lane: recounts and overflow recomputes"
     return out

@@ -1460,6 +1460,7 @@ int bifhip_release(void)
 {
     RC_TRY(abandon_pending_peaks());
     if (g_side.s) HIP_TRY(hipStreamSynchronize(g_side.s));  // clustering still reading rows
+    RC_TRY(release_training_staging());
     g_pool.release();
     return BIFHIP_OK;
 }

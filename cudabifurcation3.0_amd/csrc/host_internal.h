@@ -122,6 +122,8 @@ int pool_get(const char* key, size_t count, T** out)
 }
 
 int ensure_device();
+// Frees the surrogate training's pinned staging buffers (bifhip_release).
+int release_training_staging();
 int make_grid(Grid& G, int nPts, int dimension, const double* ranges, const int* mut,
               const double* ic, int nIC, const double* values, int nVal);
 int finish();

@@ -265,7 +265,57 @@ double ms_since(std::chrono::steady_clock::time_point t0)
         .count();
 }
 
+// Training uploads one shuffled sample order per epoch.  It is built in one
+// of two pinned host buffers and copied asynchronously; a buffer is written
+// again only after the copy that read it has completed (its event).  The
+// training graph is captured and replayed on one long-lived non-blocking
+// stream.
+struct TrainStaging {
+    hipStream_t cs = nullptr;
+    int* perm[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    size_t cap = 0;                   // ints per buffer
+};
+TrainStaging g_train;
+
+int train_staging(size_t n)
+{
+    TrainStaging& T = g_train;
+    if (!T.cs) HIP_TRY(hipStreamCreateWithFlags(&T.cs, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b)
+        if (!T.copied[b]) HIP_TRY(hipEventCreateWithFlags(&T.copied[b], hipEventDisableTiming));
+    if (T.cap >= n) return BIFHIP_OK;
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipEventSynchronize(T.copied[b]));
+        if (T.perm[b]) HIP_TRY(hipHostFree(T.perm[b]));
+        T.perm[b] = nullptr;
+    }
+    T.cap = 0;
+    for (int b = 0; b < 2; ++b)
+        HIP_TRY(hipHostMalloc((void**)&T.perm[b], sizeof(int) * n, hipHostMallocDefault));
+    T.cap = n;
+    return BIFHIP_OK;
+}
+
 }  // namespace
+
+namespace bifhip {
+namespace host {
+// Frees the training staging buffers once their copies have completed.
+int release_training_staging()
+{
+    TrainStaging& T = g_train;
+    if (T.cs) HIP_TRY(hipStreamSynchronize(T.cs));
+    for (int b = 0; b < 2; ++b) {
+        if (T.copied[b]) HIP_TRY(hipEventSynchronize(T.copied[b]));
+        if (T.perm[b]) HIP_TRY(hipHostFree(T.perm[b]));
+        T.perm[b] = nullptr;
+    }
+    T.cap = 0;
+    return BIFHIP_OK;
+}
+}  // namespace host
+}  // namespace bifhip
 
 extern "C" {
 
@@ -296,32 +346,50 @@ int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int 
     T.wLle = (float)wLle;
     T.chaosStd = (float)chaosStd;
     T.seed = seed;
-    hipStream_t s = g.stream;
     std::mt19937_64 rng(seed ^ 0x9e3779b97f4a7c15ull);
-    std::vector<int> perm(n);
+    RC_TRY(train_staging((size_t)n));
+    // graph replay on the capture stream, direct launches on the library's
+    hipStream_t s = useGraph ? g_train.cs : g.stream;
+    if (useGraph) {
+        // the capture stream starts after the library stream's work (labels)
+        hipEvent_t ready;
+        HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        hipError_t e1 = hipEventRecord(ready, g.stream);
+        hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(s, ready, 0) : e1;
+        (void)hipEventDestroy(ready);
+        HIP_TRY(e2);
+    }
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipStream_t cs = nullptr;
-    if (useGraph) HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     int rc = BIFHIP_OK;
     for (int e = 0; e < epochs && rc == BIFHIP_OK; ++e) {
-        std::iota(perm.begin(), perm.end(), 0);
+        // epoch e's order in staging buffer e % 2, once the copy that read
+        // it two epochs ago has completed
+        const int b = e & 1;
+        int* perm = g_train.perm[b];
+        if (hipEventSynchronize(g_train.copied[b]) != hipSuccess) {
+            rc = fail(1, "staging wait");
+            break;
+        }
+        std::iota(perm, perm + n, 0);
         for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[rng() % (uint64_t)(i + 1)]);
+        if (hipMemcpyAsync(d_perm, perm, sizeof(int) * steps * batch, hipMemcpyHostToDevice,
+                           s) != hipSuccess ||
+            hipEventRecord(g_train.copied[b], s) != hipSuccess) {
+            rc = fail(1, "order upload");
+            break;
+        }
         if (useGraph) {
-            rc = hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * steps * batch,
-                                hipMemcpyHostToDevice, cs) == hipSuccess ? BIFHIP_OK
-                                                                         : fail(1, "perm upload");
-            if (rc) break;
             if (!exec) {
-                if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                     rc = fail(1, "graph capture failed");
                     break;
                 }
                 int crc = BIFHIP_OK;
-                if (launch_fill_f32(3, N.loss, 0.0f, cs) != hipSuccess) crc = fail(1, "fill");
+                if (launch_fill_f32(3, N.loss, 0.0f, s) != hipSuccess) crc = fail(1, "fill");
                 for (int k = 0; k < steps && crc == BIFHIP_OK; ++k)
-                    crc = train_step(N, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, cs);
-                hipError_t ce = hipStreamEndCapture(cs, &graph);
+                    crc = train_step(N, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, s);
+                hipError_t ce = hipStreamEndCapture(s, &graph);
                 if (crc != BIFHIP_OK) { rc = crc; break; }
                 if (ce != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) !=
                                             hipSuccess) {
@@ -329,25 +397,21 @@ int bifhip_mlp_train_device(int nLat, const int* d_cls, const float* d_lle, int 
                     break;
                 }
             }
-            Timed t("mlp_train", cs);
-            if (hipGraphLaunch(exec, cs) != hipSuccess) rc = fail(1, "graph launch failed");
-        } else {
-            HIP_TRY(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * steps * batch,
-                                   hipMemcpyHostToDevice, s));
             Timed t("mlp_train", s);
-            HIP_TRY(launch_fill_f32(3, N.loss, 0.0f, s));
+            if (hipGraphLaunch(exec, s) != hipSuccess) rc = fail(1, "graph launch failed");
+        } else {
+            Timed t("mlp_train", s);
+            if (launch_fill_f32(3, N.loss, 0.0f, s) != hipSuccess) rc = fail(1, "fill");
             for (int k = 0; k < steps && rc == BIFHIP_OK; ++k)
                 rc = train_step(N, T, nLat, d_perm + (size_t)k * batch, d_cls, d_lle, s);
         }
     }
-    if (cs) {
-        (void)hipStreamSynchronize(cs);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipStreamDestroy(cs);
-    }
+    // the graph and its executable outlive their last replay
+    hipError_t se = hipStreamSynchronize(s);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     RC_TRY(rc);
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(se);
     if (loss_out) {
         HIP_TRY(hipMemcpy(loss_out, N.loss, sizeof(float) * 3, hipMemcpyDeviceToHost));
         if (steps > 0) {

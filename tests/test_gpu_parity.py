@@ -257,6 +257,45 @@ def test_dbscan_device_sparse_sets_bit_exact(orc, bh, dev):
         assert got[b] == orc.dbscan(amps[b, :P], ivls[b, :P], 0.01), (b, P)
 
 
+@pytest.mark.parametrize("eps", [0.0, 0.003, 0.01, 0.05, float("inf")])
+def test_dbscan_device_chaotic_sets_eps_and_nonfinite(orc, bh, dev, eps):
+    """Chaotic-like sets whose isolated peaks the wave kernel finds with its
+    amplitude prefilter: for every eps (0: no pair is near; inf: every
+    finite pair is), amplitudes on the prefilter bound sqrt(T)(1 + 2^-20)
+    and just inside it, exact duplicates, and NaN / infinite amplitudes."""
+    rng = np.random.default_rng(31)
+    sizes = [2, 3, 40, 63, 64, 65, 100, 128, 129, 190, 256]
+    nb, stride = 4 * len(sizes), 256
+    amps = np.zeros((nb, stride))
+    ivls = np.zeros((nb, stride))
+    cnt = np.zeros(nb, dtype=np.int32)
+    for b in range(nb):
+        P = sizes[b % len(sizes)]
+        a = rng.random(P) * 2.0
+        iv = rng.integers(1, 6, size=P) * 0.01
+        k = rng.integers(0, P, size=max(1, P // 6))
+        src = rng.integers(0, P, size=len(k))
+        a[k] = a[src] + rng.choice([0.0, 0.00999, 0.0100001, 0.009, 1e-9], size=len(k))
+        iv[k] = iv[src]
+        if b % 4 == 1 and P > 3:
+            a[rng.integers(0, P)] = np.nan
+            a[rng.integers(0, P)] = np.inf
+        if b % 4 == 2:
+            a[-1] = a[0]                     # the last peak is never absorbed
+            iv[-1] = iv[0]
+        amps[b, :P], ivls[b, :P], cnt[b] = a, iv, P
+    say("dbscan chaotic sets eps", eps, nb)
+    out = bh.DeviceArray.full((nb,), -99, np.int32)
+    A, I, Cn = _t(bh, amps.ravel(), np.float64), _t(bh, ivls.ravel(), np.float64), _t(
+        bh, cnt, np.int32)
+    bh.dbscan_device(A.ptr, I.ptr, stride, nb, Cn.ptr, 100000, eps, out.ptr)
+    bh.synchronize()
+    got = out.to_host()
+    for b in range(nb):
+        P = cnt[b]
+        assert got[b] == orc.dbscan(amps[b, :P], ivls[b, :P], eps), (b, P, eps)
+
+
 def test_dbscan_device_large_peak_sets(orc, bh, dev):
     """Cells beyond the wave kernel's 256 peaks: up to 16384 peaks take the
     workgroup kernel (peaks staged in LDS up to 3072, read from the rows

@@ -60,15 +60,32 @@ __device__ __forceinline__ void loss_row(const float* o, int y, float target, fl
     for (int k = kChaosCol + 1; k < kHead; ++k) d[k] = (bf16)0.0f;
 }
 
+// sum_z part[z * stride + i] over z < splits, added in z order; the loads
+// go out eight at a time so their latencies overlap (one load in flight at
+// a time made the sum wait for every partial in turn)
+__device__ __forceinline__ float sum_partials(const float* __restrict__ part, int splits,
+                                              long long stride, int i)
+{
+    float s = 0;
+    int z = 0;
+    for (; z + 8 <= splits; z += 8) {
+        float t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = part[(long long)(z + u) * stride + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += t[u];
+    }
+    for (; z < splits; ++z) s += part[(long long)z * stride + i];
+    return s;
+}
+
 // Sum of split partials in a fixed order: out[i] = sum_z part[z][i].
 __global__ void k_sum_splits(int n, int splits, const float* __restrict__ part,
                              float* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float s = 0;
-    for (int z = 0; z < splits; ++z) s += part[(long long)z * n + i];
-    out[i] = s;
+    out[i] = sum_partials(part, splits, n, i);
 }
 
 // Adam step on the FP32 master parameters with the gradient summed from its
@@ -83,26 +100,29 @@ __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, floa
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // every load is issued before the first wait: the entry's flags and
+    // state, then the partials (summed for held entries too, and unused)
+    const bool kept = keep[i] != 0;
+    const int bi = bfIndex[i];
+    const int ti = bfIndexT ? bfIndexT[i] : -1;
+    const float m0 = m[i], v0 = v[i], p0 = p[i];
+    const float gi = sum_partials(gpart, splits, stride, i);
     // bias corrections of step t (1-based) read from device memory, so a
     // captured graph replays correctly step after step
     const float t = (float)(*step);
     const float c1 = 1.0f / (1.0f - powf(b1, t));
     const float c2 = 1.0f / (1.0f - powf(b2, t));
-    if (!keep[i]) {
-        p[i] = 0;
-    } else {
-        float gi = 0;
-        for (int z = 0; z < splits; ++z) gi += gpart[(long long)z * stride + i];
-        const float mi = b1 * m[i] + (1 - b1) * gi;
-        const float vi = b2 * v[i] + (1 - b2) * gi * gi;
+    float pn = 0;
+    if (kept) {
+        const float mi = b1 * m0 + (1 - b1) * gi;
+        const float vi = b2 * v0 + (1 - b2) * gi * gi;
         m[i] = mi;
         v[i] = vi;
-        p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+        pn = p0 - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
     }
-    const int bi = bfIndex[i];
-    if (bi >= 0) wbf[bi] = (bf16)p[i];
-    const int ti = bfIndexT ? bfIndexT[i] : -1;
-    if (ti >= 0) wbfT[ti] = (bf16)p[i];
+    p[i] = pn;
+    if (bi >= 0) wbf[bi] = (bf16)pn;
+    if (ti >= 0) wbfT[ti] = (bf16)pn;
 }
 
 // ------------------------------------------------------ fused training step
@@ -159,19 +179,33 @@ __device__ __forceinline__ void store_tile(bf16* rm, int ld, bf16* tr, int u0, i
 
 // Fourier features of kRows rows (k_features' arithmetic) into the
 // row-major layout and, when xt is given, the transposed one; cell of row r
-// = list ? list[row0 + r] : base + row0 + r; rows from nrows on are zero
+// = list ? list[row0 + r] : base + row0 + r; rows from nrows on are zero.
+// Each of the workgroup's 256 threads takes one row (its cell read and split
+// into lattice coordinates once) and every fourth feature of it.
+constexpr int kFeatThreadsPerRow = 256 / kRows;
+static_assert(kFeatThreadsPerRow * kRows == 256, "features: 256 threads per workgroup");
 __device__ __forceinline__ void features_rows(const int* __restrict__ list, long long base,
                                               long long row0, int nrows, int nLat,
                                               const float* __restrict__ Bff, bf16* x, bf16* xt)
 {
     const float inv = nLat > 1 ? 1.0f / (float)(nLat - 1) : 0.0f;
-    for (int t = threadIdx.x; t < kRows * (kFeat / 2); t += blockDim.x) {
-        const int r = t / (kFeat / 2), f = t % (kFeat / 2);
+    const int r = threadIdx.x / kFeatThreadsPerRow;
+    const bool live = r < nrows;
+    float u = 0.0f, v = 0.0f;
+    if (live) {
+        const long long cell = list ? (long long)list[row0 + r] : base + row0 + r;
+        if (cell <= 0x7fffffffLL) {
+            const int c = (int)cell;
+            u = (float)(c % nLat) * inv;
+            v = (float)(c / nLat) * inv;
+        } else {
+            u = (float)(cell % nLat) * inv;
+            v = (float)(cell / nLat) * inv;
+        }
+    }
+    for (int f = threadIdx.x % kFeatThreadsPerRow; f < kFeat / 2; f += kFeatThreadsPerRow) {
         float sv = 0.0f, cv = 0.0f;
-        if (r < nrows) {
-            const long long cell = list ? (long long)list[row0 + r] : base + row0 + r;
-            const float u = (float)(cell % nLat) * inv;
-            const float v = (float)(cell / nLat) * inv;
+        if (live) {
             const float ph = 6.283185307179586f * (Bff[2 * f] * u + Bff[2 * f + 1] * v);
             sincosf(ph, &sv, &cv);
         }

@@ -143,11 +143,11 @@ def cpu_baseline(n, rows, repeats=3):
 
 # HBM bytes per cell of k_lle_peaks at the benchmark's settings, from the
 # FETCH_SIZE / WRITE_SIZE counter runs of the plain 1024^2 map scaled by the
-# same counters on a 1 GiB copy (FETCH x 2.0, WRITE x 1.0): 446 KiB fetched
+# same counters on a 1 GiB copy (FETCH x 2.0, WRITE x 1.0): 479 KiB fetched
 # and 1 495 424 KiB written per 1 048 576-cell launch.  Algorithmic bytes are
 # about 1 372 per cell (85 peaks x 16 B, plus 12 B of results): the peaks
 # reach the rows four at a time from an LDS stage, so the writes are 1.06x.
-LLE_PEAKS_HBM_BYTES_PER_CELL = (446 + 1495424) * 1024 / 1048576
+LLE_PEAKS_HBM_BYTES_PER_CELL = (479 + 1495424) * 1024 / 1048576
 
 
 def roofline(bh, cells_total, kernel="lle_peaks"):
@@ -187,11 +187,12 @@ def tiles_fp32_vs_fp64(bh, np, f32, f64):
             "spectrum_sum_rel_err_p99": float(np.quantile(ssum, 0.99))}
 
 
-def run_config4(bh, np, steps=1, warmup=0):
+def run_config4(bh, np, steps=1, warmup=0, keep=None):
     """Config 4: 4096 tiles of 16x16 cells on the 4096^2 lattice, seeded
     lengths U[200, 2000] time units, FP32 variational state, FP64
     Gram-Schmidt every 0.5; autotuned, one persistent graph-replayed launch
-    per map; then the same tiles with the FP64 tile kernel for the error."""
+    per map; then the same tiles with the FP64 tile kernel for the error.
+    `keep` (a dict) receives the tiles and the FP64 spectra."""
     nt, w, n_pts = 4096, 16, 4096
     tiles = bh.tiles_generate(nt, w, w, n_pts, 0, 1, 200.0, 2000.0, LLE["NT"])
     nsteps = int(np.asarray(tiles[2], dtype=np.int64).sum()) * int(LLE["NT"] / H) * w * w
@@ -227,6 +228,8 @@ def run_config4(bh, np, steps=1, warmup=0):
         bh.set_tile_precision(bh.PREC_FP32)
         bh.set_tile_config(block, cpl, 0)
         bh.set_tile_lanes(lanes)
+    if keep is not None:
+        keep.update(tiles=tiles, w=w, n_pts=n_pts, f64=f64)
     flop = TILE_STEP_FLOP * nsteps * steps
     ach = flop / (kms / 1000) / 1e12
     cells = nt * w * w
@@ -245,7 +248,8 @@ def run_config4(bh, np, steps=1, warmup=0):
 def run_tiles(bh, np, args, out):
     """Lyapunov spectrum of uneven refinement tiles (config 4), with the
     FP32 spectrum's error against the FP64 tile kernel."""
-    r = run_config4(bh, np, args.steps, args.warmup)
+    keep = {}
+    r = run_config4(bh, np, args.steps, args.warmup, keep)
     out.update(metric="grid points/s, uneven-tile Lyapunov spectrum (FP32 state, FP64 GS)",
                value=r["value"], unit="grid points/s", ms_per_step=r["ms_per_map"], dtype="f32",
                config={"workload": "4096 tiles of 16x16 cells on the Rossler 4096x4096 (a, c) "
@@ -255,6 +259,9 @@ def run_tiles(bh, np, args, out):
     out["fp64_tiles_ms"] = r["fp64_tiles_ms"]
     out["fp32_vs_fp64"] = r["fp32_vs_fp64"]
     out["roofline"] = r["roofline"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = tiles_cpu_baseline(np, keep["tiles"], keep["w"], keep["n_pts"],
+                                                 keep["f64"])
 
 
 LLE32_STEP_FLOP = 11         # FP32 fused-multiply-add form of the Rossler step (FMA = 2)
@@ -325,6 +332,8 @@ def run_lle(bh, np, args, out):
                        "note": f"{LLE32_STEP_FLOP} FLOP per state step, {LLE32_DEV_FLOP} per "
                                f"deviation step (FMA = 2); built without packed FP32 "
                                f"instructions"}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = lle_cpu_baseline(np, n, lle, a)
 
 
 def run_config3(bh, np):
@@ -429,6 +438,77 @@ def dbscan_cpu_baseline(np, n, got):
             "cpu_model": cpu_model(), "kind": "port",
             "sample": f"rows {rows} of the {n}x{n} map, {dt:.2f} s",
             "rows_bit_exact_vs_gpu": exact}
+
+
+def lle_cpu_baseline(np, n, lle, got64, rows=32, repeats=3):
+    """CPU baseline of the 2D LLE map: the oracle (FP64, OpenMP over cells,
+    two trajectories per cell as the reference steps them) on `rows` evenly
+    spaced whole rows, median of `repeats` runs, and its values compared
+    with the GPU FP64 map's rows (NaN cells as a class, else <= 1e-12)."""
+    import orclib as O
+    O.lib()
+    sel = list(range(0, n, max(1, n // rows)))[:rows]
+    times, want = [], None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        want = [O.lle_cells(2, lle["tMax"], lle["NT"], n, H, lle["eps"], IC, RANGES, [0, 2],
+                            0.0, lle["transient"], VALS, O.PERT_AXIS_X, 0, r * n, n)
+                for r in sel]
+        times.append(time.perf_counter() - t0)
+    got = got64.reshape(n, n)
+    nan_ok, err = True, 0.0
+    for r, w in zip(sel, want):
+        gn, wn = np.isnan(got[r]), np.isnan(w)
+        nan_ok &= bool(np.array_equal(gn, wn))
+        if (~wn).any():
+            err = max(err, float(np.abs(got[r][~wn] - w[~wn]).max()))
+    cells = len(sel) * n
+    dt = sorted(times)[len(times) // 2]
+    return {"value": cells / dt, "unit": "grid points/s", "cores": O.lib().orc_threads(),
+            "cpu_model": cpu_model(), "kind": "port",
+            "build": "gcc -O3 -ffp-contract=off -fopenmp", "runs_s": times,
+            "sample": f"{cells} cells ({len(sel)} evenly spaced rows of the {n}x{n} lattice), "
+                      f"FP64 LLE per cell, median of {repeats} runs: {dt:.1f} s",
+            "fp64_rows_vs_gpu": {"nan_cells_agree": nan_ok, "max_abs_err": err}}
+
+
+def tiles_cpu_baseline(np, tiles, w, n_pts, f64, ntiles=128, repeats=3):
+    """CPU baseline of the uneven tiles: the FP64 variational oracle (the
+    reference's Gram-Schmidt every 0.5 time units, OpenMP over cells) on
+    `ntiles` tiles spread over the length range, median of `repeats` runs,
+    and its spectra compared with the GPU FP64 tile kernel's."""
+    import orclib as O
+    O.lib()
+    oi, oj, nb = tiles
+    nt = len(nb)
+    order = np.argsort(nb)
+    sel = order[np.linspace(0, nt - 1, ntiles).astype(int)]
+    a, c, blocks = [], [], []
+    for t in sel:
+        for cell in range(w * w):
+            i, j = oi[t] + cell % w, oj[t] + cell // w
+            a.append(O.value_by_idx(j * n_pts + i, n_pts, RANGES[0], RANGES[1], 0))
+            c.append(O.value_by_idx(j * n_pts + i, n_pts, RANGES[2], RANGES[3], 1))
+            blocks.append(nb[t])
+    nNT = int(LLE["NT"] / H)
+    times, want = [], None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        want = O.ls_var_cells(a, [VALS[1]] * len(a), c, IC, H, 0, nNT, blocks)
+        times.append(time.perf_counter() - t0)
+    got = f64.reshape(nt, w * w, 3)[sel].reshape(-1, 3)
+    fo = np.isfinite(want).all(axis=1)
+    steps = int(np.asarray(blocks, dtype=np.int64).sum()) * nNT
+    dt = sorted(times)[len(times) // 2]
+    return {"value": len(a) / dt, "unit": "grid points/s", "cores": O.lib().orc_threads(),
+            "cpu_model": cpu_model(), "kind": "port",
+            "build": "gcc -O3 -ffp-contract=off -fopenmp", "runs_s": times,
+            "sample": f"{len(a)} cells ({ntiles} tiles spread over the length range, "
+                      f"{steps} integration steps), FP64 variational spectrum, median of "
+                      f"{repeats} runs: {dt:.1f} s",
+            "fp64_tiles_vs_gpu": {
+                "finite_cells_agree": bool(np.array_equal(np.isfinite(got).all(axis=1), fo)),
+                "max_abs_err": float(np.abs(got[fo] - want[fo]).max()) if fo.any() else 0.0}}
 
 
 def run_dbscan_fp32(bh, np, args, d, dargs, got64, ms64):

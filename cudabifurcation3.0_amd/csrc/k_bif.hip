@@ -433,6 +433,101 @@ __device__ __forceinline__ void isolated_peaks(int P, double T, const double (&a
     for (int w = 0; w < NS; ++w) Z[w] = __builtin_amdgcn_ballot_w64(kw[w] < P && cnt[w] == 0);
 }
 
+// Hashed amplitude buckets for the isolated-peak test: per wavefront
+// kDbBuckets chain heads (peak index + 1, 0 = empty) and one link per peak.
+constexpr int kDbBuckets = 1024;
+constexpr int kDbWalkCap = 64;          // walk steps before the all-pairs test
+
+// Value of peak j (per lane) held in lane j%64 of register set j/64, read
+// with ds_bpermute; every lane of the wave must be active.
+template <int NS>
+__device__ __forceinline__ double gather_peak(const double (&v)[NS], int j)
+{
+    const int addr = (j & (kWave - 1)) << 2;
+    double x = 0.0;
+#pragma unroll
+    for (int w = 0; w < NS; ++w) {
+        const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v[w]));
+        const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v[w]));
+        if ((j >> 6) == w) x = __hiloint2double(hi, lo);
+    }
+    return x;
+}
+
+// The isolated-peak test of isolated_peaks with hashed buckets instead of
+// all pairs.  A near pair has |dx| < B (see isolated_peaks), so its
+// amplitudes fall in the same or adjacent buckets of width 2B:
+// floor(a / 2B) differs by at most one (where a and a' are distinct and
+// near, ulp(a) < B keeps a / 2B below 2^51, so its rounding moves it by at
+// most 1/4; where ulp(a) >= B only equal amplitudes are near).  Bucket
+// floor(a / 2B) mod kDbBuckets holds a chain of the valid anchors j < P-1
+// with finite amplitudes; distant amplitudes that share a slot only add
+// candidates, each tested exactly.  Peak k walks the chains of its slot and
+// the two beside it until it meets an anchor j != k near it.  Non-finite
+// amplitudes are near nothing: they are isolated and never anchors.
+// Returns false, with Z incomplete, when some lane's walk is longer than
+// kDbWalkCap steps; the caller then runs the all-pairs test.  Requires T > 0 and every lane of the wave active.
+template <int NS>
+__device__ __forceinline__ bool isolated_peaks_hashed(int P, double T, const double (&a)[NS],
+                                                      const double (&b)[NS],
+                                                      const int (&kw)[NS], int lane,
+                                                      int* __restrict__ heads,
+                                                      unsigned short* __restrict__ links,
+                                                      uint64_t (&Z)[NS])
+{
+    const int last = P - 1;
+    const double invW = 0.5 / (sqrt(T) * (1.0 + 0x1p-20));
+    int slot[NS];                      // -1: no bucket (k >= P or non-finite)
+#pragma unroll
+    for (int w = 0; w < NS; ++w) {
+        const double qd = floor(a[w] * invW);
+        const bool ok = kw[w] < P && isfinite(a[w]) && isfinite(qd);
+        slot[w] = ok ? (int)(qd - (double)kDbBuckets * floor(qd * (1.0 / kDbBuckets))) : -1;
+    }
+    // empty chains, then every anchor pushed on its chain
+#pragma unroll
+    for (int i = 0; i < kDbBuckets / (4 * kWave); ++i)
+        reinterpret_cast<int4*>(heads)[i * kWave + lane] = int4{0, 0, 0, 0};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int w = 0; w < NS; ++w)
+        if (slot[w] >= 0 && kw[w] < last)
+            links[kw[w]] = (unsigned short)atomicExch(&heads[slot[w]], kw[w] + 1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // walk, one 64-peak word at a time: bucket d of {slot-1, slot, slot+1}
+    // (d = 0..2), node j (-1: this chain is exhausted)
+#pragma unroll
+    for (int w = 0; w < NS; ++w) {
+        bool done = slot[w] < 0, hit = false;
+        int d = 0;
+        int j = done ? -1 : heads[(slot[w] + kDbBuckets - 1) & (kDbBuckets - 1)] - 1;
+        for (int step = 0;; ++step) {
+            if (__builtin_amdgcn_ballot_w64(!done) == 0) break;
+            if (step == kDbWalkCap) return false;
+            if (!done && j < 0) {
+                // chain exhausted: the next bucket's head, or the end
+                if (++d == 3) done = true;
+                else j = heads[(slot[w] + d + kDbBuckets - 1) & (kDbBuckets - 1)] - 1;
+            }
+            const int jj = j < 0 ? 0 : j;
+            const double aj = gather_peak<NS>(a, jj);
+            const double bj = gather_peak<NS>(b, jj);
+            if (!done && j >= 0) {
+                if (j != kw[w] && near(aj, bj, a[w], b[w], T)) {
+                    hit = true;
+                    done = true;
+                } else {
+                    j = (int)links[j] - 1;
+                }
+            }
+        }
+        Z[w] = __builtin_amdgcn_ballot_w64(kw[w] < P && !hit);
+    }
+    return true;
+}
+
 // Peak k's value v held in lane k%64 of register set k/64 (k wave-uniform).
 template <int NS>
 __device__ __forceinline__ double peak_value(const double (&v)[NS], int k)
@@ -467,7 +562,8 @@ __device__ __forceinline__ double peak_value(const double (&v)[NS], int k)
 template <int NS>
 __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
                                                 const double* __restrict__ rb, int P, double T,
-                                                int lane)
+                                                int lane, int* __restrict__ heads,
+                                                unsigned short* __restrict__ links)
 {
     // lanes past the last peak get an infinite amplitude: never near and
     // never a candidate of the isolated-peak test
@@ -568,7 +664,9 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
         if (rem == 0) break;
         if (q == last) return cluster - 1;     // i = P ends the reference's loop
         if (nn == 0 && isoSeeds >= 0 && ++isoSeeds == kDbIsoSeeds) {
-            isolated_peaks<NS>(P, T, a, b, kw, Z);
+            if (!(T > 0.0) ||
+                !isolated_peaks_hashed<NS>(P, T, a, b, kw, lane, heads, links, Z))
+                isolated_peaks<NS>(P, T, a, b, kw, Z);
 #pragma unroll
             for (int w = 0; w < NS; ++w)
                 if (w == (last >> 6)) zlast = ((Z[w] >> (last & 63)) & 1ull) != 0;
@@ -637,6 +735,11 @@ __global__ __launch_bounds__(256, 8) void k_dbscan_wave(int nCells, const int* _
     if (cell >= nCells) return;          // wave-uniform
     const int lane = lane_id();
     const int P = __builtin_amdgcn_readfirstlane(counts[cell]);
+    // each wavefront's bucket heads and peak links (isolated_peaks_hashed)
+    __shared__ __attribute__((aligned(16))) int s_heads[4 * kDbBuckets];
+    __shared__ unsigned short s_links[4 * kDbMaxP];
+    int* heads = s_heads + (threadIdx.x / kWave) * kDbBuckets;
+    unsigned short* links = s_links + (threadIdx.x / kWave) * kDbMaxP;
     int res;
     if (P <= 0) res = 0;                 // -1 (diverged) and 0
     else if (P == 1) res = 1;
@@ -652,10 +755,10 @@ __global__ __launch_bounds__(256, 8) void k_dbscan_wave(int nCells, const int* _
         const double* ra = amps + (long long)cell * rowStride;
         const double* rb = ivls + (long long)cell * rowStride;
         const int nsets = (P + kWave - 1) / kWave;
-        if (nsets == 1) res = dbscan_wave_cell<1>(ra, rb, P, T, lane);
-        else if (nsets == 2) res = dbscan_wave_cell<2>(ra, rb, P, T, lane);
-        else if (nsets == 3) res = dbscan_wave_cell<3>(ra, rb, P, T, lane);
-        else res = dbscan_wave_cell<4>(ra, rb, P, T, lane);
+        if (nsets == 1) res = dbscan_wave_cell<1>(ra, rb, P, T, lane, heads, links);
+        else if (nsets == 2) res = dbscan_wave_cell<2>(ra, rb, P, T, lane, heads, links);
+        else if (nsets == 3) res = dbscan_wave_cell<3>(ra, rb, P, T, lane, heads, links);
+        else res = dbscan_wave_cell<4>(ra, rb, P, T, lane, heads, links);
     }
     if (lane == 0) out[cell] = res;
 }

@@ -541,34 +541,39 @@ __device__ __forceinline__ double peak_value(const double (&v)[NS], int k)
 
 // DBSCAN of one cell by one wavefront, specialised on the number NS of
 // 64-peak words (P <= 64*NS).  Peak k lives in lane k%64 of register set
-// k/64.  The label machine is wave-uniform, one 64-bit mask per word:
-//   U  unlabelled peaks k < P-1, the only ones a scan can absorb (whether
-//      P-1 is still unlabelled is the flag lastFree, whether it is isolated
-//      the flag zlast)
-//   S  stack slots that hold their own index
-//   Z  isolated peaks, once they are known (else 0)
-// Every mask that depends on an index (the indices below nn, those at or
-// after i, a range, all but one index) is the ballot of one vector compare
-// of the lane's peak index, so the scalar unit, which the CU's waves share,
-// only combines masks and finds bits.
+// k/64.  The label machine's sets are kept per peak, in the lane that holds
+// the peak, as keys that make each search one vector compare:
+//   uk  the peak's index while it is unlabelled and below P-1 (the only
+//       peaks a scan can absorb), else -1 (whether P-1 is still unlabelled
+//       is the flag lastFree, whether it is isolated the flag zlast)
+//   zf  the peak is isolated (once known)
+//   sk  the peak's index while its stack slot holds its own index, else
+//       INT_MAX
+// Updates (absorb a row, clear a popped slot or a seed, label a run) are
+// vector selects; each search (the next seed, the highest set slot below
+// nn, a count) is one ballot of one compare per word and a bit scan, so the
+// scalar unit, which the CU's waves share, only finds bits and steers the
+// loop.
 //  - a neighbour row is built, one ballot per word, only when the machine
 //    scans it: for a cluster seed or a popped index; nothing is scanned
 //    once every absorbable index is labelled.  A period-1 orbit scans one
 //    row, a period-2 orbit about a quarter of them.
 //  - after kDbIsoSeeds seeds with empty rows (a chaotic cell, whose peaks
 //    are mostly isolated) the isolated peaks are found at once
-//    (isolated_peaks), and from then on runs of isolated seeds are labelled
-//    without scans.
+//    (isolated_peaks_hashed, else isolated_peaks), and from then on runs of
+//    isolated seeds are labelled without scans.
 template <int NS>
 __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
                                                 const double* __restrict__ rb, int P, double T,
                                                 int lane, int* __restrict__ heads,
                                                 unsigned short* __restrict__ links)
 {
+    constexpr int kNoSlot = 0x7fffffff;
     // lanes past the last peak get an infinite amplitude: never near and
     // never a candidate of the isolated-peak test
     double a[NS], b[NS];
-    int kw[NS];                       // the lane's peak index in word w
+    int kw[NS], uk[NS], sk[NS];
+    bool zf[NS];
 #pragma unroll
     for (int w = 0; w < NS; ++w) {
         kw[w] = w * kWave + lane;
@@ -576,13 +581,14 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
         b[w] = kw[w] < P ? rb[kw[w]] : 0.0;
     }
     const int last = P - 1;
-    uint64_t U[NS], S[NS], Z[NS];
 #pragma unroll
     for (int w = 0; w < NS; ++w) {
-        U[w] = __builtin_amdgcn_ballot_w64(kw[w] < last);
-        S[w] = 0;
-        Z[w] = 0;
+        uk[w] = kw[w] < last ? kw[w] : -1;
+        sk[w] = kNoSlot;
+        zf[w] = false;
     }
+    bool lastFree = true, zlast = false;
+    int isoSeeds = 0;                  // seeds with empty rows before zf is known
     // The reference's loop (i = 0 .. P-1, a stack pop jumping to i = slot or
     // 0, then i + 1) runs as two phases.  Steps that provably change nothing
     // but i and nn are taken in bulk, with the state the reference reaches
@@ -594,8 +600,8 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
     //    exactly when unlabelled: the machine stops there.
     //  - walk (empty stack): the unlabelled indices >= i are visited in
     //    order.  An isolated index (empty row) only opens a cluster of its
-    //    own, so once Z is known every isolated unlabelled index before the
-    //    next non-isolated one is labelled at once (with Z = 0 the next
+    //    own, so once zf is known every isolated unlabelled index before the
+    //    next non-isolated one is labelled at once (with no zf the next
     //    unlabelled index is the seed).  A seed at P-1 ends the reference's
     //    loop after its scan.
     //  - pops: slot nn-1 is popped.  A pop of an unset slot sets i = 0 and
@@ -603,8 +609,26 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
     //    has no unlabelled neighbour left: the run of unset slots above the
     //    highest set slot hs below nn is popped at once, leaving i = 1, and
     //    hs is popped next (or, with no set slot, the walk resumes at 1).
-    bool lastFree = true, zlast = false;
-    int isoSeeds = 0;                  // seeds with empty rows before Z is known
+    // Scan of the row of peak (ai, bi): the unlabelled peaks near it leave
+    // uk and enter sk; returns how many.  A lane's distance sum is made
+    // infinite where its peak is labelled, so one compare tests both.
+    auto scan = [&](double ai, double bi) {
+        int c = 0;
+#pragma unroll
+        for (int w = 0; w < NS; ++w) {
+            const float fx = (float)(a[w] - ai);
+            const float fy = (float)(b[w] - bi);
+            const double sy = (double)fy * (double)fy;
+            double sd = fma((double)fx, (double)fx, sy);
+            sd = uk[w] >= 0 ? sd : __builtin_inf();
+            const uint64_t m = __builtin_amdgcn_ballot_w64(sd < T);
+            const bool got = ((m >> lane) & 1ull) != 0;
+            uk[w] = got ? -1 : uk[w];
+            sk[w] = got ? kw[w] : sk[w];
+            c += __popcll(m);
+        }
+        return c;
+    };
     int nn = 0, cluster = 0, i = 0, rem = last;
     int guard = 4 * P + 64;
     for (;;) {
@@ -613,7 +637,8 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
         unsigned qu = ~0u;                 // next non-isolated unlabelled >= i
 #pragma unroll
         for (int w = 0; w < NS; ++w) {
-            const uint64_t m = (U[w] & ~Z[w]) & __builtin_amdgcn_ballot_w64(kw[w] >= i);
+            const int open = zf[w] ? -1 : uk[w];
+            const uint64_t m = __builtin_amdgcn_ballot_w64(open >= i);
             const unsigned f = m != 0 ? (unsigned)__builtin_ctzll(m) : ~0u;
             qu = min(qu, f | (unsigned)(w * kWave));
         }
@@ -624,10 +649,10 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
             int c = 0;
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
-                const uint64_t m = U[w] & Z[w] &
-                                   __builtin_amdgcn_ballot_w64((unsigned)(kw[w] - i) <
-                                                               (unsigned)(q - i));
-                U[w] &= ~m;
+                const int iso = zf[w] ? uk[w] : -1;
+                const uint64_t m =
+                    __builtin_amdgcn_ballot_w64((unsigned)(iso - i) < (unsigned)(q - i));
+                uk[w] = ((m >> lane) & 1ull) != 0 ? -1 : uk[w];
                 c += __popcll(m);
             }
             cluster += c;
@@ -642,34 +667,27 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
         ++cluster;
         if (q < last) {
 #pragma unroll
-            for (int w = 0; w < NS; ++w) U[w] &= __builtin_amdgcn_ballot_w64(kw[w] != q);
+            for (int w = 0; w < NS; ++w) uk[w] = uk[w] == q ? -1 : uk[w];
             if (--rem == 0) break;
         } else {
             lastFree = false;
         }
-        // scan the seed's row: its unlabelled neighbours k < P-1 (q itself
-        // is labelled, so k != q holds through U)
-        {
-            const double ai = peak_value<NS>(a, q);
-            const double bi = peak_value<NS>(b, q);
-#pragma unroll
-            for (int w = 0; w < NS; ++w) {
-                const uint64_t m = __builtin_amdgcn_ballot_w64(near(ai, bi, a[w], b[w], T)) & U[w];
-                U[w] &= ~m;
-                S[w] |= m;
-                nn += __popcll(m);
-            }
-        }
+        // the seed's row: its unlabelled neighbours k < P-1 (q itself is
+        // labelled, so k != q holds through uk)
+        nn = scan(peak_value<NS>(a, q), peak_value<NS>(b, q));
         rem -= nn;
         if (rem == 0) break;
         if (q == last) return cluster - 1;     // i = P ends the reference's loop
         if (nn == 0 && isoSeeds >= 0 && ++isoSeeds == kDbIsoSeeds) {
+            uint64_t Z[NS];
             if (!(T > 0.0) ||
                 !isolated_peaks_hashed<NS>(P, T, a, b, kw, lane, heads, links, Z))
                 isolated_peaks<NS>(P, T, a, b, kw, Z);
 #pragma unroll
-            for (int w = 0; w < NS; ++w)
+            for (int w = 0; w < NS; ++w) {
+                zf[w] = ((Z[w] >> lane) & 1ull) != 0;
                 if (w == (last >> 6)) zlast = ((Z[w] >> (last & 63)) & 1ull) != 0;
+            }
             isoSeeds = -1;                 // known
         }
         i = q + 1;
@@ -679,7 +697,7 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
             int hs = -1;                   // highest set slot below nn
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
-                const uint64_t m = S[w] & __builtin_amdgcn_ballot_w64(kw[w] < nn);
+                const uint64_t m = __builtin_amdgcn_ballot_w64(sk[w] < nn);
                 if (m != 0) hs = w * kWave + 63 - __builtin_clzll(m);
             }
             if (hs < 0) {
@@ -687,16 +705,10 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
                 i = 1;
                 break;
             }
-            const double ai = peak_value<NS>(a, hs);
-            const double bi = peak_value<NS>(b, hs);
-            int c = 0;
 #pragma unroll
-            for (int w = 0; w < NS; ++w) {
-                const uint64_t m = __builtin_amdgcn_ballot_w64(near(ai, bi, a[w], b[w], T)) & U[w];
-                U[w] = (U[w] & ~m);
-                S[w] = (S[w] | m) & __builtin_amdgcn_ballot_w64(kw[w] != hs);
-                c += __popcll(m);
-            }
+            for (int w = 0; w < NS; ++w) sk[w] = sk[w] == hs ? kNoSlot : sk[w];
+            // hs is labelled, so the scan cannot put it back in sk
+            const int c = scan(peak_value<NS>(a, hs), peak_value<NS>(b, hs));
             nn = hs + c;
             rem -= c;
             if (rem == 0) break;

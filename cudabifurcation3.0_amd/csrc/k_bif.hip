@@ -621,11 +621,10 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
             const double sy = (double)fy * (double)fy;
             double sd = fma((double)fx, (double)fx, sy);
             sd = uk[w] >= 0 ? sd : __builtin_inf();
-            const uint64_t m = __builtin_amdgcn_ballot_w64(sd < T);
-            const bool got = ((m >> lane) & 1ull) != 0;
+            const bool got = sd < T;
             uk[w] = got ? -1 : uk[w];
             sk[w] = got ? kw[w] : sk[w];
-            c += __popcll(m);
+            c += __popcll(__builtin_amdgcn_ballot_w64(got));
         }
         return c;
     };
@@ -650,10 +649,9 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
 #pragma unroll
             for (int w = 0; w < NS; ++w) {
                 const int iso = zf[w] ? uk[w] : -1;
-                const uint64_t m =
-                    __builtin_amdgcn_ballot_w64((unsigned)(iso - i) < (unsigned)(q - i));
-                uk[w] = ((m >> lane) & 1ull) != 0 ? -1 : uk[w];
-                c += __popcll(m);
+                const bool inRun = (unsigned)(iso - i) < (unsigned)(q - i);
+                uk[w] = inRun ? -1 : uk[w];
+                c += __popcll(__builtin_amdgcn_ballot_w64(inRun));
             }
             cluster += c;
             rem -= c;

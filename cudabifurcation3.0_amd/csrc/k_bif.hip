@@ -383,6 +383,12 @@ __device__ __forceinline__ bool near(double xa, double ya, double xb, double yb,
 constexpr int kDbR = 4;                  // register sets: P <= 64*kDbR
 constexpr int kDbIsoSeeds = 2;           // isolated seeds before the isolated-peak pass
 constexpr int kDbMaxP = kWave * kDbR;
+// k_dbscan_wave workgroup size: one wavefront.  A cell's machine takes from
+// a few dozen to thousands of steps, and a workgroup keeps its LDS until its
+// last wavefront ends, so with four cells per workgroup the finished
+// wavefronts' slots stayed idle behind the slowest (2.61 against 2.26 ms per
+// 1M cells of the Rossler lattice; two per workgroup 2.39 ms).
+constexpr int kDbBlock = kWave;
 
 // Returned by dbscan_wave_cell if its machine has not finished after 4P+64
 // steps.  It finishes in at most 3P+4 (each step seeds, pops or labels), so
@@ -732,7 +738,7 @@ __device__ __forceinline__ int dbscan_wave_cell(const double* __restrict__ ra,
 // register budget that allows them (78 scalar registers, some spilled to
 // vector lanes) costs less than it gains: without the bound the kernel used
 // 112 scalar registers and ran 8% slower.
-__global__ __launch_bounds__(256, 8) void k_dbscan_wave(int nCells, const int* __restrict__ counts,
+__global__ __launch_bounds__(kDbBlock, 8) void k_dbscan_wave(int nCells, const int* __restrict__ counts,
                                                      const double* __restrict__ amps,
                                                      const double* __restrict__ ivls,
                                                      long long rowStride, int cap, int maxPeaks,
@@ -746,8 +752,8 @@ __global__ __launch_bounds__(256, 8) void k_dbscan_wave(int nCells, const int* _
     const int lane = lane_id();
     const int P = __builtin_amdgcn_readfirstlane(counts[cell]);
     // each wavefront's bucket heads and peak links (isolated_peaks_hashed)
-    __shared__ __attribute__((aligned(16))) int s_heads[4 * kDbBuckets];
-    __shared__ unsigned short s_links[4 * kDbMaxP];
+    __shared__ __attribute__((aligned(16))) int s_heads[kDbBlock / kWave * kDbBuckets];
+    __shared__ unsigned short s_links[kDbBlock / kWave * kDbMaxP];
     int* heads = s_heads + (threadIdx.x / kWave) * kDbBuckets;
     unsigned short* links = s_links + (threadIdx.x / kWave) * kDbMaxP;
     int res;
@@ -1293,7 +1299,8 @@ hipError_t launch_dbscan_wave(int nCells, const int* counts, const double* amps,
 {
     if (nCells <= 0) return hipSuccess;
     const long long threads = (long long)nCells * kWave;
-    hipLaunchKernelGGL(k_dbscan_wave, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_dbscan_wave, dim3((unsigned)((threads + kDbBlock - 1) / kDbBlock)),
+                       dim3(kDbBlock), 0, st,
                        nCells, counts, amps, ivls, rowStride, cap, maxPeaks, T, out, overflow,
                        nOverflow, listBase);
     return hipGetLastError();

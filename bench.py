@@ -143,11 +143,11 @@ def cpu_baseline(n, rows, repeats=3):
 
 # HBM bytes per cell of k_lle_peaks at the benchmark's settings, from the
 # FETCH_SIZE / WRITE_SIZE counter runs of the plain 1024^2 map scaled by the
-# same counters on a 1 GiB copy (FETCH x 2.0, WRITE x 1.0): 474 KiB fetched
+# same counters on a 1 GiB copy (FETCH x 2.0, WRITE x 1.0): 478 KiB fetched
 # and 1 495 424 KiB written per 1 048 576-cell launch.  Algorithmic bytes are
 # about 1 372 per cell (85 peaks x 16 B, plus 12 B of results): the peaks
 # reach the rows four at a time from an LDS stage, so the writes are 1.06x.
-LLE_PEAKS_HBM_BYTES_PER_CELL = (474 + 1495424) * 1024 / 1048576
+LLE_PEAKS_HBM_BYTES_PER_CELL = (478 + 1495424) * 1024 / 1048576
 
 
 def roofline(bh, cells_total, kernel="lle_peaks"):

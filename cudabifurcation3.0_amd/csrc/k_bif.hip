@@ -383,6 +383,8 @@ __device__ __forceinline__ bool near(double xa, double ya, double xb, double yb,
 constexpr int kDbR = 4;                  // register sets: P <= 64*kDbR
 constexpr int kDbIsoSeeds = 2;           // isolated seeds before the isolated-peak pass
 constexpr int kDbMaxP = kWave * kDbR;
+// most cells per launch of a one-wavefront-per-cell kernel (2^30 work-items)
+constexpr int kWaveCellsPerLaunch = 1 << 24;
 // k_dbscan_wave workgroup size: one wavefront.  A cell's machine takes from
 // a few dozen to thousands of steps, and a workgroup keeps its LDS until its
 // last wavefront ends, so with four cells per workgroup the finished
@@ -1297,13 +1299,19 @@ hipError_t launch_dbscan_wave(int nCells, const int* counts, const double* amps,
                               double T, int* out, int* overflow, int* nOverflow, int listBase,
                               hipStream_t st)
 {
-    if (nCells <= 0) return hipSuccess;
-    const long long threads = (long long)nCells * kWave;
-    hipLaunchKernelGGL(k_dbscan_wave, dim3((unsigned)((threads + kDbBlock - 1) / kDbBlock)),
-                       dim3(kDbBlock), 0, st,
-                       nCells, counts, amps, ivls, rowStride, cap, maxPeaks, T, out, overflow,
-                       nOverflow, listBase);
-    return hipGetLastError();
+    // launches of at most kWaveCellsPerLaunch cells keep the grid's work-item
+    // count (64 per cell) within 32 bits
+    for (int c0 = 0; c0 < nCells; c0 += kWaveCellsPerLaunch) {
+        const int n = min(nCells - c0, kWaveCellsPerLaunch);
+        const long long threads = (long long)n * kWave;
+        hipLaunchKernelGGL(k_dbscan_wave, dim3((unsigned)((threads + kDbBlock - 1) / kDbBlock)),
+                           dim3(kDbBlock), 0, st, n, counts + c0, amps + c0 * rowStride,
+                           ivls + c0 * rowStride, rowStride, cap, maxPeaks, T, out + c0,
+                           overflow, nOverflow, listBase + c0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_dbscan_lane(int nList, const int* list, const int* counts, const double* amps,
@@ -1321,10 +1329,21 @@ hipError_t launch_dbscan_block(int nList, const int* list, const int* counts, co
                                const double* ivls, long long rowStride, double T, int* out,
                                const int* outIndex, hipStream_t st)
 {
-    if (nList <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dbscan_block, dim3(nList), dim3(256), 0, st, nList, list, counts, amps,
-                       ivls, rowStride, T, out, outIndex);
-    return hipGetLastError();
+    // one workgroup of 256 per entry: launches of at most 2^22 entries keep
+    // the grid's work-item count within 32 bits
+    constexpr int kChunk = 1 << 22;
+    for (int c0 = 0; c0 < nList; c0 += kChunk) {
+        const int n = min(nList - c0, kChunk);
+        const long long r0 = list ? 0 : c0;      // without a list, entry t is cell t
+        hipLaunchKernelGGL(k_dbscan_block, dim3(n), dim3(256), 0, st, n,
+                           list ? list + c0 : nullptr, counts + r0, amps + r0 * rowStride,
+                           ivls + r0 * rowStride, rowStride, T,
+                           (list || outIndex) ? out : out + c0,
+                           outIndex ? outIndex + c0 : nullptr);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 int dbscan_block_max_peaks() { return kBlkMaxP; }
@@ -1334,12 +1353,18 @@ hipError_t launch_kde_wave(int nCells, const int* counts, const double* amps, lo
                            int* out, const int* outIndex, int* overflow, int* nOverflow,
                            int listBase, hipStream_t st)
 {
-    if (nCells <= 0) return hipSuccess;
-    const long long threads = (long long)nCells * kWave;
-    hipLaunchKernelGGL(k_kde_wave, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
-                       nCells, counts, amps, rowStride, cap, maxPeaks, kdeSampling, I1, I2, H,
-                       out, outIndex, overflow, nOverflow, listBase);
-    return hipGetLastError();
+    for (int c0 = 0; c0 < nCells; c0 += kWaveCellsPerLaunch) {
+        const int n = min(nCells - c0, kWaveCellsPerLaunch);
+        const long long threads = (long long)n * kWave;
+        hipLaunchKernelGGL(k_kde_wave, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
+                           n, counts + c0, amps + c0 * rowStride, rowStride, cap, maxPeaks,
+                           kdeSampling, I1, I2, H, outIndex ? out : out + c0,
+                           outIndex ? outIndex + c0 : nullptr, overflow, nOverflow,
+                           listBase + c0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_pack_rows(long long nCells, int rowCap, int stride, const double* src,

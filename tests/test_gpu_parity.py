@@ -296,6 +296,49 @@ def test_dbscan_device_chaotic_sets_eps_and_nonfinite(orc, bh, dev, eps):
         assert got[b] == orc.dbscan(amps[b, :P], ivls[b, :P], eps), (b, P, eps)
 
 
+def test_wave_kernels_past_one_launch(orc, bh, dev):
+    """More cells than one launch of the one-wavefront-per-cell DBSCAN and
+    KDE kernels covers (2^24, whose 64 work-items per cell fill 2^30): the
+    cells on both sides of the launch boundary and the last ones equal the
+    oracle's, so the second launch reads and writes the right rows."""
+    n0 = 1 << 24
+    nb, stride = n0 + 160, 4
+    rng = np.random.default_rng(13)
+    cnt = np.zeros(nb, np.int32)
+    amps = np.zeros((nb, stride))
+    ivls = np.zeros((nb, stride))
+    idx = np.r_[np.arange(n0 - 96, nb), [0, 1, 777]]
+    for c in idx:
+        P = int(rng.integers(-1, stride + 1))
+        cnt[c] = P
+        if P > 0:
+            base = rng.choice([0.5, 2.0, 6.0], size=P)
+            amps[c, :P] = base + rng.choice([0.0, 0.004, 0.02], size=P)
+            ivls[c, :P] = rng.integers(1, 4, size=P) * 0.01
+    say("wave kernels past one launch", nb)
+    A, I, Cn = _t(bh, amps.ravel(), np.float64), _t(bh, ivls.ravel(), np.float64), _t(
+        bh, cnt, np.int32)
+    out = bh.DeviceArray.full((nb,), -99, np.int32)
+    bh.dbscan_device(A.ptr, I.ptr, stride, nb, Cn.ptr, 100000, 0.01, out.ptr)
+    bh.synchronize()
+    got = out.to_host()
+    for c in idx:
+        P = int(cnt[c])
+        want = 0 if P <= 0 else orc.dbscan(amps[c, :P], ivls[c, :P], 0.01)
+        assert got[c] == want, ("dbscan", int(c), P)
+    assert not (got == -99).any()
+    out2 = bh.DeviceArray.full((nb,), -99, np.int32)
+    bh.kde_device(A.ptr, stride, nb, Cn.ptr, 100, 10, 0.0, 8.0, 0.05, out2.ptr)
+    bh.synchronize()
+    got2 = out2.to_host()
+    want2 = np.array([0 if cnt[c] <= 0 else orc.kde(amps[c, :cnt[c]], 100, 10, 0.0, 8.0, 0.05)
+                      for c in idx])
+    # expf may differ from glibc's in the last ulp: the same agreement bar as
+    # the KDE fuzz (a misplaced second launch misses most of its cells)
+    assert (got2[idx] == want2).mean() >= 0.995
+    assert not (got2 == -99).any()
+
+
 def test_dbscan_device_large_peak_sets(orc, bh, dev):
     """Cells beyond the wave kernel's 256 peaks: up to 16384 peaks take the
     workgroup kernel (peaks staged in LDS up to 3072, read from the rows

@@ -20,6 +20,8 @@ Workloads (one step = one whole map, inputs resident, outputs in HBM):
              the plain FP64 LLE map, with its error and sign agreement.
   dbscan     the 512^2 DBSCAN peak-count map (capped at 32), plain FP64,
              with sampled rows checked bit for bit against the oracle.
+  kde        the 512^2 KDE peak-count map, plain FP64, with sampled rows'
+             count agreement with the oracle.
   bif1d      the 1000-point 1D bifurcation diagram through the reference's
              driver, every cell checked against the oracle (CPU baseline).
   tiles      Lyapunov spectrum of 4096 uneven 16x16 refinement tiles (FP32
@@ -419,6 +421,66 @@ def run_dbscan(bh, np, args, out):
                        "avg_launch_ms": kms / max(nl, 1), "launches": nl,
                        "note": "12 FLOP per Rossler step, 150 000 steps per cell; no-contraction "
                                "FP64 reaches at most 50% of the FMA-counted peak"}
+
+
+KDE = {"sampling": 10, "I1": -15.0, "I2": 15.0, "smooth": 0.05}
+
+
+def run_kde(bh, np, args, out):
+    """2D bifurcation map: KDE peak count of the x peaks per cell, plain
+    FP64 trajectories, Rossler n x n over the (a, c) box, transient 1000,
+    record 500, kdeSampling 10 over [-15, 15], smoothing 0.05.  The CPU
+    baseline leg times the oracle on three rows and states their count
+    agreement with the map (the oracle's expf is glibc's, so KDE counts are
+    compared as agreement)."""
+    n = args.n or 512
+    cells = n * n
+    d = bh.DeviceArray((cells,), np.int32)
+    kargs = (d.ptr, PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(), PEAK["transient"],
+             VALS, 1, KDE["sampling"], KDE["I1"], KDE["I2"], KDE["smooth"])
+    for _ in range(args.warmup):
+        bh.kde_map_device(*kargs)
+    bh.synchronize()
+    bh.set_timing(True)
+    bh.reset_kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bh.kde_map_device(*kargs)
+    bh.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    fam = "bif_peaks2" if bh.kernel_time("bif_peaks2")[1] else "bif_peaks"
+    kms, nl = bh.kernel_time(fam)
+    kdems, _ = bh.kernel_time("kde")
+    got = d.to_host().reshape(n, n)
+    ach = BIF_FLOP_PER_CELL * cells * args.steps / (kms / 1000) / 1e12
+    out.update(metric="grid points/s, 2D KDE peak-count map (plain FP64)",
+               value=cells / (ms / 1000), unit="grid points/s", ms_per_step=ms, dtype="f64",
+               config={"workload": f"2D bifurcation map, Rossler {n}x{n} (a, c) lattice, KDE "
+                                   f"peak count of x, plain FP64",
+                       "grid": n, "ranges": RANGES, "h": H, "peak": PEAK, "kde": KDE})
+    out["kernels_ms_per_step"] = {fam: kms / args.steps, "kde": kdems / args.steps}
+    if not args.no_cpu_baseline:
+        import orclib as O
+        O.lib()
+        rows = [0, n // 3, n - 1]
+        t0 = time.perf_counter()
+        wants = [O.kde_cells(PEAK["tMax"], n, H, IC, RANGES, [0, 2], 0, 0.0, n_rec(),
+                             PEAK["transient"], VALS, 1, KDE["sampling"], KDE["I1"], KDE["I2"],
+                             KDE["smooth"], r * n, n) for r in rows]
+        dt = time.perf_counter() - t0
+        agree = float(np.mean([np.mean(got[r] == w) for r, w in zip(rows, wants)]))
+        out["cpu_baseline"] = {"value": len(rows) * n / dt, "unit": "grid points/s",
+                               "cores": O.lib().orc_threads(), "cpu_model": cpu_model(),
+                               "kind": "port", "sample": f"rows {rows} of the {n}x{n} map, "
+                                                         f"{dt:.2f} s",
+                               "rows_count_agreement_vs_gpu": agree}
+    out["roofline"] = {"bound": "valu_fp64", "kernel": "k_" + fam, "achieved": ach,
+                       "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                       "frac": ach / (FP64_VALU_PEAK / 1e12), "traffic": None,
+                       "avg_launch_ms": kms / max(nl, 1), "launches": nl,
+                       "note": "12 FLOP per Rossler step, 150 000 steps per cell; the KDE pass "
+                               "skips the terms that are exactly +0 (peaks beyond 15 smoothing "
+                               "widths of every sample point of a 64-point batch)"}
 
 
 def dbscan_cpu_baseline(np, n, got):
@@ -979,7 +1041,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="surrogate",
-                    choices=["surrogate", "plain", "lle", "tiles", "dbscan", "bif1d", "copy"])
+                    choices=["surrogate", "plain", "lle", "tiles", "dbscan", "kde", "bif1d", "copy"])
     ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
     ap.add_argument("--coarse", type=int, default=256)
     ap.add_argument("--nt", type=float, default=LLE["NT"],
@@ -1055,7 +1117,7 @@ def main():
            "data": "synthetic (a, c) parameter lattice of the Rossler system"}
 
     extra = {"tiles": run_tiles, "copy": run_copy, "lle": run_lle, "dbscan": run_dbscan,
-             "bif1d": run_bif1d}
+             "kde": run_kde, "bif1d": run_bif1d}
     if args.workload in extra:
         extra[args.workload](bh, np, args, out)
         print(json.dumps(out), flush=True)

@@ -88,6 +88,7 @@ def _declare(L):
         "bifhip_LLE_to_buffer": [_I] + drv_ly + [_PD],
         "bifhip_LS_to_buffer": [_I] + drv_ly + [_PD],
         "bifhip_bifurcation2DDBSCAN_device": drv_db + [_VP],
+        "bifhip_bifurcation2DKDE_device": drv_kde + [_VP],
         "bifhip_LLE_device": [_I] + drv_ly + [_VP],
         "bifhip_peak_finder_device": [_VP, _I, _I, _VP, _VP, _VP, _D],
         "bifhip_dbscan_device": [_VP, _VP, _I, _I, _VP, _I, _D, _VP],
@@ -512,6 +513,16 @@ def ls_map(dimension, tMax, NT, nPts, h, eps, initialConditions, ranges, indices
 
 
 # ------------------------------------------------- device-level entry points
+
+def kde_map_device(d_out, tMax, nPts, h, initialConditions, ranges, indicesOfMutVars,
+                   writableVar, maxValue, maxAmountOfPeaks, transientTime, values, preScaller,
+                   kdeSampling, i1, i2, smooth):
+    ic, r, m, v = _f64(initialConditions), _f64(ranges), _i32(indicesOfMutVars), _f64(values)
+    _check(load().bifhip_bifurcation2DKDE_device(tMax, nPts, h, len(ic), _pd(ic), _pd(r), _pi(m),
+                                                 writableVar, maxValue, maxAmountOfPeaks,
+                                                 transientTime, _pd(v), len(v), preScaller,
+                                                 kdeSampling, i1, i2, smooth, d_out))
+
 
 def dbscan_map_device(d_out, tMax, nPts, h, initialConditions, ranges, indicesOfMutVars,
                       writableVar, maxValue, maxAmountOfPeaks, transientTime, values, preScaller,

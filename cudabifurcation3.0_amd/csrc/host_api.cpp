@@ -1940,6 +1940,17 @@ int bifhip_bifurcation2DDBSCAN_device(double tMax, int nPts, double h, int nIC, 
                         transientTime, values, nVal, pre, eps, 0, 0, 0, 0, d_out);
 }
 
+int bifhip_bifurcation2DKDE_device(double tMax, int nPts, double h, int nIC, const double* ic,
+                                   const double* ranges, const int* mut, int wv,
+                                   double maxValue, int maxPeaks, double transientTime,
+                                   const double* values, int nVal, int pre, int kdeSampling,
+                                   double I1, double I2, double H, int* d_out)
+{
+    if (!d_out) return fail(BIFHIP_E_ARG, "null d_out");
+    return cluster_impl(1, tMax, nPts, h, nIC, ic, ranges, mut, wv, maxValue, maxPeaks,
+                        transientTime, values, nVal, pre, 0.0, kdeSampling, I1, I2, H, d_out);
+}
+
 int bifhip_LLE_device(int dimension, double tMax, double NT, int nPts, double h, double eps,
                       const double* ic, int nIC, const double* ranges, const int* mut, int wv,
                       double maxValue, double transientTime, const double* values, int nVal,

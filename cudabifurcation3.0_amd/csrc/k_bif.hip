@@ -979,6 +979,24 @@ __device__ __forceinline__ double kde_term(double pm, double delt, double H)
     return (double)(float)exp((double)arg);
 }
 
+// A term is exactly +0 once |pm - delt| > 15|H|: then |t| > 14.99, the
+// float argument is below -112, and exp of it (< 2^-161) rounds to +0 in
+// float (expf underflows to 0 below -103.97).  Adding +0 leaves the sum
+// unchanged, so such terms may be skipped without changing any sum.  NaN
+// differences never compare greater and are evaluated; infinite ones give
+// +0 either way; H = 0 skips exactly the terms that are +0 (d / 0 = +-inf).
+__device__ __forceinline__ double kde_skip_distance(double H) { return 15.0 * fabs(H); }
+
+// Lane l receives v of lane l - k (lanes below k receive lane 0's); every
+// lane of the wave must be active.
+__device__ __forceinline__ double lane_shift_up_d(double v, int k)
+{
+    const int src = max((int)lane_id() - k, 0);
+    const int lo = __builtin_amdgcn_ds_bpermute(src * 4, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(src * 4, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
 // Rows hold up to cap peaks; cells with cap < P <= maxPeaks are appended to
 // overflow[] (listBase + cell) for recomputation.  Output to
 // out[outIndex[cell]] when outIndex is given.
@@ -1009,20 +1027,61 @@ __global__ __launch_bounds__(256) void k_kde_wave(int nCells, const int* __restr
         const double* pk = amps + (long long)cell * rowStride;
         const double k1 = (double)(kdeSampling * P);
         const double k2 = (I2 - I1) / (k1 - 1);
-        double pp = 0, p = 0, cur = 0;
+        const double X = kde_skip_distance(H);
+        double pp = 0, p = 0, cur = 0;     // the machine's last three sums
         bool strange = false;
         int result = 0;
         for (int base = 0; (double)base < k1 - 1; base += kWave) {
             const int w = base + lane;
-            double val = 0;
-            if ((double)w < k1 - 1) {
-                const double delt = w * k2 + I1;
-                for (int m = 0; m < P; ++m) val += kde_term(pk[m], delt, H);
-            }
+            // lanes 0 .. wend-1 hold sample points w < k1 - 1
             const int wend = min(kWave, (int)ceil(k1 - 1) - base);
+            const double delt = w * k2 + I1;
+            // delt is monotone in w, so with finite ends every point of the
+            // batch lies in [lo, hi].  A peak with pm - hi > X (or pm - lo
+            // < -X) then has pm - delt beyond X for every lane, and adds +0
+            // to every sum (kde_skip_distance): the wave skips it.
+            const double d0 = readlane_d(delt, 0), d1 = readlane_d(delt, wend - 1);
+            const bool bounded = isfinite(d0) && isfinite(d1) && isfinite(X);
+            const double lo = fmin(d0, d1), hi = fmax(d0, d1);
+            double val = 0;
+            for (int c0 = 0; c0 < P; c0 += kWave) {
+                const int j = c0 + lane;
+                const double pj = j < P ? pk[j] : 0.0;
+                const bool far = bounded && ((pj - hi) > X || (pj - lo) < -X);
+                uint64_t live = __builtin_amdgcn_ballot_w64(j < P && !far);
+                while (live != 0) {            // the live peaks, in the reference's order
+                    const int l = __builtin_ctzll(live);
+                    live &= live - 1;
+                    val += kde_term(readlane_d(pj, l), delt, H);
+                }
+            }
+            // The reference's local-maximum machine over this batch's sums.
+            // With no plateau open (strange) and none opening here, it only
+            // counts the points with pp < p > cur: lane l's (pp, p, cur) are
+            // the sums at w-2, w-1, w, from the lanes below it or the state.
+            const double v1 = lane_shift_up_d(val, 1), v2 = lane_shift_up_d(val, 2);
+            const double pl = lane >= 1 ? v1 : cur;
+            const double ppl = lane >= 2 ? v2 : (lane == 1 ? cur : p);
+            const bool eval = lane < wend && w >= 2 && pl > ppl;
+            if (!strange && __builtin_amdgcn_ballot_w64(eval && pl == val) == 0) {
+                result += __popcll(__builtin_amdgcn_ballot_w64(eval && pl > val));
+                if (wend >= 3) {
+                    pp = readlane_d(val, wend - 3);
+                    p = readlane_d(val, wend - 2);
+                } else if (wend == 2) {
+                    pp = cur;
+                    p = readlane_d(val, 0);
+                } else {
+                    pp = p;
+                    p = cur;
+                }
+                cur = readlane_d(val, wend - 1);
+                continue;
+            }
+            // a plateau is open or opens in this batch: step the machine
+            // sum by sum
             for (int l = 0; l < wend; ++l) {
                 const int ww = base + l;
-                if (!((double)ww < k1 - 1)) break;
                 pp = p;
                 p = cur;
                 cur = readlane_d(val, l);

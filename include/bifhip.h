@@ -312,6 +312,15 @@ int bifhip_bifurcation2DDBSCAN_device(double tMax, int nPts, double h,
                                       double transientTime, const double* values,
                                       int amountOfValues, int preScaller, double eps,
                                       int* d_out);
+int bifhip_bifurcation2DKDE_device(double tMax, int nPts, double h,
+                                   int amountOfInitialConditions,
+                                   const double* initialConditions, const double* ranges,
+                                   const int* indicesOfMutVars, int writableVar,
+                                   double maxValue, int maxAmountOfPeaks, double transientTime,
+                                   const double* values, int amountOfValues, int preScaller,
+                                   int kdeSampling, double kdeSamplesInterval1,
+                                   double kdeSamplesInterval2, double kdeSamplesSmooth,
+                                   int* d_out);
 
 int bifhip_LLE_device(int dimension, double tMax, double NT, int nPts, double h, double eps,
                       const double* initialConditions, int amountOfInitialConditions,

@@ -509,6 +509,54 @@ def test_kde_device_agreement(orc, bh, dev):
     assert agree >= 0.995
 
 
+def test_kde_device_plateaus_zero_runs_and_special_values(orc, bh, dev):
+    """Sets that drive the KDE machine's rarer rules, against the oracle:
+    sample grids aligned with symmetric peak pairs (sums exactly equal at
+    neighbouring points: plateaus that open and close across 64-point
+    batches), samples far from every peak (runs of exact-zero sums), NaN
+    and infinite peaks, H = 0 and H < 0, a reversed interval, one point per
+    peak (kdeSampling 1), and cells with more than 64 peaks."""
+    rng = np.random.default_rng(17)
+    cases = []
+    for t in range(360):
+        P = int(rng.integers(3, 150))
+        kind = t % 6
+        if kind == 0:
+            a = (rng.integers(0, 7, size=P) * 0.25).astype(np.float64)
+            cases.append((a, 2, -1.0, -1.0 + (2.0 * P - 1) * 0.125, 0.25))
+        elif kind == 1:
+            a = np.where(np.arange(P) % 2 == 1, 1.0, 3.0)
+            cases.append((a, int(rng.choice([1, 2, 3])), 0.0, 4.0, 0.5))
+        elif kind == 2:
+            a = rng.random(P) * 12
+            cases.append((a, 10, 100.0, 130.0, 0.05))
+        elif kind == 3:
+            a = rng.random(P) * 12
+            a[int(rng.integers(0, P))] = [np.nan, np.inf, -np.inf][t % 3]
+            cases.append((a, 10, -15.0, 15.0, float(rng.choice([0.05, -0.05]))))
+        elif kind == 4:
+            a = (rng.integers(0, 5, size=P) * 2.0)
+            cases.append((a, 10, 15.0, -15.0, float(rng.choice([0.0, 0.05, 0.2]))))
+        else:
+            a = rng.integers(0, 3, size=P) * 3.0 + (rng.random(P) - 0.5) * 1e-3
+            cases.append((a, int(rng.choice([1, 10, 20])), -15.0, 15.0, 0.05))
+    say("kde special sets", len(cases))
+    stride = 160
+    got = []
+    for a, ks, i1, i2, hh in cases:
+        amps = np.zeros(stride)
+        amps[:len(a)] = a
+        A, Cn = _t(bh, amps, np.float64), _t(bh, np.array([len(a)], np.int32), np.int32)
+        out = bh.DeviceArray((1,), np.int32).zero_()
+        bh.kde_device(A.ptr, stride, 1, Cn.ptr, 1000, ks, i1, i2, hh, out.ptr)
+        bh.synchronize()
+        got.append(int(out.to_host()[0]))
+    want = [orc.kde(a, 1000, ks, i1, i2, hh) for a, ks, i1, i2, hh in cases]
+    agree = float(np.mean(np.array(got) == np.array(want)))
+    say("kde special agreement", agree)
+    assert agree >= 0.995
+
+
 def test_kde_map_agreement(orc, bh):
     say("kde map 24^2")
     m = bh.kde_map(200.0, 24, 0.01, IC, R2, [0, 2], 0, 0.0, 1000, 200.0, VALS, 1, 10, -15.0,

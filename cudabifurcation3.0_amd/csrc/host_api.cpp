@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -17,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <numeric>
 #include <vector>
 
@@ -675,52 +677,132 @@ int lyap_spec(LyapSpec& S, int dimension, double tMax, double NT, int nPts, doub
     return BIFHIP_OK;
 }
 
+static void direction_of(bool ls, long long gi, long long chunk, int D, double* z)
+{
+    const long long local = chunk > 0 ? gi % chunk : gi;
+    if (!ls) {
+        const uint32_t id = (uint32_t)local;
+        double zp = 0;
+        for (int i = 0; i < D; ++i) {
+            if (g.sweepIC) {
+                // LLEKernelICCUDA, Library/cudaLibrary.cu:846
+                const int idx = (int)local;
+                const double arg = 0.2171828 * (i + 1) + idx + (0.2171828 + i * idx);
+                z[i] = sinf((float)arg) * 0.5;
+            } else {
+                const int32_t arg = (int32_t)(id * ((uint32_t)i * id + 1u) + 1u);
+                z[i] = 0.5 * (double)sinf((float)arg);
+            }
+            zp += z[i] * z[i];
+        }
+        zp = std::sqrt(zp);
+        for (int i = 0; i < D; ++i) z[i] /= zp;
+    } else {
+        const int idx = (int)local;
+        for (int j = 0; j < D; ++j) {
+            double zp = 0;
+            for (int i = 0; i < D; ++i) {
+                const double arg = 0.2171828 * (i + 1) * (j + 1) + idx + (0.2171828 + i * j * idx);
+                z[j * D + i] = sinf((float)arg) * 0.5;
+                zp += z[j * D + i] * z[j * D + i];
+            }
+            zp = std::sqrt(zp);
+            for (int i = 0; i < D; ++i) z[j * D + i] /= zp;
+        }
+    }
+}
+
+// Directions of n cells, cell t's flattened index cellIndex(t), into out
+// (D or D*D values per cell).  Each cell's values depend only on its own
+// index, so the table is filled by up to 16 host threads.
+// The table lives in pinned host memory that is reused across calls: the
+// host threads fill it and one DMA copies it (dir_upload).  dir_table waits
+// for the previous copy out of it before handing it out again.
+struct DirStaging {
+    double* p = nullptr;
+    size_t cap = 0;
+    hipEvent_t copied = nullptr;
+    bool pending = false;
+} g_dirStage;
+
+static int dir_table(size_t n, double** out)
+{
+    DirStaging& D = g_dirStage;
+    if (D.pending) {
+        HIP_TRY(hipEventSynchronize(D.copied));
+        D.pending = false;
+    }
+    if (!D.copied) HIP_TRY(hipEventCreateWithFlags(&D.copied, hipEventDisableTiming));
+    if (n > D.cap) {
+        if (D.p) HIP_TRY(hipHostFree(D.p));
+        D.p = nullptr;
+        D.cap = 0;
+        HIP_TRY(hipHostMalloc((void**)&D.p, sizeof(double) * std::max<size_t>(n, 1),
+                              hipHostMallocDefault));
+        D.cap = n;
+    }
+    *out = D.p;
+    return BIFHIP_OK;
+}
+
+static int dir_upload(double* d_dirs, size_t n, hipStream_t st)
+{
+    DirStaging& D = g_dirStage;
+    HIP_TRY(hipMemcpyAsync(d_dirs, D.p, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(D.copied, st));
+    D.pending = true;
+    return BIFHIP_OK;
+}
+
+int release_dir_staging()
+{
+    DirStaging& D = g_dirStage;
+    if (D.pending) HIP_TRY(hipEventSynchronize(D.copied));
+    D.pending = false;
+    if (D.p) HIP_TRY(hipHostFree(D.p));
+    D.p = nullptr;
+    D.cap = 0;
+    if (D.copied) HIP_TRY(hipEventDestroy(D.copied));
+    D.copied = nullptr;
+    return BIFHIP_OK;
+}
+
+// Fills the pinned table with the directions of n cells and copies it to
+// d_dirs on st.
+template <class Idx>
+static int directions_of(bool ls, int n, long long chunk, Idx cellIndex, double* d_dirs,
+                         hipStream_t st)
+{
+    const int D = model_dim(g.model);
+    const int per = ls ? D * D : D;
+    double* o = nullptr;
+    RC_TRY(dir_table((size_t)n * per, &o));
+    auto fill = [&](int t0, int t1) {
+        for (int t = t0; t < t1; ++t) direction_of(ls, cellIndex(t), chunk, D, o + (size_t)t * per);
+    };
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int T = std::max(1, std::min({16, hw, n / 8192}));
+    if (T == 1) {
+        fill(0, n);
+    } else {
+        std::vector<std::thread> th;
+        th.reserve(T);
+        for (int k = 0; k < T; ++k)
+            th.emplace_back(fill, (int)((long long)n * k / T), (int)((long long)n * (k + 1) / T));
+        for (auto& t : th) t.join();
+    }
+    return dir_upload(d_dirs, (size_t)n * per, st);
+}
+
 // Host table of reference perturbation directions for cells [off, off+n):
 // LLE: 0.5*sinf((int)(idx*(i*idx+1)+1)) normalised (Library/cudaLibrary.cu:720-732);
 //      sweeps over initial conditions: 0.5*sinf(0.2171828(i+1) + idx +
 //      (0.2171828 + i*idx)) normalised (Library/cudaLibrary.cu:844-855)
 // LS:  0.5*sinf(0.2171828(i+1)(j+1) + idx + (0.2171828 + i*j*idx)) rows
 //      normalised (Library/cudaLibrary.cu:1025-1040)
-void directions(bool ls, long long off, int n, long long chunk, std::vector<double>& out)
+int directions(bool ls, long long off, int n, long long chunk, double* d_dirs, hipStream_t st)
 {
-    const int D = model_dim(g.model);
-    const int per = ls ? D * D : D;
-    out.resize((size_t)n * per);
-    for (int t = 0; t < n; ++t) {
-        const long long gi = off + t;
-        const long long local = chunk > 0 ? gi % chunk : gi;
-        double* z = out.data() + (size_t)t * per;
-        if (!ls) {
-            const uint32_t id = (uint32_t)local;
-            double zp = 0;
-            for (int i = 0; i < D; ++i) {
-                if (g.sweepIC) {
-                    // LLEKernelICCUDA, Library/cudaLibrary.cu:846
-                    const int idx = (int)local;
-                    const double arg = 0.2171828 * (i + 1) + idx + (0.2171828 + i * idx);
-                    z[i] = sinf((float)arg) * 0.5;
-                } else {
-                    const int32_t arg = (int32_t)(id * ((uint32_t)i * id + 1u) + 1u);
-                    z[i] = 0.5 * (double)sinf((float)arg);
-                }
-                zp += z[i] * z[i];
-            }
-            zp = std::sqrt(zp);
-            for (int i = 0; i < D; ++i) z[i] /= zp;
-        } else {
-            const int idx = (int)local;
-            for (int j = 0; j < D; ++j) {
-                double zp = 0;
-                for (int i = 0; i < D; ++i) {
-                    const double arg = 0.2171828 * (i + 1) * (j + 1) + idx + (0.2171828 + i * j * idx);
-                    z[j * D + i] = sinf((float)arg) * 0.5;
-                    zp += z[j * D + i] * z[j * D + i];
-                }
-                zp = std::sqrt(zp);
-                for (int i = 0; i < D; ++i) z[j * D + i] /= zp;
-            }
-        }
-    }
+    return directions_of(ls, n, chunk, [off](int t) { return off + t; }, d_dirs, st);
 }
 
 int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* d_out)
@@ -734,7 +816,6 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
     if (!ls && g.precision == BIFHIP_PREC_FP32 && g.model == BIFHIP_MODEL_MEMRISTIVE)
         return fail(BIFHIP_E_ARG, "the FP32 LLE path supports the Rossler and linear models");
     double* d_dirs = nullptr;
-    std::vector<double> hdirs;
     if (refDirs) RC_TRY(pool_get("dirs", (size_t)std::min(chunk, cells) * per, &d_dirs));
     for (long long off = 0; off < cells; off += chunk) {
         const int n = (int)std::min(chunk, cells - off);
@@ -752,9 +833,7 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
         A.nBlocks = S.nBlocks;
         A.dirs = nullptr;
         if (refDirs) {
-            directions(ls, off, n, dirChunk, hdirs);
-            HIP_TRY(hipMemcpyAsync(d_dirs, hdirs.data(), sizeof(double) * hdirs.size(),
-                                   hipMemcpyHostToDevice, g.stream));
+            RC_TRY(directions(ls, off, n, dirChunk, d_dirs, g.stream));
             A.dirs = d_dirs;
         }
         if (ls) {
@@ -764,7 +843,6 @@ int lyap_map_device(const LyapSpec& S, bool ls, long long defaultChunk, double* 
             Timed t(g.precision == BIFHIP_PREC_FP32 ? "lle32" : "lle", nullptr, A.nCells);
             HIP_TRY(launch_lle(g.model, g.precision, A, S.maxValue != 0, d_out + off, g.stream));
         }
-        if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
     }
     return BIFHIP_OK;
 }
@@ -837,7 +915,6 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
     const double T = dbscan_threshold(epsDb);
     const bool refDirs = g.pert == BIFHIP_PERT_REFERENCE;
     const long long dirChunk = g.dirChunk >= 0 ? g.dirChunk : 200000;
-    std::vector<double> hdirs;
     if (refDirs) RC_TRY(pool_get("f_dirs", (size_t)pass * 3, &d_dirs));
 
     auto integrate = [&](long long off, int m, const PeakRows& r) -> int {
@@ -851,20 +928,14 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
         F.nBlocks = L.nBlocks;
         if (refDirs) {
             // directions of the listed cells (their flattened indices)
-            hdirs.resize((size_t)m * 3);
-            std::vector<double> one;
             std::vector<int> seg;
             if (!h_list && d_list) {
                 seg.resize((size_t)m);
                 HIP_TRY(hipMemcpy(seg.data(), d_list + off, sizeof(int) * m, hipMemcpyDeviceToHost));
             }
-            for (int i = 0; i < m; ++i) {
-                const long long gi = h_list ? h_list[off + i] : (d_list ? seg[i] : base + off + i);
-                directions(false, gi, 1, dirChunk, one);
-                std::memcpy(hdirs.data() + (size_t)i * 3, one.data(), sizeof(double) * 3);
-            }
-            HIP_TRY(hipMemcpyAsync(d_dirs, hdirs.data(), sizeof(double) * hdirs.size(),
-                                   hipMemcpyHostToDevice, g.stream));
+            RC_TRY(directions_of(false, m, dirChunk, [&](int i) -> long long {
+                return h_list ? h_list[off + i] : (d_list ? seg[i] : base + off + i);
+            }, d_dirs, g.stream));
             F.dirs = d_dirs;
         }
         {
@@ -872,7 +943,6 @@ int fused_cells_device(const PeakMapSpec& S, const LyapSpec& L, int maxPeaks, do
             HIP_TRY(launch_lle_peaks(g.model, S.wv, F, d_lle + off, r.counts, r.amps, r.ivls,
                                      g.stream));
         }
-        if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
         return BIFHIP_OK;
     };
     auto cluster = [&](long long off, int m, const PeakRows& r, hipStream_t st) -> int {
@@ -1020,7 +1090,6 @@ int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const d
     const bool refDirs = g.pert == BIFHIP_PERT_REFERENCE;
     const long long dirChunk = g.dirChunk >= 0 ? g.dirChunk : 200000;
     double* d_dirs = nullptr;
-    std::vector<double> hdirs, one;
     std::vector<int> seg;
     if (refDirs) RC_TRY(pool_get("lr_dirs", (size_t)std::min<long long>(chunk, n) * 3, &d_dirs));
     for (long long off = 0; off < n; off += chunk) {
@@ -1042,20 +1111,14 @@ int surrogate_resolve_lle(int nPts, double h, int nIC, const double* ic, const d
             // directions of the listed cells (their flattened indices)
             seg.resize((size_t)m);
             HIP_TRY(hipMemcpy(seg.data(), d_list + off, sizeof(int) * m, hipMemcpyDeviceToHost));
-            hdirs.resize((size_t)m * 3);
-            for (int i = 0; i < m; ++i) {
-                directions(false, seg[i], 1, dirChunk, one);
-                std::memcpy(hdirs.data() + (size_t)i * 3, one.data(), sizeof(double) * 3);
-            }
-            HIP_TRY(hipMemcpyAsync(d_dirs, hdirs.data(), sizeof(double) * hdirs.size(),
-                                   hipMemcpyHostToDevice, g.stream));
+            RC_TRY(directions_of(false, m, dirChunk, [&](int i) -> long long { return seg[i]; },
+                                 d_dirs, g.stream));
             A.dirs = d_dirs;
         }
         {
             Timed t(precision == BIFHIP_PREC_FP32 ? "lle32" : "lle", nullptr, m);
             HIP_TRY(launch_lle(g.model, precision, A, false, d_lle + off, g.stream));
         }
-        if (refDirs) HIP_TRY(hipStreamSynchronize(g.stream));  // host table reused
     }
     return BIFHIP_OK;
 }
@@ -1461,6 +1524,7 @@ int bifhip_release(void)
     RC_TRY(abandon_pending_peaks());
     if (g_side.s) HIP_TRY(hipStreamSynchronize(g_side.s));  // clustering still reading rows
     RC_TRY(release_training_staging());
+    RC_TRY(release_dir_staging());
     g_pool.release();
     return BIFHIP_OK;
 }

@@ -618,6 +618,42 @@ def test_ls2d_fp64_matches_oracle(orc, bh):
     _close_with_nan_class(got, want, 1e-12, "LS2D")
 
 
+def test_reference_directions_from_host_threads(orc, bh):
+    """Maps large enough that the host builds the reference perturbation
+    directions (glibc sinf) on several threads: LLE2D on 160^2 cells, LS2D
+    on 136^2 and the fused LLE + peak-count map on 160^2 equal the oracle on
+    rows spread over the threads' ranges (LLE / LS within 1e-12, peak
+    counts exactly)."""
+    n = 160
+    say("LLE2D reference directions", n)
+    got = bh.lle_map(2, 20.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 5.0, VALS).reshape(n, n)
+    for r in (0, 53, 107, 159):
+        want = orc.lle_cells(2, 20.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 5.0, VALS,
+                             orc.PERT_REFERENCE, 200000, r * n, n)
+        _close_with_nan_class(got[r], want, 1e-12, f"LLE2D row {r}")
+    m = 136
+    say("LS2D reference directions", m)
+    ls = bh.ls_map(2, 10.0, 0.5, m, 0.01, 1e-6, IC, R2, [0, 2], 0.0, 5.0, VALS)
+    for r in (0, 67, 68, 135):
+        want = orc.ls_cells(2, 10.0, 0.5, m, 0.01, 1e-6, IC, R2, [0, 2], 0.0, 5.0, VALS,
+                            orc.PERT_REFERENCE, 0, r * m, m)
+        _close_with_nan_class(ls[r * m:(r + 1) * m].ravel(), np.asarray(want).ravel(), 1e-12,
+                              f"LS2D row {r}")
+    say("fused map reference directions", n)
+    d_peak = bh.DeviceArray((n * n,), np.int32).zero_()
+    d_lle = bh.DeviceArray((n * n,), np.float64).zero_()
+    bh.lle_peak_map_device(d_peak.ptr, d_lle.ptr, n, 0.01, IC, R2, [0, 2], VALS, 10.0, 5.0, 0,
+                           100000, 0.01, 20.0, 0.5, 2.0, 1e-8)
+    f_peak, f_lle = d_peak.to_host().reshape(n, n), d_lle.to_host().reshape(n, n)
+    for r in (0, 80, 159):
+        want = orc.lle_cells(2, 20.0, 0.5, n, 0.01, 1e-8, IC, R2, [0, 2], 0.0, 2.0, VALS,
+                             orc.PERT_REFERENCE, 200000, r * n, n)
+        _close_with_nan_class(f_lle[r], want, 1e-12, f"fused LLE row {r}")
+        wp = orc.dbscan_cells(10.0, n, 0.01, IC, R2, [0, 2], 0, 0.0, 100000, 5.0, VALS, 1, 0.01,
+                              r * n, n)
+        np.testing.assert_array_equal(f_peak[r], wp, err_msg=f"fused peaks row {r}")
+
+
 def test_linear_model_closed_forms_gpu(bh):
     import math
     say("linear model closed forms")

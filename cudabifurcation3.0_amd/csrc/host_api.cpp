@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <charconv>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1143,6 +1144,29 @@ int surrogate_resolve(int nPts, double h, int nIC, const double* ic, const doubl
 
 // ------------------------------------------------------------ CSV output
 
+// A value's text as the reference's stream writes it: printf's %.*g.
+// std::to_chars in general format with a precision is specified to give
+// printf's text (checked against snprintf on 2e7 values at precisions 1, 6,
+// 12, 15 and 17) and is several times faster; non-finite values and other
+// precisions keep snprintf.
+constexpr int kFmtBuf = 384;
+
+static int fmt_g(char* t, double v, int prec)
+{
+    if (std::isfinite(v) && prec >= 1 && prec <= 17) {
+        const auto r = std::to_chars(t, t + kFmtBuf, v, std::chars_format::general, prec);
+        return (int)(r.ptr - t);
+    }
+    const int n = std::snprintf(t, kFmtBuf, "%.*g", std::min(prec, 300), v);
+    return std::min(n, kFmtBuf - 1);
+}
+
+static int fmt_d(char* t, int v)
+{
+    const auto r = std::to_chars(t, t + kFmtBuf, v);
+    return (int)(r.ptr - t);
+}
+
 struct Out {
     FILE* f = nullptr;
     std::string buf;
@@ -1157,15 +1181,13 @@ struct Out {
     }
     void num(double v, int prec)
     {
-        char t[64];
-        int n = snprintf(t, sizeof(t), "%.*g", prec, v);
-        buf.append(t, (size_t)n);
+        char t[kFmtBuf];
+        buf.append(t, (size_t)fmt_g(t, v, prec));
     }
     void num(int v)
     {
-        char t[32];
-        int n = snprintf(t, sizeof(t), "%d", v);
-        buf.append(t, (size_t)n);
+        char t[kFmtBuf];
+        buf.append(t, (size_t)fmt_d(t, v));
     }
     void str(const char* s) { buf.append(s); }
     bool flush()
@@ -1212,36 +1234,75 @@ int file_error(const std::string& path)
     return fail(BIFHIP_E_FILE, "Output file open error: %s", path.c_str());
 }
 
-int write_map_int(const std::string& path, int nPts, const double* ranges, int hdr, int prec,
-                  const int* v, long long n)
+// Writes n values with the reference's row logic.  Value i's text is
+// preceded by ", " when i > 0 and then by a line break when i % nPts == 0
+// (RowWriter's output for nPts >= 1), so ranges of the index can be
+// formatted independently: large maps are formatted by up to 16 host
+// threads, a round of ranges at a time, and written in order.
+template <class Fmt>
+static int write_rows(const std::string& path, int nPts, const double* ranges, int hdr,
+                      int prec, long long n, Fmt fmtValue)
 {
     Out o;
     if (!o.open(path, "w")) return file_error(path);
     if (hdr) header(o, ranges, prec);
-    RowWriter rw(nPts);
-    for (long long i = 0; i < n; ++i) {
-        rw.sep(o);
-        o.num(v[i]);
-        if (o.buf.size() > (1u << 22) && !o.flush()) return file_error(path);
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int T = (int)std::max(1LL, std::min<long long>(std::min(16, hw), n / 65536));
+    if (nPts <= 0 || T == 1) {
+        RowWriter rw(nPts);
+        char t[kFmtBuf];
+        for (long long i = 0; i < n; ++i) {
+            rw.sep(o);
+            o.buf.append(t, (size_t)fmtValue(i, t));
+            if (o.buf.size() > (1u << 22) && !o.flush()) return file_error(path);
+        }
+        if (!o.close()) return file_error(path);
+        return BIFHIP_OK;
+    }
+    if (!o.flush()) return file_error(path);
+    const long long part = std::min<long long>(1LL << 18, (n + T - 1) / T);  // per thread and round
+    std::vector<std::string> text(T);
+    auto fmtRange = [&](int k, long long i0, long long i1) {
+        std::string& out = text[k];
+        out.clear();
+        char t[kFmtBuf];
+        for (long long i = i0; i < i1; ++i) {
+            if (i != 0) {
+                out.append(", ", 2);
+                if (i % nPts == 0) out.push_back('\n');
+            }
+            out.append(t, (size_t)fmtValue(i, t));
+        }
+    };
+    for (long long r0 = 0; r0 < n; r0 += part * T) {
+        std::vector<std::thread> th;
+        th.reserve(T);
+        for (int k = 0; k < T; ++k) {
+            const long long i0 = std::min(n, r0 + part * k), i1 = std::min(n, i0 + part);
+            th.emplace_back(fmtRange, k, i0, i1);
+        }
+        for (auto& t : th) t.join();
+        for (int k = 0; k < T; ++k)
+            if (!text[k].empty() && fwrite(text[k].data(), 1, text[k].size(), o.f) != text[k].size())
+                return file_error(path);
     }
     if (!o.close()) return file_error(path);
     return BIFHIP_OK;
 }
 
+int write_map_int(const std::string& path, int nPts, const double* ranges, int hdr, int prec,
+                  const int* v, long long n)
+{
+    return write_rows(path, nPts, ranges, hdr, prec, n,
+                      [v](long long i, char* t) { return fmt_d(t, v[i]); });
+}
+
 int write_map_double(const std::string& path, int nPts, const double* ranges, int hdr, int prec,
                      const double* v, long long n, int stride, int offset)
 {
-    Out o;
-    if (!o.open(path, "w")) return file_error(path);
-    if (hdr) header(o, ranges, prec);
-    RowWriter rw(nPts);
-    for (long long i = 0; i < n; ++i) {
-        rw.sep(o);
-        o.num(v[i * stride + offset], prec);
-        if (o.buf.size() > (1u << 22) && !o.flush()) return file_error(path);
-    }
-    if (!o.close()) return file_error(path);
-    return BIFHIP_OK;
+    return write_rows(path, nPts, ranges, hdr, prec, n, [=](long long i, char* t) {
+        return fmt_g(t, v[i * stride + offset], prec);
+    });
 }
 
 int write_bif1d(const std::string& path, int nPts, const double* ranges, const int* counts,

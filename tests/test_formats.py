@@ -91,3 +91,36 @@ def test_map3d_layout(tmp_path):
     r, v = F.read_map3d(p)
     np.testing.assert_array_equal(r, [[0, 1], [2, 3], [4, 5]])
     np.testing.assert_array_equal(v, vol)
+
+
+def _mixed_doubles(rng, n):
+    """Magnitudes over the whole double range, subnormals, signed zeros,
+    infinities and NaN, short decimals and exact integers."""
+    v = rng.normal(size=n) * 10.0 ** rng.integers(-300, 300, size=n)
+    k = rng.integers(0, 10, size=n)
+    v[k == 1] = np.round(v[k == 1], 3)
+    v[k == 2] = rng.integers(-10 ** 6, 10 ** 6, size=int((k == 2).sum()))
+    v[k == 3] = rng.normal(size=int((k == 3).sum())) * 1e-310
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, 1.7976931348623157e308])
+    v[k == 4] = special[rng.integers(0, len(special), size=int((k == 4).sum()))]
+    return v
+
+
+def test_map_writers_threaded_text_equals_the_reference_layout(tmp_path):
+    """Maps large enough to be formatted by several host threads (1024^2
+    values; 1000 values per row, so row breaks fall inside the threads'
+    ranges): the file equals the reference writer's row logic with printf
+    %.*g values, at precisions 6 and 12, and %d for counts."""
+    rng = np.random.default_rng(5)
+    ranges = [0.05, 0.45, 2.0, 10.0]
+    p = tmp_path / "big.csv"
+    for nPts, count in ((1024, 1024 * 1024), (1000, 1000 * 1000)):
+        v = _mixed_doubles(rng, count)
+        for prec in (6, 12):
+            B.write_map_double(p, nPts, ranges, v, precision=prec)
+            want = _ref_map_text(nPts, ranges, v, prec, fmt=lambda x, q=prec: "%.*g" % (q, x))
+            assert p.read_text() == want, (nPts, prec)
+    ints = rng.integers(-2 ** 31, 2 ** 31, size=1024 * 1024).astype(np.int32)
+    ints[:3] = [0, -2 ** 31, 2 ** 31 - 1]
+    B.write_map_int(p, 1024, ranges, ints, header=True, precision=6)
+    assert p.read_text() == _ref_map_text(1024, ranges, ints, 6)

@@ -22,6 +22,8 @@ Workloads (one step = one whole map, inputs resident, outputs in HBM):
              with sampled rows checked bit for bit against the oracle.
   kde        the 512^2 KDE peak-count map, plain FP64, with sampled rows'
              count agreement with the oracle.
+  drivers    every file-writing driver of the reference at its configuration
+             size (host inputs, CSV written): wall and kernel time per call.
   bif1d      the 1000-point 1D bifurcation diagram through the reference's
              driver, every cell checked against the oracle (CPU baseline).
   tiles      Lyapunov spectrum of 4096 uneven 16x16 refinement tiles (FP32
@@ -609,6 +611,75 @@ def run_dbscan_fp32(bh, np, args, d, dargs, got64, ms64):
                          "frac": ach / (FP32_VALU_PEAK / 1e12), "launches": nl}}
 
 
+def run_drivers(bh, np, args, out):
+    """The reference's file-writing drivers (Library/hostLibrary.cuh), each
+    called as a drop-in caller would at its configuration's size: host
+    inputs in, the CSV file written.  Per driver: wall time per call, the
+    kernel time inside it and the file size.  The line's value is the LLE2D
+    driver's grid points/s (1024^2, reference perturbation directions)."""
+    import os
+    import tempfile
+    ic3 = (3, IC)
+    lle_common = (LLE["tMax"], LLE["NT"])
+    drivers = {
+        "bifurcation1D": (bh.bifurcation1D, 1000, ("bif_peaks",),
+                          (PEAK["tMax"], 1000, H, 3, IC, [2.0, 7.0], [2], 0, 0.0,
+                           PEAK["transient"], [0.2, 0.2, 0.0], 3, 1)),
+        "bifurcation2DDBSCAN": (bh.bifurcation2DDBSCAN, 512 * 512,
+                                ("bif_peaks", "bif_peaks2", "dbscan"),
+                                (PEAK["tMax"], 512, H, *ic3, RANGES, [0, 2], 0, 0.0, n_rec(),
+                                 PEAK["transient"], VALS, 3, 1, PEAK["eps"])),
+        "bifurcation2DKDE": (bh.bifurcation2DKDE, 512 * 512, ("bif_peaks", "bif_peaks2", "kde"),
+                             (PEAK["tMax"], 512, H, *ic3, RANGES, [0, 2], 0, 0.0, n_rec(),
+                              PEAK["transient"], VALS, 3, 1, KDE["sampling"], KDE["I1"],
+                              KDE["I2"], KDE["smooth"])),
+        "LLE1D": (bh.LLE1D, 1000, ("lle",),
+                  (*lle_common, 1000, H, LLE["eps"], IC, 3, [2.0, 7.0], [2], 0, 0.0,
+                   LLE["transient"], [0.2, 0.2, 0.0], 3)),
+        "LLE2D": (bh.LLE2D, 1024 * 1024, ("lle",),
+                  (*lle_common, 1024, H, LLE["eps"], IC, 3, RANGES, [0, 2], 0, 0.0,
+                   LLE["transient"], VALS, 3)),
+        "LS1D": (bh.LS1D, 1000, ("ls",),
+                 (*lle_common, 1000, H, 1e-6, IC, 3, [2.0, 7.0], [2], 0, 0.0,
+                  LLE["transient"], [0.2, 0.2, 0.0], 3)),
+        "LS2D": (bh.LS2D, 512 * 512, ("ls",),
+                 (*lle_common, 512, H, 1e-6, IC, 3, RANGES, [0, 2], 0, 0.0,
+                  LLE["transient"], VALS, 3)),
+    }
+    res = {}
+    bh.set_perturbation(bh.PERT_REFERENCE)       # the drivers' own default
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "mat.csv")
+        bh.set_output_path(path)
+        bh.set_timing(True)
+        for name, (fn, cells, fams, dargs) in drivers.items():
+            for f in os.listdir(tmp):
+                os.remove(os.path.join(tmp, f))
+            fn(*dargs)                                     # warm-up
+            bh.reset_kernel_times()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn(*dargs)
+            ms = (time.perf_counter() - t0) * 1000 / args.steps
+            kms = sum(bh.kernel_time(f)[0] for f in fams) / args.steps
+            res[name] = {"cells": cells, "ms_per_call": ms, "kernel_ms": kms,
+                         "host_ms": ms - kms,
+                         "file_mb": sum(os.path.getsize(os.path.join(tmp, f))
+                                        for f in os.listdir(tmp)) / 1e6,
+                         "grid_points_per_s": cells / (ms / 1000)}
+            print(f"driver {name}: {ms:.1f} ms per call, kernels {kms:.1f} ms", file=sys.stderr,
+                  flush=True)
+    lle = res["LLE2D"]
+    out.update(metric="grid points/s, LLE2D through the reference's file-writing driver",
+               value=lle["grid_points_per_s"], unit="grid points/s",
+               ms_per_step=lle["ms_per_call"], dtype="f64",
+               config={"workload": "the reference's drivers at their configuration sizes, host "
+                                   "inputs, CSV files written; LLE2D on the Rossler 1024x1024 "
+                                   "(a, c) lattice, transient 500, measure 1000, NT 0.5, "
+                                   "reference perturbation directions"})
+    out["drivers"] = res
+
+
 def run_bif1d(bh, np, args, out):
     """1D bifurcation diagram: local maxima of x for c over [2, 7] in 1000
     points (a = b = 0.2), transient 1000, record 500, plain FP64 through the
@@ -1041,7 +1112,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="surrogate",
-                    choices=["surrogate", "plain", "lle", "tiles", "dbscan", "kde", "bif1d", "copy"])
+                    choices=["surrogate", "plain", "lle", "tiles", "dbscan", "kde", "bif1d", "drivers",
+                             "copy"])
     ap.add_argument("--n", type=int, default=None, help="fine lattice (4096 / 1024)")
     ap.add_argument("--coarse", type=int, default=256)
     ap.add_argument("--nt", type=float, default=LLE["NT"],
@@ -1117,7 +1189,7 @@ def main():
            "data": "synthetic (a, c) parameter lattice of the Rossler system"}
 
     extra = {"tiles": run_tiles, "copy": run_copy, "lle": run_lle, "dbscan": run_dbscan,
-             "kde": run_kde, "bif1d": run_bif1d}
+             "kde": run_kde, "bif1d": run_bif1d, "drivers": run_drivers}
     if args.workload in extra:
         extra[args.workload](bh, np, args, out)
         print(json.dumps(out), flush=True)

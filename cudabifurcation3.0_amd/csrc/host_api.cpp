@@ -1247,7 +1247,7 @@ static int write_rows(const std::string& path, int nPts, const double* ranges, i
     if (!o.open(path, "w")) return file_error(path);
     if (hdr) header(o, ranges, prec);
     const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-    const int T = (int)std::max(1LL, std::min<long long>(std::min(16, hw), n / 65536));
+    const int T = (int)std::max(1LL, std::min<long long>(std::min(16, hw), n / 16384));
     if (nPts <= 0 || T == 1) {
         RowWriter rw(nPts);
         char t[kFmtBuf];
@@ -1305,21 +1305,45 @@ int write_map_double(const std::string& path, int nPts, const double* ranges, in
     });
 }
 
+// One line per peak: "parameter, peak" at %.12g.  The points' lines are
+// independent, so ranges of points are formatted by host threads and
+// written in order.
 int write_bif1d(const std::string& path, int nPts, const double* ranges, const int* counts,
                 const double* peaks, int stride)
 {
     Out o;
     if (!o.open(path, "w")) return file_error(path);
-    for (int k = 0; k < nPts; ++k) {
-        const double pv = value_by_idx(k, nPts, ranges[0], ranges[1], 0);
-        for (int j = 0; j < counts[k]; ++j) {
-            o.num(pv, 12);
-            o.str(", ");
-            o.num(peaks[(long long)k * stride + j], 12);
-            o.str("\n");
+    long long lines = 0;
+    for (int k = 0; k < nPts; ++k) lines += std::max(0, counts[k]);
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int T = (int)std::max(1LL, std::min<long long>(std::min(16, hw), lines / 8192));
+    std::vector<std::string> text(T);
+    auto fmtPoints = [&](int t, int k0, int k1) {
+        std::string& out = text[t];
+        char a[kFmtBuf], b[kFmtBuf];
+        for (int k = k0; k < k1; ++k) {
+            const int la = fmt_g(a, value_by_idx(k, nPts, ranges[0], ranges[1], 0), 12);
+            for (int j = 0; j < counts[k]; ++j) {
+                out.append(a, (size_t)la);
+                out.append(", ", 2);
+                out.append(b, (size_t)fmt_g(b, peaks[(long long)k * stride + j], 12));
+                out.push_back('\n');
+            }
         }
-        if (o.buf.size() > (1u << 22) && !o.flush()) return file_error(path);
+    };
+    if (T == 1) {
+        fmtPoints(0, 0, nPts);
+    } else {
+        std::vector<std::thread> th;
+        th.reserve(T);
+        for (int t = 0; t < T; ++t)
+            th.emplace_back(fmtPoints, t, (int)((long long)nPts * t / T),
+                            (int)((long long)nPts * (t + 1) / T));
+        for (auto& x : th) x.join();
     }
+    for (int t = 0; t < T; ++t)
+        if (!text[t].empty() && fwrite(text[t].data(), 1, text[t].size(), o.f) != text[t].size())
+            return file_error(path);
     if (!o.close()) return file_error(path);
     return BIFHIP_OK;
 }

@@ -124,3 +124,22 @@ def test_map_writers_threaded_text_equals_the_reference_layout(tmp_path):
     ints[:3] = [0, -2 ** 31, 2 ** 31 - 1]
     B.write_map_int(p, 1024, ranges, ints, header=True, precision=6)
     assert p.read_text() == _ref_map_text(1024, ranges, ints, 6)
+
+
+def test_bif1d_writer_threaded_text(tmp_path):
+    """A 1D diagram large enough to be formatted by several host threads
+    (1000 points, about 50 000 peak lines, some points with none): one line
+    "parameter, peak" per peak at %.12g, points in order."""
+    rng = np.random.default_rng(9)
+    n, stride = 1000, 120
+    counts = rng.integers(0, 100, size=n).astype(np.int32)
+    counts[::37] = 0
+    peaks = _mixed_doubles(rng, n * stride).reshape(n, stride)
+    p = tmp_path / "bif.csv"
+    B.write_bif1d(p, n, [2.0, 7.0], counts, peaks)
+    want = []
+    for k in range(n):
+        pv = "%.12g" % B.value_by_idx(k, n, 2.0, 7.0, 0)
+        for j in range(counts[k]):
+            want.append(f"{pv}, {'%.12g' % peaks[k, j]}\n")
+    assert p.read_text() == "".join(want)

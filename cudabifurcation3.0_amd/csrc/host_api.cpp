@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <numeric>
 #include <vector>
@@ -716,6 +717,29 @@ static void direction_of(bool ls, long long gi, long long chunk, int D, double* 
 // Directions of n cells, cell t's flattened index cellIndex(t), into out
 // (D or D*D values per cell).  Each cell's values depend only on its own
 // index, so the table is filled by up to 16 host threads.
+// Runs work(k) for k in [0, parts) on that many host threads; a part whose
+// thread cannot be started runs on the calling thread.
+template <class W>
+static void run_parts(int parts, W work)
+{
+    if (parts <= 1) {
+        work(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(parts);
+    int started = 0;
+    for (; started < parts; ++started) {
+        try {
+            th.emplace_back(work, started);
+        } catch (const std::system_error&) {
+            break;
+        }
+    }
+    for (int k = started; k < parts; ++k) work(k);
+    for (auto& t : th) t.join();
+}
+
 // The table lives in pinned host memory that is reused across calls: the
 // host threads fill it and one DMA copies it (dir_upload).  dir_table waits
 // for the previous copy out of it before handing it out again.
@@ -783,15 +807,9 @@ static int directions_of(bool ls, int n, long long chunk, Idx cellIndex, double*
     };
     const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
     const int T = std::max(1, std::min({16, hw, n / 8192}));
-    if (T == 1) {
-        fill(0, n);
-    } else {
-        std::vector<std::thread> th;
-        th.reserve(T);
-        for (int k = 0; k < T; ++k)
-            th.emplace_back(fill, (int)((long long)n * k / T), (int)((long long)n * (k + 1) / T));
-        for (auto& t : th) t.join();
-    }
+    run_parts(T, [&](int k) {
+        fill((int)((long long)n * k / T), (int)((long long)n * (k + 1) / T));
+    });
     return dir_upload(d_dirs, (size_t)n * per, st);
 }
 
@@ -1275,15 +1293,13 @@ static int write_rows(const std::string& path, int nPts, const double* ranges, i
         }
     };
     for (long long r0 = 0; r0 < n; r0 += part * T) {
-        std::vector<std::thread> th;
-        th.reserve(T);
-        for (int k = 0; k < T; ++k) {
+        run_parts(T, [&](int k) {
             const long long i0 = std::min(n, r0 + part * k), i1 = std::min(n, i0 + part);
-            th.emplace_back(fmtRange, k, i0, i1);
-        }
-        for (auto& t : th) t.join();
+            fmtRange(k, i0, i1);
+        });
         for (int k = 0; k < T; ++k)
-            if (!text[k].empty() && fwrite(text[k].data(), 1, text[k].size(), o.f) != text[k].size())
+            if (!text[k].empty() &&
+                fwrite(text[k].data(), 1, text[k].size(), o.f) != text[k].size())
                 return file_error(path);
     }
     if (!o.close()) return file_error(path);
@@ -1331,16 +1347,9 @@ int write_bif1d(const std::string& path, int nPts, const double* ranges, const i
             }
         }
     };
-    if (T == 1) {
-        fmtPoints(0, 0, nPts);
-    } else {
-        std::vector<std::thread> th;
-        th.reserve(T);
-        for (int t = 0; t < T; ++t)
-            th.emplace_back(fmtPoints, t, (int)((long long)nPts * t / T),
-                            (int)((long long)nPts * (t + 1) / T));
-        for (auto& x : th) x.join();
-    }
+    run_parts(T, [&](int t) {
+        fmtPoints(t, (int)((long long)nPts * t / T), (int)((long long)nPts * (t + 1) / T));
+    });
     for (int t = 0; t < T; ++t)
         if (!text[t].empty() && fwrite(text[t].data(), 1, text[t].size(), o.f) != text[t].size())
             return file_error(path);

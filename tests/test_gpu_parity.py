@@ -564,6 +564,12 @@ def test_kde_map_agreement(orc, bh):
     want = orc.kde_cells(200.0, 24, 0.01, IC, R2, [0, 2], 0, 0.0, 1000, 200.0, VALS, 1, 10,
                          -15.0, 15.0, 0.05, 0, 576)
     assert (m == want).mean() >= 0.995
+    # the device-output entry point writes the same map
+    d = bh.DeviceArray((576,), np.int32).zero_()
+    bh.kde_map_device(d.ptr, 200.0, 24, 0.01, IC, R2, [0, 2], 0, 0.0, 1000, 200.0, VALS, 1, 10,
+                      -15.0, 15.0, 0.05)
+    bh.synchronize()
+    np.testing.assert_array_equal(d.to_host(), m)
 
 
 # --------------------------------------------------------- Gram-Schmidt

@@ -514,11 +514,57 @@ __device__ __forceinline__ void infer_post_row(const float* o, float lleMean, fl
     lle[r] = l;
 }
 
-// Fused inference: a workgroup takes kRows rows (cell = list ? list[row] :
-// base + row) through the features, the three layers and the head with the
-// activations in LDS (the forward arithmetic of k_train_fused), then writes
-// the per-row predictions (cls != null) and/or the raw head outputs
-// (rawOut != null, [rows][kHead]).
+// Hidden layer forward with this wave's weight tiles already in registers
+// (w[i][kk]: unit tile 2*wave+i, k = 32*kk + kq..+7): the MFMA sequence,
+// bias, ReLU and BF16 rounding of fwd_hidden, so the activations are
+// bit-identical to it.
+template <int K>
+__device__ __forceinline__ void fwd_hidden_reg(const bf16x8 (&w)[2][K / 32],
+                                               const float* __restrict__ b, const bf16* in,
+                                               int ldIn, bf16* outRm, int wave, int li, int kq,
+                                               int rq)
+{
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < K / 32; ++kk) {
+        bf16x8 x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = ld8(in + (j * 16 + li) * ldIn + kk * 32 + kq);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[i][kk], x[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int u0 = (2 * wave + i) * 16 + rq;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float t = acc[i][j][e] + b[u0 + e];
+                v[e] = t > 0 ? t : 0.0f;
+            }
+            store_tile(outRm, kLdH, nullptr, u0, j * 16 + li, v);
+        }
+    }
+}
+
+// Fused inference, persistent: each workgroup loads its waves' weight tiles
+// into registers once (W1, W2 and W3: two 16-unit tiles per wave; Wh: both
+// tiles in every wave, 112 VGPRs in all) and then takes kRows-row tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... (cell = list ? list[row] : base +
+// row) through the features, the three layers and the head with the
+// activations in LDS.  The products, in the order of k_train_fused's forward
+// pass, give the same outputs as one workgroup per tile; only the weights'
+// global reads are gone.  Writes the per-row predictions (cls != null)
+// and/or the raw head outputs (rawOut != null, [rows][kHead]).
 __global__ __launch_bounds__(256) void k_infer_fused(
     long long rows, long long base, const int* __restrict__ list, int nLat,
     const float* __restrict__ Bff, const bf16* __restrict__ wbf, const float* __restrict__ p,
@@ -532,26 +578,62 @@ __global__ __launch_bounds__(256) void k_infer_fused(
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, kq = (lane >> 4) * 8, rq = (lane >> 4) * 4;
-    const long long row0 = (long long)blockIdx.x * kRows;
-    const int nrows = (int)min((long long)kRows, rows - row0);
 
-    features_rows(list, base, row0, nrows, nLat, Bff, sX, nullptr);
-    __syncthreads();
-    fwd_hidden<kFeat>(wbf + qW1, p + pB1, sX, kLdX, sH[0], nullptr, wave, li, kq, rq);
-    __syncthreads();
-    fwd_hidden<kHid>(wbf + qW2, p + pB2, sH[0], kLdH, sH[1], nullptr, wave, li, kq, rq);
-    __syncthreads();
-    fwd_hidden<kHid>(wbf + qW3, p + pB3, sH[1], kLdH, sH[0], nullptr, wave, li, kq, rq);
-    __syncthreads();
-    head_rows(wbf + qWh, p + pBh, sH[0], sOut, kHead + 1, wave, li, kq, rq);
-    __syncthreads();
-    if (rawOut)
-        for (int t = tid; t < nrows * kHead; t += 256)
-            rawOut[(row0 + t / kHead) * kHead + t % kHead] =
-                sOut[(t / kHead) * (kHead + 1) + t % kHead];
-    if (cls && tid < nrows)
-        infer_post_row(sOut + tid * (kHead + 1), lleMean, lleStd, chaosThr, row0 + tid, cls,
-                       chaos, conf, confChaos, lle);
+    bf16x8 w1[2][kFeat / 32], w2[2][kHid / 32], w3[2][kHid / 32], wh[2][kHid / 32];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int u = (2 * wave + i) * 16 + li;
+#pragma unroll
+        for (int kk = 0; kk < kFeat / 32; ++kk) w1[i][kk] = ld8(wbf + qW1 + u * kFeat + kk * 32 + kq);
+#pragma unroll
+        for (int kk = 0; kk < kHid / 32; ++kk) {
+            w2[i][kk] = ld8(wbf + qW2 + u * kHid + kk * 32 + kq);
+            w3[i][kk] = ld8(wbf + qW3 + u * kHid + kk * 32 + kq);
+            wh[i][kk] = ld8(wbf + qWh + (i * 16 + li) * kHid + kk * 32 + kq);
+        }
+    }
+    const long long tiles = (rows + kRows - 1) / kRows;
+    for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const long long row0 = tile * kRows;
+        const int nrows = (int)min((long long)kRows, rows - row0);
+        features_rows(list, base, row0, nrows, nLat, Bff, sX, nullptr);
+        __syncthreads();
+        fwd_hidden_reg<kFeat>(w1, p + pB1, sX, kLdX, sH[0], wave, li, kq, rq);
+        __syncthreads();
+        fwd_hidden_reg<kHid>(w2, p + pB2, sH[0], kLdH, sH[1], wave, li, kq, rq);
+        __syncthreads();
+        fwd_hidden_reg<kHid>(w3, p + pB3, sH[1], kLdH, sH[0], wave, li, kq, rq);
+        __syncthreads();
+        // head: this wave's row tile, both unit tiles (head_rows' sequence)
+        {
+            f32x4 acc[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+#pragma unroll
+            for (int kk = 0; kk < kHid / 32; ++kk) {
+                const bf16x8 x = ld8(sH[0] + (wave * 16 + li) * kLdH + kk * 32 + kq);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wh[i][kk], x, acc[i], 0, 0, 0);
+            }
+            const int r = wave * 16 + li;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int u = i * 16 + rq + e;
+                    sOut[r * (kHead + 1) + u] = acc[i][e] + p[pBh + u];
+                }
+        }
+        __syncthreads();
+        if (rawOut)
+            for (int t = tid; t < nrows * kHead; t += 256)
+                rawOut[(row0 + t / kHead) * kHead + t % kHead] =
+                    sOut[(t / kHead) * (kHead + 1) + t % kHead];
+        if (cls && tid < nrows)
+            infer_post_row(sOut + tid * (kHead + 1), lleMean, lleStd, chaosThr, row0 + tid, cls,
+                           chaos, conf, confChaos, lle);
+        // the next tile writes sX and sH[0] only after its first barrier, by
+        // which every wave has finished reading sOut and sH[0] of this one
+    }
 }
 
 __global__ void k_fill_f64(int n, double* p, double v)
@@ -842,7 +924,19 @@ hipError_t launch_infer_fused(long long rows, long long base, const int* list, i
                               float* rawOut, hipStream_t st)
 {
     if (rows <= 0) return hipSuccess;
-    const long long blocks = (rows + kRows - 1) / kRows;
+    const long long tiles = (rows + kRows - 1) / kRows;
+    // persistent: as many workgroups as the device holds at once
+    static int resident = 0;
+    if (resident == 0) {
+        int dev = 0, cus = 0, perCu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_infer_fused, 256, 0);
+        if (e != hipSuccess) return e;
+        resident = std::max(1, cus * std::max(1, perCu));
+    }
+    const long long blocks = std::min<long long>(tiles, resident);
     hipLaunchKernelGGL(k_infer_fused, dim3((unsigned)blocks), dim3(256), 0, st, rows, base, list,
                        nLat, Bff, wbf, p, lleMean, lleStd, chaosThr, cls, chaos, conf, confChaos,
                        lle, rawOut);

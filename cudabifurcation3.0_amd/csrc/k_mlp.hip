@@ -147,6 +147,8 @@ __global__ void k_adam(int n, float* __restrict__ p, float* __restrict__ m, floa
 // Rows past `rows` enter with zero features and a zero loss gradient and
 // add nothing.
 constexpr int kRows = 64;
+// k_train_fused: eight waves per 64-row workgroup, one 16-unit tile each
+constexpr int kTrainThreads = 512;
 constexpr int kTr = kRows + 8;          // transposed row length
 constexpr int kLdX = kFeat + 8;
 constexpr int kLdH = kHid + 8;
@@ -180,14 +182,15 @@ __device__ __forceinline__ void store_tile(bf16* rm, int ld, bf16* tr, int u0, i
 // Fourier features of kRows rows (k_features' arithmetic) into the
 // row-major layout and, when xt is given, the transposed one; cell of row r
 // = list ? list[row0 + r] : base + row0 + r; rows from nrows on are zero.
-// Each of the workgroup's 256 threads takes one row (its cell read and split
-// into lattice coordinates once) and every fourth feature of it.
-constexpr int kFeatThreadsPerRow = 256 / kRows;
-static_assert(kFeatThreadsPerRow * kRows == 256, "features: 256 threads per workgroup");
+// Each of the workgroup's NT threads takes one row (its cell read and split
+// into lattice coordinates once) and every (NT / kRows)-th feature of it.
+template <int NT>
 __device__ __forceinline__ void features_rows(const int* __restrict__ list, long long base,
                                               long long row0, int nrows, int nLat,
                                               const float* __restrict__ Bff, bf16* x, bf16* xt)
 {
+    constexpr int kFeatThreadsPerRow = NT / kRows;
+    static_assert(kFeatThreadsPerRow * kRows == NT, "features: NT threads per workgroup");
     const float inv = nLat > 1 ? 1.0f / (float)(nLat - 1) : 0.0f;
     const int r = threadIdx.x / kFeatThreadsPerRow;
     const bool live = r < nrows;
@@ -219,58 +222,56 @@ __device__ __forceinline__ void features_rows(const int* __restrict__ list, long
     }
 }
 
-// head outputs (FP32, + bias) of the kRows rows held row-major in h3: this
-// wave's row tile, both unit tiles
-__device__ __forceinline__ void head_rows(const bf16* __restrict__ Wh, const float* __restrict__ bh,
+// head outputs (FP32, + bias) of the kRows rows held row-major in h3, one
+// 16 x 16 tile per wave of eight: row tile (wave & 3), unit tile (wave >> 2)
+__device__ __forceinline__ void head_tile(const bf16* __restrict__ Wh, const float* __restrict__ bh,
                                           const bf16* h3, float* out, int ldOut, int wave, int li,
                                           int kq, int rq)
 {
-    f32x4 acc[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+    const int rt = wave & 3, ut = wave >> 2;
+    f32x4 acc = f32x4{0, 0, 0, 0};
 #pragma unroll
     for (int k0 = 0; k0 < kHid; k0 += 32) {
-        const bf16x8 x = ld8(h3 + (wave * 16 + li) * kLdH + k0 + kq);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                ld8(Wh + (i * 16 + li) * kHid + k0 + kq), x, acc[i], 0, 0, 0);
+        const bf16x8 x = ld8(h3 + (rt * 16 + li) * kLdH + k0 + kq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(Wh + (ut * 16 + li) * kHid + k0 + kq), x,
+                                                      acc, 0, 0, 0);
     }
-    const int r = wave * 16 + li;
+    const int r = rt * 16 + li;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int u = i * 16 + rq + e;
-            out[r * ldOut + u] = acc[i][e] + bh[u];
-        }
+    for (int e = 0; e < 4; ++e) {
+        const int u = ut * 16 + rq + e;
+        out[r * ldOut + u] = acc[e] + bh[u];
+    }
 }
 
-// hidden layer forward: out = relu(W in^T + b), W [kHid][K] (global BF16)
-template <int K>
+// hidden layer forward: out = relu(W in^T + b), W [kHid][K] (global BF16);
+// this wave's unit tiles are NI * wave .. NI * wave + NI - 1, all row tiles
+template <int K, int NI>
 __device__ __forceinline__ void fwd_hidden(const bf16* __restrict__ W, const float* __restrict__ b,
                                            const bf16* in, int ldIn, bf16* outRm, bf16* outTr,
                                            int wave, int li, int kq, int rq)
 {
-    f32x4 acc[2][4];
+    f32x4 acc[NI][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
 #pragma unroll
     for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 a[2], x[4];
+        bf16x8 a[NI], x[4];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = ld8(W + ((2 * wave + i) * 16 + li) * K + k0 + kq);
+        for (int i = 0; i < NI; ++i) a[i] = ld8(W + ((NI * wave + i) * 16 + li) * K + k0 + kq);
 #pragma unroll
         for (int j = 0; j < 4; ++j) x[j] = ld8(in + (j * 16 + li) * ldIn + k0 + kq);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < NI; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], x[j], acc[i][j], 0, 0, 0);
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int u0 = (2 * wave + i) * 16 + rq;
+    for (int i = 0; i < NI; ++i) {
+        const int u0 = (NI * wave + i) * 16 + rq;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             float v[4];
@@ -285,24 +286,24 @@ __device__ __forceinline__ void fwd_hidden(const bf16* __restrict__ W, const flo
 }
 
 // input gradient of a hidden layer, masked by the layer input's ReLU:
-// acc(k, r) = sum_m WT[k][m] dY[r][m] for this wave's 2 x 4 tiles
-template <int M>
+// acc(k, r) = sum_m WT[k][m] dY[r][m] for this wave's NI x 4 tiles
+template <int M, int NI>
 __device__ __forceinline__ void bwd_input(const bf16* __restrict__ WT, const bf16* dY, int ldY,
-                                          f32x4 (&acc)[2][4], int wave, int li, int kq)
+                                          f32x4 (&acc)[NI][4], int wave, int li, int kq)
 {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
 #pragma unroll
     for (int m0 = 0; m0 < M; m0 += 32) {
-        bf16x8 a[2], y[4];
+        bf16x8 a[NI], y[4];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = ld8(WT + ((2 * wave + i) * 16 + li) * M + m0 + kq);
+        for (int i = 0; i < NI; ++i) a[i] = ld8(WT + ((NI * wave + i) * 16 + li) * M + m0 + kq);
 #pragma unroll
         for (int j = 0; j < 4; ++j) y[j] = ld8(dY + (j * 16 + li) * ldY + m0 + kq);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < NI; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], y[j], acc[i][j], 0, 0, 0);
@@ -310,12 +311,13 @@ __device__ __forceinline__ void bwd_input(const bf16* __restrict__ WT, const bf1
 }
 
 // mask by (h > 0) from the row-major activation, then store both layouts
-__device__ __forceinline__ void store_masked(const f32x4 (&acc)[2][4], const bf16* hRm,
+template <int NI>
+__device__ __forceinline__ void store_masked(const f32x4 (&acc)[NI][4], const bf16* hRm,
                                              bf16* outRm, bf16* outTr, int wave, int li, int rq)
 {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int u0 = (2 * wave + i) * 16 + rq;
+    for (int i = 0; i < NI; ++i) {
+        const int u0 = (NI * wave + i) * 16 + rq;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int r = j * 16 + li;
@@ -329,8 +331,7 @@ __device__ __forceinline__ void store_masked(const f32x4 (&acc)[2][4], const bf1
 }
 
 // weight gradient dW[M x N] (+)= sum over the kRows rows of dY^T X from the
-// transposed layouts; this wave's rows of tiles are {2 wave, 2 wave + 1}
-// (TI = 2) or all of them with columns {2 wave, 2 wave + 1} (head, TI = M/16)
+// transposed layouts: tiles (i0 + i, j0 + j), i < TI, j < TJ
 template <int TI, int TJ>
 __device__ __forceinline__ void wgrad(const bf16* dYT, const bf16* XT, int i0, int j0,
                                       float* __restrict__ g, int ldG, int li, int kq, int rq)
@@ -364,7 +365,7 @@ __device__ __forceinline__ void wgrad(const bf16* dYT, const bf16* XT, int i0, i
 
 // list[row]: lattice cell of each batch row (features); targets are read at
 // clsAll/lleAll[tlist ? tlist[row] : list[row]].
-__global__ __launch_bounds__(256) void k_train_fused(
+__global__ __launch_bounds__(kTrainThreads) void k_train_fused(
     int rows, int nLat, const int* __restrict__ list, const int* __restrict__ tlist,
     const float* __restrict__ Bff,
     const bf16* __restrict__ wbf, const bf16* __restrict__ wbfT, const float* __restrict__ p,
@@ -388,17 +389,17 @@ __global__ __launch_bounds__(256) void k_train_fused(
     if (step && blockIdx.x == 0 && tid == 0) *step += 1;
 
     // Fourier features; rows past the batch are zero
-    features_rows(list, 0, row0, nrows, nLat, Bff, sX, sXT);
+    features_rows<kTrainThreads>(list, 0, row0, nrows, nLat, Bff, sX, sXT);
     __syncthreads();
 
-    // forward
-    fwd_hidden<kFeat>(wbf + qW1, p + pB1, sX, kLdX, sH[0], sHT[0], wave, li, kq, rq);
+    // forward: each of the eight waves takes one 16-unit tile of every row
+    fwd_hidden<kFeat, 1>(wbf + qW1, p + pB1, sX, kLdX, sH[0], sHT[0], wave, li, kq, rq);
     __syncthreads();
-    fwd_hidden<kHid>(wbf + qW2, p + pB2, sH[0], kLdH, sH[1], sHT[1], wave, li, kq, rq);
+    fwd_hidden<kHid, 1>(wbf + qW2, p + pB2, sH[0], kLdH, sH[1], sHT[1], wave, li, kq, rq);
     __syncthreads();
-    fwd_hidden<kHid>(wbf + qW3, p + pB3, sH[1], kLdH, sH[2], sHT[2], wave, li, kq, rq);
+    fwd_hidden<kHid, 1>(wbf + qW3, p + pB3, sH[1], kLdH, sH[2], sHT[2], wave, li, kq, rq);
     __syncthreads();
-    head_rows(wbf + qWh, p + pBh, sH[2], sOut, kHead + 1, wave, li, kq, rq);
+    head_tile(wbf + qWh, p + pBh, sH[2], sOut, kHead + 1, wave, li, kq, rq);
     __syncthreads();
 
     // loss gradient, one row per lane of wave 0
@@ -437,30 +438,31 @@ __global__ __launch_bounds__(256) void k_train_fused(
     }
     __syncthreads();
 
-    // head: dWh = dOut^T H3; dH3 = (dOut Wh) masked by H3
-    wgrad<2, 2>(sDOT, sHT[2], 0, 2 * wave, gp + pWh, kHid, li, kq, rq);
-    f32x4 acc[2][4];
-    bwd_input<kHead>(wbfT + tWh, sDO, kLdO, acc, wave, li, kq);
+    // head: dWh = dOut^T H3 (this wave: both head tiles of H3 unit tile
+    // `wave`); dH3 = (dOut Wh) masked by H3
+    wgrad<2, 1>(sDOT, sHT[2], 0, wave, gp + pWh, kHid, li, kq, rq);
+    f32x4 acc[1][4];
+    bwd_input<kHead, 1>(wbfT + tWh, sDO, kLdO, acc, wave, li, kq);
     __syncthreads();   // H3 (both layouts) is read for the last time above
-    store_masked(acc, sH[2], sH[2], sHT[2], wave, li, rq);
+    store_masked<1>(acc, sH[2], sH[2], sHT[2], wave, li, rq);
     __syncthreads();
     // layer 3: dW3 = dH3^T H2; dH2 = (dH3 W3) masked by H2
-    wgrad<2, 8>(sHT[2], sHT[1], 2 * wave, 0, gp + pW3, kHid, li, kq, rq);
-    bwd_input<kHid>(wbfT + tW3, sH[2], kLdH, acc, wave, li, kq);
+    wgrad<1, 8>(sHT[2], sHT[1], wave, 0, gp + pW3, kHid, li, kq, rq);
+    bwd_input<kHid, 1>(wbfT + tW3, sH[2], kLdH, acc, wave, li, kq);
     __syncthreads();
-    store_masked(acc, sH[1], sH[1], sHT[1], wave, li, rq);
+    store_masked<1>(acc, sH[1], sH[1], sHT[1], wave, li, rq);
     __syncthreads();
     // layer 2: dW2 = dH2^T H1; dH1 = (dH2 W2) masked by H1
-    wgrad<2, 8>(sHT[1], sHT[0], 2 * wave, 0, gp + pW2, kHid, li, kq, rq);
-    bwd_input<kHid>(wbfT + tW2, sH[1], kLdH, acc, wave, li, kq);
+    wgrad<1, 8>(sHT[1], sHT[0], wave, 0, gp + pW2, kHid, li, kq, rq);
+    bwd_input<kHid, 1>(wbfT + tW2, sH[1], kLdH, acc, wave, li, kq);
     __syncthreads();
-    store_masked(acc, sH[0], sH[0], sHT[0], wave, li, rq);
+    store_masked<1>(acc, sH[0], sH[0], sHT[0], wave, li, rq);
     __syncthreads();
     // layer 1: dW1 = dH1^T X
-    wgrad<2, 4>(sHT[0], sXT, 2 * wave, 0, gp + pW1, kFeat, li, kq, rq);
+    wgrad<1, 4>(sHT[0], sXT, wave, 0, gp + pW1, kFeat, li, kq, rq);
 
     // bias gradients: column sums over the rows of dOut, dH3, dH2, dH1
-    for (int c = tid; c < kHead + 3 * kHid; c += 256) {
+    for (int c = tid; c < kHead + 3 * kHid; c += kTrainThreads) {
         const bf16* src;
         int off;
         if (c < kHead) { src = sDOT + c * kTr; off = pBh + c; }
@@ -596,7 +598,7 @@ __global__ __launch_bounds__(256) void k_infer_fused(
     for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const long long row0 = tile * kRows;
         const int nrows = (int)min((long long)kRows, rows - row0);
-        features_rows(list, base, row0, nrows, nLat, Bff, sX, nullptr);
+        features_rows<256>(list, base, row0, nrows, nLat, Bff, sX, nullptr);
         __syncthreads();
         fwd_hidden_reg<kFeat>(w1, p + pB1, sX, kLdX, sH[0], wave, li, kq, rq);
         __syncthreads();
@@ -950,8 +952,8 @@ hipError_t launch_train_fused(int rows, int nLat, const int* list, const int* tl
                               hipStream_t st)
 {
     if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_train_fused, dim3(train_fused_blocks(rows)), dim3(256), 0, st, rows,
-                       nLat, list, tlist, Bff, wbf, wbfT, p, cls, lle, wLle, chaosStd, gpart,
+    hipLaunchKernelGGL(k_train_fused, dim3(train_fused_blocks(rows)), dim3(kTrainThreads), 0, st,
+                       rows, nLat, list, tlist, Bff, wbf, wbfT, p, cls, lle, wLle, chaosStd, gpart,
                        stride, loss, step);
     return hipGetLastError();
 }

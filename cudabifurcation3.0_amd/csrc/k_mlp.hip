@@ -61,13 +61,21 @@ __device__ __forceinline__ void loss_row(const float* o, int y, float target, fl
 }
 
 // sum_z part[z * stride + i] over z < splits, added in z order; the loads
-// go out eight at a time so their latencies overlap (one load in flight at
-// a time made the sum wait for every partial in turn)
+// go out sixteen (then eight) at a time so their latencies overlap (one
+// load in flight at a time made the sum wait for every partial in turn;
+// sixteen against eight: 20 epochs of 2048-row steps 15.75 against 15.98 ms)
 __device__ __forceinline__ float sum_partials(const float* __restrict__ part, int splits,
                                               long long stride, int i)
 {
     float s = 0;
     int z = 0;
+    for (; z + 16 <= splits; z += 16) {
+        float t[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) t[u] = part[(long long)(z + u) * stride + i];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += t[u];
+    }
     for (; z + 8 <= splits; z += 8) {
         float t[8];
 #pragma unroll

@@ -557,6 +557,35 @@ def test_kde_device_plateaus_zero_runs_and_special_values(orc, bh, dev):
     assert agree >= 0.995
 
 
+def test_kde_device_many_peak_chunks(orc, bh, dev):
+    """Peak sets of many 64-peak chunks (the kernel loads a cell's peaks 64
+    at a time for every batch of sample points): sizes on both sides of
+    chunk boundaries up to 1 000 peaks, clustered and spread amplitudes, a
+    NaN in a late chunk; counts against the oracle."""
+    rng = np.random.default_rng(23)
+    sizes = [63, 64, 65, 127, 128, 129, 255, 256, 257, 640, 1000]
+    stride = 1024
+    got, want = [], []
+    for k, P in enumerate(sizes):
+        if k % 2:
+            a = rng.integers(0, 6, size=P) * 1.5 + rng.normal(size=P) * 0.01
+        else:
+            a = rng.random(P) * 12
+        if P == 640:
+            a[600] = np.nan
+        amps = np.zeros(stride)
+        amps[:P] = a
+        A, Cn = _t(bh, amps, np.float64), _t(bh, np.array([P], np.int32), np.int32)
+        out = bh.DeviceArray((1,), np.int32).zero_()
+        bh.kde_device(A.ptr, stride, 1, Cn.ptr, 100000, 2, -15.0, 15.0, 0.05, out.ptr)
+        bh.synchronize()
+        got.append(int(out.to_host()[0]))
+        want.append(orc.kde(a, 100000, 2, -15.0, 15.0, 0.05))
+    say("kde many chunks", got, want)
+    assert float(np.mean(np.array(got) == np.array(want))) >= 0.9
+    assert sum(g != w for g, w in zip(got, want)) <= 1
+
+
 def test_kde_map_agreement(orc, bh):
     say("kde map 24^2")
     m = bh.kde_map(200.0, 24, 0.01, IC, R2, [0, 2], 0, 0.0, 1000, 200.0, VALS, 1, 10, -15.0,
